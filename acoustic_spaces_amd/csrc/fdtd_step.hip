@@ -1,0 +1,1030 @@
+// The wave solver's time step on MI355X (gfx950): one fused launch per step.
+//
+// The scheme is the reference's (cpp/include/acoustic_spaces/fdtd.hpp:17-75, run() at
+// 786-904): a two-level pressure leapfrog, p+ = (2 - G) p + sum_k g_k p_k - p-, over the
+// grid's interior cells; the special cells (boundary, fitted-wall and source cells) by
+// their CSR links, admittance-filter branch states and source; the split-field matched
+// layer; and the probes, read before the update. Every update keeps the reference's
+// operations and their order, compiled without contraction (-ffp-contract=off) and with
+// single-precision denormals flushed (-fgpu-flush-denormals-to-zero), as the reference
+// runs with FTZ/DAZ (fdtd.hpp:314-336): in float the fields equal the reference's bit for
+// bit. The probes are summed in double, as the reference's record() (fdtd.hpp:3525-3534).
+//
+// Layout in HBM. The two pressure fields are float (or double) arrays of nx x ny x nz
+// cells, x fastest, each row padded to a pitch of px cells (a multiple of the row
+// alignment, 64 floats = 256 bytes by default), so that a wavefront's 64 lanes read one
+// aligned row segment. The cell classes are one byte per cell. The update of the
+// interior runs over tiles of 64 x TY cells of a plane, marching TZ planes along z with
+// the planes below and above the current one held in registers; only tiles holding
+// interior cells are launched (a list made at set-up). Special cells, their links and
+// branch states are CSR arrays with the links' neighbour offsets precomputed for the
+// padded grid; the matched layer's nine state values are nine planes of its cells
+// (structure of arrays).
+//
+// Work of one step, by block: interior tiles, then specials, then layer cells, then the
+// probes. The four sets write disjoint cells of the next field and read only the current
+// field (and their own cells of the previous one), so one launch carries them all.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "asfdtd.h"
+
+namespace {
+
+constexpr int kTileX = 64;   // cells of a tile along x: one wavefront
+constexpr int kTileRows = 4; // rows (y) of a tile: one wavefront each
+constexpr int kBlock = kTileX * kTileRows;
+
+constexpr int kSteps[18][3] = {
+    {-1, 0, 0},  {1, 0, 0},  {0, -1, 0}, {0, 1, 0},  {0, 0, -1}, {0, 0, 1},
+    {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0},  {-1, 0, -1}, {1, 0, -1},
+    {-1, 0, 1},  {1, 0, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1}};
+
+// Everything one step's launch reads, by value (kernel arguments).
+template <typename T>
+struct StepArgs {
+    T *__restrict__ a;        // previous pressure, overwritten by the next
+    const T *__restrict__ b;  // current pressure
+    T *__restrict__ wa;       // branch velocities, the same way
+    const T *__restrict__ wb;
+    // interior
+    const uint8_t *__restrict__ types;
+    const float *__restrict__ scale;  // padded, or null
+    const uint32_t *__restrict__ tiles;
+    uint32_t tile_blocks;
+    uint32_t tiles_x, tiles_y;
+    int nx, ny, nz, px;
+    int tile_z;
+    long long sxy;
+    T axial, diagonal, centre, total;
+    // specials, in blocks of kBlock sorted by their counts of links and states; each
+    // block's links and states in a slab of (its widest count) x kBlock, slot-major
+    uint32_t specials, special_blocks;
+    const uint32_t *__restrict__ sp_cell;
+    const uint8_t *__restrict__ sp_links;   // links of each special
+    const uint8_t *__restrict__ sp_states;  // branch states of each special
+    const uint32_t *__restrict__ link_base; // per block: its link slab
+    const uint32_t *__restrict__ state_base;
+    const int32_t *__restrict__ link_offset;
+    const float *__restrict__ link_weight;
+    const float *__restrict__ coef_a;
+    const float *__restrict__ coef_b;
+    const float *__restrict__ coef_inv;
+    const float *__restrict__ source;
+    const float *__restrict__ state_weight;
+    const float *__restrict__ state_now;
+    const float *__restrict__ state_before;
+    const float *__restrict__ state_gain;
+    const float *__restrict__ signal;  // device, indexed by step
+    // matched layer
+    uint32_t layer_cells, layer_blocks;
+    const uint32_t *__restrict__ layer_cell;
+    const uint64_t *__restrict__ layer_at;  // x | y << 21 | z << 42
+    T *__restrict__ layer_state;            // 9 planes of layer_cells
+    const float *__restrict__ layer_scale;  // or null
+    const float *__restrict__ keep[6];
+    const float *__restrict__ push[6];
+    // probes
+    uint32_t probes;
+    const uint32_t *__restrict__ probe_start;
+    const uint32_t *__restrict__ probe_cell;
+    const double *__restrict__ probe_weight;
+    double *__restrict__ out;
+    unsigned long long out_stride;
+    unsigned long long step;  // index into signal and out
+};
+
+template <typename T, bool WIDE, bool SCALED>
+__device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t block) {
+    const uint32_t tile = s.tiles[block];
+    const uint32_t tx = tile % s.tiles_x;
+    const uint32_t rest = tile / s.tiles_x;
+    const uint32_t ty = rest % s.tiles_y;
+    const uint32_t tz = rest / s.tiles_y;
+    const int x = static_cast<int>(tx) * kTileX + static_cast<int>(threadIdx.x % kTileX);
+    const int y = 1 + static_cast<int>(ty) * kTileRows + static_cast<int>(threadIdx.x / kTileX);
+    if (x >= s.nx || y >= s.ny - 1) return;
+    const int z0 = 1 + static_cast<int>(tz) * s.tile_z;
+    const int z1 = min(z0 + s.tile_z, s.nz - 1);
+    // 32-bit indices: the padded grid holds fewer than 2^31 cells (checked at set-up).
+    const int sy = s.px, sz = static_cast<int>(s.sxy);
+    int i = x + y * sy + z0 * sz;
+    const T *__restrict__ b = s.b;
+    if constexpr (!WIDE) {
+        // The column in chunks of kUnroll planes: every load of a chunk (the planes above,
+        // the four in-plane neighbours, the previous pressure and the classes) is issued
+        // before any arithmetic, so a wavefront keeps some thirty loads in flight instead
+        // of waiting twice a plane. Each address lies in the padded grid whatever the
+        // cell's class (x < nx <= px, 1 <= y <= ny - 2, 1 <= z <= nz - 2), so the loads need
+        // no guard; only interior cells are written.
+        constexpr int kUnroll = 4;
+        T col[kUnroll + 2];  // planes z - 1, z, z + 1, ..., z + kUnroll
+        col[0] = b[i - sz];
+        col[1] = b[i];
+        for (int z = z0; z < z1; z += kUnroll, i += kUnroll * sz) {
+            const int count = min(kUnroll, z1 - z);
+            T xm[kUnroll], xp[kUnroll], ym[kUnroll], yp[kUnroll], prev[kUnroll];
+            uint8_t cls[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (u < count) {
+                    const int j = i + u * sz;
+                    col[u + 2] = b[j + sz];
+                    xm[u] = b[j - 1];
+                    xp[u] = b[j + 1];
+                    ym[u] = b[j - sy];
+                    yp[u] = b[j + sy];
+                    prev[u] = s.a[j];
+                    cls[u] = s.types[j];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (u < count && cls[u] == 1) {
+                    // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
+                    T sum = xm[u] + xp[u];
+                    sum = sum + ym[u];
+                    sum = sum + yp[u];
+                    sum = sum + col[u];
+                    sum = sum + col[u + 2];
+                    T next;
+                    if constexpr (SCALED) {
+                        const T sc = static_cast<T>(s.scale[i + u * sz]);
+                        const T c0 = static_cast<T>(2) - s.total * sc;
+                        const T ax = sc * s.axial;
+                        next = c0 * col[u + 1] + ax * sum;
+                    } else {
+                        next = s.centre * col[u + 1] + s.axial * sum;
+                    }
+                    s.a[i + u * sz] = next - prev[u];
+                }
+            }
+            if (count == kUnroll) {
+                col[0] = col[kUnroll];
+                col[1] = col[kUnroll + 1];
+            }
+        }
+        return;
+    }
+    T below = b[i - sz];
+    T here = b[i];
+    for (int z = z0; z < z1; ++z, i += sz) {
+        const T above = b[i + sz];
+        if (s.types[i] == 1) {
+            // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
+            T sum = b[i - 1] + b[i + 1];
+            sum = sum + b[i - sy];
+            sum = sum + b[i + sy];
+            sum = sum + below;
+            sum = sum + above;
+            T next;
+            if constexpr (SCALED) {
+                const T sc = static_cast<T>(s.scale[i]);
+                const T c0 = static_cast<T>(2) - s.total * sc;
+                const T ax = sc * s.axial;
+                next = c0 * here + ax * sum;
+            } else {
+                next = s.centre * here + s.axial * sum;
+            }
+            if constexpr (WIDE) {
+                T edges = b[i - 1 - sy] + b[i + 1 - sy];
+                edges = edges + b[i - 1 + sy];
+                edges = edges + b[i + 1 + sy];
+                edges = edges + b[i - 1 - sz];
+                edges = edges + b[i + 1 - sz];
+                edges = edges + b[i - 1 + sz];
+                edges = edges + b[i + 1 + sz];
+                edges = edges + b[i - sy - sz];
+                edges = edges + b[i + sy - sz];
+                edges = edges + b[i - sy + sz];
+                edges = edges + b[i + sy + sz];
+                if constexpr (SCALED) {
+                    const T dg = static_cast<T>(s.scale[i]) * s.diagonal;
+                    next = next + dg * edges;
+                } else {
+                    next = next + s.diagonal * edges;
+                }
+            }
+            s.a[i] = next - s.a[i];
+        }
+        below = here;
+        here = above;
+    }
+}
+
+// A special cell (fdtd.hpp:860-881): its links summed in their order, its branch states'
+// terms, the source, coef_inv applied last; then its states' velocities. The links and
+// states sit in the block's slabs (slot-major, so a wavefront's loads of a slot coalesce),
+// read in chunks whose loads are all issued before the sums.
+template <typename T>
+__device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t block) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t k = block * kBlock + t;
+    if (k >= s.specials) return;
+    const int i = static_cast<int>(s.sp_cell[k]);
+    const int nl = s.sp_links[k], ns = s.sp_states[k];
+    const T *__restrict__ b = s.b;
+    const uint32_t lb = s.link_base[block] + t;
+    constexpr int kLinkChunk = 6;
+    T sum = 0;
+    for (int c = 0; c < nl; c += kLinkChunk) {
+        int32_t off[kLinkChunk];
+        float w[kLinkChunk];
+        T v[kLinkChunk];
+#pragma unroll
+        for (int u = 0; u < kLinkChunk; ++u) {
+            if (c + u < nl) {
+                off[u] = s.link_offset[lb + (c + u) * kBlock];
+                w[u] = s.link_weight[lb + (c + u) * kBlock];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLinkChunk; ++u) {
+            if (c + u < nl) v[u] = b[i + off[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kLinkChunk; ++u) {
+            if (c + u < nl) sum += static_cast<T>(w[u]) * v[u];
+        }
+    }
+    const uint32_t sb = s.state_base[block] + t;
+    T flow = 0;
+    for (int j = 0; j < ns; ++j) {
+        const uint32_t q = sb + j * kBlock;
+        const T own = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
+        flow += static_cast<T>(s.state_weight[q]) * own;
+    }
+    const T previous = s.a[i];
+    T acc = static_cast<T>(s.coef_a[k]) * b[i];
+    acc = acc + sum;
+    acc = acc - static_cast<T>(s.coef_b[k]) * previous;
+    acc = acc - flow;
+    acc = acc + static_cast<T>(s.source[k]) * static_cast<T>(s.signal[s.step]);
+    const T next = acc * static_cast<T>(s.coef_inv[k]);
+    s.a[i] = next;
+    for (int j = 0; j < ns; ++j) {
+        // The term again: wa has not changed since it was taken.
+        const uint32_t q = sb + j * kBlock;
+        const T own = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
+        s.wa[q] = s.wa[q] + ((next - previous) * static_cast<T>(s.state_gain[q]) + own);
+    }
+}
+
+// A matched-layer cell (fdtd.hpp:2242-2266): its six face velocities, then the three
+// parts of its pressure, from the current field into the next.
+template <typename T>
+__device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
+    const long long i = s.layer_cell[c];
+    const uint64_t at = s.layer_at[c];
+    const uint32_t coord[3] = {static_cast<uint32_t>(at & 0x1FFFFF), static_cast<uint32_t>((at >> 21) & 0x1FFFFF),
+                               static_cast<uint32_t>(at >> 42)};
+    const long long stride[3] = {1, s.px, s.sxy};
+    const uint32_t L = s.layer_cells;
+    T *__restrict__ st = s.layer_state;
+    const T *__restrict__ b = s.b;
+    const T bi = b[i];
+    T total = 0;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        const uint32_t x = coord[axis];
+        const float *fk = s.keep[2 * axis + 1];
+        const float *fp = s.push[2 * axis + 1];
+        T below = st[(3 + 2 * axis) * static_cast<size_t>(L) + c];
+        T above = st[(4 + 2 * axis) * static_cast<size_t>(L) + c];
+        below = static_cast<T>(fk[x]) * below - static_cast<T>(fp[x]) * (bi - b[i - stride[axis]]);
+        above = static_cast<T>(fk[x + 1]) * above - static_cast<T>(fp[x + 1]) * (b[i + stride[axis]] - bi);
+        st[(3 + 2 * axis) * static_cast<size_t>(L) + c] = below;
+        st[(4 + 2 * axis) * static_cast<size_t>(L) + c] = above;
+        T part = st[axis * static_cast<size_t>(L) + c];
+        if (s.layer_scale != nullptr) {
+            const T ps = static_cast<T>(s.push[2 * axis][x]) * static_cast<T>(s.layer_scale[c]);
+            part = static_cast<T>(s.keep[2 * axis][x]) * part - ps * (above - below);
+        } else {
+            part = static_cast<T>(s.keep[2 * axis][x]) * part - static_cast<T>(s.push[2 * axis][x]) * (above - below);
+        }
+        st[axis * static_cast<size_t>(L) + c] = part;
+        total += part;
+    }
+    s.a[i] = total;
+}
+
+// A probe (fdtd.hpp:3525-3534): its cells' current pressures, weighted, summed in double
+// in the order of its cells.
+template <typename T>
+__device__ __forceinline__ void record_probe(const StepArgs<T> &s, uint32_t p) {
+    double value = 0.0;
+    for (uint32_t k = s.probe_start[p]; k < s.probe_start[p + 1]; ++k) {
+        value += s.probe_weight[k] * static_cast<double>(s.b[s.probe_cell[k]]);
+    }
+    s.out[p * s.out_stride + s.step] = value;
+}
+
+// One step in one launch: special blocks first (their gathers are the longest chains of
+// loads, so they start first), then interior tiles, layer cells and the probes.
+template <typename T, bool WIDE, bool SCALED>
+__global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
+    uint32_t block = blockIdx.x;
+    if (block < s.special_blocks) {
+        special_cell<T>(s, block);
+        return;
+    }
+    block -= s.special_blocks;
+    if (block < s.tile_blocks) {
+        interior_tile<T, WIDE, SCALED>(s, block);
+        return;
+    }
+    block -= s.tile_blocks;
+    if (block < s.layer_blocks) {
+        const uint32_t c = block * kBlock + threadIdx.x;
+        if (c < s.layer_cells) layer_cell<T>(s, c);
+        return;
+    }
+    block -= s.layer_blocks;
+    const uint32_t p = block * kBlock + threadIdx.x;
+    if (p < s.probes) record_probe<T>(s, p);
+}
+
+// The same work as four launches, one per kind of cell (the split launch shape).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void special_kernel(const StepArgs<T> s) {
+    special_cell<T>(s, blockIdx.x);
+}
+
+template <typename T, bool WIDE, bool SCALED>
+__global__ __launch_bounds__(kBlock) void interior_kernel(const StepArgs<T> s) {
+    interior_tile<T, WIDE, SCALED>(s, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void layer_kernel(const StepArgs<T> s) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < s.layer_cells) layer_cell<T>(s, c);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void probe_kernel(const StepArgs<T> s) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < s.probes) record_probe<T>(s, p);
+}
+
+thread_local std::string g_create_error;
+
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct as_fdtd {
+    int precision = 32;
+    int wide = 0, scaled = 0;
+    int nx = 0, ny = 0, nz = 0, px = 0, tile_z = 16;
+    long long sxy = 0;
+    size_t n_pad = 0;
+    uint32_t tiles_x = 0, tiles_y = 0;
+    uint32_t tile_count = 0, specials = 0, states = 0, layer_cells = 0, probes = 0;
+    uint64_t interior_cells = 0;
+    double axial = 0, diagonal = 0, centre = 0, total = 0;
+    unsigned long long done = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocations;
+    // device arrays
+    void *u0 = nullptr, *u1 = nullptr, *w0 = nullptr, *w1 = nullptr, *layer_state = nullptr;
+    uint8_t *types = nullptr;
+    float *scale = nullptr;
+    uint32_t *tiles = nullptr;
+    uint32_t *sp_cell = nullptr, *link_base = nullptr, *state_base = nullptr;
+    uint8_t *sp_links = nullptr, *sp_states = nullptr;
+    int32_t *link_offset = nullptr;
+    float *link_weight = nullptr, *coef_a = nullptr, *coef_b = nullptr, *coef_inv = nullptr, *source = nullptr;
+    float *state_weight = nullptr, *state_now = nullptr, *state_before = nullptr, *state_gain = nullptr;
+    size_t state_slots = 0;   // the state slabs' length (states and padding)
+    int split = 0;            // 1: four launches a step instead of one
+    uint32_t *layer_cell = nullptr;
+    uint64_t *layer_at = nullptr;
+    float *layer_scale = nullptr;
+    float *keep[6] = {}, *push[6] = {};
+    uint32_t *probe_start = nullptr, *probe_cell = nullptr;
+    double *probe_weight = nullptr;
+    // scratch for run() on host pointers
+    float *signal_buf = nullptr;
+    size_t signal_cap = 0;
+    double *out_buf = nullptr;
+    size_t out_cap = 0;
+    // graphs of graph_steps steps (see ensure_graphs)
+    uint32_t graph_steps = 64;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    float *sig_stage = nullptr;
+    double *out_stage = nullptr;
+    std::string error;
+};
+
+namespace {
+
+struct Failure {
+    int code;
+    std::string what;
+};
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw Failure{AS_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template <typename U>
+U *upload(as_fdtd *h, const std::vector<U> &v) {
+    if (v.empty()) return nullptr;
+    void *p = nullptr;
+    check(hipMalloc(&p, v.size() * sizeof(U)), "hipMalloc");
+    h->allocations.push_back(p);
+    check(hipMemcpy(p, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice), "hipMemcpy");
+    return static_cast<U *>(p);
+}
+
+void *zeros(as_fdtd *h, size_t bytes) {
+    void *p = nullptr;
+    check(hipMalloc(&p, std::max<size_t>(bytes, 256)), "hipMalloc");
+    h->allocations.push_back(p);
+    check(hipMemset(p, 0, std::max<size_t>(bytes, 256)), "hipMemset");
+    return p;
+}
+
+void refuse(const std::string &what) { throw Failure{AS_EINVAL, what}; }
+
+void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tuning) {
+    if (precision != 32 && precision != 64) refuse("precision is 32 or 64");
+    if (L.stencil != 7 && L.stencil != 19) refuse("the stencil has 7 or 19 points");
+    if (L.nx < 3 || L.ny < 3 || L.nz < 3) refuse("the grid is too small");
+    if (L.n_weights < (L.stencil == 7 ? 6u : 18u)) refuse("the layout's weights are too few for its stencil");
+    if (L.types == nullptr) refuse("the layout has no cell types");
+    h->precision = precision;
+    h->wide = L.stencil == 19;
+    h->scaled = L.scale != nullptr;
+    h->nx = static_cast<int>(L.nx);
+    h->ny = static_cast<int>(L.ny);
+    h->nz = static_cast<int>(L.nz);
+    int align = tuning && tuning->row_align > 0 ? tuning->row_align : 64;
+    if (align != 1 && align != 16 && align != 32 && align != 64) refuse("the row alignment is 1, 16, 32 or 64 cells");
+    h->tile_z = tuning && tuning->tile_z > 0 ? tuning->tile_z : 16;
+    if (tuning && tuning->tile_y > 0 && tuning->tile_y != kTileRows) refuse("tile_y is 4 (one wavefront a row)");
+    if (tuning && tuning->graph_steps != 0) h->graph_steps = tuning->graph_steps < 0 ? 0u : static_cast<uint32_t>(tuning->graph_steps);
+    if (h->graph_steps > 4096) refuse("at most 4096 steps a graph");
+    if (tuning && tuning->split > 0) h->split = 1;
+    const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
+    const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);
+    const uint64_t n_pad = px * ny * nz;
+    if (n_pad >= (1ull << 31) || nx >= (1u << 21) || ny >= (1u << 21) || nz >= (1u << 21)) {
+        throw Failure{AS_ERANGE, "the grid has too many cells for 32-bit indices"};
+    }
+    h->px = static_cast<int>(px);
+    h->sxy = static_cast<long long>(px * ny);
+    h->n_pad = n_pad;
+    const auto pad = [&](uint64_t i) -> uint64_t {
+        const uint64_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+        return x + px * (y + ny * z);
+    };
+    h->axial = L.weights[0];
+    h->diagonal = h->wide ? L.weights[6] : 0.0;
+    h->centre = L.centre;
+    h->total = L.total;
+
+    // Cell classes, padded, and the tiles that hold interior cells.
+    std::vector<uint8_t> types(n_pad, 0);
+    std::vector<float> scale;
+    if (h->scaled) scale.assign(n_pad, 1.0f);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t p = pad(i);
+        types[p] = L.types[i];
+        if (h->scaled) scale[p] = L.scale[i];
+    }
+    h->tiles_x = static_cast<uint32_t>((nx + kTileX - 1) / kTileX);
+    h->tiles_y = static_cast<uint32_t>((ny - 2 + kTileRows - 1) / kTileRows);
+    const uint32_t tiles_z = static_cast<uint32_t>((nz - 2 + h->tile_z - 1) / h->tile_z);
+    std::vector<uint32_t> tiles;
+    uint64_t interior = 0;
+    for (uint32_t tz = 0; tz < tiles_z; ++tz) {
+        for (uint32_t ty = 0; ty < h->tiles_y; ++ty) {
+            for (uint32_t tx = 0; tx < h->tiles_x; ++tx) {
+                uint64_t count = 0;
+                const uint64_t z0 = 1 + static_cast<uint64_t>(tz) * h->tile_z;
+                const uint64_t z1 = std::min<uint64_t>(z0 + h->tile_z, nz - 1);
+                const uint64_t y0 = 1 + static_cast<uint64_t>(ty) * kTileRows;
+                const uint64_t y1 = std::min<uint64_t>(y0 + kTileRows, ny - 1);
+                const uint64_t x0 = static_cast<uint64_t>(tx) * kTileX;
+                const uint64_t x1 = std::min<uint64_t>(x0 + kTileX, nx);
+                for (uint64_t z = z0; z < z1; ++z)
+                    for (uint64_t y = y0; y < y1; ++y)
+                        for (uint64_t x = x0; x < x1; ++x) count += types[x + px * (y + ny * z)] == 1;
+                if (count) tiles.push_back(tx + h->tiles_x * (ty + h->tiles_y * tz));
+                interior += count;
+            }
+        }
+    }
+    h->tile_count = static_cast<uint32_t>(tiles.size());
+    h->interior_cells = interior;
+
+    // Specials: sorted by their counts of links and states (a special's update reads only
+    // its own cell, links and states, so their order is free), in blocks of kBlock; each
+    // block's links and states in slabs of (its largest count) x kBlock, slot-major. The
+    // links are offsets on the padded grid; each state carries its branch's coefficients.
+    const uint64_t K = L.specials;
+    if (K > 0 && (L.special == nullptr || L.link_start == nullptr || L.state_start == nullptr)) {
+        refuse("the layout's special cells are incomplete");
+    }
+    if (K >= (1ull << 31) || L.links >= (1ull << 32) || L.states >= (1ull << 32)) {
+        throw Failure{AS_ERANGE, "too many special cells, links or states for 32-bit indices"};
+    }
+    if (K > 0 && (L.link_start[0] != 0 || L.state_start[0] != 0 || L.link_start[K] != L.links ||
+                  L.state_start[K] != L.states)) {
+        refuse("the layout's link or state ranges do not match their arrays");
+    }
+    std::vector<int32_t> offset(18);
+    for (int k = 0; k < 18; ++k) {
+        offset[static_cast<size_t>(k)] = static_cast<int32_t>(kSteps[k][0] + kSteps[k][1] * static_cast<long long>(px) +
+                                                              kSteps[k][2] * static_cast<long long>(px * ny));
+    }
+    std::vector<uint64_t> order(K);
+    for (uint64_t k = 0; k < K; ++k) {
+        order[k] = k;
+        const int64_t i = L.special[k];
+        if (i < 0 || static_cast<uint64_t>(i) >= n) refuse("a special cell lies outside the grid");
+        const uint64_t x = static_cast<uint64_t>(i) % nx, y = (static_cast<uint64_t>(i) / nx) % ny,
+                       z = static_cast<uint64_t>(i) / (nx * ny);
+        if (x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz) {
+            refuse("a special cell lies on the grid's edge");
+        }
+        if (L.link_start[k + 1] < L.link_start[k] || L.state_start[k + 1] < L.state_start[k]) {
+            refuse("the layout's link or state ranges run backwards");
+        }
+        if (L.link_start[k + 1] - L.link_start[k] > 18) refuse("a special cell has more than 18 links");
+        if (L.state_start[k + 1] - L.state_start[k] > 255) refuse("a special cell has more than 255 branch states");
+    }
+    const auto links_of = [&](uint64_t k) { return L.link_start[k + 1] - L.link_start[k]; };
+    const auto states_of = [&](uint64_t k) { return L.state_start[k + 1] - L.state_start[k]; };
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t p, uint64_t q) {
+        return links_of(p) != links_of(q) ? links_of(p) < links_of(q) : states_of(p) < states_of(q);
+    });
+    const uint64_t blocks = (K + kBlock - 1) / kBlock;
+    std::vector<uint32_t> sp_cell(K), link_base(blocks), state_base(blocks);
+    std::vector<uint8_t> sp_links(K), sp_states(K);
+    std::vector<float> coef_a(K), coef_b(K), coef_inv(K), source(K);
+    uint64_t link_slots = 0, state_slots = 0;
+    for (uint64_t B = 0; B < blocks; ++B) {
+        uint64_t wl = 0, ws = 0;
+        for (uint64_t t = 0; t < kBlock && B * kBlock + t < K; ++t) {
+            wl = std::max(wl, links_of(order[B * kBlock + t]));
+            ws = std::max(ws, states_of(order[B * kBlock + t]));
+        }
+        link_base[B] = static_cast<uint32_t>(link_slots);
+        state_base[B] = static_cast<uint32_t>(state_slots);
+        link_slots += wl * kBlock;
+        state_slots += ws * kBlock;
+        if (link_slots >= (1ull << 32) || state_slots >= (1ull << 32)) {
+            throw Failure{AS_ERANGE, "too many links or states for 32-bit indices"};
+        }
+    }
+    std::vector<int32_t> link_offset(link_slots, 0);
+    std::vector<float> link_weight(link_slots, 0.0f);
+    std::vector<float> state_weight(state_slots, 0.0f), now(state_slots, 0.0f), before(state_slots, 0.0f),
+        gain(state_slots, 0.0f);
+    for (uint64_t k = 0; k < K; ++k) {
+        const uint64_t src = order[k], B = k / kBlock, t = k % kBlock;
+        sp_cell[k] = static_cast<uint32_t>(pad(static_cast<uint64_t>(L.special[src])));
+        sp_links[k] = static_cast<uint8_t>(links_of(src));
+        sp_states[k] = static_cast<uint8_t>(states_of(src));
+        coef_a[k] = L.coef_a[src];
+        coef_b[k] = L.coef_b[src];
+        coef_inv[k] = L.coef_inv[src];
+        source[k] = L.source[src];
+        for (uint64_t u = 0; u < links_of(src); ++u) {
+            const uint64_t l = L.link_start[src] + u;
+            const int d = L.link_direction[l];
+            if (d < 0 || d >= 18) refuse("a link has no direction");
+            const uint64_t slot = link_base[B] + u * kBlock + t;
+            link_offset[slot] = offset[static_cast<size_t>(d)];
+            link_weight[slot] = L.link_weight[l];
+        }
+        for (uint64_t u = 0; u < states_of(src); ++u) {
+            const uint64_t j = L.state_start[src] + u;
+            const int32_t br = L.state_branch[j];
+            if (br < 0 || static_cast<uint64_t>(br) >= L.branches) refuse("a branch state has no branch");
+            const uint64_t slot = state_base[B] + u * kBlock + t;
+            state_weight[slot] = L.state_weight[j];
+            now[slot] = L.branch_now[br];
+            before[slot] = L.branch_before[br];
+            gain[slot] = L.branch_gain[br];
+        }
+    }
+    h->specials = static_cast<uint32_t>(K);
+    h->states = static_cast<uint32_t>(L.states);
+    h->state_slots = state_slots;
+
+    // Matched layer.
+    std::vector<uint32_t> layer_cell(L.layer_cells);
+    std::vector<uint64_t> layer_at(L.layer_cells);
+    for (uint64_t c = 0; c < L.layer_cells; ++c) {
+        const int64_t i = L.layer_cell[c];
+        if (i < 0 || static_cast<uint64_t>(i) >= n) refuse("a layer cell lies outside the grid");
+        const uint64_t x = static_cast<uint64_t>(i) % nx, y = (static_cast<uint64_t>(i) / nx) % ny,
+                       z = static_cast<uint64_t>(i) / (nx * ny);
+        if (x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz) {
+            refuse("a layer cell lies on the grid's edge");
+        }
+        layer_cell[c] = static_cast<uint32_t>(pad(static_cast<uint64_t>(i)));
+        layer_at[c] = x | (y << 21) | (z << 42);
+    }
+    h->layer_cells = static_cast<uint32_t>(L.layer_cells);
+
+    // Probes.
+    std::vector<uint32_t> probe_start(L.probes + 1), probe_cell(L.taps);
+    std::vector<double> probe_weight(L.probe_weights, L.probe_weights + L.taps);
+    for (uint64_t p = 0; p <= L.probes; ++p) probe_start[p] = static_cast<uint32_t>(L.probe_start[p]);
+    for (uint64_t t = 0; t < L.taps; ++t) {
+        if (L.probe_cells[t] >= n) refuse("a probe reads a cell outside the grid");
+        probe_cell[t] = static_cast<uint32_t>(pad(L.probe_cells[t]));
+    }
+    h->probes = static_cast<uint32_t>(L.probes);
+
+    // Device copies.
+    const size_t e = precision == 64 ? sizeof(double) : sizeof(float);
+    check(hipStreamCreate(&h->stream), "hipStreamCreate");
+    h->types = upload(h, types);
+    h->scale = upload(h, scale);
+    h->tiles = upload(h, tiles);
+    h->sp_cell = upload(h, sp_cell);
+    h->sp_links = upload(h, sp_links);
+    h->sp_states = upload(h, sp_states);
+    h->link_base = upload(h, link_base);
+    h->state_base = upload(h, state_base);
+    h->link_offset = upload(h, link_offset);
+    h->link_weight = upload(h, link_weight);
+    h->coef_a = upload(h, coef_a);
+    h->coef_b = upload(h, coef_b);
+    h->coef_inv = upload(h, coef_inv);
+    h->source = upload(h, source);
+    h->state_weight = upload(h, state_weight);
+    h->state_now = upload(h, now);
+    h->state_before = upload(h, before);
+    h->state_gain = upload(h, gain);
+    h->layer_cell = upload(h, layer_cell);
+    h->layer_at = upload(h, layer_at);
+    if (L.layer_scale != nullptr && L.layer_cells > 0) {
+        h->layer_scale = upload(h, std::vector<float>(L.layer_scale, L.layer_scale + L.layer_cells));
+    }
+    if (L.layer_cells > 0) {
+        const uint64_t len[3] = {nx + 1, ny + 1, nz + 1};
+        for (int k = 0; k < 6; ++k) {
+            if (L.keep[k] == nullptr || L.push[k] == nullptr) refuse("the matched layer's coefficients are missing");
+            h->keep[k] = upload(h, std::vector<float>(L.keep[k], L.keep[k] + len[k / 2]));
+            h->push[k] = upload(h, std::vector<float>(L.push[k], L.push[k] + len[k / 2]));
+        }
+    }
+    h->probe_start = upload(h, probe_start);
+    h->probe_cell = upload(h, probe_cell);
+    h->probe_weight = upload(h, probe_weight);
+    h->u0 = zeros(h, n_pad * e);
+    h->u1 = zeros(h, n_pad * e);
+    h->w0 = zeros(h, state_slots * e);
+    h->w1 = zeros(h, state_slots * e);
+    h->layer_state = zeros(h, 9 * L.layer_cells * e);
+}
+
+template <typename T>
+StepArgs<T> step_args(const as_fdtd *h) {
+    StepArgs<T> s{};
+    s.types = h->types;
+    s.scale = h->scale;
+    s.tiles = h->tiles;
+    s.tile_blocks = h->tile_count;
+    s.tiles_x = h->tiles_x;
+    s.tiles_y = h->tiles_y;
+    s.nx = h->nx;
+    s.ny = h->ny;
+    s.nz = h->nz;
+    s.px = h->px;
+    s.tile_z = h->tile_z;
+    s.sxy = h->sxy;
+    s.axial = static_cast<T>(static_cast<float>(h->axial));
+    s.diagonal = static_cast<T>(static_cast<float>(h->diagonal));
+    s.centre = static_cast<T>(static_cast<float>(h->centre));
+    s.total = static_cast<T>(static_cast<float>(h->total));
+    s.specials = h->specials;
+    s.special_blocks = (h->specials + kBlock - 1) / kBlock;
+    s.sp_cell = h->sp_cell;
+    s.sp_links = h->sp_links;
+    s.sp_states = h->sp_states;
+    s.link_base = h->link_base;
+    s.state_base = h->state_base;
+    s.link_offset = h->link_offset;
+    s.link_weight = h->link_weight;
+    s.coef_a = h->coef_a;
+    s.coef_b = h->coef_b;
+    s.coef_inv = h->coef_inv;
+    s.source = h->source;
+    s.state_weight = h->state_weight;
+    s.state_now = h->state_now;
+    s.state_before = h->state_before;
+    s.state_gain = h->state_gain;
+    s.layer_cells = h->layer_cells;
+    s.layer_blocks = (h->layer_cells + kBlock - 1) / kBlock;
+    s.layer_cell = h->layer_cell;
+    s.layer_at = h->layer_at;
+    s.layer_state = static_cast<T *>(h->layer_state);
+    s.layer_scale = h->layer_scale;
+    for (int k = 0; k < 6; ++k) {
+        s.keep[k] = h->keep[k];
+        s.push[k] = h->push[k];
+    }
+    s.probes = h->probes;
+    s.probe_start = h->probe_start;
+    s.probe_cell = h->probe_cell;
+    s.probe_weight = h->probe_weight;
+    return s;
+}
+
+template <typename T>
+void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, unsigned long long index, unsigned long long k,
+                hipStream_t stream) {
+    T *u0 = static_cast<T *>(h->u0), *u1 = static_cast<T *>(h->u1);
+    T *w0 = static_cast<T *>(h->w0), *w1 = static_cast<T *>(h->w1);
+    const bool even = (index & 1ull) == 0;  // the reference's swaps (fdtd.hpp:883-885, 900-903)
+    s.a = even ? u0 : u1;
+    s.b = even ? u1 : u0;
+    s.wa = even ? w0 : w1;
+    s.wb = even ? w1 : w0;
+    s.step = k;
+    if (h->split) {
+        if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
+        if (s.tile_blocks) {
+            if (h->wide && h->scaled) {
+                hipLaunchKernelGGL((interior_kernel<T, true, true>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+            } else if (h->wide) {
+                hipLaunchKernelGGL((interior_kernel<T, true, false>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+            } else if (h->scaled) {
+                hipLaunchKernelGGL((interior_kernel<T, false, true>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+            } else {
+                hipLaunchKernelGGL((interior_kernel<T, false, false>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+            }
+        }
+        if (s.layer_blocks) hipLaunchKernelGGL((layer_kernel<T>), dim3(s.layer_blocks), dim3(kBlock), 0, stream, s);
+        const uint32_t probe_blocks = (h->probes + kBlock - 1) / kBlock;
+        if (probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
+        return;
+    }
+    (void)blocks;
+    if (h->wide && h->scaled) {
+        hipLaunchKernelGGL((step_kernel<T, true, true>), dim3(blocks), dim3(kBlock), 0, stream, s);
+    } else if (h->wide) {
+        hipLaunchKernelGGL((step_kernel<T, true, false>), dim3(blocks), dim3(kBlock), 0, stream, s);
+    } else if (h->scaled) {
+        hipLaunchKernelGGL((step_kernel<T, false, true>), dim3(blocks), dim3(kBlock), 0, stream, s);
+    } else {
+        hipLaunchKernelGGL((step_kernel<T, false, false>), dim3(blocks), dim3(kBlock), 0, stream, s);
+    }
+}
+
+template <typename T>
+uint32_t block_count(const as_fdtd *h, const StepArgs<T> &s) {
+    const uint32_t probe_blocks = (h->probes + kBlock - 1) / kBlock;
+    return s.tile_blocks + s.special_blocks + s.layer_blocks + probe_blocks;
+}
+
+// Two graphs of graph_steps steps each, one starting on an even step and one on an odd, on
+// staging buffers for the drive and the probes' output: a run replays them, copying its
+// drive in and the probes out between replays, so the host launches one graph where it
+// would launch graph_steps kernels.
+template <typename T>
+void ensure_graphs(as_fdtd *h) {
+    if (h->graph_exec[0] != nullptr) return;
+    const uint32_t G = h->graph_steps;
+    check(hipMalloc(reinterpret_cast<void **>(&h->sig_stage), G * sizeof(float)), "hipMalloc");
+    check(hipMemset(h->sig_stage, 0, G * sizeof(float)), "hipMemset");
+    const size_t out_len = static_cast<size_t>(std::max<uint32_t>(h->probes, 1)) * G;
+    check(hipMalloc(reinterpret_cast<void **>(&h->out_stage), out_len * sizeof(double)), "hipMalloc");
+    for (int parity = 0; parity < 2; ++parity) {
+        StepArgs<T> s = step_args<T>(h);
+        s.signal = h->sig_stage;
+        s.out = h->out_stage;
+        s.out_stride = G;
+        const uint32_t blocks = block_count(h, s);
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        for (uint32_t k = 0; k < G; ++k) launch_one<T>(h, s, blocks, static_cast<unsigned long long>(parity) + k, k, h->stream);
+        check(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
+        check(hipGraphInstantiate(&h->graph_exec[parity], graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        check(hipGraphDestroy(graph), "hipGraphDestroy");
+    }
+}
+
+template <typename T>
+void launch_steps(as_fdtd *h, const float *d_signal, size_t steps, double *d_out) {
+    StepArgs<T> s = step_args<T>(h);
+    const uint32_t blocks = block_count(h, s);
+    if (blocks == 0) return;
+    size_t k = 0;
+    const uint32_t G = h->graph_steps;
+    if (G > 0 && steps >= G) {
+        ensure_graphs<T>(h);
+        for (; k + G <= steps; k += G) {
+            check(hipMemcpyAsync(h->sig_stage, d_signal + k, G * sizeof(float), hipMemcpyDeviceToDevice, h->stream),
+                  "hipMemcpyAsync");
+            check(hipGraphLaunch(h->graph_exec[(h->done + k) & 1ull], h->stream), "hipGraphLaunch");
+            if (h->probes > 0) {
+                check(hipMemcpy2DAsync(d_out + k, steps * sizeof(double), h->out_stage, G * sizeof(double),
+                                       G * sizeof(double), h->probes, hipMemcpyDeviceToDevice, h->stream),
+                      "hipMemcpy2DAsync");
+            }
+        }
+    }
+    s.signal = d_signal;
+    s.out = d_out;
+    s.out_stride = steps;
+    for (; k < steps; ++k) launch_one<T>(h, s, blocks, h->done + k, k, h->stream);
+    check(hipGetLastError(), "step launch");
+    h->done += steps;
+}
+
+int fail(as_fdtd *h, const Failure &f) {
+    if (h) h->error = f.what;
+    return f.code;
+}
+
+}  // namespace
+
+extern "C" {
+
+as_fdtd *as_fdtd_create(const as_layout *layout, int precision, const as_tuning *tuning) {
+    g_create_error.clear();
+    if (layout == nullptr) {
+        g_create_error = "no layout";
+        return nullptr;
+    }
+    as_fdtd *h = new (std::nothrow) as_fdtd();
+    if (h == nullptr) {
+        g_create_error = "out of host memory";
+        return nullptr;
+    }
+    try {
+        build(h, *layout, precision, tuning);
+        return h;
+    } catch (const Failure &f) {
+        g_create_error = f.what;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+    }
+    as_fdtd_destroy(h);
+    return nullptr;
+}
+
+const char *as_fdtd_create_error(void) { return g_create_error.c_str(); }
+
+int as_fdtd_run_device(as_fdtd *h, const float *d_signal, size_t steps, double *d_out) {
+    if (h == nullptr) return AS_EINVAL;
+    try {
+        if (steps == 0) return AS_OK;
+        if (h->precision == 64) {
+            launch_steps<double>(h, d_signal, steps, d_out);
+        } else {
+            launch_steps<float>(h, d_signal, steps, d_out);
+        }
+        return AS_OK;
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+int as_fdtd_run(as_fdtd *h, const float *signal, size_t steps, double *out) {
+    if (h == nullptr) return AS_EINVAL;
+    try {
+        if (steps == 0) return AS_OK;
+        if (signal == nullptr || (out == nullptr && h->probes > 0)) refuse("run needs a signal and room for the probes");
+        // Pieces of at most this many steps, so the device's output stays small.
+        const size_t piece = 1u << 16;
+        if (h->signal_cap < std::min(steps, piece)) {
+            if (h->signal_buf) (void)hipFree(h->signal_buf);
+            h->signal_buf = nullptr;
+            h->signal_cap = std::min(steps, piece);
+            check(hipMalloc(reinterpret_cast<void **>(&h->signal_buf), h->signal_cap * sizeof(float)), "hipMalloc");
+        }
+        const size_t out_need = std::max<size_t>(1, h->probes) * std::min(steps, piece);
+        if (h->out_cap < out_need) {
+            if (h->out_buf) (void)hipFree(h->out_buf);
+            h->out_buf = nullptr;
+            h->out_cap = out_need;
+            check(hipMalloc(reinterpret_cast<void **>(&h->out_buf), h->out_cap * sizeof(double)), "hipMalloc");
+        }
+        std::vector<double> chunk;
+        for (size_t done = 0; done < steps; done += piece) {
+            const size_t m = std::min(piece, steps - done);
+            check(hipMemcpyAsync(h->signal_buf, signal + done, m * sizeof(float), hipMemcpyHostToDevice, h->stream),
+                  "hipMemcpyAsync");
+            if (h->precision == 64) {
+                launch_steps<double>(h, h->signal_buf, m, h->out_buf);
+            } else {
+                launch_steps<float>(h, h->signal_buf, m, h->out_buf);
+            }
+            chunk.resize(static_cast<size_t>(h->probes) * m);
+            if (!chunk.empty()) {
+                check(hipMemcpyAsync(chunk.data(), h->out_buf, chunk.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                     h->stream),
+                      "hipMemcpyAsync");
+            }
+            check(hipStreamSynchronize(h->stream), "step");
+            for (uint32_t p = 0; p < h->probes; ++p) {
+                std::memcpy(out + static_cast<size_t>(p) * steps + done, chunk.data() + static_cast<size_t>(p) * m,
+                            m * sizeof(double));
+            }
+        }
+        return AS_OK;
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+int as_fdtd_reset(as_fdtd *h) {
+    if (h == nullptr) return AS_EINVAL;
+    try {
+        const size_t e = h->precision == 64 ? sizeof(double) : sizeof(float);
+        check(hipMemsetAsync(h->u0, 0, h->n_pad * e, h->stream), "hipMemsetAsync");
+        check(hipMemsetAsync(h->u1, 0, h->n_pad * e, h->stream), "hipMemsetAsync");
+        check(hipMemsetAsync(h->w0, 0, std::max<size_t>(h->state_slots * e, 256), h->stream), "hipMemsetAsync");
+        check(hipMemsetAsync(h->w1, 0, std::max<size_t>(h->state_slots * e, 256), h->stream), "hipMemsetAsync");
+        check(hipMemsetAsync(h->layer_state, 0, std::max<size_t>(9ull * h->layer_cells * e, 256), h->stream),
+              "hipMemsetAsync");
+        check(hipStreamSynchronize(h->stream), "reset");
+        h->done = 0;
+        return AS_OK;
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+int as_fdtd_field(as_fdtd *h, double *out) {
+    if (h == nullptr || out == nullptr) return AS_EINVAL;
+    try {
+        const bool even = (h->done & 1ull) == 0;
+        const void *b = even ? h->u1 : h->u0;
+        const size_t e = h->precision == 64 ? sizeof(double) : sizeof(float);
+        std::vector<unsigned char> host(h->n_pad * e);
+        check(hipMemcpyAsync(host.data(), b, host.size(), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(h->stream), "field");
+        const size_t nx = static_cast<size_t>(h->nx), ny = static_cast<size_t>(h->ny), nz = static_cast<size_t>(h->nz);
+        const size_t px = static_cast<size_t>(h->px);
+        for (size_t z = 0; z < nz; ++z)
+            for (size_t y = 0; y < ny; ++y)
+                for (size_t x = 0; x < nx; ++x) {
+                    const size_t p = x + px * (y + ny * z);
+                    const size_t i = x + nx * (y + ny * z);
+                    if (h->precision == 64) {
+                        out[i] = reinterpret_cast<const double *>(host.data())[p];
+                    } else {
+                        out[i] = static_cast<double>(reinterpret_cast<const float *>(host.data())[p]);
+                    }
+                }
+        return AS_OK;
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+void *as_fdtd_stream(as_fdtd *h) { return h ? static_cast<void *>(h->stream) : nullptr; }
+
+int as_fdtd_info(const as_fdtd *h, uint64_t *out) {
+    if (h == nullptr || out == nullptr) return AS_EINVAL;
+    out[0] = h->interior_cells;
+    out[1] = h->specials;
+    out[2] = h->states;
+    out[3] = h->layer_cells;
+    out[4] = h->probes;
+    out[5] = h->tile_count;
+    out[6] = static_cast<uint64_t>(h->px);
+    out[7] = h->done;
+    return AS_OK;
+}
+
+const char *as_fdtd_last_error(const as_fdtd *h) { return h ? h->error.c_str() : "no handle"; }
+
+void as_fdtd_destroy(as_fdtd *h) {
+    if (h == nullptr) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocations) (void)hipFree(p);
+    if (h->signal_buf) (void)hipFree(h->signal_buf);
+    if (h->out_buf) (void)hipFree(h->out_buf);
+    for (auto &g : h->graph_exec)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (h->sig_stage) (void)hipFree(h->sig_stage);
+    if (h->out_stage) (void)hipFree(h->out_stage);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

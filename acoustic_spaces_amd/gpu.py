@@ -1,0 +1,102 @@
+"""The wave solver's stepper on MI355X: the drop-in for the reference's GpuWaveSolver.
+
+The reference picks its stepper at one line of wave_responses,
+``stepper = GpuWaveSolver(solver) if on_gpu else solver`` (engine/wave.py:1178); the
+stepper reads ``solver.layout()`` once and offers ``run(signal) -> (probes, steps)`` and
+``probes`` (engine/gpu.py:445-612). ``HipWaveSolver`` offers the same, over the HIP
+kernels of libasfdtd.so, for a layout from the reference's ``_native.WaveSolver`` or from
+this package's own set-up (acoustic_spaces_amd.solver.WaveSolver).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from acoustic_spaces_amd import native
+
+
+class HipWaveSolver:
+    """Steps the grid of ``solver`` (anything with ``layout()``, fresh or reset, or a layout
+    dict) on the GPU: :meth:`run` does what the solver's ``run`` does, from the GPU's own
+    state. ``precision`` 32 is the reference's float scheme, bit for bit; 64 is the same
+    scheme in double. ``tile_z``, ``row_align`` and ``graph_steps`` set the launch shape (0: the
+    library's defaults; graph_steps < 0 launches every step from the host); ``split`` launches
+    each kind of cell on its own instead of one launch per step."""
+
+    def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
+                 graph_steps: int = 0, split: bool = False) -> None:
+        lib = native.load()
+        layout = solver if isinstance(solver, dict) else solver.layout()
+        struct, keep = native.pack_layout(layout)
+        tuning = native.Tuning(0, int(tile_z), int(row_align), int(graph_steps), 1 if split else 0)
+        h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
+        if not h:
+            message = lib.as_fdtd_create_error().decode()
+            if "hip" in message.lower() or "device" in message.lower():
+                raise RuntimeError(message)
+            raise ValueError(message)
+        self._h = h
+        self._lib = lib
+        self._probes = int(struct.probes)
+        self.shape = (int(struct.nx), int(struct.ny), int(struct.nz))
+        self.precision = int(precision)
+        del keep  # the library holds its own copies
+
+    @property
+    def probes(self) -> int:
+        return self._probes
+
+    def _check(self, code: int):
+        if code != native.AS_OK:
+            native.raise_for(code, self._lib.as_fdtd_last_error(self._h).decode())
+
+    def run(self, signal: np.ndarray) -> np.ndarray:
+        """Advances one step per sample of ``signal`` (added at the source) and returns
+        the pressure at every probe before each step (probes, steps)."""
+        signal = np.ascontiguousarray(signal, dtype=np.float32).reshape(-1)
+        out = np.empty((self._probes, len(signal)), np.float64)
+        if len(signal):
+            self._check(self._lib.as_fdtd_run(self._h, signal.ctypes.data, len(signal), out.ctypes.data))
+        return out
+
+    def run_device(self, signal_ptr: int, steps: int, out_ptr: int) -> None:
+        """Enqueues ``steps`` steps on the solver's stream, on device pointers (the drive,
+        float32[steps]; the probes' output, float64[probes, steps]); does not wait."""
+        self._check(self._lib.as_fdtd_run_device(self._h, ctypes.c_void_p(signal_ptr), int(steps),
+                                                 ctypes.c_void_p(out_ptr)))
+
+    def reset(self) -> None:
+        """Zeroes the fields and states (WaveSolver::reset, fdtd.hpp:906-915)."""
+        self._check(self._lib.as_fdtd_reset(self._h))
+
+    def field(self) -> np.ndarray:
+        """The current pressure (the next step's probes read it), float64, x fastest."""
+        out = np.empty(int(np.prod(self.shape)), np.float64)
+        self._check(self._lib.as_fdtd_field(self._h, out.ctypes.data))
+        return out
+
+    @property
+    def stream(self) -> int:
+        """The HIP stream the steps run on (for events)."""
+        return int(self._lib.as_fdtd_stream(self._h) or 0)
+
+    def info(self) -> dict:
+        out = np.zeros(8, np.uint64)
+        self._check(self._lib.as_fdtd_info(self._h, out.ctypes.data))
+        keys = ("interior_cells", "specials", "states", "layer_cells", "probes", "tiles",
+                "row_pitch", "steps_done")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.as_fdtd_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,170 @@
+"""ctypes binding of libasfdtd.so (include/asfdtd.h).
+
+The library is built in-tree by ``acoustic_spaces_amd.build`` (or __graft_entry__.build());
+loading it when it is missing raises, loudly: there is no other path to the kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIBRARY = Path(__file__).resolve().parent / "libasfdtd.so"
+
+P = ctypes.c_void_p
+U64 = ctypes.c_uint64
+SIZE = ctypes.c_size_t
+
+AS_OK, AS_EINVAL, AS_EDEVICE, AS_ERANGE = 0, -1, -2, -3
+
+
+class Layout(ctypes.Structure):
+    """as_layout (include/asfdtd.h)."""
+
+    _fields_ = [
+        ("nx", U64), ("ny", U64), ("nz", U64), ("stencil", U64),
+        ("weights", P), ("n_weights", U64), ("centre", ctypes.c_float), ("total", ctypes.c_float),
+        ("scale", P), ("layer_scale", P), ("types", P),
+        ("specials", U64), ("special", P), ("link_start", P), ("links", U64),
+        ("link_direction", P), ("link_weight", P),
+        ("coef_a", P), ("coef_b", P), ("coef_inv", P), ("source", P),
+        ("state_start", P), ("states", U64), ("state_branch", P), ("state_weight", P),
+        ("branches", U64), ("branch_now", P), ("branch_before", P), ("branch_gain", P),
+        ("layer_cells", U64), ("layer_cell", P), ("keep", P * 6), ("push", P * 6),
+        ("probes", U64), ("probe_start", P), ("taps", U64), ("probe_cells", P),
+        ("probe_weights", P),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    """as_tuning (include/asfdtd.h)."""
+
+    _fields_ = [("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int), ("row_align", ctypes.c_int),
+                ("graph_steps", ctypes.c_int), ("split", ctypes.c_int)]
+
+
+_lib = None
+
+#: Every symbol include/asfdtd.h declares.
+SYMBOLS = (
+    "as_fdtd_create", "as_fdtd_create_error", "as_fdtd_run", "as_fdtd_run_device",
+    "as_fdtd_reset", "as_fdtd_field", "as_fdtd_stream", "as_fdtd_info", "as_fdtd_last_error",
+    "as_fdtd_destroy",
+)
+
+
+def load():
+    """The library, its functions typed; raises when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY.exists():
+        raise RuntimeError(
+            f"the HIP wave-solver library {LIBRARY} is not built: run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (or python -m acoustic_spaces_amd.build)"
+        )
+    lib = ctypes.CDLL(str(LIBRARY))
+    lib.as_fdtd_create.restype = P
+    lib.as_fdtd_create.argtypes = [ctypes.POINTER(Layout), ctypes.c_int, ctypes.POINTER(Tuning)]
+    lib.as_fdtd_create_error.restype = ctypes.c_char_p
+    lib.as_fdtd_create_error.argtypes = []
+    lib.as_fdtd_run.restype = ctypes.c_int
+    lib.as_fdtd_run.argtypes = [P, P, SIZE, P]
+    lib.as_fdtd_run_device.restype = ctypes.c_int
+    lib.as_fdtd_run_device.argtypes = [P, P, SIZE, P]
+    lib.as_fdtd_reset.restype = ctypes.c_int
+    lib.as_fdtd_reset.argtypes = [P]
+    lib.as_fdtd_field.restype = ctypes.c_int
+    lib.as_fdtd_field.argtypes = [P, P]
+    lib.as_fdtd_stream.restype = P
+    lib.as_fdtd_stream.argtypes = [P]
+    lib.as_fdtd_info.restype = ctypes.c_int
+    lib.as_fdtd_info.argtypes = [P, P]
+    lib.as_fdtd_last_error.restype = ctypes.c_char_p
+    lib.as_fdtd_last_error.argtypes = [P]
+    lib.as_fdtd_destroy.restype = None
+    lib.as_fdtd_destroy.argtypes = [P]
+    _lib = lib
+    return lib
+
+
+def raise_for(code: int, message: str):
+    """The reference's exceptions for the library's codes: ValueError for a refused
+    argument or range (engine/gpu.py:440-441, 495-504), RuntimeError for the device."""
+    if code == AS_OK:
+        return
+    if code in (AS_EINVAL, AS_ERANGE):
+        raise ValueError(message)
+    raise RuntimeError(message)
+
+
+_AXES = ("x", "y", "z")
+
+_TYPES = {
+    "weights": np.float32, "scale": np.float32, "layer_scale": np.float32, "types": np.uint8,
+    "special": np.int64, "link_start": np.uint64, "link_direction": np.uint8,
+    "link_weight": np.float32, "coef_a": np.float32, "coef_b": np.float32,
+    "coef_inv": np.float32, "source": np.float32, "state_start": np.uint64,
+    "state_branch": np.int32, "state_weight": np.float32, "branch_now": np.float32,
+    "branch_before": np.float32, "branch_gain": np.float32, "layer_cells": np.int64,
+    "probe_start": np.uint64, "probe_cells": np.uint64, "probe_weights": np.float64,
+}
+
+
+def pack_layout(layout: dict):
+    """An as_layout over the arrays of a layout dict (WaveSolver::layout's names,
+    reference fdtd.hpp:727-782); returns (struct, arrays kept alive)."""
+    flows = [k for k in ("flow_x", "flow_y", "flow_mask") if len(layout.get(k, ())) > 0]
+    if flows:
+        raise ValueError("the HIP stepper steps still air only: moving air is stepped on the CPU, "
+                         "as the reference's own GPU stepper leaves it (engine/wave.py:1159-1178)")
+    A = {}
+    for name, dt in _TYPES.items():
+        A[name] = np.ascontiguousarray(layout.get(name, np.zeros(0, dt)), dtype=dt)
+    for k in ("keep", "push"):
+        for ax in _AXES:
+            for face in ("", "face_"):
+                key = f"{k}_{face}{ax}"
+                A[key] = np.ascontiguousarray(layout.get(key, np.zeros(0)), dtype=np.float32)
+    shape = [int(v) for v in layout["shape"]]
+    n = shape[0] * shape[1] * shape[2]
+    if len(A["types"]) != n:
+        raise ValueError("the layout's cell types do not cover its grid")
+    s = Layout()
+    s.nx, s.ny, s.nz = shape
+    s.stencil = int(np.asarray(layout["stencil"]).reshape(-1)[0])
+    s.weights = A["weights"].ctypes.data
+    s.n_weights = len(A["weights"])
+    s.centre = float(np.asarray(layout["centre"]).reshape(-1)[0])
+    s.total = float(np.asarray(layout["total"]).reshape(-1)[0])
+    scaled = int(np.asarray(layout.get("scaled", [0])).reshape(-1)[0])
+    if scaled and len(A["scale"]) != n:
+        raise ValueError("the layout's speeds do not cover its grid")
+    s.scale = A["scale"].ctypes.data if scaled else None
+    s.layer_scale = A["layer_scale"].ctypes.data if len(A["layer_scale"]) else None
+    s.types = A["types"].ctypes.data
+    s.specials = len(A["special"])
+    s.links = len(A["link_direction"])
+    s.states = len(A["state_branch"])
+    s.branches = len(A["branch_now"])
+    s.layer_cells = len(A["layer_cells"])
+    s.probes = len(A["probe_start"]) - 1 if len(A["probe_start"]) else 0
+    s.taps = len(A["probe_cells"])
+    if len(A["link_start"]) != s.specials + 1 or len(A["state_start"]) != s.specials + 1:
+        raise ValueError("the layout's link and state ranges do not match its special cells")
+    for name in ("special", "link_start", "link_direction", "link_weight", "coef_a", "coef_b",
+                 "coef_inv", "source", "state_start", "state_branch", "state_weight",
+                 "branch_now", "branch_before", "branch_gain", "probe_start", "probe_cells",
+                 "probe_weights"):
+        setattr(s, name, A[name].ctypes.data if len(A[name]) else None)
+    s.layer_cell = A["layer_cells"].ctypes.data if len(A["layer_cells"]) else None
+    for a, ax in enumerate(_AXES):
+        for j, face in enumerate(("", "face_")):
+            keep, push = A[f"keep_{face}{ax}"], A[f"push_{face}{ax}"]
+            if s.layer_cells and (len(keep) != shape[a] + 1 or len(push) != shape[a] + 1):
+                raise ValueError("the matched layer's coefficients do not cover the grid")
+            s.keep[2 * a + j] = keep.ctypes.data if len(keep) else None
+            s.push[2 * a + j] = push.ctypes.data if len(push) else None
+    return s, A

@@ -1,0 +1,420 @@
+"""Synthetic scenes for the wave solver: triangle meshes, grids and drives.
+
+The meshes are closed triangle surfaces with one material per face, the input of the
+reference's ``_native.WaveSolver(bvh, face_material, ...)`` (cpp/bindings/native_module.cpp:
+1238-1285). Grids are laid out as the reference's tests lay them out (tests/test_wave.py:
+59-107): the origin two cells and a fraction below the model, so that no plane of cell
+centres meets a wall, and two cells of margin past it.
+
+Workloads, with the sizes the solver is measured on:
+
+- ``shoebox``: a 6 x 4 x 3 m room at 5 cm (125 x 85 x 65 cells), six walls with constant
+  absorption drawn from U(0.05, 0.4), a Gaussian pulse, four receivers;
+- ``hall``: a 30 x 20 x 12 m hall, side walls splayed 5 degrees, a ceiling of six tilted
+  planes, at 10 cm (about 7.2M air cells), first-order admittance filters per face;
+- ``outdoor``: a ground block with a rigid box barrier under open sky, 25 cm cells,
+  with a 16-cell matched layer and a two-pole ground admittance.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+C = 343.0
+COURANT = 1.0 / math.sqrt(3.0)  # 7-point stencil (reference wave.py:116-130)
+ISOTROPIC_COURANT = math.sqrt(0.75)  # 19-point stencil
+GRID_SHIFT = np.array([0.0137, 0.0219, 0.0291])  # tests/test_wave.py:95
+
+
+@dataclasses.dataclass
+class Mesh:
+    """A triangle mesh: vertices (V, 3) float64, faces (F, 3) int64 and each face's
+    material (F,) int64; ``names`` holds the materials' names."""
+
+    vertices: np.ndarray
+    faces: np.ndarray
+    face_material: np.ndarray
+    names: list
+
+    @property
+    def materials(self) -> int:
+        return len(self.names)
+
+
+def _quad(v, a, b, c, d):
+    """Two triangles of the planar quad a-b-c-d (in order round it)."""
+    return [(v[a], v[b], v[c]), (v[a], v[c], v[d])]
+
+
+def _mesh_from_triangles(tris, mats, names) -> Mesh:
+    verts: list = []
+    index: dict = {}
+    faces = []
+    for tri in tris:
+        row = []
+        for p in tri:
+            key = tuple(float(x) for x in p)
+            if key not in index:
+                index[key] = len(verts)
+                verts.append(key)
+            row.append(index[key])
+        faces.append(row)
+    return Mesh(
+        np.array(verts, dtype=np.float64),
+        np.array(faces, dtype=np.int64),
+        np.array(mats, dtype=np.int64),
+        list(names),
+    )
+
+
+def box_mesh(size, low=(0.0, 0.0, 0.0), inward=True) -> Mesh:
+    """A closed box from ``low`` to ``low + size``; materials floor, ceiling, then the
+    walls y = low, x = high, y = high, x = low (as the reference's box_mesh orders them,
+    geometry/builders.py:3-10). Faces point into the box when ``inward``."""
+    x0, y0, z0 = (float(v) for v in low)
+    x1, y1, z1 = x0 + float(size[0]), y0 + float(size[1]), z0 + float(size[2])
+    v = {
+        0: (x0, y0, z0), 1: (x1, y0, z0), 2: (x1, y1, z0), 3: (x0, y1, z0),
+        4: (x0, y0, z1), 5: (x1, y0, z1), 6: (x1, y1, z1), 7: (x0, y1, z1),
+    }
+    # Each quad listed counterclockwise seen from inside the box.
+    quads = [
+        (0, 1, 2, 3),  # floor (normal +z, into the box)
+        (4, 7, 6, 5),  # ceiling
+        (0, 4, 5, 1),  # y = y0
+        (1, 5, 6, 2),  # x = x1
+        (2, 6, 7, 3),  # y = y1
+        (3, 7, 4, 0),  # x = x0
+    ]
+    tris, mats = [], []
+    for m, q in enumerate(quads):
+        a, b, c, d = q if inward else q[::-1]
+        tris += _quad(v, a, b, c, d)
+        mats += [m, m]
+    return _mesh_from_triangles(tris, mats, ["floor", "ceiling", "wall_1", "wall_2", "wall_3", "wall_4"])
+
+
+def hall_mesh() -> Mesh:
+    """A 30 x 20 x 12 m hall: its side walls splayed 5 degrees outwards from the stage
+    end (x = 0) to the back (x = 30), and a ceiling of six planes, each tilted along x.
+    Materials: floor, the six ceiling planes, the two side walls, the stage and back
+    walls (11). Faces point into the hall."""
+    length, width = 30.0, 20.0
+    splay = math.tan(math.radians(5.0))
+    heights = [10.2, 11.6, 10.4, 12.0, 10.8, 11.7, 10.5]  # ceiling over x = 0, 5, ..., 30
+
+    def y_low(x):
+        return -splay * x
+
+    def y_high(x):
+        return width + splay * x
+
+    xs = [5.0 * k for k in range(7)]
+    tris, mats = [], []
+    names = ["floor"] + [f"ceiling_{k + 1}" for k in range(6)] + ["side_1", "side_2", "stage", "back"]
+    # Floor, counterclockwise from above (normal +z).
+    floor = [(0.0, y_low(0.0), 0.0), (length, y_low(length), 0.0), (length, y_high(length), 0.0),
+             (0.0, y_high(0.0), 0.0)]
+    tris += [(floor[0], floor[1], floor[2]), (floor[0], floor[2], floor[3])]
+    mats += [0, 0]
+    # Ceiling planes (normal pointing down into the hall).
+    for k in range(6):
+        xa, xb = xs[k], xs[k + 1]
+        a = (xa, y_low(xa), heights[k])
+        b = (xb, y_low(xb), heights[k + 1])
+        c = (xb, y_high(xb), heights[k + 1])
+        d = (xa, y_high(xa), heights[k])
+        tris += [(a, d, c), (a, c, b)]
+        mats += [1 + k, 1 + k]
+    # Side walls: vertical polygons under the ceiling's profile, as fans from a floor corner.
+    for side, (yf, m) in enumerate(((y_low, 7), (y_high, 8))):
+        bottom = [(x, yf(x), 0.0) for x in (0.0, length)]
+        top = [(x, yf(x), h) for x, h in zip(xs, heights)]
+        poly = [bottom[0], bottom[1]] + top[::-1]  # around the wall
+        centre = bottom[0]
+        for j in range(1, len(poly) - 1):
+            t = (centre, poly[j], poly[j + 1])
+            # side_1 (y low) faces +y into the hall; side_2 faces -y.
+            n = np.cross(np.subtract(t[1], t[0]), np.subtract(t[2], t[0]))
+            inward_y = 1.0 if side == 0 else -1.0
+            if n[1] * inward_y < 0:
+                t = (t[0], t[2], t[1])
+            tris.append(t)
+            mats.append(m)
+    # Stage wall (x = 0, faces +x) and back wall (x = 30, faces -x).
+    for x, m, sign in ((0.0, 9, 1.0), (length, 10, -1.0)):
+        h = heights[0] if x == 0.0 else heights[-1]
+        q = [(x, y_low(x), 0.0), (x, y_high(x), 0.0), (x, y_high(x), h), (x, y_low(x), h)]
+        for t in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
+            n = np.cross(np.subtract(t[1], t[0]), np.subtract(t[2], t[0]))
+            if n[0] * sign < 0:
+                t = (t[0], t[2], t[1])
+            tris.append(t)
+            mats.append(m)
+    return _mesh_from_triangles(tris, mats, names)
+
+
+def merge(*meshes: Mesh) -> Mesh:
+    """One mesh of several, their materials numbered one after another."""
+    verts, faces, mats, names = [], [], [], []
+    vbase = mbase = 0
+    for m in meshes:
+        verts.append(m.vertices)
+        faces.append(m.faces + vbase)
+        mats.append(m.face_material + mbase)
+        names += m.names
+        vbase += len(m.vertices)
+        mbase += m.materials
+    return Mesh(np.vstack(verts), np.vstack(faces), np.concatenate(mats), names)
+
+
+@dataclasses.dataclass
+class Grid:
+    origin: np.ndarray
+    cell: float
+    shape: tuple
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.shape))
+
+    def centre(self, index) -> np.ndarray:
+        """The centre of the cell at grid coordinates ``index`` (x, y, z)."""
+        return self.origin + (np.asarray(index, dtype=float) + 0.5) * self.cell
+
+
+def grid_round(low, high, cell: float, layer: int = 0) -> Grid:
+    """The grid round the box from ``low`` to ``high`` as the reference's tests and
+    wave_responses lay it out (tests/test_wave.py:94-96, engine/wave.py:955-966):
+    origin ``low - cell (2 + layer + (0.0137, 0.0219, 0.0291))``, shape
+    ``ceil((high - origin) / cell) + 2 + layer``."""
+    low = np.asarray(low, dtype=float)
+    high = np.asarray(high, dtype=float)
+    origin = low - cell * (2.0 + layer + GRID_SHIFT)
+    shape = np.ceil((high - origin) / cell).astype(np.int64) + 2 + layer
+    return Grid(origin, float(cell), tuple(int(s) for s in shape))
+
+
+def normal_incidence_admittance(alpha):
+    """The specific admittance of a locally reacting wall of normal-incidence absorption
+    ``alpha`` with real impedance: beta = (1 - sqrt(1 - alpha)) / (1 + sqrt(1 - alpha))."""
+    r = np.sqrt(1.0 - np.asarray(alpha, dtype=float))
+    return (1.0 - r) / (1.0 + r)
+
+
+def gaussian_drive(rate: float, steps: int, sigma_s: float = 0.6e-3, delay_sigmas: float = 5.0):
+    """A Gaussian pulse of width ``sigma_s`` seconds, peak 1 at ``delay_sigmas`` widths,
+    sampled at the grid's rate (float32, one value per step)."""
+    t = np.arange(steps, dtype=np.float64) / rate
+    t0 = delay_sigmas * sigma_s
+    return np.exp(-0.5 * ((t - t0) / sigma_s) ** 2).astype(np.float32)
+
+
+@dataclasses.dataclass
+class Scene:
+    """Everything the solver's constructor takes, for one run."""
+
+    name: str
+    mesh: Mesh
+    beta: np.ndarray  # (M,) constant admittance per material
+    branches: np.ndarray  # (B, 3) m, r, q per branch
+    branch_material: np.ndarray  # (B,)
+    grid: Grid
+    source: np.ndarray
+    probes: np.ndarray  # (P, 3)
+    rate: float
+    steps: int
+    drive: np.ndarray
+    stencil: int = 7
+    layer: int = 0
+    layer_damping: float = 0.0
+    edge_admittance: float = 1.0
+
+    def solver_args(self) -> dict:
+        """The keyword arguments of the solver's constructor."""
+        return dict(
+            vertices=self.mesh.vertices,
+            faces=self.mesh.faces,
+            face_material=self.mesh.face_material,
+            beta=self.beta,
+            branches=self.branches,
+            branch_material=self.branch_material,
+            origin=self.grid.origin,
+            cell=self.grid.cell,
+            shape=self.grid.shape,
+            source=self.source,
+            probes=self.probes,
+            edge_admittance=self.edge_admittance,
+            layer=self.layer,
+            layer_damping=self.layer_damping,
+            stencil=self.stencil,
+        )
+
+
+def _points_inside(rng, low, high, margin, count):
+    low = np.asarray(low, float) + margin
+    high = np.asarray(high, float) - margin
+    return low + rng.random((count, 3)) * (high - low)
+
+
+def shoebox(size=(6.0, 4.0, 3.0), cell=0.05, seed=0, length_s=0.5, receivers=4,
+            stencil=7, absorption=None) -> Scene:
+    """The shoebox: six walls of constant absorption U(0.05, 0.4) (seeded) as
+    normal-incidence admittance, a Gaussian pulse (sigma 0.6 ms) at a seeded point at
+    least 0.5 m from the walls, ``receivers`` seeded receivers as far in."""
+    rng = np.random.default_rng(seed)
+    mesh = box_mesh(size)
+    alpha = rng.uniform(0.05, 0.4, 6) if absorption is None else np.asarray(absorption, float)
+    beta = normal_incidence_admittance(alpha)
+    grid = grid_round((0, 0, 0), size, cell)
+    courant = COURANT if stencil == 7 else ISOTROPIC_COURANT
+    rate = C / (courant * cell)
+    steps = int(round(length_s * rate))
+    margin = min(0.5, 0.25 * min(size))
+    source = _points_inside(rng, (0, 0, 0), size, margin, 1)[0]
+    probes = _points_inside(rng, (0, 0, 0), size, margin, receivers)
+    return Scene(
+        name=f"shoebox {size[0]:g}x{size[1]:g}x{size[2]:g} m at {cell:g} m",
+        mesh=mesh, beta=beta, branches=np.zeros((0, 3)), branch_material=np.zeros(0, np.int64),
+        grid=grid, source=source, probes=probes, rate=rate, steps=steps,
+        drive=gaussian_drive(rate, steps), stencil=stencil,
+    )
+
+
+def relaxation_branches(alpha_bands, bands_hz, rate):
+    """A constant admittance and one first-order (mass-free) relaxation branch per band
+    for a wall whose normal-incidence absorption is ``alpha_bands`` at ``bands_hz``.
+
+    A relaxation branch Y_j(s) = 1 / (R_j + S_j / s) (m = 0, reference impedance.py:168-211)
+    rises from 0 below its corner w_j = S_j / R_j to 1 / R_j above it. Branch j takes its
+    corner at band j's centre and its height from the rise of the target admittance from
+    band j - 1 to band j (the first band's from the constant). Centred in time
+    (fdtd.hpp:391-411) its solver coefficients are m = 0, r = R_j, q = S_j T."""
+    beta = normal_incidence_admittance(alpha_bands)
+    base = float(beta[0]) * 0.5
+    rows = []
+    level = base
+    T = 1.0 / rate
+    for j, f in enumerate(bands_hz):
+        rise = max(float(beta[j]) - level, 1e-3)
+        R = 1.0 / rise
+        w = 2.0 * math.pi * f
+        S = w * R
+        rows.append((0.0, R, S * T))
+        level += rise
+    return base, np.array(rows, dtype=float)
+
+
+def hall(cell=0.1, seed=1, length_s=2.0, receivers=16, sources=2) -> list:
+    """The hall, one Scene per source: per material, absorption U(0.02, 0.3) at 63, 125 and
+    250 Hz as a constant admittance and three relaxation branches; sources and receivers on
+    a seeded audience grid 1.2 m above the floor (sources on the stage, 1.5 m up)."""
+    rng = np.random.default_rng(seed)
+    mesh = hall_mesh()
+    lo = mesh.vertices.min(axis=0)
+    hi = mesh.vertices.max(axis=0)
+    grid = grid_round(lo, hi, cell)
+    rate = C / (COURANT * cell)
+    steps = int(round(length_s * rate))
+    bands = (63.0, 125.0, 250.0)
+    beta, rows, owners = [], [], []
+    for m in range(mesh.materials):
+        alpha = rng.uniform(0.02, 0.3, 3)
+        b, br = relaxation_branches(alpha, bands, rate)
+        beta.append(b)
+        rows.append(br)
+        owners.append(np.full(len(br), m, np.int64))
+    # Audience: a 4 x 4 grid of seats between x = 10 and 26 m, within the narrowest width.
+    seats = []
+    for i in range(4):
+        for j in range(4):
+            x = 10.0 + 4.0 * i + rng.uniform(-0.6, 0.6)
+            y = 4.0 + 4.0 * j + rng.uniform(-0.6, 0.6)
+            seats.append((x, y, 1.2 + rng.uniform(-0.05, 0.05)))
+    probes = np.array(seats[:receivers], dtype=float)
+    scenes = []
+    for k in range(sources):
+        src = np.array([2.5 + rng.uniform(-0.3, 0.3), 7.0 + 6.0 * k + rng.uniform(-0.5, 0.5),
+                        1.5 + rng.uniform(-0.1, 0.1)])
+        scenes.append(Scene(
+            name=f"hall 30x20x12 m at {cell:g} m, source {k + 1}",
+            mesh=mesh, beta=np.array(beta), branches=np.vstack(rows),
+            branch_material=np.concatenate(owners), grid=grid, source=src, probes=probes,
+            rate=rate, steps=steps, drive=gaussian_drive(rate, steps),
+        ))
+    return scenes
+
+
+def delany_bazley_admittance(f, sigma=200e3):
+    """The normalised surface admittance rho c / Z of a hard-backed-free, semi-infinite
+    porous ground of flow resistivity ``sigma`` (Pa s / m^2) by Delany and Bazley's
+    empirical characteristic impedance, Z_c / (rho c) = 1 + 9.08 X^-0.75 - 11.9j X^-0.73,
+    X = rho f / sigma (rho = 1.2 kg/m^3), with the e^{jwt} convention."""
+    X = 1.2 * np.asarray(f, float) / sigma
+    z = 1.0 + 9.08 * X ** -0.75 - 11.9j * X ** -0.73
+    return 1.0 / z
+
+
+def outdoor(cell=0.25, seed=2, length_s=1.0, receivers=32, layer=16, layer_damping=0.5,
+            size=(200.0, 200.0, 40.0)) -> Scene:
+    """The outdoor case: a ground block (its top at z = 0, 1 m thick) under open sky, a
+    3 m high, 30 m long, 1 m thick rigid box barrier at its centre, a 16-cell matched layer
+    round the grid. The grid spans ``size`` (200 x 200 x 40 m at 25 cm: 800 x 800 x 160
+    cells) with the block inside its layer and margin. The ground's admittance is a
+    constant and two relaxation branches fitted to Delany-Bazley at sigma 200 kPa s/m^2."""
+    rng = np.random.default_rng(seed)
+    n = np.round(np.asarray(size) / cell).astype(int)
+    pad = layer + 2
+    # The block spans the grid inside its pad; the grid's origin is shifted as usual.
+    origin = -cell * (pad + GRID_SHIFT)
+    grid = Grid(origin, float(cell), tuple(int(v) for v in n))
+    gx = (n[0] - 2 * pad) * cell
+    gy = (n[1] - 2 * pad) * cell
+    thick = 1.0
+    ground = box_mesh((gx, gy, thick), low=(0.0, 0.0, -thick), inward=False)
+    ground_top = 0.0
+    # Shift so that the block's bottom lies pad cells above the grid's bottom.
+    shift_z = (pad + 1) * cell + thick
+    ground.vertices = ground.vertices + np.array([0.0, 0.0, shift_z])
+    ground_top = shift_z
+    cx, cy = gx / 2, gy / 2
+    barrier = box_mesh((1.0, 30.0, 3.0), low=(cx - 0.5, cy - 15.0, ground_top), inward=False)
+    # The barrier's bottom lies on the ground: drop its floor face (two triangles).
+    keep = barrier.face_material != 0
+    barrier = Mesh(barrier.vertices, barrier.faces[keep], barrier.face_material[keep] - 1, barrier.names[1:])
+    mesh = merge(ground, barrier)
+    nm = mesh.materials
+    rate = C / (COURANT * cell)
+    steps = int(round(length_s * rate))
+    # Ground materials: the first six (the block's faces); barrier rigid.
+    f = np.array([63.0, 125.0])
+    y = delany_bazley_admittance(f)
+    base = float(np.real(y).min()) * 0.5
+    rows = []
+    level = base
+    T = 1.0 / rate
+    for fj, yj in zip(f, np.real(y)):
+        rise = max(float(yj) - level, 1e-3)
+        R = 1.0 / rise
+        rows.append((0.0, R, 2.0 * math.pi * fj * R * T))
+        level += rise
+    beta = np.zeros(nm)
+    beta[:6] = base
+    branches, owners = [], []
+    for m in range(6):
+        branches.append(np.array(rows))
+        owners.append(np.full(len(rows), m, np.int64))
+    source = np.array([cx - 20.0 + rng.uniform(-0.1, 0.1), cy + rng.uniform(-0.1, 0.1), ground_top + 0.5])
+    xs = np.linspace(cx + 10.0, cx + 150.0 * (gx / 2 - 10) / 150.0, receivers)
+    xs = np.minimum(xs, gx - 8.0)
+    probes = np.column_stack([xs, np.full(receivers, cy), np.full(receivers, ground_top + 1.5)])
+    return Scene(
+        name=f"outdoor {size[0]:g}x{size[1]:g}x{size[2]:g} m at {cell:g} m, layer {layer}",
+        mesh=mesh, beta=beta, branches=np.vstack(branches), branch_material=np.concatenate(owners),
+        grid=grid, source=source, probes=probes, rate=rate, steps=steps,
+        drive=gaussian_drive(rate, steps), layer=layer, layer_damping=layer_damping,
+        edge_admittance=1.0,
+    )

@@ -1,0 +1,125 @@
+/*
+ * asfdtd.h — C ABI of the MI355X wave-solver library (libasfdtd.so).
+ *
+ * Two halves:
+ *
+ *  1. The stepper. It replaces the reference's WebGPU stepper GpuWaveSolver
+ *     (src/acoustic_spaces/engine/gpu.py:445-612), which wave_responses picks at
+ *     `stepper = GpuWaveSolver(solver) if on_gpu else solver` (engine/wave.py:1178). Like
+ *     it, it takes the solver's layout (WaveSolver::layout, cpp/include/acoustic_spaces/
+ *     fdtd.hpp:727-782) once and steps it: run(signal) records every probe's pressure
+ *     before each step and keeps the state across calls (WaveSolver::run, fdtd.hpp:786-904).
+ *
+ *  2. The set-up. It replaces the reference's WaveSolver constructor on walls on cell
+ *     faces (fdtd.hpp:360-690; bound as _native.WaveSolver, cpp/bindings/
+ *     native_module.cpp:1238-1285): from a triangle mesh, materials and a grid it lays out
+ *     the grid and hands out the same layout.
+ *
+ * Plain pointers and sizes only. Functions return 0 on success and a negative code on
+ * failure; the message is then in as_fdtd_last_error / as_setup_last_error. A handle is
+ * used by one host thread at a time.
+ */
+#ifndef ASFDTD_H
+#define ASFDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define AS_OK 0
+#define AS_EINVAL (-1)   /* an argument or the layout is refused (reference: ValueError) */
+#define AS_EDEVICE (-2)  /* a HIP call failed (no device, out of memory, launch error) */
+#define AS_ERANGE (-3)   /* a grid or table beyond the library's 32-bit indices */
+
+/*
+ * The layout: the arrays WaveSolver::layout hands out (fdtd.hpp:727-782), each a pointer
+ * and its length in elements, in the reference's types. keep/push are per axis, the
+ * centres' then the faces' coefficients: keep_x, keep_face_x, keep_y, keep_face_y,
+ * keep_z, keep_face_z (each of length n_axis + 1, or 0 without a matched layer).
+ * Moving air (flow_*, wind_*, loop_*) is not part of it: the reference steps moving air on
+ * the CPU only (engine/wave.py:1159-1178), and so does this library refuse it.
+ */
+typedef struct as_layout {
+    uint64_t nx, ny, nz;
+    uint64_t stencil;              /* 7 or 19 */
+    const float *weights;          /* [6] or [18]: the update's link weights */
+    uint64_t n_weights;
+    float centre, total;
+    const float *scale;            /* [nx ny nz] each cell's s, or NULL (layout "scaled" 0) */
+    const float *layer_scale;      /* [layer_cells] or NULL */
+    const uint8_t *types;          /* [nx ny nz] 0 solid, 1 interior, 2 special, 3 layer */
+    uint64_t specials;
+    const int64_t *special;        /* [specials] cell of each special */
+    const uint64_t *link_start;    /* [specials + 1] */
+    uint64_t links;
+    const uint8_t *link_direction; /* [links] 0..17 */
+    const float *link_weight;      /* [links] */
+    const float *coef_a, *coef_b, *coef_inv, *source; /* [specials] */
+    const uint64_t *state_start;   /* [specials + 1] */
+    uint64_t states;
+    const int32_t *state_branch;   /* [states] */
+    const float *state_weight;     /* [states] */
+    uint64_t branches;
+    const float *branch_now, *branch_before, *branch_gain; /* [branches] */
+    uint64_t layer_cells;
+    const int64_t *layer_cell;     /* [layer_cells] */
+    const float *keep[6];
+    const float *push[6];
+    uint64_t probes;
+    const uint64_t *probe_start;   /* [probes + 1] */
+    uint64_t taps;
+    const uint64_t *probe_cells;   /* [taps] */
+    const double *probe_weights;   /* [taps] */
+} as_layout;
+
+typedef struct as_fdtd as_fdtd;
+
+/* Launch shape of the step kernels (0 = the library's default for the field). */
+typedef struct as_tuning {
+    int tile_y;        /* rows of a tile (the tile is 64 cells along x) */
+    int tile_z;        /* planes a tile marches through */
+    int row_align;     /* row pitch in floats: a multiple of this (1, 16, 32, 64) */
+    int graph_steps;   /* steps a captured hipGraph replays (0 = 64; negative: no graphs) */
+    int split;         /* 1: one launch per kind of cell (specials, interior, layer, probes) */
+} as_tuning;
+
+/* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the
+ * same scheme in double, coefficients widened). NULL on failure (see as_fdtd_create_error).
+ * Replaces GpuWaveSolver.__init__ (engine/gpu.py:445-545). */
+as_fdtd *as_fdtd_create(const as_layout *layout, int precision, const as_tuning *tuning);
+const char *as_fdtd_create_error(void);
+
+/* Advances `steps` steps, adding signal[k] at the source on step k, and writes every
+ * probe's pressure before each step to out[probe * steps + k]; host pointers. The state
+ * continues across calls. Replaces GpuWaveSolver.run (engine/gpu.py:546-612) and
+ * WaveSolver::run (fdtd.hpp:786-904). */
+int as_fdtd_run(as_fdtd *h, const float *signal, size_t steps, double *out);
+
+/* The same on device pointers, enqueued on the handle's stream without waiting. */
+int as_fdtd_run_device(as_fdtd *h, const float *d_signal, size_t steps, double *d_out);
+
+/* Zeroes the state (WaveSolver::reset, fdtd.hpp:906-915). */
+int as_fdtd_reset(as_fdtd *h);
+
+/* The current pressure, the field the next step's probes read, widened to double
+ * (nx ny nz values, x fastest). */
+int as_fdtd_field(as_fdtd *h, double *out);
+
+/* The handle's HIP stream (a hipStream_t), for the caller's events. */
+void *as_fdtd_stream(as_fdtd *h);
+
+/* Counts: [0] cells stepped as interior, [1] specials, [2] states, [3] layer cells,
+ * [4] probes, [5] active tiles, [6] row pitch, [7] steps done. */
+int as_fdtd_info(const as_fdtd *h, uint64_t *out);
+
+const char *as_fdtd_last_error(const as_fdtd *h);
+void as_fdtd_destroy(as_fdtd *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASFDTD_H */

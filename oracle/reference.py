@@ -1,0 +1,155 @@
+"""Test infrastructure: the reference's own wave solver, through oracle/_ref/libasref.so.
+
+The library is built by ``make -C oracle ref`` from the reference's header-only C++
+(cpp/include/acoustic_spaces/fdtd.hpp, read in place) and this repository's wrapper
+ref_harness.cpp. It is the checker of the product's layout producer and of the oracle's
+stepping, and the CPU baseline that bench.py times; the product never loads it.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBRARY = HERE / "_ref" / "libasref.so"
+
+_KINDS = {0: np.uint8, 1: np.int32, 2: np.int64, 3: np.uint64, 4: np.float32, 5: np.float64}
+_lib = None
+
+
+def available() -> bool:
+    return LIBRARY.exists()
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY.exists():
+        raise FileNotFoundError(f"{LIBRARY} is not built (make -C oracle ref)")
+    lib = ctypes.CDLL(str(LIBRARY))
+    P = ctypes.c_void_p
+    S = ctypes.c_size_t
+    D = ctypes.c_double
+    lib.asref_create.restype = P
+    lib.asref_create.argtypes = [P, S, P, S, P, P, S, P, P, S, P, D, S, S, S, P, P, S, D, S, D, S,
+                                 S, P, S, ctypes.c_int, ctypes.c_char_p, S]
+    lib.asref_layout_count.restype = ctypes.c_int
+    lib.asref_layout_count.argtypes = [P]
+    lib.asref_layout_entry.restype = ctypes.c_int
+    lib.asref_layout_entry.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(S),
+                                       ctypes.POINTER(P)]
+    lib.asref_run.argtypes = [P, P, S, P]
+    lib.asref_reset.argtypes = [P]
+    lib.asref_destroy.argtypes = [P]
+    lib.asref_info.argtypes = [P, P]
+    lib.asref_open_sides.argtypes = [P, P]
+    lib.asref_probe_found.restype = ctypes.c_int
+    lib.asref_probe_found.argtypes = [P, S]
+    lib.asref_probe_in_sight.restype = ctypes.c_int
+    lib.asref_probe_in_sight.argtypes = [P, S]
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class ReferenceSolver:
+    """The reference's ``WaveSolver`` (fdtd.hpp:339-3601) built from a scene's arrays, with
+    the Python contract of its binding: ``layout()``, ``run(signal)``, ``reset()`` and the
+    counts (cpp/bindings/native_module.cpp:1561-1707)."""
+
+    def __init__(self, vertices, faces, face_material, beta, branches, branch_material, origin,
+                 cell, shape, source, probes, edge_admittance=1.0, layer=0, layer_damping=0.0,
+                 threads=1, stencil=7, speed=None, probes_per_listener=1, fitted_walls=False):
+        lib = _load()
+        self._keep = [
+            np.ascontiguousarray(vertices, np.float64),
+            np.ascontiguousarray(faces, np.int64),
+            np.ascontiguousarray(face_material, np.int64),
+            np.ascontiguousarray(beta, np.float64),
+            np.ascontiguousarray(np.asarray(branches, np.float64).reshape(-1, 3)),
+            np.ascontiguousarray(branch_material, np.int64),
+            np.ascontiguousarray(origin, np.float64),
+            np.ascontiguousarray(source, np.float64),
+            np.ascontiguousarray(np.asarray(probes, np.float64).reshape(-1, 3)),
+        ]
+        v, f, fm, b, br, bm, o, s, p = self._keep
+        sp = None if speed is None else np.ascontiguousarray(speed, np.float32)
+        err = ctypes.create_string_buffer(4096)
+        nx, ny, nz = (int(x) for x in shape)
+        h = lib.asref_create(_ptr(v), len(v), _ptr(f), len(f), _ptr(fm), _ptr(b), len(b), _ptr(br),
+                             _ptr(bm), len(bm), _ptr(o), float(cell), nx, ny, nz, _ptr(s), _ptr(p),
+                             len(p), float(edge_admittance), int(layer), float(layer_damping),
+                             int(threads), int(stencil), _ptr(sp), int(probes_per_listener),
+                             1 if fitted_walls else 0, err, len(err))
+        if not h:
+            raise ValueError(err.value.decode())
+        self._h = h
+        self.shape = (nx, ny, nz)
+        info = np.zeros(8, np.uint64)
+        lib.asref_info(h, _ptr(info))
+        (self.air_cells, self.boundary_cells, self.branch_states, self.layer_cells, self.probes,
+         self.threads) = (int(x) for x in info[:6])
+        self.open = bool(info[6])
+        self.source_on_face = bool(info[7])
+        sides = np.zeros(6, np.int32)
+        lib.asref_open_sides(h, _ptr(sides))
+        self.open_sides = tuple(bool(x) for x in sides)
+
+    @classmethod
+    def from_scene(cls, scene, threads=1, **kw):
+        a = scene.solver_args()
+        return cls(threads=threads, **a, **kw)
+
+    def layout(self) -> dict:
+        lib = _load()
+        out = {}
+        name = ctypes.c_char_p()
+        kind = ctypes.c_int()
+        count = ctypes.c_size_t()
+        data = ctypes.c_void_p()
+        for k in range(lib.asref_layout_count(self._h)):
+            lib.asref_layout_entry(self._h, k, ctypes.byref(name), ctypes.byref(kind),
+                                   ctypes.byref(count), ctypes.byref(data))
+            dtype = np.dtype(_KINDS[kind.value])
+            n = count.value
+            if n:
+                buf = (ctypes.c_char * (n * dtype.itemsize)).from_address(data.value)
+                arr = np.frombuffer(buf, dtype=dtype).copy()
+            else:
+                arr = np.zeros(0, dtype)
+            out[name.value.decode()] = arr
+        return out
+
+    def run(self, signal) -> np.ndarray:
+        sig = np.ascontiguousarray(signal, np.float32)
+        out = np.zeros((self.probes, len(sig)), np.float64)
+        _load().asref_run(self._h, _ptr(sig), len(sig), _ptr(out))
+        return out
+
+    def reset(self):
+        _load().asref_reset(self._h)
+
+    def probe_found(self, p) -> bool:
+        return bool(_load().asref_probe_found(self._h, p))
+
+    def probe_in_sight(self, p) -> bool:
+        return bool(_load().asref_probe_in_sight(self._h, p))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.asref_destroy(h)
+            self._h = None
+
+
+def host_cores() -> int:
+    return len(os.sched_getaffinity(0))

@@ -1,0 +1,52 @@
+"""Small scenes shared by the tests: each a Scene (acoustic_spaces_amd.scenes) sized so the
+oracle steps it in seconds, covering what the reference's own tests cover (both stencils,
+branch states, a speed per cell, the matched layer, probes next to the source)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from acoustic_spaces_amd import scenes
+
+
+def with_branches(scene, per_material=((0.0, 5.0, 0.3), (0.002, 3.0, 0.5))):
+    """The scene with a relaxation and a resonant branch on every material."""
+    rows, owners = [], []
+    for m in range(scene.mesh.materials):
+        for r in per_material:
+            rows.append(r)
+            owners.append(m)
+    scene.branches = np.array(rows, float)
+    scene.branch_material = np.array(owners, np.int64)
+    return scene
+
+
+def small_box(stencil=7, seed=3, cell=0.1, size=(1.9, 1.4, 1.1), branches=True):
+    sc = scenes.shoebox(size=size, cell=cell, seed=seed, length_s=0.1, stencil=stencil)
+    return with_branches(sc) if branches else sc
+
+
+def open_block(layer=4, seed=5, cell=0.1):
+    """A rigid block in open air with a matched layer round the grid: the source and the
+    probes outside the block."""
+    rng = np.random.default_rng(seed)
+    block = scenes.box_mesh((0.6, 0.5, 0.4), low=(0.4, 0.4, 0.3), inward=False)
+    grid = scenes.grid_round((0.0, 0.0, 0.0), (1.4, 1.3, 1.0), cell, layer=layer)
+    rate = scenes.C / (scenes.COURANT * cell)
+    steps = 300
+    source = np.array([0.15, 0.2, 0.5]) + rng.uniform(-0.02, 0.02, 3)
+    probes = np.array([[1.2, 1.1, 0.8], [0.2, 1.1, 0.2], [1.25, 0.2, 0.5]])
+    sc = scenes.Scene(
+        name="open block", mesh=block, beta=np.full(block.materials, 0.05),
+        branches=np.zeros((0, 3)), branch_material=np.zeros(0, np.int64), grid=grid,
+        source=source, probes=probes, rate=rate, steps=steps,
+        drive=scenes.gaussian_drive(rate, steps, sigma_s=2e-3), layer=layer, layer_damping=0.5,
+    )
+    return with_branches(sc, per_material=((0.0, 4.0, 0.2),))
+
+
+def speeds_for(scene, seed=7):
+    """A speed of sound per cell in (0.9, 1]: s = r^2 scales each cell's links."""
+    rng = np.random.default_rng(seed)
+    n = scene.grid.cells
+    return (0.9 + 0.1 * rng.random(n)).astype(np.float32)

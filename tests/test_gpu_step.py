@@ -1,0 +1,90 @@
+"""The HIP stepper against the oracle, on the GPU: every probe's pressure before every
+step, and the field after the run, equal to the reference's float scheme bit for bit
+(the oracle is itself pinned to the reference's run(), tests/test_oracle.py); the double
+scheme equal to the oracle's double restatement. Layouts come from the reference's own
+set-up (oracle/_ref), as the reference's GpuWaveSolver takes them from
+``_native.WaveSolver`` (engine/wave.py:1178)."""
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _layout(scene, **kw):
+    from oracle.reference import ReferenceSolver
+
+    return ReferenceSolver.from_scene(scene, **kw).layout()
+
+
+CASES = {
+    "box7": lambda: (cases.small_box(), {}),
+    "box19": lambda: (cases.small_box(stencil=19), {}),
+    "open_layer": lambda: (cases.open_block(), {}),
+    "speed": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.small_box()),
+    "speed19": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.small_box(stencil=19)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("precision", [32, 64])
+def test_steps_equal_the_oracle(reference_available, name, precision):
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    scene, kw = CASES[name]()
+    layout = _layout(scene, **kw)
+    print(f"case {name} precision {precision} shape {scene.grid.shape}", flush=True)
+    gpu = HipWaveSolver(layout, precision=precision)
+    cpu = OracleSolver(layout, precision=precision)
+    drive = scene.drive[: scene.steps]
+    # Split so that the fields' parity flips both ways across calls (tests/test_gpu.py:56-85).
+    for piece in (drive[:101], drive[101:224], drive[224:]):
+        a = gpu.run(piece)
+        b = cpu.run(piece)
+        assert a.shape == b.shape
+        assert np.array_equal(a, b), np.abs(a - b).max()
+    assert np.array_equal(gpu.field(), cpu.field())
+
+
+def test_reset_starts_over(reference_available):
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    scene = cases.small_box()
+    layout = _layout(scene)
+    gpu = HipWaveSolver(layout)
+    first = gpu.run(scene.drive[:150])
+    gpu.reset()
+    again = gpu.run(scene.drive[:150])
+    assert np.array_equal(first, again)
+
+
+@pytest.mark.parametrize("tile_z,row_align", [(4, 1), (16, 16), (64, 64)])
+def test_launch_shape_does_not_change_results(reference_available, tile_z, row_align):
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    scene = cases.small_box(stencil=7)
+    layout = _layout(scene)
+    gpu = HipWaveSolver(layout, tile_z=tile_z, row_align=row_align)
+    cpu = OracleSolver(layout)
+    assert np.array_equal(gpu.run(scene.drive[:200]), cpu.run(scene.drive[:200]))
+
+
+def test_shoebox_full_grid(reference_available):
+    """The 6 x 4 x 3 m shoebox at 5 cm (125 x 85 x 65 cells, 576,000 air cells), 400 steps."""
+    from acoustic_spaces_amd import scenes
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    scene = cases.with_branches(scenes.shoebox())
+    layout = _layout(scene, threads=8)
+    gpu = HipWaveSolver(layout)
+    cpu = OracleSolver(layout)
+    d = scene.drive[:400]
+    a = gpu.run(d[:201])
+    b = cpu.run(d[:201])
+    assert np.array_equal(a, b)
+    assert np.array_equal(gpu.run(d[201:]), cpu.run(d[201:]))
