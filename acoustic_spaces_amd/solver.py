@@ -1,0 +1,236 @@
+"""The wave solver's set-up: the reference's ``_native.WaveSolver`` constructor and layout.
+
+``WaveSolver(bvh, face_material, beta, branches, branch_material, origin, cell, nx, ny, nz,
+source, probes, ...)`` takes the reference binding's arguments (cpp/bindings/
+native_module.cpp:1238-1285, 1561-1580) and lays the grid out on the host, in C++
+(csrc/setup.cpp, through libasfdtd.so): walls on cell faces, both stencils, a speed of sound
+per cell, the matched layer, admittance branches. ``layout()`` hands out the same arrays as
+the reference's (WaveSolver::layout, fdtd.hpp:727-782), to be stepped by
+``acoustic_spaces_amd.gpu.HipWaveSolver``. ``run`` steps it on the GPU, so that the object is
+also the reference solver's drop-in where it is stepped directly.
+
+Not here: moving air (the reference steps it on the CPU only), fitted walls (refused with
+a ValueError), and sources or probes that lie on a face (the reference's leans): refused.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from acoustic_spaces_amd import native
+
+P = ctypes.c_void_p
+S = ctypes.c_size_t
+D = ctypes.c_double
+
+_setup_lib = None
+
+
+def _lib():
+    global _setup_lib
+    if _setup_lib is None:
+        lib = native.load()
+        lib.as_setup_create.restype = P
+        lib.as_setup_create.argtypes = [P, S, P, S, P, P, S, P, P, S, P, D, S, S, S, P, P, S, D, S, D,
+                                        S, P, S, ctypes.c_int]
+        lib.as_setup_create_error.restype = ctypes.c_char_p
+        lib.as_setup_create_error.argtypes = []
+        lib.as_setup_layout.restype = ctypes.c_int
+        lib.as_setup_layout.argtypes = [P, ctypes.POINTER(native.Layout)]
+        lib.as_setup_info.restype = ctypes.c_int
+        lib.as_setup_info.argtypes = [P, P]
+        lib.as_setup_probe.restype = ctypes.c_int
+        lib.as_setup_probe.argtypes = [P, S, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        lib.as_setup_destroy.restype = None
+        lib.as_setup_destroy.argtypes = [P]
+        _setup_lib = lib
+    return _setup_lib
+
+
+class Bvh:
+    """The mesh a WaveSolver is laid out over: ``Bvh(vertices (V, 3), faces (F, 3))``, as the
+    reference's ``_native.Bvh`` is built (native_module.cpp:1422-1434)."""
+
+    def __init__(self, vertices, faces, split: str = "sah"):
+        if split not in ("sah", "median"):
+            raise ValueError("Bvh: split must be 'sah' or 'median'")
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        self.faces = np.ascontiguousarray(faces, dtype=np.int64).reshape(-1, 3)
+        if len(self.faces) == 0:
+            raise ValueError("Bvh: the mesh has no faces")
+        if self.faces.min() < 0 or self.faces.max() >= len(self.vertices):
+            raise ValueError("Bvh: face refers to a missing vertex")
+
+    def face_count(self) -> int:
+        return len(self.faces)
+
+
+def _array(ptr, count, dtype):
+    dtype = np.dtype(dtype)
+    if count == 0 or not ptr:
+        return np.zeros(0, dtype)
+    buf = (ctypes.c_char * (count * dtype.itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dtype).copy()
+
+
+class WaveSolver:
+    """The grid of the reference's WaveSolver, laid out here (see the module's notes)."""
+
+    def __init__(self, bvh, face_material, beta, branches, branch_material, origin, cell, nx, ny,
+                 nz, source, probes, edge_admittance=1.0, layer=0, layer_damping=0.0, threads=0,
+                 stencil=7, speed=None, flow_x=None, flow_y=None, probes_per_listener=1,
+                 fitted_walls=False, vertices=None, faces=None):
+        if fitted_walls:
+            raise ValueError("fitted walls: not laid out by this set-up; take walls on cell faces")
+        if flow_x is not None or flow_y is not None:
+            raise ValueError("moving air: stepped on the CPU only, as the reference's GPU stepper "
+                             "leaves it; leave the air's motion out")
+        lib = _lib()
+        fm = np.ascontiguousarray(face_material, np.int64)
+        if len(fm) != bvh.face_count():
+            raise ValueError("WaveSolver: one material per face")
+        b = np.ascontiguousarray(beta, np.float64)
+        br = np.ascontiguousarray(np.asarray(branches, np.float64).reshape(-1, 3))
+        bm = np.ascontiguousarray(branch_material, np.int64)
+        if len(br) != len(bm):
+            raise ValueError("WaveSolver: m, r and q of each branch, and its material")
+        o = np.ascontiguousarray(origin, np.float64).reshape(3)
+        s = np.ascontiguousarray(source, np.float64).reshape(3)
+        p = np.ascontiguousarray(np.asarray(probes, np.float64).reshape(-1, 3))
+        n = int(nx) * int(ny) * int(nz)
+        sp = None
+        if speed is not None:
+            sp = np.ascontiguousarray(speed, np.float32)
+            if len(sp) != n:
+                raise ValueError("WaveSolver: one speed per cell of the grid")
+        self._keep = (bvh, fm, b, br, bm, o, s, p, sp)
+        ptr = lambda a: None if a is None or len(a) == 0 else a.ctypes.data  # noqa: E731
+        h = lib.as_setup_create(
+            bvh.vertices.ctypes.data, len(bvh.vertices), bvh.faces.ctypes.data, len(bvh.faces),
+            fm.ctypes.data, b.ctypes.data, len(b), ptr(br), ptr(bm), len(bm), o.ctypes.data,
+            float(cell), int(nx), int(ny), int(nz), s.ctypes.data, ptr(p), len(p),
+            float(edge_admittance), int(layer), float(layer_damping), int(stencil), ptr(sp),
+            int(probes_per_listener), int(threads))
+        if not h:
+            raise ValueError(lib.as_setup_create_error().decode())
+        self._h = h
+        self._shape = (int(nx), int(ny), int(nz))
+        self._stencil = int(stencil)
+        info = np.zeros(8, np.uint64)
+        lib.as_setup_info(h, info.ctypes.data)
+        self.air_cells = int(info[0])
+        self.boundary_cells = int(info[1])
+        self.branch_states = int(info[2])
+        self.layer_cells = int(info[3])
+        self._probes = int(info[4])
+        self.open = bool(info[5])
+        self.open_sides = tuple(bool((int(info[6]) >> k) & 1) for k in range(6))
+        self.source_on_face = False
+        self.flowing = False
+        self._stepper = None
+        self._layout = None
+
+    @classmethod
+    def from_scene(cls, scene, threads=0, **kw):
+        """The solver of an ``acoustic_spaces_amd.scenes.Scene``."""
+        a = scene.solver_args()
+        nx, ny, nz = a.pop("shape")
+        bvh = Bvh(a.pop("vertices"), a.pop("faces"))
+        return cls(bvh, a.pop("face_material"), a.pop("beta"), a.pop("branches"),
+                   a.pop("branch_material"), a.pop("origin"), a.pop("cell"), nx, ny, nz,
+                   a.pop("source"), a.pop("probes"), threads=threads, **a, **kw)
+
+    @property
+    def probes(self) -> int:
+        return self._probes
+
+    @property
+    def stencil(self) -> int:
+        return self._stencil
+
+    def probe_found(self, p: int) -> bool:
+        found = ctypes.c_int()
+        if _lib().as_setup_probe(self._h, int(p), ctypes.byref(found), None) != 0:
+            raise IndexError(p)
+        return bool(found.value)
+
+    def probe_in_sight(self, p: int) -> bool:
+        sight = ctypes.c_int()
+        if _lib().as_setup_probe(self._h, int(p), None, ctypes.byref(sight)) != 0:
+            raise IndexError(p)
+        return bool(sight.value)
+
+    def layout(self) -> dict:
+        """The layout's arrays by the reference's names and types (fdtd.hpp:727-782)."""
+        if self._layout is not None:
+            return {k: v.copy() for k, v in self._layout.items()}
+        L = native.Layout()
+        if _lib().as_setup_layout(self._h, ctypes.byref(L)) != 0:
+            raise RuntimeError("as_setup_layout failed")
+        n = L.nx * L.ny * L.nz
+        u64, f32 = np.uint64, np.float32
+        out = {
+            "shape": np.array([L.nx, L.ny, L.nz], u64),
+            "stencil": np.array([L.stencil], u64),
+            "weights": _array(L.weights, L.n_weights, f32),
+            "centre": np.array([L.centre], f32),
+            "total": np.array([L.total], f32),
+            "scaled": np.array([1 if L.scale else 0], u64),
+            "scale": _array(L.scale, n if L.scale else 0, f32),
+            "layer_scale": _array(L.layer_scale, L.layer_cells if L.layer_scale else 0, f32),
+            "types": _array(L.types, n, np.uint8),
+            "special": _array(L.special, L.specials, np.int64),
+            "link_start": _array(L.link_start, L.specials + 1, u64),
+            "link_direction": _array(L.link_direction, L.links, np.uint8),
+            "link_weight": _array(L.link_weight, L.links, f32),
+            "coef_a": _array(L.coef_a, L.specials, f32),
+            "coef_b": _array(L.coef_b, L.specials, f32),
+            "coef_inv": _array(L.coef_inv, L.specials, f32),
+            "source": _array(L.source, L.specials, f32),
+            "state_start": _array(L.state_start, L.specials + 1, u64),
+            "state_branch": _array(L.state_branch, L.states, np.int32),
+            "state_weight": _array(L.state_weight, L.states, f32),
+            "branch_now": _array(L.branch_now, L.branches, f32),
+            "branch_before": _array(L.branch_before, L.branches, f32),
+            "branch_gain": _array(L.branch_gain, L.branches, f32),
+            "layer_cells": _array(L.layer_cell, L.layer_cells, np.int64),
+            "flow_x": np.zeros(0, f32), "flow_y": np.zeros(0, f32),
+            "flow_mask": np.zeros(0, np.uint8),
+        }
+        for ax in "xyz":
+            out[f"wind_{ax}"] = np.zeros(0, f32)
+            out[f"loop_{ax}"] = np.zeros(0, f32)
+        lengths = (L.nx + 1, L.ny + 1, L.nz + 1)
+        for a, ax in enumerate("xyz"):
+            for j, face in enumerate(("", "face_")):
+                m = lengths[a] if L.layer_cells else 0
+                out[f"keep_{face}{ax}"] = _array(L.keep[2 * a + j], m, f32)
+                out[f"push_{face}{ax}"] = _array(L.push[2 * a + j], m, f32)
+        out["probe_start"] = _array(L.probe_start, L.probes + 1, u64)
+        out["probe_cells"] = _array(L.probe_cells, L.taps, u64)
+        out["probe_weights"] = _array(L.probe_weights, L.taps, np.float64)
+        self._layout = out
+        return {k: v.copy() for k, v in out.items()}
+
+    # Stepping, on the GPU (the reference's run/reset, fdtd.hpp:786-915).
+    def _gpu(self):
+        if self._stepper is None:
+            from acoustic_spaces_amd.gpu import HipWaveSolver
+
+            self._stepper = HipWaveSolver(self.layout())
+        return self._stepper
+
+    def run(self, signal) -> np.ndarray:
+        return self._gpu().run(signal)
+
+    def reset(self) -> None:
+        if self._stepper is not None:
+            self._stepper.reset()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _setup_lib is not None:
+            _setup_lib.as_setup_destroy(h)
+            self._h = None

@@ -50,3 +50,13 @@ def speeds_for(scene, seed=7):
     rng = np.random.default_rng(seed)
     n = scene.grid.cells
     return (0.9 + 0.1 * rng.random(n)).astype(np.float32)
+
+
+def probes_near_source(seed=9):
+    """Probes within three cells of the source (read trilinearly) and next to walls."""
+    sc = small_box(seed=seed)
+    s = sc.source
+    c = sc.grid.cell
+    sc.probes = np.array([s + [1.3 * c, 0.4 * c, -0.7 * c], s + [-2.1 * c, 1.1 * c, 0.3 * c],
+                          [0.03, 0.71, 0.55], [1.0, 0.05, 1.07]])
+    return sc
