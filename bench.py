@@ -1,0 +1,176 @@
+"""Throughput of the wave solver's time step on one MI355X.
+
+A step is one time step of the FDTD scheme over the whole grid of the hall workload: the
+30 x 20 x 12 m hall with a ceiling of six tilted planes and side walls splayed 5 degrees, at
+10 cm cells (305 x 257 x 125 cells, 7,568,516 air cells), walls on cell faces with a
+constant admittance and three first-order (relaxation) admittance branches per material,
+one source, 16 receivers; 2 s of response is 11,882 steps. The grid is laid out by this
+package's set-up (csrc/setup.cpp) and stepped by the HIP kernels (csrc/fdtd_step.hip);
+the drive and the probes' output stay in device memory over the timed region.
+
+Printed: one JSON line with Mcell-updates/s (air cells x steps / s), the step kernel's
+roofline (algorithmic bytes per step over its time per step, against HBM's 8 TB/s) and
+the reference's own CPU solver (built from its headers into oracle/_ref) timed on this
+host's cores on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+PER_AIR_CELL = 12  # bytes: read the current and previous pressure, write the next (float)
+PER_LAYER_CELL = 80  # bytes: the current pressure, nine state values read and written, the next
+PER_STATE = 12  # bytes: a branch velocity read twice-over and written (float)
+
+
+def algorithmic_bytes(info: dict, air_cells: int, layer_cells: int, states: int) -> int:
+    """Bytes one step must move at least: 12 per air cell outside the matched layer, 80 per
+    layer cell, 12 per branch state (coefficient and index reads not counted)."""
+    return PER_AIR_CELL * (air_cells - layer_cells) + PER_LAYER_CELL * layer_cells + PER_STATE * states
+
+
+def cpu_baseline(scene, target_s: float = 12.0) -> dict:
+    """The reference's CPU solver (oracle/_ref) on all host cores, on a bounded run of the
+    same workload; the oracle's one-thread port if the reference's build is absent."""
+    cores = len(os.sched_getaffinity(0))
+    from oracle import reference
+
+    if reference.available():
+        solver = reference.ReferenceSolver.from_scene(scene, threads=cores)
+        kind, used = "reference", solver.threads
+        air = solver.air_cells
+    else:
+        from acoustic_spaces_amd.solver import WaveSolver
+        from oracle.stepper import OracleSolver
+
+        solver = OracleSolver(WaveSolver.from_scene(scene).layout())
+        kind, used = "port", 1
+        air = int(np.count_nonzero(WaveSolver.from_scene(scene).layout()["types"]))
+    drive = scene.drive
+    t = time.perf_counter()
+    solver.run(drive[:8])
+    first = max(time.perf_counter() - t, 1e-6) / 8
+    steps = int(min(len(drive) - 8, max(16, target_s / first)))
+    t = time.perf_counter()
+    solver.run(drive[8:8 + steps])
+    elapsed = time.perf_counter() - t
+    return {
+        "value": round(air * steps / elapsed / 1e6, 1),
+        "unit": "Mcell-updates/s",
+        "cores": int(used),
+        "kind": kind,
+        "sample": f"{steps} steps of the same hall grid ({air:,} air cells) after 8, "
+                  f"{elapsed:.1f} s, threads={used}",
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=11882)
+    ap.add_argument("--warmup", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--graph-steps", type=int, default=0)
+    ap.add_argument("--tile-z", type=int, default=0)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the solver is built and measured at N=1 only "
+              f"(--gpus {args.gpus} asked for); a multi-GPU run is not part of this build.",
+              file=sys.stderr)
+        return 2
+
+    from acoustic_spaces_amd import hip, scenes
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    scene = scenes.hall()[0]
+    K, W = args.steps, args.warmup
+    need = W + K
+    drive = scene.drive
+    if need > len(drive):
+        drive = scenes.gaussian_drive(scene.rate, need)
+    t = time.perf_counter()
+    solver = WaveSolver.from_scene(scene)
+    layout = solver.layout()
+    setup_s = time.perf_counter() - t
+    stepper = HipWaveSolver(layout, graph_steps=args.graph_steps, tile_z=args.tile_z)
+    info = stepper.info()
+    probes = stepper.probes
+    d_drive = hip.DeviceArray(drive[:need].astype(np.float32))
+    d_out = hip.DeviceArray(np.zeros((probes, need), np.float64))
+    stream = stepper.stream
+    # Warm-up, then exactly K timed steps between synchronisations, on the solver's stream.
+    stepper.run_device(d_drive.ptr, W, d_out.ptr)
+    hip.synchronize()
+    start, stop = hip.Event(), hip.Event()
+    start.record(stream)
+    stepper.run_device(d_drive.ptr + 4 * W, K, d_out.ptr + 8 * probes * W)
+    stop.record(stream)
+    hip.synchronize()
+    ms = start.elapsed_ms(stop)
+    out = d_out.to_host()
+    if not np.all(np.isfinite(out)):
+        print("non-finite probe output", file=sys.stderr)
+        return 1
+
+    air = solver.air_cells
+    per_step_s = ms / 1e3 / K
+    value = air / per_step_s / 1e6
+    step_bytes = algorithmic_bytes(info, air, solver.layer_cells, solver.branch_states)
+    achieved = step_bytes / per_step_s / 1e9
+    line = {
+        "metric": "FDTD Mcell-updates/s (hall, plain solver)",
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms / K, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "hall 30x20x12 m, ceiling of 6 tilted planes, side walls splayed 5 deg, "
+                        f"10 cm cells {scene.grid.shape[0]}x{scene.grid.shape[1]}x{scene.grid.shape[2]}, "
+                        f"{air:,} air cells, walls on cell faces, 3 relaxation branches per material, "
+                        "1 source, 16 receivers",
+            "air_cells": air,
+            "branch_states": solver.branch_states,
+            "special_cells": info["specials"],
+            "setup_s": round(setup_s, 2),
+            "graph_steps": args.graph_steps or 64,
+            "precision": "float (bit-identical to the reference's CPU scheme)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_step": step_bytes,
+            "kernel": "step_kernel<float,false,false> (one fused launch per step)",
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(scene)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
