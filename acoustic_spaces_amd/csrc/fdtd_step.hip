@@ -43,6 +43,10 @@ namespace {
 constexpr int kTileX = 64;   // cells of a tile along x: one wavefront
 constexpr int kTileRows = 4; // rows (y) of a tile: one wavefront each
 constexpr int kBlock = kTileX * kTileRows;
+constexpr uint32_t kProbesPerBlock = kBlock / kTileX;  // one wavefront a probe
+constexpr uint32_t kXcds = 8;  // MI355X's dies: blocks go to them round robin by index
+constexpr uint32_t kAllInterior = 1u << 31;  // a tile's flag: every cell of it is interior
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;    // a block with no tile
 
 constexpr int kSteps[18][3] = {
     {-1, 0, 0},  {1, 0, 0},  {0, -1, 0}, {0, 1, 0},  {0, 0, -1}, {0, 0, 1},
@@ -103,9 +107,72 @@ struct StepArgs {
     unsigned long long step;  // index into signal and out
 };
 
+// The 7-point column of one lane: planes z0 to z1 - 1 of cell (x, y), in chunks of kUnroll
+// planes. Every load of a chunk (the planes above, the four in-plane neighbours, the
+// previous pressure and, unless the whole tile is interior, the classes) is issued before
+// any arithmetic, so a wavefront keeps some thirty loads in flight instead of waiting
+// twice a plane. Each address lies in the padded grid whatever the cell's class
+// (x < nx <= px, 1 <= y <= ny - 2, 1 <= z <= nz - 2), so the loads need no guard; only
+// interior cells are written.
+template <typename T, bool SCALED, bool ALL>
+__device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int z0, int z1) {
+    constexpr int kUnroll = 4;
+    const int sy = s.px, sz = static_cast<int>(s.sxy);
+    const T *__restrict__ b = s.b;
+    T col[kUnroll + 2];  // planes z - 1, z, z + 1, ..., z + kUnroll
+    col[0] = b[i - sz];
+    col[1] = b[i];
+    for (int z = z0; z < z1; z += kUnroll, i += kUnroll * sz) {
+        const int count = min(kUnroll, z1 - z);
+        T xm[kUnroll], xp[kUnroll], ym[kUnroll], yp[kUnroll], prev[kUnroll];
+        uint8_t cls[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (u < count) {
+                const int j = i + u * sz;
+                col[u + 2] = b[j + sz];
+                xm[u] = b[j - 1];
+                xp[u] = b[j + 1];
+                ym[u] = b[j - sy];
+                yp[u] = b[j + sy];
+                prev[u] = s.a[j];
+                if constexpr (!ALL) cls[u] = s.types[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (u < count && (ALL || cls[u] == 1)) {
+                // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
+                T sum = xm[u] + xp[u];
+                sum = sum + ym[u];
+                sum = sum + yp[u];
+                sum = sum + col[u];
+                sum = sum + col[u + 2];
+                T next;
+                if constexpr (SCALED) {
+                    const T sc = static_cast<T>(s.scale[i + u * sz]);
+                    const T c0 = static_cast<T>(2) - s.total * sc;
+                    const T ax = sc * s.axial;
+                    next = c0 * col[u + 1] + ax * sum;
+                } else {
+                    next = s.centre * col[u + 1] + s.axial * sum;
+                }
+                s.a[i + u * sz] = next - prev[u];
+            }
+        }
+        if (count == kUnroll) {
+            col[0] = col[kUnroll];
+            col[1] = col[kUnroll + 1];
+        }
+    }
+}
+
 template <typename T, bool WIDE, bool SCALED>
 __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t block) {
-    const uint32_t tile = s.tiles[block];
+    const uint32_t entry = s.tiles[block];
+    if (entry == kNoTile) return;
+    const bool all = (entry & kAllInterior) != 0;
+    const uint32_t tile = entry & ~kAllInterior;
     const uint32_t tx = tile % s.tiles_x;
     const uint32_t rest = tile / s.tiles_x;
     const uint32_t ty = rest % s.tiles_y;
@@ -120,58 +187,10 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
     int i = x + y * sy + z0 * sz;
     const T *__restrict__ b = s.b;
     if constexpr (!WIDE) {
-        // The column in chunks of kUnroll planes: every load of a chunk (the planes above,
-        // the four in-plane neighbours, the previous pressure and the classes) is issued
-        // before any arithmetic, so a wavefront keeps some thirty loads in flight instead
-        // of waiting twice a plane. Each address lies in the padded grid whatever the
-        // cell's class (x < nx <= px, 1 <= y <= ny - 2, 1 <= z <= nz - 2), so the loads need
-        // no guard; only interior cells are written.
-        constexpr int kUnroll = 4;
-        T col[kUnroll + 2];  // planes z - 1, z, z + 1, ..., z + kUnroll
-        col[0] = b[i - sz];
-        col[1] = b[i];
-        for (int z = z0; z < z1; z += kUnroll, i += kUnroll * sz) {
-            const int count = min(kUnroll, z1 - z);
-            T xm[kUnroll], xp[kUnroll], ym[kUnroll], yp[kUnroll], prev[kUnroll];
-            uint8_t cls[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                if (u < count) {
-                    const int j = i + u * sz;
-                    col[u + 2] = b[j + sz];
-                    xm[u] = b[j - 1];
-                    xp[u] = b[j + 1];
-                    ym[u] = b[j - sy];
-                    yp[u] = b[j + sy];
-                    prev[u] = s.a[j];
-                    cls[u] = s.types[j];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                if (u < count && cls[u] == 1) {
-                    // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
-                    T sum = xm[u] + xp[u];
-                    sum = sum + ym[u];
-                    sum = sum + yp[u];
-                    sum = sum + col[u];
-                    sum = sum + col[u + 2];
-                    T next;
-                    if constexpr (SCALED) {
-                        const T sc = static_cast<T>(s.scale[i + u * sz]);
-                        const T c0 = static_cast<T>(2) - s.total * sc;
-                        const T ax = sc * s.axial;
-                        next = c0 * col[u + 1] + ax * sum;
-                    } else {
-                        next = s.centre * col[u + 1] + s.axial * sum;
-                    }
-                    s.a[i + u * sz] = next - prev[u];
-                }
-            }
-            if (count == kUnroll) {
-                col[0] = col[kUnroll];
-                col[1] = col[kUnroll + 1];
-            }
+        if (all) {
+            interior_column<T, SCALED, true>(s, i, z0, z1);
+        } else {
+            interior_column<T, SCALED, false>(s, i, z0, z1);
         }
         return;
     }
@@ -223,23 +242,59 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
 
 // A special cell (fdtd.hpp:860-881): its links summed in their order, its branch states'
 // terms, the source, coef_inv applied last; then its states' velocities. The links and
-// states sit in the block's slabs (slot-major, so a wavefront's loads of a slot coalesce),
-// read in chunks whose loads are all issued before the sums.
+// states sit in the block's slabs (slot-major, so a wavefront's loads of a slot coalesce).
+// The loads come in two waves: first everything indexed by the special itself (its cell,
+// coefficients, first chunk of link offsets and weights, first chunk of states), then
+// everything indexed by its cell (its pressures and its links' far ends); chunks of
+// links and states past the first, rare, follow one by one.
 template <typename T>
 __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t block) {
+    constexpr int kLinkChunk = 6, kStateChunk = 4;
     const uint32_t t = threadIdx.x;
     const uint32_t k = block * kBlock + t;
     if (k >= s.specials) return;
     const int i = static_cast<int>(s.sp_cell[k]);
     const int nl = s.sp_links[k], ns = s.sp_states[k];
-    const T *__restrict__ b = s.b;
+    const float ca = s.coef_a[k], cb = s.coef_b[k], ci = s.coef_inv[k], src = s.source[k];
+    const float sig = s.signal[s.step];
     const uint32_t lb = s.link_base[block] + t;
-    constexpr int kLinkChunk = 6;
+    const uint32_t sb = s.state_base[block] + t;
+    const T *__restrict__ b = s.b;
+    int32_t off[kLinkChunk];
+    float w[kLinkChunk];
+#pragma unroll
+    for (int u = 0; u < kLinkChunk; ++u) {
+        if (u < nl) {
+            off[u] = s.link_offset[lb + u * kBlock];
+            w[u] = s.link_weight[lb + u * kBlock];
+        }
+    }
+    float sw[kStateChunk], sn[kStateChunk], sbf[kStateChunk];
+    T swa[kStateChunk], swb[kStateChunk];
+#pragma unroll
+    for (int u = 0; u < kStateChunk; ++u) {
+        if (u < ns) {
+            const uint32_t q = sb + u * kBlock;
+            sw[u] = s.state_weight[q];
+            sn[u] = s.state_now[q];
+            sbf[u] = s.state_before[q];
+            swa[u] = s.wa[q];
+            swb[u] = s.wb[q];
+        }
+    }
+    const T here = b[i];
+    const T previous = s.a[i];
+    T v[kLinkChunk];
+#pragma unroll
+    for (int u = 0; u < kLinkChunk; ++u) {
+        if (u < nl) v[u] = b[i + off[u]];
+    }
     T sum = 0;
-    for (int c = 0; c < nl; c += kLinkChunk) {
-        int32_t off[kLinkChunk];
-        float w[kLinkChunk];
-        T v[kLinkChunk];
+#pragma unroll
+    for (int u = 0; u < kLinkChunk; ++u) {
+        if (u < nl) sum += static_cast<T>(w[u]) * v[u];
+    }
+    for (int c = kLinkChunk; c < nl; c += kLinkChunk) {
 #pragma unroll
         for (int u = 0; u < kLinkChunk; ++u) {
             if (c + u < nl) {
@@ -256,26 +311,42 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
             if (c + u < nl) sum += static_cast<T>(w[u]) * v[u];
         }
     }
-    const uint32_t sb = s.state_base[block] + t;
+    T own[kStateChunk];
     T flow = 0;
-    for (int j = 0; j < ns; ++j) {
-        const uint32_t q = sb + j * kBlock;
-        const T own = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
-        flow += static_cast<T>(s.state_weight[q]) * own;
+#pragma unroll
+    for (int u = 0; u < kStateChunk; ++u) {
+        if (u < ns) {
+            own[u] = static_cast<T>(sn[u]) * swb[u] - static_cast<T>(sbf[u]) * swa[u];
+            flow += static_cast<T>(sw[u]) * own[u];
+        }
     }
-    const T previous = s.a[i];
-    T acc = static_cast<T>(s.coef_a[k]) * b[i];
+    for (int c = kStateChunk; c < ns; ++c) {
+        const uint32_t q = sb + c * kBlock;
+        const T term = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
+        flow += static_cast<T>(s.state_weight[q]) * term;
+    }
+    T acc = static_cast<T>(ca) * here;
     acc = acc + sum;
-    acc = acc - static_cast<T>(s.coef_b[k]) * previous;
+    acc = acc - static_cast<T>(cb) * previous;
     acc = acc - flow;
-    acc = acc + static_cast<T>(s.source[k]) * static_cast<T>(s.signal[s.step]);
-    const T next = acc * static_cast<T>(s.coef_inv[k]);
+    acc = acc + static_cast<T>(src) * static_cast<T>(sig);
+    const T next = acc * static_cast<T>(ci);
     s.a[i] = next;
-    for (int j = 0; j < ns; ++j) {
-        // The term again: wa has not changed since it was taken.
-        const uint32_t q = sb + j * kBlock;
-        const T own = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
-        s.wa[q] = s.wa[q] + ((next - previous) * static_cast<T>(s.state_gain[q]) + own);
+    if (ns > 0) {
+        const T rise = next - previous;
+#pragma unroll
+        for (int u = 0; u < kStateChunk; ++u) {
+            if (u < ns) {
+                const uint32_t q = sb + u * kBlock;
+                s.wa[q] = swa[u] + (rise * static_cast<T>(s.state_gain[q]) + own[u]);
+            }
+        }
+        for (int c = kStateChunk; c < ns; ++c) {
+            // The term again: wa has not changed since it was taken.
+            const uint32_t q = sb + c * kBlock;
+            const T term = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
+            s.wa[q] = s.wa[q] + (rise * static_cast<T>(s.state_gain[q]) + term);
+        }
     }
 }
 
@@ -317,15 +388,38 @@ __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
     s.a[i] = total;
 }
 
-// A probe (fdtd.hpp:3525-3534): its cells' current pressures, weighted, summed in double
-// in the order of its cells.
+// The probes (fdtd.hpp:3525-3534), one wavefront each: its lanes read the probe's cells
+// and form the products w * p in double at once, into the block's shared memory; then one
+// lane sums them in the order of the cells, as the reference sums them. Every thread of
+// the block reaches both barriers.
 template <typename T>
-__device__ __forceinline__ void record_probe(const StepArgs<T> &s, uint32_t p) {
-    double value = 0.0;
-    for (uint32_t k = s.probe_start[p]; k < s.probe_start[p + 1]; ++k) {
-        value += s.probe_weight[k] * static_cast<double>(s.b[s.probe_cell[k]]);
+__device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t block) {
+    __shared__ double products[kBlock];
+    const uint32_t wave = threadIdx.x / kTileX, lane = threadIdx.x % kTileX;
+    const uint32_t p = block * (kBlock / kTileX) + wave;
+    const bool active = p < s.probes;
+    const uint32_t k0 = active ? s.probe_start[p] : 0, k1 = active ? s.probe_start[p + 1] : 0;
+    // Every wave of the block runs the same number of rounds (the block's longest probe).
+    uint32_t most = k1 - k0;
+    for (int w = 0; w < kBlock / kTileX; ++w) {
+        const uint32_t q = block * (kBlock / kTileX) + static_cast<uint32_t>(w);
+        if (q < s.probes) most = max(most, s.probe_start[q + 1] - s.probe_start[q]);
     }
-    s.out[p * s.out_stride + s.step] = value;
+    double value = 0.0;
+    for (uint32_t base = 0; base < most; base += kTileX) {
+        const uint32_t k = k0 + base + lane;
+        double product = 0.0;
+        if (k < k1) product = s.probe_weight[k] * static_cast<double>(s.b[s.probe_cell[k]]);
+        products[threadIdx.x] = product;
+        __syncthreads();
+        if (lane == 0 && k0 + base < k1) {
+            const uint32_t count = min(static_cast<uint32_t>(kTileX), k1 - k0 - base);
+            const double *mine = products + wave * kTileX;
+            for (uint32_t l = 0; l < count; ++l) value += mine[l];
+        }
+        __syncthreads();
+    }
+    if (active && lane == 0) s.out[p * s.out_stride + s.step] = value;
 }
 
 // One step in one launch: special blocks first (their gathers are the longest chains of
@@ -349,8 +443,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
         return;
     }
     block -= s.layer_blocks;
-    const uint32_t p = block * kBlock + threadIdx.x;
-    if (p < s.probes) record_probe<T>(s, p);
+    record_probes<T>(s, block);
 }
 
 // The same work as four launches, one per kind of cell (the split launch shape).
@@ -372,8 +465,7 @@ __global__ __launch_bounds__(kBlock) void layer_kernel(const StepArgs<T> s) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void probe_kernel(const StepArgs<T> s) {
-    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p < s.probes) record_probe<T>(s, p);
+    record_probes<T>(s, blockIdx.x);
 }
 
 thread_local std::string g_create_error;
@@ -410,6 +502,7 @@ struct as_fdtd {
     float *state_weight = nullptr, *state_now = nullptr, *state_before = nullptr, *state_gain = nullptr;
     size_t state_slots = 0;   // the state slabs' length (states and padding)
     int split = 0;            // 1: four launches a step instead of one
+    int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order
     uint32_t *layer_cell = nullptr;
     uint64_t *layer_at = nullptr;
     float *layer_scale = nullptr;
@@ -479,6 +572,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (tuning && tuning->graph_steps != 0) h->graph_steps = tuning->graph_steps < 0 ? 0u : static_cast<uint32_t>(tuning->graph_steps);
     if (h->graph_steps > 4096) refuse("at most 4096 steps a graph");
     if (tuning && tuning->split > 0) h->split = 1;
+    if (tuning && tuning->tile_order > 0) h->tile_order = 1;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
     const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);
     const uint64_t n_pad = px * ny * nz;
@@ -509,6 +603,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->tiles_x = static_cast<uint32_t>((nx + kTileX - 1) / kTileX);
     h->tiles_y = static_cast<uint32_t>((ny - 2 + kTileRows - 1) / kTileRows);
     const uint32_t tiles_z = static_cast<uint32_t>((nz - 2 + h->tile_z - 1) / h->tile_z);
+    if (static_cast<uint64_t>(h->tiles_x) * h->tiles_y * tiles_z >= kAllInterior) refuse("too many tiles");
     std::vector<uint32_t> tiles;
     uint64_t interior = 0;
     for (uint32_t tz = 0; tz < tiles_z; ++tz) {
@@ -524,13 +619,30 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
                 for (uint64_t z = z0; z < z1; ++z)
                     for (uint64_t y = y0; y < y1; ++y)
                         for (uint64_t x = x0; x < x1; ++x) count += types[x + px * (y + ny * z)] == 1;
-                if (count) tiles.push_back(tx + h->tiles_x * (ty + h->tiles_y * tz));
+                if (count) {
+                    uint32_t id = tx + h->tiles_x * (ty + h->tiles_y * tz);
+                    if (count == (z1 - z0) * (y1 - y0) * (x1 - x0)) id |= kAllInterior;
+                    tiles.push_back(id);
+                }
                 interior += count;
             }
         }
     }
-    h->tile_count = static_cast<uint32_t>(tiles.size());
     h->interior_cells = interior;
+    // Blocks go to the eight dies round robin by index (block b to die b mod 8), each die
+    // with its own L2: so that a die's tiles are neighbours, block r takes tile
+    // (r mod 8) * per + r / 8 of the list (x fastest, then y, then z): die d steps the d-th
+    // slab of planes, and only the slabs' boundary planes are read by two dies.
+    if (h->tile_order == 0 && !tiles.empty()) {
+        const size_t per = (tiles.size() + kXcds - 1) / kXcds;
+        std::vector<uint32_t> ordered(per * kXcds, kNoTile);
+        for (size_t r = 0; r < ordered.size(); ++r) {
+            const size_t idx = (r % kXcds) * per + r / kXcds;
+            if (idx < tiles.size()) ordered[r] = tiles[idx];
+        }
+        tiles.swap(ordered);
+    }
+    h->tile_count = static_cast<uint32_t>(tiles.size());
 
     // Specials: sorted by their counts of links and states (a special's update reads only
     // its own cell, links and states, so their order is free), in blocks of kBlock; each
@@ -718,7 +830,9 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.centre = static_cast<T>(static_cast<float>(h->centre));
     s.total = static_cast<T>(static_cast<float>(h->total));
     s.specials = h->specials;
+    // Whole multiples of the dies, so the interior's blocks start on die 0 (see the tiles).
     s.special_blocks = (h->specials + kBlock - 1) / kBlock;
+    s.special_blocks = (s.special_blocks + kXcds - 1) / kXcds * kXcds;
     s.sp_cell = h->sp_cell;
     s.sp_links = h->sp_links;
     s.sp_states = h->sp_states;
@@ -776,7 +890,7 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, unsigned long long 
             }
         }
         if (s.layer_blocks) hipLaunchKernelGGL((layer_kernel<T>), dim3(s.layer_blocks), dim3(kBlock), 0, stream, s);
-        const uint32_t probe_blocks = (h->probes + kBlock - 1) / kBlock;
+        const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
         if (probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
         return;
     }
@@ -794,7 +908,7 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, unsigned long long 
 
 template <typename T>
 uint32_t block_count(const as_fdtd *h, const StepArgs<T> &s) {
-    const uint32_t probe_blocks = (h->probes + kBlock - 1) / kBlock;
+    const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
     return s.tile_blocks + s.special_blocks + s.layer_blocks + probe_blocks;
 }
 

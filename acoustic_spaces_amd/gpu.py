@@ -26,11 +26,12 @@ class HipWaveSolver:
     each kind of cell on its own instead of one launch per step."""
 
     def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
-                 graph_steps: int = 0, split: bool = False) -> None:
+                 graph_steps: int = 0, split: bool = False, tile_order: int = 0) -> None:
         lib = native.load()
         layout = solver if isinstance(solver, dict) else solver.layout()
         struct, keep = native.pack_layout(layout)
-        tuning = native.Tuning(0, int(tile_z), int(row_align), int(graph_steps), 1 if split else 0)
+        tuning = native.Tuning(0, int(tile_z), int(row_align), int(graph_steps), 1 if split else 0,
+                               int(tile_order))
         h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
         if not h:
             message = lib.as_fdtd_create_error().decode()

@@ -42,7 +42,8 @@ class Tuning(ctypes.Structure):
     """as_tuning (include/asfdtd.h)."""
 
     _fields_ = [("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int), ("row_align", ctypes.c_int),
-                ("graph_steps", ctypes.c_int), ("split", ctypes.c_int)]
+                ("graph_steps", ctypes.c_int), ("split", ctypes.c_int),
+                ("tile_order", ctypes.c_int)]
 
 
 _lib = None

@@ -76,6 +76,35 @@ def cpu_baseline(scene, target_s: float = 12.0) -> dict:
     }
 
 
+def calibrate() -> int:
+    """hipMemcpy device to device (the runtime's copy kernel) of 1 GiB, 96 MiB and 32 MiB,
+    20 times each after a warm-up: GB/s counted as bytes read plus bytes written."""
+    import ctypes
+
+    from acoustic_spaces_amd import hip
+
+    rt = hip.runtime()
+    rows = []
+    for mib in (1024, 96, 32):
+        n = mib << 20
+        a = hip.DeviceArray(np.zeros(n // 4, np.float32))
+        b = hip.DeviceArray(np.zeros(n // 4, np.float32))
+        rt.hipMemcpy(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), n, 3)
+        hip.synchronize()
+        start, stop = hip.Event(), hip.Event()
+        start.record(0)
+        for _ in range(20):
+            rt.hipMemcpy(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), n, 3)
+        stop.record(0)
+        ms = start.elapsed_ms(stop) / 20
+        rows.append({"bytes_copied": n, "us_per_copy": round(ms * 1e3, 1),
+                     "GBps_read_plus_write": round(2 * n / (ms * 1e-3) / 1e9, 1)})
+        a.free()
+        b.free()
+    print(json.dumps({"calibration": "hipMemcpy device to device", "runs": rows}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,7 +113,12 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="time the runtime's own device-to-device copy kernel instead (the bandwidth "
+                         "reference and the rocprofv3 counter scale)")
     args = ap.parse_args(argv)
+    if args.calibrate:
+        return calibrate()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the solver is built and measured at N=1 only "
               f"(--gpus {args.gpus} asked for); a multi-GPU run is not part of this build.",

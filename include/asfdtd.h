@@ -85,6 +85,7 @@ typedef struct as_tuning {
     int row_align;     /* row pitch in floats: a multiple of this (1, 16, 32, 64) */
     int graph_steps;   /* steps a captured hipGraph replays (0 = 64; negative: no graphs) */
     int split;         /* 1: one launch per kind of cell (specials, interior, layer, probes) */
+    int tile_order;    /* 0: interior tiles grouped by die (XCD), 1: in grid order */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the
