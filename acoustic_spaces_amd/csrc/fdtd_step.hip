@@ -32,6 +32,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -47,6 +48,9 @@ constexpr uint32_t kProbesPerBlock = kBlock / kTileX;  // one wavefront a probe
 constexpr uint32_t kXcds = 8;  // MI355X's dies: blocks go to them round robin by index
 constexpr uint32_t kAllInterior = 1u << 31;  // a tile's flag: every cell of it is interior
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;    // a block with no tile
+// Cell classes on the device: the layout's 0 solid, 1 interior, 2 special (here: a general
+// special, stepped by its own blocks), 3 layer; and 4, a face special (stepped in its tile).
+constexpr uint8_t kClassInterior = 1, kClassFace = 4;
 
 constexpr int kSteps[18][3] = {
     {-1, 0, 0},  {1, 0, 0},  {0, -1, 0}, {0, 1, 0},  {0, 0, -1}, {0, 0, 1},
@@ -89,6 +93,13 @@ struct StepArgs {
     const float *__restrict__ state_before;
     const float *__restrict__ state_gain;
     const float *__restrict__ signal;  // device, indexed by step
+    // face specials, stepped in the tile pass (see face_special)
+    const uint32_t *__restrict__ face_tile;    // per tile block: its first record (and one past the last)
+    const uint4 *__restrict__ face_record;     // cell, links | states << 6 | coefficients << 10,
+                                               // where its states start, its first four states' kinds
+    const uint16_t *__restrict__ face_state_kind;  // per state: its row of face_branch
+    const float4 *__restrict__ face_coef;      // coef_a, coef_b, coef_inv, source
+    const float4 *__restrict__ face_branch;    // now, before, gain, weight
     // matched layer
     uint32_t layer_cells, layer_blocks;
     const uint32_t *__restrict__ layer_cell;
@@ -106,6 +117,84 @@ struct StepArgs {
     unsigned long long out_stride;
     unsigned long long step;  // index into signal and out
 };
+
+// A face special (see the tiles' notes): a special cell whose links are axial and weigh
+// as the interior's. Its record gives its cell, its links (a mask over -x, +x, -y, +y, -z,
+// +z), its coefficients (an index into a small table of coef_a, coef_b, coef_inv and
+// source) and its branch states (their count, and where they start in the states' arrays;
+// each state an index into a small table of now, before, gain and weight). The arithmetic
+// is special_cell's, in its order. A tile's face specials are stepped after its column
+// pass, by all its lanes at once, while the tile's pressures are still in the caches.
+template <typename T>
+__device__ __forceinline__ void face_special(const StepArgs<T> &s, uint32_t r) {
+    const uint4 rec = s.face_record[r];  // cell, links | states << 6 | coefficients << 10, states' start, kinds
+    const int i = static_cast<int>(rec.x);
+    const uint32_t desc = rec.y;
+    const uint32_t sbase = rec.z;
+    const uint32_t links = desc & 63u, ns = (desc >> 6) & 15u, ci_index = desc >> 10;
+    const float sig = s.signal[s.step];
+    const int sy = s.px, sz = static_cast<int>(s.sxy);
+    const T *__restrict__ b = s.b;
+    const float4 c = s.face_coef[ci_index];  // coef_a, coef_b, coef_inv, source
+    const T here = b[i], previous = s.a[i];
+    const T xm = b[i - 1], xp = b[i + 1], ym = b[i - sy], yp = b[i + sy], below = b[i - sz], above = b[i + sz];
+    const T w = s.axial;
+    T sum = 0;
+    if (links & 1u) sum += w * xm;
+    if (links & 2u) sum += w * xp;
+    if (links & 4u) sum += w * ym;
+    if (links & 8u) sum += w * yp;
+    if (links & 16u) sum += w * below;
+    if (links & 32u) sum += w * above;
+    constexpr int kChunk = 4;
+    float4 kind[kChunk];
+    T wa[kChunk], wb[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        if (static_cast<uint32_t>(u) < ns) {
+            // The first four states' kinds ride in the record (255: look it up).
+            uint32_t kid = (rec.w >> (8 * u)) & 0xFFu;
+            if (kid == 0xFFu) kid = s.face_state_kind[sbase + u];
+            kind[u] = s.face_branch[kid];  // now, before, gain, weight
+            wa[u] = s.wa[sbase + u];
+            wb[u] = s.wb[sbase + u];
+        }
+    }
+    T own[kChunk];
+    T flow = 0;
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        if (static_cast<uint32_t>(u) < ns) {
+            own[u] = static_cast<T>(kind[u].x) * wb[u] - static_cast<T>(kind[u].y) * wa[u];
+            flow += static_cast<T>(kind[u].w) * own[u];
+        }
+    }
+    for (uint32_t j = kChunk; j < ns; ++j) {
+        const uint32_t q = sbase + j;
+        const float4 k = s.face_branch[s.face_state_kind[q]];
+        const T term = static_cast<T>(k.x) * s.wb[q] - static_cast<T>(k.y) * s.wa[q];
+        flow += static_cast<T>(k.w) * term;
+    }
+    T acc = static_cast<T>(c.x) * here;
+    acc = acc + sum;
+    acc = acc - static_cast<T>(c.y) * previous;
+    acc = acc - flow;
+    acc = acc + static_cast<T>(c.w) * static_cast<T>(sig);
+    const T next = acc * static_cast<T>(c.z);
+    s.a[i] = next;
+    const T rise = next - previous;
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        if (static_cast<uint32_t>(u) < ns) s.wa[sbase + u] = wa[u] + (rise * static_cast<T>(kind[u].z) + own[u]);
+    }
+    for (uint32_t j = kChunk; j < ns; ++j) {
+        // The term again: wa has not changed since it was taken.
+        const uint32_t q = sbase + j;
+        const float4 k = s.face_branch[s.face_state_kind[q]];
+        const T term = static_cast<T>(k.x) * s.wb[q] - static_cast<T>(k.y) * s.wa[q];
+        s.wa[q] = s.wa[q] + (rise * static_cast<T>(k.z) + term);
+    }
+}
 
 // The 7-point column of one lane: planes z0 to z1 - 1 of cell (x, y), in chunks of kUnroll
 // planes. Every load of a chunk (the planes above, the four in-plane neighbours, the
@@ -141,7 +230,7 @@ __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            if (u < count && (ALL || cls[u] == 1)) {
+            if (u < count && (ALL || cls[u] == kClassInterior)) {
                 // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
                 T sum = xm[u] + xp[u];
                 sum = sum + ym[u];
@@ -179,7 +268,7 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
     const uint32_t tz = rest / s.tiles_y;
     const int x = static_cast<int>(tx) * kTileX + static_cast<int>(threadIdx.x % kTileX);
     const int y = 1 + static_cast<int>(ty) * kTileRows + static_cast<int>(threadIdx.x / kTileX);
-    if (x >= s.nx || y >= s.ny - 1) return;
+    const bool inside = x < s.nx && y < s.ny - 1;
     const int z0 = 1 + static_cast<int>(tz) * s.tile_z;
     const int z1 = min(z0 + s.tile_z, s.nz - 1);
     // 32-bit indices: the padded grid holds fewer than 2^31 cells (checked at set-up).
@@ -187,18 +276,26 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
     int i = x + y * sy + z0 * sz;
     const T *__restrict__ b = s.b;
     if constexpr (!WIDE) {
-        if (all) {
-            interior_column<T, SCALED, true>(s, i, z0, z1);
-        } else {
-            interior_column<T, SCALED, false>(s, i, z0, z1);
+        if (inside) {
+            if (all) {
+                interior_column<T, SCALED, true>(s, i, z0, z1);
+            } else {
+                interior_column<T, SCALED, false>(s, i, z0, z1);
+            }
+        }
+        if (!all) {
+            // The tile's face specials, by all the block's lanes.
+            const uint32_t r1 = s.face_tile[block + 1];
+            for (uint32_t r = s.face_tile[block] + threadIdx.x; r < r1; r += kBlock) face_special<T>(s, r);
         }
         return;
     }
+    if (!inside) return;
     T below = b[i - sz];
     T here = b[i];
     for (int z = z0; z < z1; ++z, i += sz) {
         const T above = b[i + sz];
-        if (s.types[i] == 1) {
+        if (s.types[i] == kClassInterior) {
             // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
             T sum = b[i - 1] + b[i + 1];
             sum = sum + b[i - sy];
@@ -480,7 +577,7 @@ struct DeviceBuffer {
 struct as_fdtd {
     int precision = 32;
     int wide = 0, scaled = 0;
-    int nx = 0, ny = 0, nz = 0, px = 0, tile_z = 16;
+    int nx = 0, ny = 0, nz = 0, px = 0, tile_z = 8;
     long long sxy = 0;
     size_t n_pad = 0;
     uint32_t tiles_x = 0, tiles_y = 0;
@@ -500,7 +597,12 @@ struct as_fdtd {
     int32_t *link_offset = nullptr;
     float *link_weight = nullptr, *coef_a = nullptr, *coef_b = nullptr, *coef_inv = nullptr, *source = nullptr;
     float *state_weight = nullptr, *state_now = nullptr, *state_before = nullptr, *state_gain = nullptr;
-    size_t state_slots = 0;   // the state slabs' length (states and padding)
+    size_t state_slots = 0;   // the states' arrays' length (general slabs, then face states)
+    uint32_t face_specials = 0;
+    uint32_t *face_tile = nullptr;
+    uint4 *face_record = nullptr;
+    uint16_t *face_state_kind = nullptr;
+    float4 *face_coef = nullptr, *face_branch = nullptr;
     int split = 0;            // 1: four launches a step instead of one
     int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order
     uint32_t *layer_cell = nullptr;
@@ -567,7 +669,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->nz = static_cast<int>(L.nz);
     int align = tuning && tuning->row_align > 0 ? tuning->row_align : 64;
     if (align != 1 && align != 16 && align != 32 && align != 64) refuse("the row alignment is 1, 16, 32 or 64 cells");
-    h->tile_z = tuning && tuning->tile_z > 0 ? tuning->tile_z : 16;
+    h->tile_z = tuning && tuning->tile_z > 0 ? tuning->tile_z : 8;
     if (tuning && tuning->tile_y > 0 && tuning->tile_y != kTileRows) refuse("tile_y is 4 (one wavefront a row)");
     if (tuning && tuning->graph_steps != 0) h->graph_steps = tuning->graph_steps < 0 ? 0u : static_cast<uint32_t>(tuning->graph_steps);
     if (h->graph_steps > 4096) refuse("at most 4096 steps a graph");
@@ -591,42 +693,107 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->centre = L.centre;
     h->total = L.total;
 
-    // Cell classes, padded, and the tiles that hold interior cells.
+    // Cell classes, padded.
     std::vector<uint8_t> types(n_pad, 0);
     std::vector<float> scale;
     if (h->scaled) scale.assign(n_pad, 1.0f);
+    uint64_t marked_special = 0;
     for (uint64_t i = 0; i < n; ++i) {
         const uint64_t p = pad(i);
+        if (L.types[i] > 3) refuse("a cell's type is not 0, 1, 2 or 3");
         types[p] = L.types[i];
+        marked_special += L.types[i] == 2;
         if (h->scaled) scale[p] = L.scale[i];
     }
+
+    // Specials: checked, then each either a face special (its links axial, each weighing
+    // the interior's axial weight, in the order of the directions; no speed per cell; at
+    // most 15 states), stepped in its tile, or a general one, stepped by its own blocks.
+    const uint64_t K = L.specials;
+    if (K > 0 && (L.special == nullptr || L.link_start == nullptr || L.state_start == nullptr)) {
+        refuse("the layout's special cells are incomplete");
+    }
+    if (K >= (1ull << 31) || L.links >= (1ull << 32) || L.states >= (1ull << 32)) {
+        throw Failure{AS_ERANGE, "too many special cells, links or states for 32-bit indices"};
+    }
+    if (K > 0 && (L.link_start[0] != 0 || L.state_start[0] != 0 || L.link_start[K] != L.links ||
+                  L.state_start[K] != L.states)) {
+        refuse("the layout's link or state ranges do not match their arrays");
+    }
+    if (marked_special != K) refuse("the layout's special cells and its cells of type 2 differ");
+    const auto links_of = [&](uint64_t k) { return L.link_start[k + 1] - L.link_start[k]; };
+    const auto states_of = [&](uint64_t k) { return L.state_start[k + 1] - L.state_start[k]; };
+    std::vector<uint8_t> face(K, 0);
+    std::vector<uint8_t> seen(K ? n_pad : 0, 0);
+    const float axial = L.weights[0];
+    for (uint64_t k = 0; k < K; ++k) {
+        const int64_t i = L.special[k];
+        if (i < 0 || static_cast<uint64_t>(i) >= n) refuse("a special cell lies outside the grid");
+        const uint64_t x = static_cast<uint64_t>(i) % nx, y = (static_cast<uint64_t>(i) / nx) % ny,
+                       z = static_cast<uint64_t>(i) / (nx * ny);
+        if (x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz) {
+            refuse("a special cell lies on the grid's edge");
+        }
+        if (L.types[i] != 2) refuse("a special cell is not of type 2");
+        const uint64_t p = pad(static_cast<uint64_t>(i));
+        if (seen[p]) refuse("a cell is special twice");
+        seen[p] = 1;
+        if (L.link_start[k + 1] < L.link_start[k] || L.state_start[k + 1] < L.state_start[k]) {
+            refuse("the layout's link or state ranges run backwards");
+        }
+        if (links_of(k) > 18) refuse("a special cell has more than 18 links");
+        if (states_of(k) > 255) refuse("a special cell has more than 255 branch states");
+        for (uint64_t l = L.link_start[k]; l < L.link_start[k + 1]; ++l) {
+            if (L.link_direction[l] >= 18) refuse("a link has no direction");
+        }
+        for (uint64_t j = L.state_start[k]; j < L.state_start[k + 1]; ++j) {
+            if (L.state_branch[j] < 0 || static_cast<uint64_t>(L.state_branch[j]) >= L.branches) {
+                refuse("a branch state has no branch");
+            }
+        }
+        bool ok = !h->wide && !h->scaled && states_of(k) <= 15;
+        int last = -1;
+        for (uint64_t l = L.link_start[k]; ok && l < L.link_start[k + 1]; ++l) {
+            const int d = L.link_direction[l];
+            ok = d < 6 && d > last && L.link_weight[l] == axial;
+            last = d;
+        }
+        face[k] = ok ? 1 : 0;
+        if (ok) types[p] = kClassFace;
+    }
+    (void)seen;
+
+    // The tiles: those holding interior cells or face specials.
     h->tiles_x = static_cast<uint32_t>((nx + kTileX - 1) / kTileX);
     h->tiles_y = static_cast<uint32_t>((ny - 2 + kTileRows - 1) / kTileRows);
     const uint32_t tiles_z = static_cast<uint32_t>((nz - 2 + h->tile_z - 1) / h->tile_z);
     if (static_cast<uint64_t>(h->tiles_x) * h->tiles_y * tiles_z >= kAllInterior) refuse("too many tiles");
+    const auto tile_box = [&](uint32_t id, uint64_t *lo, uint64_t *hi) {
+        const uint32_t tx = id % h->tiles_x, ty = (id / h->tiles_x) % h->tiles_y, tz = id / (h->tiles_x * h->tiles_y);
+        lo[2] = 1 + static_cast<uint64_t>(tz) * h->tile_z;
+        hi[2] = std::min<uint64_t>(lo[2] + h->tile_z, nz - 1);
+        lo[1] = 1 + static_cast<uint64_t>(ty) * kTileRows;
+        hi[1] = std::min<uint64_t>(lo[1] + kTileRows, ny - 1);
+        lo[0] = static_cast<uint64_t>(tx) * kTileX;
+        hi[0] = std::min<uint64_t>(lo[0] + kTileX, nx);
+    };
     std::vector<uint32_t> tiles;
     uint64_t interior = 0;
-    for (uint32_t tz = 0; tz < tiles_z; ++tz) {
-        for (uint32_t ty = 0; ty < h->tiles_y; ++ty) {
-            for (uint32_t tx = 0; tx < h->tiles_x; ++tx) {
-                uint64_t count = 0;
-                const uint64_t z0 = 1 + static_cast<uint64_t>(tz) * h->tile_z;
-                const uint64_t z1 = std::min<uint64_t>(z0 + h->tile_z, nz - 1);
-                const uint64_t y0 = 1 + static_cast<uint64_t>(ty) * kTileRows;
-                const uint64_t y1 = std::min<uint64_t>(y0 + kTileRows, ny - 1);
-                const uint64_t x0 = static_cast<uint64_t>(tx) * kTileX;
-                const uint64_t x1 = std::min<uint64_t>(x0 + kTileX, nx);
-                for (uint64_t z = z0; z < z1; ++z)
-                    for (uint64_t y = y0; y < y1; ++y)
-                        for (uint64_t x = x0; x < x1; ++x) count += types[x + px * (y + ny * z)] == 1;
-                if (count) {
-                    uint32_t id = tx + h->tiles_x * (ty + h->tiles_y * tz);
-                    if (count == (z1 - z0) * (y1 - y0) * (x1 - x0)) id |= kAllInterior;
-                    tiles.push_back(id);
+    for (uint32_t id = 0; id < h->tiles_x * h->tiles_y * tiles_z; ++id) {
+        uint64_t lo[3], hi[3], inner = 0, faces = 0;
+        tile_box(id, lo, hi);
+        for (uint64_t z = lo[2]; z < hi[2]; ++z)
+            for (uint64_t y = lo[1]; y < hi[1]; ++y)
+                for (uint64_t x = lo[0]; x < hi[0]; ++x) {
+                    const uint8_t c = types[x + px * (y + ny * z)];
+                    inner += c == kClassInterior;
+                    faces += c == kClassFace;
                 }
-                interior += count;
-            }
+        if (inner + faces) {
+            const bool all = inner == (hi[2] - lo[2]) * (hi[1] - lo[1]) * (hi[0] - lo[0]);
+            tiles.push_back(id | (all ? kAllInterior : 0u));
         }
+        interior += inner;
     }
     h->interior_cells = interior;
     // Blocks go to the eight dies round robin by index (block b to die b mod 8), each die
@@ -644,55 +811,30 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     }
     h->tile_count = static_cast<uint32_t>(tiles.size());
 
-    // Specials: sorted by their counts of links and states (a special's update reads only
-    // its own cell, links and states, so their order is free), in blocks of kBlock; each
-    // block's links and states in slabs of (its largest count) x kBlock, slot-major. The
-    // links are offsets on the padded grid; each state carries its branch's coefficients.
-    const uint64_t K = L.specials;
-    if (K > 0 && (L.special == nullptr || L.link_start == nullptr || L.state_start == nullptr)) {
-        refuse("the layout's special cells are incomplete");
-    }
-    if (K >= (1ull << 31) || L.links >= (1ull << 32) || L.states >= (1ull << 32)) {
-        throw Failure{AS_ERANGE, "too many special cells, links or states for 32-bit indices"};
-    }
-    if (K > 0 && (L.link_start[0] != 0 || L.state_start[0] != 0 || L.link_start[K] != L.links ||
-                  L.state_start[K] != L.states)) {
-        refuse("the layout's link or state ranges do not match their arrays");
-    }
+    // General specials: sorted by their counts of links and states (a special's update reads
+    // only its own cell, links and states, so their order is free), in blocks of kBlock;
+    // each block's links and states in slabs of (its largest count) x kBlock, slot-major.
+    // The links are offsets on the padded grid; each state carries its branch's coefficients.
     std::vector<int32_t> offset(18);
     for (int k = 0; k < 18; ++k) {
         offset[static_cast<size_t>(k)] = static_cast<int32_t>(kSteps[k][0] + kSteps[k][1] * static_cast<long long>(px) +
                                                               kSteps[k][2] * static_cast<long long>(px * ny));
     }
-    std::vector<uint64_t> order(K);
-    for (uint64_t k = 0; k < K; ++k) {
-        order[k] = k;
-        const int64_t i = L.special[k];
-        if (i < 0 || static_cast<uint64_t>(i) >= n) refuse("a special cell lies outside the grid");
-        const uint64_t x = static_cast<uint64_t>(i) % nx, y = (static_cast<uint64_t>(i) / nx) % ny,
-                       z = static_cast<uint64_t>(i) / (nx * ny);
-        if (x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz) {
-            refuse("a special cell lies on the grid's edge");
-        }
-        if (L.link_start[k + 1] < L.link_start[k] || L.state_start[k + 1] < L.state_start[k]) {
-            refuse("the layout's link or state ranges run backwards");
-        }
-        if (L.link_start[k + 1] - L.link_start[k] > 18) refuse("a special cell has more than 18 links");
-        if (L.state_start[k + 1] - L.state_start[k] > 255) refuse("a special cell has more than 255 branch states");
-    }
-    const auto links_of = [&](uint64_t k) { return L.link_start[k + 1] - L.link_start[k]; };
-    const auto states_of = [&](uint64_t k) { return L.state_start[k + 1] - L.state_start[k]; };
+    std::vector<uint64_t> order;
+    for (uint64_t k = 0; k < K; ++k)
+        if (!face[k]) order.push_back(k);
+    const uint64_t G = order.size();
     std::stable_sort(order.begin(), order.end(), [&](uint64_t p, uint64_t q) {
         return links_of(p) != links_of(q) ? links_of(p) < links_of(q) : states_of(p) < states_of(q);
     });
-    const uint64_t blocks = (K + kBlock - 1) / kBlock;
-    std::vector<uint32_t> sp_cell(K), link_base(blocks), state_base(blocks);
-    std::vector<uint8_t> sp_links(K), sp_states(K);
-    std::vector<float> coef_a(K), coef_b(K), coef_inv(K), source(K);
+    const uint64_t blocks = (G + kBlock - 1) / kBlock;
+    std::vector<uint32_t> sp_cell(G), link_base(blocks), state_base(blocks);
+    std::vector<uint8_t> sp_links(G), sp_states(G);
+    std::vector<float> coef_a(G), coef_b(G), coef_inv(G), source(G);
     uint64_t link_slots = 0, state_slots = 0;
     for (uint64_t B = 0; B < blocks; ++B) {
         uint64_t wl = 0, ws = 0;
-        for (uint64_t t = 0; t < kBlock && B * kBlock + t < K; ++t) {
+        for (uint64_t t = 0; t < kBlock && B * kBlock + t < G; ++t) {
             wl = std::max(wl, links_of(order[B * kBlock + t]));
             ws = std::max(ws, states_of(order[B * kBlock + t]));
         }
@@ -700,7 +842,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         state_base[B] = static_cast<uint32_t>(state_slots);
         link_slots += wl * kBlock;
         state_slots += ws * kBlock;
-        if (link_slots >= (1ull << 32) || state_slots >= (1ull << 32)) {
+        if (link_slots >= (1ull << 32) || state_slots >= (1ull << 31)) {
             throw Failure{AS_ERANGE, "too many links or states for 32-bit indices"};
         }
     }
@@ -708,7 +850,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     std::vector<float> link_weight(link_slots, 0.0f);
     std::vector<float> state_weight(state_slots, 0.0f), now(state_slots, 0.0f), before(state_slots, 0.0f),
         gain(state_slots, 0.0f);
-    for (uint64_t k = 0; k < K; ++k) {
+    for (uint64_t k = 0; k < G; ++k) {
         const uint64_t src = order[k], B = k / kBlock, t = k % kBlock;
         sp_cell[k] = static_cast<uint32_t>(pad(static_cast<uint64_t>(L.special[src])));
         sp_links[k] = static_cast<uint8_t>(links_of(src));
@@ -719,16 +861,13 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         source[k] = L.source[src];
         for (uint64_t u = 0; u < links_of(src); ++u) {
             const uint64_t l = L.link_start[src] + u;
-            const int d = L.link_direction[l];
-            if (d < 0 || d >= 18) refuse("a link has no direction");
             const uint64_t slot = link_base[B] + u * kBlock + t;
-            link_offset[slot] = offset[static_cast<size_t>(d)];
+            link_offset[slot] = offset[L.link_direction[l]];
             link_weight[slot] = L.link_weight[l];
         }
         for (uint64_t u = 0; u < states_of(src); ++u) {
             const uint64_t j = L.state_start[src] + u;
             const int32_t br = L.state_branch[j];
-            if (br < 0 || static_cast<uint64_t>(br) >= L.branches) refuse("a branch state has no branch");
             const uint64_t slot = state_base[B] + u * kBlock + t;
             state_weight[slot] = L.state_weight[j];
             now[slot] = L.branch_now[br];
@@ -736,9 +875,79 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             gain[slot] = L.branch_gain[br];
         }
     }
-    h->specials = static_cast<uint32_t>(K);
+    h->specials = static_cast<uint32_t>(G);
+    h->face_specials = static_cast<uint32_t>(K - G);
     h->states = static_cast<uint32_t>(L.states);
-    h->state_slots = state_slots;
+
+    // Face specials' records, tile block by tile block (each tile's in the order of its
+    // cells). Their states follow the general specials' slabs in the states' arrays.
+    std::vector<std::pair<uint64_t, uint64_t>> by_cell;  // (padded cell, special)
+    for (uint64_t k = 0; k < K; ++k)
+        if (face[k]) by_cell.emplace_back(pad(static_cast<uint64_t>(L.special[k])), k);
+    std::sort(by_cell.begin(), by_cell.end());
+    const auto bits = [](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    std::map<std::array<uint32_t, 4>, uint32_t> coef_index, branch_index;
+    std::vector<float> face_coef, face_branch;
+    const auto index_of = [&](std::map<std::array<uint32_t, 4>, uint32_t> &table, std::vector<float> &rows,
+                              const float (&v)[4]) {
+        const std::array<uint32_t, 4> key{bits(v[0]), bits(v[1]), bits(v[2]), bits(v[3])};
+        const auto it = table.find(key);
+        if (it != table.end()) return it->second;
+        const uint32_t id = static_cast<uint32_t>(table.size());
+        table.emplace(key, id);
+        rows.insert(rows.end(), v, v + 4);
+        return id;
+    };
+    std::vector<uint32_t> face_tile(static_cast<size_t>(h->tile_count) + 1, 0), face_cell, face_desc, face_state, face_kinds;
+    std::vector<uint16_t> face_state_kind;
+    uint64_t face_slots = state_slots;
+    for (uint32_t blk = 0; blk < h->tile_count; ++blk) {
+        face_tile[blk] = static_cast<uint32_t>(face_desc.size());
+        const uint32_t entry = tiles[blk];
+        if (entry == kNoTile || (entry & kAllInterior)) continue;
+        uint64_t lo[3], hi[3];
+        tile_box(entry & ~kAllInterior, lo, hi);
+        for (uint64_t z = lo[2]; z < hi[2]; ++z)
+            for (uint64_t y = lo[1]; y < hi[1]; ++y)
+                for (uint64_t x = lo[0]; x < hi[0]; ++x) {
+                    const uint64_t p = x + px * (y + ny * z);
+                    if (types[p] != kClassFace) continue;
+                    const auto it = std::lower_bound(by_cell.begin(), by_cell.end(), std::make_pair(p, uint64_t{0}));
+                    const uint64_t k = it->second;
+                    uint32_t mask = 0;
+                    for (uint64_t l = L.link_start[k]; l < L.link_start[k + 1]; ++l) mask |= 1u << L.link_direction[l];
+                    const float cv[4] = {L.coef_a[k], L.coef_b[k], L.coef_inv[k], L.source[k]};
+                    const uint32_t ci = index_of(coef_index, face_coef, cv);
+                    if (ci >= (1u << 22)) throw Failure{AS_ERANGE, "too many distinct special coefficients"};
+                    face_cell.push_back(static_cast<uint32_t>(p));
+                    face_desc.push_back(mask | static_cast<uint32_t>(states_of(k)) << 6 | ci << 10);
+                    face_state.push_back(static_cast<uint32_t>(face_slots));
+                    uint32_t packed = 0;
+                    for (uint64_t j = L.state_start[k]; j < L.state_start[k + 1]; ++j) {
+                        const int32_t br = L.state_branch[j];
+                        const float bv[4] = {L.branch_now[br], L.branch_before[br], L.branch_gain[br], L.state_weight[j]};
+                        const uint32_t bi = index_of(branch_index, face_branch, bv);
+                        if (bi > 0xFFFFu) throw Failure{AS_ERANGE, "too many distinct branch states"};
+                        const uint64_t u = j - L.state_start[k];
+                        if (u < 4) packed |= std::min<uint32_t>(bi, 0xFFu) << (8 * u);
+                        face_state_kind.push_back(static_cast<uint16_t>(bi));
+                        ++face_slots;
+                    }
+                    for (uint64_t u = states_of(k); u < 4; ++u) packed |= 0xFFu << (8 * u);
+                    face_kinds.push_back(packed);
+                    if (face_slots >= (1ull << 31)) throw Failure{AS_ERANGE, "too many states"};
+                }
+    }
+    face_tile[h->tile_count] = static_cast<uint32_t>(face_desc.size());
+    if (face_desc.size() != K - G) throw Failure{AS_EINVAL, "internal: a face special was not met by a tile"};
+    // The face states' kinds are indexed from the start of the states' arrays.
+    std::vector<uint16_t> kinds(face_slots, 0);
+    std::copy(face_state_kind.begin(), face_state_kind.end(), kinds.begin() + static_cast<ptrdiff_t>(state_slots));
+    h->state_slots = face_slots;
 
     // Matched layer.
     std::vector<uint32_t> layer_cell(L.layer_cells);
@@ -787,6 +996,20 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->state_now = upload(h, now);
     h->state_before = upload(h, before);
     h->state_gain = upload(h, gain);
+    h->face_tile = upload(h, face_tile);
+    {
+        std::vector<uint32_t> records(4 * face_cell.size());
+        for (size_t r = 0; r < face_cell.size(); ++r) {
+            records[4 * r] = face_cell[r];
+            records[4 * r + 1] = face_desc[r];
+            records[4 * r + 2] = face_state[r];
+            records[4 * r + 3] = face_kinds[r];
+        }
+        h->face_record = reinterpret_cast<uint4 *>(upload(h, records));
+    }
+    h->face_state_kind = upload(h, kinds);
+    h->face_coef = reinterpret_cast<float4 *>(upload(h, face_coef));
+    h->face_branch = reinterpret_cast<float4 *>(upload(h, face_branch));
     h->layer_cell = upload(h, layer_cell);
     h->layer_at = upload(h, layer_at);
     if (L.layer_scale != nullptr && L.layer_cells > 0) {
@@ -805,8 +1028,8 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->probe_weight = upload(h, probe_weight);
     h->u0 = zeros(h, n_pad * e);
     h->u1 = zeros(h, n_pad * e);
-    h->w0 = zeros(h, state_slots * e);
-    h->w1 = zeros(h, state_slots * e);
+    h->w0 = zeros(h, face_slots * e);
+    h->w1 = zeros(h, face_slots * e);
     h->layer_state = zeros(h, 9 * L.layer_cells * e);
 }
 
@@ -848,6 +1071,11 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.state_now = h->state_now;
     s.state_before = h->state_before;
     s.state_gain = h->state_gain;
+    s.face_tile = h->face_tile;
+    s.face_record = h->face_record;
+    s.face_state_kind = h->face_state_kind;
+    s.face_coef = h->face_coef;
+    s.face_branch = h->face_branch;
     s.layer_cells = h->layer_cells;
     s.layer_blocks = (h->layer_cells + kBlock - 1) / kBlock;
     s.layer_cell = h->layer_cell;
@@ -1115,7 +1343,7 @@ void *as_fdtd_stream(as_fdtd *h) { return h ? static_cast<void *>(h->stream) : n
 int as_fdtd_info(const as_fdtd *h, uint64_t *out) {
     if (h == nullptr || out == nullptr) return AS_EINVAL;
     out[0] = h->interior_cells;
-    out[1] = h->specials;
+    out[1] = static_cast<uint64_t>(h->specials) + h->face_specials;
     out[2] = h->states;
     out[3] = h->layer_cells;
     out[4] = h->probes;
