@@ -418,3 +418,23 @@ def outdoor(cell=0.25, seed=2, length_s=1.0, receivers=32, layer=16, layer_dampi
         drive=gaussian_drive(rate, steps), layer=layer, layer_damping=layer_damping,
         edge_admittance=1.0,
     )
+
+
+def rooms_batch(count=64, cell=0.05, seed=3, length_s=0.5, receivers=4, size_low=(3.0, 3.0, 2.5),
+                size_high=(12.0, 10.0, 5.0)) -> list:
+    """A batch of shoeboxes for the Monte Carlo study of wall absorption: dimensions
+    U(3, 12) x U(3, 10) x U(2.5, 5) m; each wall's absorption drawn from N(mu, 0.1 mu),
+    mu ~ U(0.05, 0.4), truncated to (0, 1) (drawn again until it falls inside); one source
+    and ``receivers`` receivers per room; one drive for all (one cell size, one rate)."""
+    rng = np.random.default_rng(seed)
+    rooms = []
+    for r in range(count):
+        size = tuple(float(v) for v in rng.uniform(size_low, size_high))
+        mu = rng.uniform(0.05, 0.4, 6)
+        alpha = rng.normal(mu, 0.1 * mu)
+        while np.any((alpha <= 0.0) | (alpha >= 1.0)):
+            bad = (alpha <= 0.0) | (alpha >= 1.0)
+            alpha[bad] = rng.normal(mu[bad], 0.1 * mu[bad])
+        rooms.append(shoebox(size=size, cell=cell, seed=int(rng.integers(1 << 30)), length_s=length_s,
+                             receivers=receivers, absorption=alpha))
+    return rooms

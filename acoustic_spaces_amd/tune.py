@@ -1,0 +1,83 @@
+"""The stepper's autotuner: tile depth and steps per graph, chosen by timing them.
+
+For a layout it times each candidate launch shape (tile_z planes per tile, graph_steps
+steps per captured graph) over a short run on the GPU, with hipEvents on the stepper's
+stream, and keeps the fastest. Results are cached by the layout's signature (its grid,
+cell counts and stencil) in a JSON file, as the reference keeps its own device timings
+(engine/tuning.py:76-180); the reference tunes only the CPU/GPU choice, this tunes the
+kernels' shape.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+
+CANDIDATES = tuple({"tile_z": tz, "graph_steps": gs} for tz in (4, 8, 12, 16) for gs in (32, 64, 128))
+
+
+def cache_path() -> Path:
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return Path(base) / "acoustic_spaces_amd" / "tuning.json"
+
+
+def signature(layout: dict) -> str:
+    shape = "x".join(str(int(v)) for v in layout["shape"])
+    air = int(np.count_nonzero(np.asarray(layout["types"])))
+    return (f"{shape}|air{air}|sp{len(layout['special'])}|st{len(layout['state_branch'])}|"
+            f"ly{len(layout['layer_cells'])}|s{int(layout['stencil'][0])}|sc{int(layout['scaled'][0])}")
+
+
+def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int = 32, **shape) -> float:
+    """Microseconds per step of the layout under one launch shape."""
+    from acoustic_spaces_amd import hip
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    stepper = HipWaveSolver(layout, precision=precision, **shape)
+    try:
+        drive = hip.DeviceArray(np.zeros(warmup + steps, np.float32))
+        out = hip.DeviceArray(np.zeros((max(stepper.probes, 1), warmup + steps), np.float64))
+        stepper.run_device(drive.ptr, warmup, out.ptr)
+        hip.synchronize()
+        start, stop = hip.Event(), hip.Event()
+        start.record(stepper.stream)
+        stepper.run_device(drive.ptr + 4 * warmup, steps, out.ptr + 8 * max(stepper.probes, 1) * warmup)
+        stop.record(stepper.stream)
+        hip.synchronize()
+        return start.elapsed_ms(stop) * 1e3 / steps
+    finally:
+        stepper.close()
+
+
+def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: bool = True,
+             precision: int = 32) -> dict:
+    """The fastest launch shape for the layout: {"tile_z", "graph_steps", "us_per_step",
+    "timings"}; from the cache when it holds this layout's signature."""
+    key = f"{signature(layout)}|p{precision}"
+    path = cache_path()
+    cache = {}
+    if use_cache and path.exists():
+        try:
+            cache = json.loads(path.read_text())
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            return cache[key]
+    timings = []
+    for shape in candidates:
+        us = time_shape(layout, steps=max(steps, shape["graph_steps"]), precision=precision, **shape)
+        timings.append({**shape, "us_per_step": round(us, 3)})
+    best = min(timings, key=lambda t: t["us_per_step"])
+    result = {"tile_z": best["tile_z"], "graph_steps": best["graph_steps"], "us_per_step": best["us_per_step"],
+              "timings": timings}
+    if use_cache:
+        try:
+            path.parent.mkdir(parents=True, exist_ok=True)
+            cache[key] = result
+            path.write_text(json.dumps(cache, indent=1))
+        except OSError:
+            pass
+    return result

@@ -105,11 +105,159 @@ def calibrate() -> int:
     return 0
 
 
+def time_steps(stepper, drive: np.ndarray, warmup: int, steps: int):
+    """Warm-up, then exactly ``steps`` steps between synchronisations, timed with hipEvents
+    on the stepper's own stream; the drive and the probes' output in device memory.
+    Returns (milliseconds, the probes' output of the timed steps)."""
+    from acoustic_spaces_amd import hip
+
+    need = warmup + steps
+    probes = max(stepper.probes, 1)
+    d_drive = hip.DeviceArray(np.ascontiguousarray(drive[:need], np.float32))
+    d_out = hip.DeviceArray(np.zeros((probes, need), np.float64))
+    stream = stepper.stream
+    if warmup:
+        stepper.run_device(d_drive.ptr, warmup, d_out.ptr)
+    hip.synchronize()
+    start, stop = hip.Event(), hip.Event()
+    start.record(stream)
+    stepper.run_device(d_drive.ptr + 4 * warmup, steps, d_out.ptr + 8 * probes * warmup)
+    stop.record(stream)
+    hip.synchronize()
+    ms = start.elapsed_ms(stop)
+    return ms, d_out.to_host()
+
+
+def measured_traffic(name: str):
+    """HBM bytes per step from the rocprofv3 FETCH_SIZE and WRITE_SIZE runs summarised in
+    profiles/<name>.json (FETCH_SIZE scaled by 2, WRITE_SIZE by 1, as a copy of known size
+    reads them), or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return int(t["fetch_kib"] * 1024 * t["fetch_scale"] + t["write_kib"] * 1024 * t["write_scale"])
+
+
+def line_for(metric, value, K, W, ms, workload, config, roofline, dtype="f32"):
+    return {
+        "metric": metric,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms / K, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": workload, **config},
+        "roofline": roofline,
+    }
+
+
+def roofline_of(step_bytes, per_step_s, traffic, kernel):
+    achieved = step_bytes / per_step_s / 1e9
+    return {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "bytes_per_step": step_bytes,
+        "kernel": kernel,
+    }
+
+
+def plain(scene, name, args, traffic_name=None):
+    """One plain solve of ``scene``: set-up, then K timed steps."""
+    from acoustic_spaces_amd import scenes as sc
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    K, W = args.steps, args.warmup
+    drive = scene.drive if W + K <= len(scene.drive) else sc.gaussian_drive(scene.rate, W + K)
+    t = time.perf_counter()
+    solver = WaveSolver.from_scene(scene)
+    layout = solver.layout()
+    setup_s = time.perf_counter() - t
+    stepper = HipWaveSolver(layout, graph_steps=args.graph_steps, tile_z=args.tile_z)
+    info = stepper.info()
+    ms, out = time_steps(stepper, drive, W, K)
+    if not np.all(np.isfinite(out)):
+        raise SystemExit("non-finite probe output")
+    air = solver.air_cells
+    per_step_s = ms / 1e3 / K
+    step_bytes = algorithmic_bytes(info, air, solver.layer_cells, solver.branch_states)
+    config = {
+        "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
+        "special_cells": info["specials"], "grid": list(scene.grid.shape), "setup_s": round(setup_s, 2),
+        "graph_steps": args.graph_steps or 64, "tile_z": args.tile_z or 8,
+        "precision": "float (bit-identical to the reference's CPU scheme)",
+    }
+    traffic = measured_traffic(traffic_name) if traffic_name else None
+    return line_for(f"FDTD Mcell-updates/s ({name}, plain solver)", air / per_step_s / 1e6, K, W, ms, scene.name,
+                    config, roofline_of(step_bytes, per_step_s, traffic,
+                                        "step_kernel<float,false,false> (one fused launch per step)"))
+
+
+def rooms(args):
+    """64 rooms: one batched solve (every room in one launch a step, its launch shape from
+    the autotuner) against 64 plain solves one after another, the same steps each."""
+    from acoustic_spaces_amd import scenes, tune
+    from acoustic_spaces_amd.batch import merge_layouts
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    K, W = args.steps, args.warmup
+    batch = scenes.rooms_batch(count=64)
+    t = time.perf_counter()
+    layouts = [WaveSolver.from_scene(r).layout() for r in batch]
+    setup_s = time.perf_counter() - t
+    drive = scenes.gaussian_drive(batch[0].rate, W + K)
+    merged, placements = merge_layouts(layouts)
+    shape = tune.autotune(merged, steps=128)
+    stepper = HipWaveSolver(merged, tile_z=shape["tile_z"], graph_steps=shape["graph_steps"])
+    info = stepper.info()
+    ms_batch, out = time_steps(stepper, drive, W, K)
+    stepper.close()
+    if not np.all(np.isfinite(out)):
+        raise SystemExit("non-finite probe output")
+    ms_seq = 0.0
+    for L in layouts:
+        s = HipWaveSolver(L)
+        ms, _ = time_steps(s, drive, W, K)
+        ms_seq += ms
+        s.close()
+    air = int(sum(np.count_nonzero(L["types"]) for L in layouts))
+    states = int(sum(len(L["state_branch"]) for L in layouts))
+    per_step_s = ms_batch / 1e3 / K
+    step_bytes = PER_AIR_CELL * air + PER_STATE * states
+    config = {
+        "rooms": len(batch), "air_cells": air, "merged_grid": [int(v) for v in merged["shape"]],
+        "setup_s": round(setup_s, 2), "tuned": {k: shape[k] for k in ("tile_z", "graph_steps", "us_per_step")},
+        "sequential_ms": round(ms_seq, 2), "batched_ms": round(ms_batch, 2),
+        "batched_over_sequential_speedup": round(ms_seq / ms_batch, 3),
+        "sequential_mcell_updates_per_s": round(air * K / (ms_seq / 1e3) / 1e6, 1),
+        "special_cells": info["specials"],
+    }
+    return line_for("FDTD Mcell-updates/s (64 rooms batched; vs 64 sequential plain solves)",
+                    air / per_step_s / 1e6, K, W, ms_batch,
+                    "64 shoeboxes U(3,12)xU(3,10)xU(2.5,5) m at 5 cm, Monte Carlo wall absorption, "
+                    "1 source and 4 receivers each, stacked into one grid", config,
+                    roofline_of(step_bytes, per_step_s, None, "step_kernel<float,false,false> over all rooms"))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=11882)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=256)
+    ap.add_argument("--workload", choices=("hall", "shoebox", "outdoor", "rooms"), default="hall")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
@@ -125,82 +273,30 @@ def main(argv=None) -> int:
               file=sys.stderr)
         return 2
 
-    from acoustic_spaces_amd import hip, scenes
-    from acoustic_spaces_amd.gpu import HipWaveSolver
-    from acoustic_spaces_amd.solver import WaveSolver
+    from acoustic_spaces_amd import scenes
 
-    scene = scenes.hall()[0]
-    K, W = args.steps, args.warmup
-    need = W + K
-    drive = scene.drive
-    if need > len(drive):
-        drive = scenes.gaussian_drive(scene.rate, need)
-    t = time.perf_counter()
-    solver = WaveSolver.from_scene(scene)
-    layout = solver.layout()
-    setup_s = time.perf_counter() - t
-    stepper = HipWaveSolver(layout, graph_steps=args.graph_steps, tile_z=args.tile_z)
-    info = stepper.info()
-    probes = stepper.probes
-    d_drive = hip.DeviceArray(drive[:need].astype(np.float32))
-    d_out = hip.DeviceArray(np.zeros((probes, need), np.float64))
-    stream = stepper.stream
-    # Warm-up, then exactly K timed steps between synchronisations, on the solver's stream.
-    stepper.run_device(d_drive.ptr, W, d_out.ptr)
-    hip.synchronize()
-    start, stop = hip.Event(), hip.Event()
-    start.record(stream)
-    stepper.run_device(d_drive.ptr + 4 * W, K, d_out.ptr + 8 * probes * W)
-    stop.record(stream)
-    hip.synchronize()
-    ms = start.elapsed_ms(stop)
-    out = d_out.to_host()
-    if not np.all(np.isfinite(out)):
-        print("non-finite probe output", file=sys.stderr)
-        return 1
-
-    air = solver.air_cells
-    per_step_s = ms / 1e3 / K
-    value = air / per_step_s / 1e6
-    step_bytes = algorithmic_bytes(info, air, solver.layer_cells, solver.branch_states)
-    achieved = step_bytes / per_step_s / 1e9
-    line = {
-        "metric": "FDTD Mcell-updates/s (hall, plain solver)",
-        "value": round(value, 1),
-        "unit": "Mcell-updates/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": round(ms / K, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "hall 30x20x12 m, ceiling of 6 tilted planes, side walls splayed 5 deg, "
-                        f"10 cm cells {scene.grid.shape[0]}x{scene.grid.shape[1]}x{scene.grid.shape[2]}, "
-                        f"{air:,} air cells, walls on cell faces, 3 relaxation branches per material, "
-                        "1 source, 16 receivers",
-            "air_cells": air,
-            "branch_states": solver.branch_states,
-            "special_cells": info["specials"],
-            "setup_s": round(setup_s, 2),
-            "graph_steps": args.graph_steps or 64,
-            "precision": "float (bit-identical to the reference's CPU scheme)",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_per_step": step_bytes,
-            "kernel": "step_kernel<float,false,false> (one fused launch per step)",
-        },
-    }
-    if not args.no_cpu:
+    if args.workload == "hall":
+        args.steps = args.steps or 11882
+        scene = scenes.hall()[0]
+        line = plain(scene, "hall", args, traffic_name="step_kernel_hall_traffic")
+        line["config"]["workload"] = (
+            "hall 30x20x12 m, ceiling of 6 tilted planes, side walls splayed 5 deg, 10 cm cells "
+            f"{scene.grid.shape[0]}x{scene.grid.shape[1]}x{scene.grid.shape[2]}, "
+            f"{line['config']['air_cells']:,} air cells, walls on cell faces, 3 relaxation branches per "
+            "material, 1 source, 16 receivers")
+    elif args.workload == "shoebox":
+        args.steps = args.steps or 5941
+        scene = scenes.shoebox()
+        line = plain(scene, "shoebox", args)
+    elif args.workload == "outdoor":
+        args.steps = args.steps or 2377
+        scene = scenes.outdoor()
+        line = plain(scene, "outdoor", args)
+    else:
+        args.steps = args.steps or 5941
+        line = rooms(args)
+        scene = None
+    if not args.no_cpu and scene is not None:
         line["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(line), flush=True)
     return 0
