@@ -116,6 +116,7 @@ struct StepArgs {
     double *__restrict__ out;
     unsigned long long out_stride;
     unsigned long long step;  // index into signal and out
+    T *__restrict__ history;  // or null: the specials' p(n+1) - p(n-1), step-major (adjoint)
 };
 
 // A face special (see the tiles' notes): a special cell whose links are axial and weigh
@@ -429,6 +430,8 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
     acc = acc + static_cast<T>(src) * static_cast<T>(sig);
     const T next = acc * static_cast<T>(ci);
     s.a[i] = next;
+    // For the adjoint's gradient: each special's p(n+1) - p(n-1) of this step.
+    if (s.history != nullptr) s.history[s.step * s.specials + k] = next - previous;
     if (ns > 0) {
         const T rise = next - previous;
 #pragma unroll
@@ -565,6 +568,112 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const StepArgs<T> s) {
     record_probes<T>(s, blockIdx.x);
 }
 
+// ---- The adjoint (time-reversed) scheme ----------------------------------------------
+//
+// For a closed grid (no matched layer, one speed of sound) the step is linear in the fields,
+// p(n+1) = L p(n) - d p(n-1) - coef_inv sum_s weight (now w(n) - before w(n-1)) + source, with
+// L the links' row operator times coef_inv, d = coef_inv coef_b; and the branch velocities
+// w(n+1) = w(n-1) + gain (p(n+1) - p(n-1)) + now w(n) - before w(n-1). Its adjoint runs the
+// same two-level recursion backwards, in Q = coef_inv P (P the adjoint of p) and W (of w):
+//   W(n) = now W(n+1) + (1 - before) W(n+2) + weight (before Q(n+2) - now Q(n+1))
+//   Q(n) = coef_inv (coef_a Q(n+1) + sum_l w_l Q(n+1)[i + l] - coef_b Q(n+2)
+//                    + sum_s gain (W(n) - W(n+2)) + (C^T dJ/dy(n))_i),
+// the interior's rows being the forward interior update itself (the links' weights are
+// pair-averaged, so the Laplacian is symmetric, fdtd.hpp:62-67). The gradient of J with
+// respect to a special's wall term g (coef_b = 1 - g, coef_inv = 1 / (1 + g)) and a state's
+// gain is, with D(n) = p(n) - p(n-2) at the special's cell,
+//   dJ/dg = -sum_n D(n) Q(n),   dJ/dgain = sum_n D(n) W(n).
+
+template <typename T>
+__device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t block) {
+    constexpr int kLinkChunk = 6;
+    const uint32_t t = threadIdx.x;
+    const uint32_t k = block * kBlock + t;
+    if (k >= s.specials) return;
+    const int i = static_cast<int>(s.sp_cell[k]);
+    const int nl = s.sp_links[k], ns = s.sp_states[k];
+    const T ca = s.coef_a[k], cb = s.coef_b[k], ci = s.coef_inv[k];
+    const uint32_t lb = s.link_base[block] + t;
+    const uint32_t sb = s.state_base[block] + t;
+    const T *__restrict__ b = s.b;
+    const T qb = b[i], qa = s.a[i];
+    T sum = 0;
+    for (int c = 0; c < nl; c += kLinkChunk) {
+        int32_t off[kLinkChunk];
+        float w[kLinkChunk];
+        T v[kLinkChunk];
+#pragma unroll
+        for (int u = 0; u < kLinkChunk; ++u) {
+            if (c + u < nl) {
+                off[u] = s.link_offset[lb + (c + u) * kBlock];
+                w[u] = s.link_weight[lb + (c + u) * kBlock];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLinkChunk; ++u) {
+            if (c + u < nl) v[u] = b[i + off[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kLinkChunk; ++u) {
+            if (c + u < nl) sum += static_cast<T>(w[u]) * v[u];
+        }
+    }
+    T gsum = 0;
+    for (int j = 0; j < ns; ++j) {
+        const uint32_t q = sb + j * kBlock;
+        const T now = s.state_now[q], before = s.state_before[q];
+        const T old = s.wa[q];
+        const T fresh = now * s.wb[q] + (static_cast<T>(1) - before) * old +
+                        static_cast<T>(s.state_weight[q]) * (before * qa - now * qb);
+        gsum += static_cast<T>(s.state_gain[q]) * (fresh - old);
+        s.wa[q] = fresh;
+    }
+    s.a[i] = ci * (ca * qb + sum - cb * qa + gsum);
+}
+
+template <typename T, bool WIDE>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(const StepArgs<T> s) {
+    const uint32_t block = blockIdx.x;
+    if (block < s.special_blocks) {
+        adjoint_special<T>(s, block);
+        return;
+    }
+    interior_tile<T, WIDE, false>(s, block - s.special_blocks);
+}
+
+// The probes' sensitivities into the adjoint field: Q[cell] += coef_inv (sum over the taps
+// on the cell of weight dJ/dy(probe, n)), one thread per distinct tap cell.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void inject_kernel(T *__restrict__ q, const uint32_t *__restrict__ cell,
+                                                       const float *__restrict__ ci, const uint32_t *__restrict__ start,
+                                                       const uint32_t *__restrict__ probe,
+                                                       const double *__restrict__ weight,
+                                                       const double *__restrict__ g, unsigned long long stride,
+                                                       unsigned long long n, uint32_t count) {
+    const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
+    if (u >= count) return;
+    double sum = 0.0;
+    for (uint32_t t = start[u]; t < start[u + 1]; ++t) sum += weight[t] * g[probe[t] * stride + n];
+    q[cell[u]] += static_cast<T>(static_cast<double>(ci[u]) * sum);
+}
+
+// The gradient's sums: each general special's D(n) Q(n) and each of its states' D(n) W(n).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void accumulate_kernel(const StepArgs<T> s, const T *__restrict__ q,
+                                                           const T *__restrict__ w, const T *__restrict__ delta,
+                                                           double *__restrict__ acc_g, double *__restrict__ acc_gain) {
+    const uint32_t block = blockIdx.x, t = threadIdx.x;
+    const uint32_t k = block * kBlock + t;
+    if (k >= s.specials) return;
+    const double d = static_cast<double>(delta[k]);
+    acc_g[k] += d * static_cast<double>(q[s.sp_cell[k]]);
+    const uint32_t sb = s.state_base[block] + t;
+    for (int j = 0; j < s.sp_states[k]; ++j) {
+        const uint32_t slot = sb + j * kBlock;
+        acc_gain[slot] += d * static_cast<double>(w[slot]);
+    }
+}
+
 thread_local std::string g_create_error;
 
 struct DeviceBuffer {
@@ -605,6 +714,21 @@ struct as_fdtd {
     float4 *face_coef = nullptr, *face_branch = nullptr;
     int split = 0;            // 1: four launches a step instead of one
     int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order
+    int no_faces = 0;         // 1: every special is general (the adjoint's handles)
+    // The adjoint (see as_fdtd_gradient): its fields, accumulators, taps and checkpoints.
+    void *q0 = nullptr, *q1 = nullptr, *v0 = nullptr, *v1 = nullptr, *history = nullptr;
+    double *acc_g = nullptr, *acc_gain = nullptr;
+    uint32_t taps_unique = 0;
+    uint32_t *tap_cell = nullptr, *tap_start = nullptr, *tap_probe = nullptr;
+    double *tap_weight = nullptr;
+    float *tap_ci = nullptr;
+    std::vector<uint64_t> sorted_to_special;  // general special (sorted) -> layout index
+    std::vector<uint64_t> state_slot;         // layout state -> its slot
+    std::vector<uint32_t> probe_start_host;
+    std::vector<uint32_t> probe_cell_host;
+    std::vector<double> probe_weight_host;
+    std::vector<float> coef_inv_host;         // per layout special
+    std::vector<uint64_t> special_cell_host;  // padded cell per layout special
     uint32_t *layer_cell = nullptr;
     uint64_t *layer_at = nullptr;
     float *layer_scale = nullptr;
@@ -675,6 +799,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (h->graph_steps > 4096) refuse("at most 4096 steps a graph");
     if (tuning && tuning->split > 0) h->split = 1;
     if (tuning && tuning->tile_order > 0) h->tile_order = 1;
+    if (tuning && tuning->no_faces > 0) h->no_faces = 1;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
     const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);
     const uint64_t n_pad = px * ny * nz;
@@ -751,7 +876,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
                 refuse("a branch state has no branch");
             }
         }
-        bool ok = !h->wide && !h->scaled && states_of(k) <= 15;
+        bool ok = !h->no_faces && !h->wide && !h->scaled && states_of(k) <= 15;
         int last = -1;
         for (uint64_t l = L.link_start[k]; ok && l < L.link_start[k + 1]; ++l) {
             const int d = L.link_direction[l];
@@ -876,6 +1001,16 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         }
     }
     h->specials = static_cast<uint32_t>(G);
+    // Where each layout special and state went, for the adjoint's sensitivities.
+    h->sorted_to_special = order;
+    h->state_slot.assign(L.states, ~0ull);
+    h->coef_inv_host.assign(L.coef_inv, L.coef_inv + K);
+    h->special_cell_host.resize(K);
+    for (uint64_t k = 0; k < K; ++k) h->special_cell_host[k] = pad(static_cast<uint64_t>(L.special[k]));
+    for (uint64_t k = 0; k < G; ++k) {
+        const uint64_t src = order[k], B = k / kBlock, t = k % kBlock;
+        for (uint64_t u = 0; u < states_of(src); ++u) h->state_slot[L.state_start[src] + u] = state_base[B] + u * kBlock + t;
+    }
     h->face_specials = static_cast<uint32_t>(K - G);
     h->states = static_cast<uint32_t>(L.states);
 
@@ -974,6 +1109,9 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         probe_cell[t] = static_cast<uint32_t>(pad(L.probe_cells[t]));
     }
     h->probes = static_cast<uint32_t>(L.probes);
+    h->probe_start_host = probe_start;
+    h->probe_cell_host = probe_cell;
+    h->probe_weight_host = probe_weight;
 
     // Device copies.
     const size_t e = precision == 64 ? sizeof(double) : sizeof(float);
@@ -1337,6 +1475,215 @@ int as_fdtd_field(as_fdtd *h, double *out) {
         return fail(h, f);
     }
 }
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout, double *out,
+              double *sens_g, double *sens_gain) {
+    if (h->layer_cells > 0) refuse("the adjoint needs a closed grid: no matched layer");
+    if (h->scaled) refuse("the adjoint needs one speed of sound for the whole grid");
+    if (h->face_specials > 0) refuse("the adjoint needs a handle made with no_faces set");
+    if (every == 0) refuse("checkpoints every 0 steps");
+    const size_t e = sizeof(T);
+    const size_t G = h->specials, slots = h->state_slots, P = h->probes;
+    const size_t segments = (steps + every - 1) / every;
+    std::vector<void *> scratch;
+    auto alloc = [&](size_t bytes) {
+        void *p = nullptr;
+        check(hipMalloc(&p, std::max<size_t>(bytes, 256)), "hipMalloc");
+        scratch.push_back(p);
+        check(hipMemsetAsync(p, 0, std::max<size_t>(bytes, 256), h->stream), "hipMemsetAsync");
+        return p;
+    };
+    struct Free {
+        std::vector<void *> &v;
+        ~Free() {
+            for (void *p : v) (void)hipFree(p);
+        }
+    } release{scratch};
+    // Checkpoints: the state before each segment's first step.
+    const size_t field = h->n_pad * e, states = slots * e;
+    char *ckpt = static_cast<char *>(alloc(segments * (2 * field + 2 * states)));
+    T *history = static_cast<T *>(alloc(every * std::max<size_t>(G, 1) * e));
+    T *q0 = static_cast<T *>(alloc(field)), *q1 = static_cast<T *>(alloc(field));
+    T *v0 = static_cast<T *>(alloc(states)), *v1 = static_cast<T *>(alloc(states));
+    double *acc_g = static_cast<double *>(alloc(std::max<size_t>(G, 1) * sizeof(double)));
+    double *acc_gain = static_cast<double *>(alloc(std::max<size_t>(slots, 1) * sizeof(double)));
+    float *d_signal = static_cast<float *>(alloc(steps * sizeof(float)));
+    double *d_out = static_cast<double *>(alloc(std::max<size_t>(P, 1) * steps * sizeof(double)));
+    double *d_g = static_cast<double *>(alloc(std::max<size_t>(P, 1) * steps * sizeof(double)));
+    check(hipMemcpyAsync(d_signal, signal, steps * sizeof(float), hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync");
+    if (P > 0) {
+        check(hipMemcpyAsync(d_g, dl_dout, P * steps * sizeof(double), hipMemcpyHostToDevice, h->stream),
+              "hipMemcpyAsync");
+    }
+    // The probes' taps, by distinct cell, each with its coef_inv (1 off the special cells).
+    std::vector<std::pair<uint32_t, std::pair<uint32_t, double>>> taps;  // (cell, (probe, weight))
+    for (uint32_t p = 0; p < P; ++p)
+        for (uint32_t t = h->probe_start_host[p]; t < h->probe_start_host[p + 1]; ++t)
+            taps.push_back({h->probe_cell_host[t], {p, h->probe_weight_host[t]}});
+    std::stable_sort(taps.begin(), taps.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    std::vector<uint32_t> tcell, tstart{0}, tprobe;
+    std::vector<double> tweight;
+    std::vector<float> tci;
+    std::vector<std::pair<uint64_t, float>> ci_of;
+    for (size_t k = 0; k < h->special_cell_host.size(); ++k) ci_of.push_back({h->special_cell_host[k], h->coef_inv_host[k]});
+    std::sort(ci_of.begin(), ci_of.end());
+    for (size_t t = 0; t < taps.size(); ++t) {
+        if (t == 0 || taps[t].first != taps[t - 1].first) {
+            if (t) tstart.push_back(static_cast<uint32_t>(tprobe.size()));
+            tcell.push_back(taps[t].first);
+            const auto it = std::lower_bound(ci_of.begin(), ci_of.end(), std::make_pair<uint64_t, float>(taps[t].first, -1.0f));
+            tci.push_back(it != ci_of.end() && it->first == taps[t].first ? it->second : 1.0f);
+        }
+        tprobe.push_back(taps[t].second.first);
+        tweight.push_back(taps[t].second.second);
+    }
+    tstart.push_back(static_cast<uint32_t>(tprobe.size()));
+    const uint32_t unique = static_cast<uint32_t>(tcell.size());
+    auto put = [&](const auto &v) {
+        using U = typename std::decay_t<decltype(v)>::value_type;
+        U *p = static_cast<U *>(alloc(std::max<size_t>(v.size(), 1) * sizeof(U)));
+        if (!v.empty()) check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync");
+        return p;
+    };
+    uint32_t *d_tcell = put(tcell), *d_tstart = put(tstart), *d_tprobe = put(tprobe);
+    double *d_tweight = put(tweight);
+    float *d_tci = put(tci);
+
+    // The forward run, saving the state before each segment.
+    StepArgs<T> s = step_args<T>(h);
+    const uint32_t blocks = block_count(h, s);
+    T *u[2] = {static_cast<T *>(h->u0), static_cast<T *>(h->u1)};
+    T *w[2] = {static_cast<T *>(h->w0), static_cast<T *>(h->w1)};
+    auto save = [&](size_t seg) {
+        char *base = ckpt + seg * (2 * field + 2 * states);
+        check(hipMemcpyAsync(base, u[0], field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        check(hipMemcpyAsync(base + field, u[1], field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        if (states) {
+            check(hipMemcpyAsync(base + 2 * field, w[0], states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+            check(hipMemcpyAsync(base + 2 * field + states, w[1], states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        }
+    };
+    auto restore = [&](size_t seg) {
+        char *base = ckpt + seg * (2 * field + 2 * states);
+        check(hipMemcpyAsync(u[0], base, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        check(hipMemcpyAsync(u[1], base + field, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        if (states) {
+            check(hipMemcpyAsync(w[0], base + 2 * field, states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+            check(hipMemcpyAsync(w[1], base + 2 * field + states, states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        }
+    };
+    // The run starts from rest.
+    for (T *f : {u[0], u[1]}) check(hipMemsetAsync(f, 0, field, h->stream), "hipMemsetAsync");
+    for (T *f : {w[0], w[1]}) check(hipMemsetAsync(f, 0, std::max<size_t>(states, 256), h->stream), "hipMemsetAsync");
+    s.signal = d_signal;
+    s.out = d_out;
+    s.out_stride = steps;
+    for (size_t seg = 0; seg < segments; ++seg) {
+        save(seg);
+        const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
+        for (size_t k = k0; k < k1; ++k) launch_one<T>(h, s, blocks, k, k, h->stream);
+    }
+    check(hipGetLastError(), "forward");
+    if (out != nullptr && P > 0) {
+        check(hipMemcpyAsync(out, d_out, P * steps * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+    }
+
+    // Backwards, segment by segment: recompute the segment from its checkpoint, keeping the
+    // specials' p(n+1) - p(n-1), then the adjoint steps n = (its last step + 1) down to its
+    // first step + 1.
+    StepArgs<T> r = step_args<T>(h);  // recompute: no probes, history on
+    r.probes = 0;
+    r.signal = d_signal;
+    r.history = history;
+    const uint32_t rblocks = block_count(h, r);
+    StepArgs<T> a = step_args<T>(h);  // adjoint
+    a.probes = 0;
+    const uint32_t ablocks = a.special_blocks + a.tile_blocks;
+    const uint32_t sblocks = (h->specials + kBlock - 1) / kBlock;
+    const uint32_t iblocks = (unique + kBlock - 1) / kBlock;
+    unsigned long long m = 0;  // adjoint steps done: their parity says which field is which
+    for (size_t seg = segments; seg-- > 0;) {
+        const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
+        restore(seg);
+        for (size_t k = k0; k < k1; ++k) {
+            r.signal = d_signal + k0;
+            launch_one<T>(h, r, rblocks, k, k - k0, h->stream);
+        }
+        // Adjoint steps n from min(k1, steps - 1) down to k0 + 1 (Q(steps) is 0: p(steps) is
+        // read by no probe; Q(0) enters no gradient).
+        const size_t top = std::min(k1, steps - 1);
+        for (size_t n = top; n >= k0 + 1 && n >= 1; --n) {
+            const bool even = (m & 1ull) == 0;
+            a.a = even ? q0 : q1;
+            a.b = even ? q1 : q0;
+            a.wa = even ? v0 : v1;
+            a.wb = even ? v1 : v0;
+            if (ablocks) {
+                if (h->wide) {
+                    hipLaunchKernelGGL((adjoint_kernel<T, true>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                } else {
+                    hipLaunchKernelGGL((adjoint_kernel<T, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                }
+            }
+            if (iblocks) {
+                hipLaunchKernelGGL((inject_kernel<T>), dim3(iblocks), dim3(kBlock), 0, h->stream, a.a, d_tcell, d_tci,
+                                   d_tstart, d_tprobe, d_tweight, d_g, static_cast<unsigned long long>(steps),
+                                   static_cast<unsigned long long>(n), unique);
+            }
+            if (sblocks) {
+                hipLaunchKernelGGL((accumulate_kernel<T>), dim3(sblocks), dim3(kBlock), 0, h->stream, a, a.a, a.wa,
+                                   history + (n - 1 - k0) * G, acc_g, acc_gain);
+            }
+            ++m;
+            if (n == 0) break;
+        }
+    }
+    check(hipGetLastError(), "adjoint");
+    std::vector<double> g(G), gain(slots);
+    if (G) check(hipMemcpyAsync(g.data(), acc_g, G * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+    if (slots) check(hipMemcpyAsync(gain.data(), acc_gain, slots * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+    check(hipStreamSynchronize(h->stream), "adjoint");
+    // The state after the run is the forward run's end: restore it for the next call.
+    h->done = 0;
+    if (sens_g != nullptr) {
+        std::fill(sens_g, sens_g + h->special_cell_host.size(), 0.0);
+        for (size_t k = 0; k < G; ++k) sens_g[h->sorted_to_special[k]] = -g[k];
+    }
+    if (sens_gain != nullptr) {
+        for (size_t j = 0; j < h->state_slot.size(); ++j) sens_gain[j] = h->state_slot[j] == ~0ull ? 0.0 : gain[h->state_slot[j]];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int as_fdtd_gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout, double *out,
+                     double *sens_g, double *sens_gain) {
+    if (h == nullptr) return AS_EINVAL;
+    try {
+        if (steps < 2) refuse("the adjoint needs at least two steps");
+        if (signal == nullptr || (dl_dout == nullptr && h->probes > 0)) refuse("the gradient needs a drive and dJ/dy");
+        if (h->precision == 64) {
+            gradient<double>(h, signal, steps, every, dl_dout, out, sens_g, sens_gain);
+        } else {
+            gradient<float>(h, signal, steps, every, dl_dout, out, sens_g, sens_gain);
+        }
+        // Leave the handle at rest, as after reset().
+        return as_fdtd_reset(h);
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+}  // extern "C"
+
+extern "C" {
 
 void *as_fdtd_stream(as_fdtd *h) { return h ? static_cast<void *>(h->stream) : nullptr; }
 

@@ -26,12 +26,13 @@ class HipWaveSolver:
     each kind of cell on its own instead of one launch per step."""
 
     def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
-                 graph_steps: int = 0, split: bool = False, tile_order: int = 0) -> None:
+                 graph_steps: int = 0, split: bool = False, tile_order: int = 0,
+                 no_faces: bool = False) -> None:
         lib = native.load()
         layout = solver if isinstance(solver, dict) else solver.layout()
         struct, keep = native.pack_layout(layout)
         tuning = native.Tuning(0, int(tile_z), int(row_align), int(graph_steps), 1 if split else 0,
-                               int(tile_order))
+                               int(tile_order), 1 if no_faces else 0)
         h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
         if not h:
             message = lib.as_fdtd_create_error().decode()
@@ -41,6 +42,8 @@ class HipWaveSolver:
         self._h = h
         self._lib = lib
         self._probes = int(struct.probes)
+        self._specials = int(struct.specials)
+        self._states = int(struct.states)
         self.shape = (int(struct.nx), int(struct.ny), int(struct.nz))
         self.precision = int(precision)
         del keep  # the library holds its own copies
@@ -67,6 +70,22 @@ class HipWaveSolver:
         float32[steps]; the probes' output, float64[probes, steps]); does not wait."""
         self._check(self._lib.as_fdtd_run_device(self._h, ctypes.c_void_p(signal_ptr), int(steps),
                                                  ctypes.c_void_p(out_ptr)))
+
+    def gradient(self, signal, dl_dout, every: int = 256):
+        """The adjoint run (see as_fdtd_gradient in include/asfdtd.h): from rest, the probes'
+        signals for ``signal`` and the gradient of a loss J given dJ/dy (probes, steps) with
+        respect to each special cell's wall term g and each branch state's gain. Returns
+        (y (probes, steps), dJ/dg (specials,), dJ/dgain (states,)). The solver is left at rest;
+        it must be made with ``no_faces=True``."""
+        signal = np.ascontiguousarray(signal, dtype=np.float32).reshape(-1)
+        steps = len(signal)
+        g = np.ascontiguousarray(dl_dout, dtype=np.float64).reshape(self._probes, steps)
+        y = np.empty((self._probes, steps), np.float64)
+        sens_g = np.zeros(self._specials, np.float64)
+        sens_gain = np.zeros(self._states, np.float64)
+        self._check(self._lib.as_fdtd_gradient(self._h, signal.ctypes.data, steps, int(every), g.ctypes.data,
+                                               y.ctypes.data, sens_g.ctypes.data, sens_gain.ctypes.data))
+        return y, sens_g, sens_gain
 
     def reset(self) -> None:
         """Zeroes the fields and states (WaveSolver::reset, fdtd.hpp:906-915)."""

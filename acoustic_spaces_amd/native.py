@@ -43,7 +43,7 @@ class Tuning(ctypes.Structure):
 
     _fields_ = [("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int), ("row_align", ctypes.c_int),
                 ("graph_steps", ctypes.c_int), ("split", ctypes.c_int),
-                ("tile_order", ctypes.c_int)]
+                ("tile_order", ctypes.c_int), ("no_faces", ctypes.c_int)]
 
 
 _lib = None
@@ -52,8 +52,12 @@ _lib = None
 SYMBOLS = (
     "as_fdtd_create", "as_fdtd_create_error", "as_fdtd_run", "as_fdtd_run_device",
     "as_fdtd_reset", "as_fdtd_field", "as_fdtd_stream", "as_fdtd_info", "as_fdtd_last_error",
-    "as_fdtd_destroy",
+    "as_fdtd_destroy", "as_fdtd_gradient",
 )
+
+#: The set-up's symbols (acoustic_spaces_amd.solver binds them).
+SETUP_SYMBOLS = ("as_setup_create", "as_setup_create_error", "as_setup_layout", "as_setup_info",
+                 "as_setup_probe", "as_setup_destroy")
 
 
 def load():
@@ -87,6 +91,8 @@ def load():
     lib.as_fdtd_last_error.argtypes = [P]
     lib.as_fdtd_destroy.restype = None
     lib.as_fdtd_destroy.argtypes = [P]
+    lib.as_fdtd_gradient.restype = ctypes.c_int
+    lib.as_fdtd_gradient.argtypes = [P, P, SIZE, SIZE, P, P, P, P]
     _lib = lib
     return lib
 

@@ -86,6 +86,7 @@ typedef struct as_tuning {
     int graph_steps;   /* steps a captured hipGraph replays (0 = 64; negative: no graphs) */
     int split;         /* 1: one launch per kind of cell (specials, interior, layer, probes) */
     int tile_order;    /* 0: interior tiles grouped by die (XCD), 1: in grid order */
+    int no_faces;      /* 1: step every special cell by its own blocks (as the adjoint needs) */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the
@@ -119,6 +120,47 @@ int as_fdtd_info(const as_fdtd *h, uint64_t *out);
 
 const char *as_fdtd_last_error(const as_fdtd *h);
 void as_fdtd_destroy(as_fdtd *h);
+
+/*
+ * The gradient of a loss J of the probes' signals with respect to each special cell's wall
+ * term g (coef_b = 1 - g, coef_inv = 1 / (1 + g), fdtd.hpp:3499-3517) and each branch
+ * state's gain (fdtd.hpp:391-411), by the adjoint (time-reversed) scheme. The reference
+ * has no adjoint (no counterpart); it serves fitting wall admittance to measured
+ * responses. From rest, `steps` steps of `signal` are run, saving the state every `every`
+ * steps; then, segment by segment from the last, the segment is recomputed from its
+ * checkpoint and the adjoint stepped back through it. dl_dout holds dJ/dy[probe][step]
+ * (probes x steps). Out: the forward probes (probes x steps, or NULL), dJ/dg per layout
+ * special (specials) and dJ/dgain per layout state (states). The handle must be made with
+ * no_faces set, on a closed grid (no matched layer) with one speed of sound. It is left at
+ * rest.
+ */
+int as_fdtd_gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout,
+                     double *out, double *sens_g, double *sens_gain);
+
+/*
+ * The set-up on walls on cell faces (the reference's WaveSolver constructor,
+ * fdtd.hpp:360-690, bound at cpp/bindings/native_module.cpp:1238-1285): the arguments
+ * of the binding, the mesh as vertices (nv x 3) and faces (nf x 3); threads 0 takes the
+ * machine's. NULL on failure (ValueError in the reference; see as_setup_create_error).
+ */
+typedef struct as_setup as_setup;
+as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
+                          const int64_t *face_material, const double *beta, size_t materials,
+                          const double *branches, const int64_t *branch_material, size_t nbranch,
+                          const double *origin, double cell, size_t nx, size_t ny, size_t nz,
+                          const double *source, const double *probes, size_t nprobes,
+                          double edge_admittance, size_t layer, double layer_damping, size_t stencil,
+                          const float *speed, size_t probes_per_listener, int threads);
+const char *as_setup_create_error(void);
+/* Points `out` at the set-up's layout arrays (WaveSolver::layout, fdtd.hpp:727-782); they
+ * live as long as the handle. */
+int as_setup_layout(const as_setup *h, as_layout *out);
+/* [0] air cells, [1] boundary cells, [2] branch states, [3] layer cells, [4] probes,
+ * [5] open, [6] open sides (bit k for -x, +x, -y, +y, -z, +z), [7] source on a face. */
+int as_setup_info(const as_setup *h, uint64_t *out);
+/* probe_found and probe_in_sight of probe p (fdtd.hpp:713-716). */
+int as_setup_probe(const as_setup *h, size_t p, int *found, int *in_sight);
+void as_setup_destroy(as_setup *h);
 
 #ifdef __cplusplus
 }

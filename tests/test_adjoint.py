@@ -1,0 +1,109 @@
+"""The adjoint (time-reversed) scheme's gradient.
+
+CPU: the adjoint's oracle (oracle/adjoint.py, a double NumPy restatement of the step with
+each special's wall term g and each state's gain given) equals the C oracle's double run.
+GPU: dJ/dg and dJ/dgain from HipWaveSolver.gradient equal central finite differences of
+the NumPy forward (h = 1e-6 on g and gain, J linear in the probes with seeded weights),
+are independent of the checkpoint interval, and the float adjoint agrees with the double.
+"""
+
+import numpy as np
+import pytest
+
+import cases
+
+
+def _layout(stencil=7):
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    sc = cases.small_box(stencil=stencil)
+    return sc, WaveSolver.from_scene(sc).layout()
+
+
+def test_numpy_forward_equals_the_double_oracle():
+    from oracle.adjoint import Forward
+    from oracle.stepper import OracleSolver
+
+    sc, layout = _layout()
+    f = Forward(layout)
+    g, gain = f.layout_coefficients(layout)
+    drive = sc.drive[:200]
+    a = f.run(drive, g, gain, coef_b=np.asarray(layout["coef_b"], np.float64),
+              coef_inv=np.asarray(layout["coef_inv"], np.float64))
+    b = OracleSolver(layout, 64).run(drive)
+    assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def _fd(f, layout, drive, gain, c, which, index, h):
+    """Central differences of J = sum(c y) in a special's wall term g or a state's gain,
+    about the layout's own float coefficients: g moves coef_b by -h and 1 / coef_inv by +h
+    (coef_b = 1 - g, coef_inv = 1 / (1 + g)), as the adjoint differentiates them."""
+    cb = np.asarray(layout["coef_b"], np.float64)
+    ci = np.asarray(layout["coef_inv"], np.float64)
+
+    def J(dg, gn):
+        return float(np.sum(c * f.run(drive, None, gn, coef_b=cb - dg, coef_inv=1.0 / (1.0 / ci + dg))))
+    zero = np.zeros(len(cb))
+    if which == "g":
+        up, dn = zero.copy(), zero.copy()
+        up[index] = h
+        dn[index] = -h
+        return (J(up, gain) - J(dn, gain)) / (2 * h)
+    up, dn = gain.copy(), gain.copy()
+    up[index] += h
+    dn[index] -= h
+    return (J(zero, up) - J(zero, dn)) / (2 * h)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stencil", [7, 19])
+def test_gradient_equals_finite_differences(stencil):
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.adjoint import Forward
+
+    sc, layout = _layout(stencil)
+    drive = sc.drive[:300]
+    rng = np.random.default_rng(5)
+    gpu = HipWaveSolver(layout, precision=64, no_faces=True)
+    c = rng.normal(size=(gpu.probes, len(drive)))
+    y, sens_g, sens_gain = gpu.gradient(drive, c, every=64)
+    f = Forward(layout)
+    g, gain = f.layout_coefficients(layout)
+    # The forward the adjoint differentiates is the stepper's own.
+    assert np.allclose(y, f.run(drive, g, gain, coef_b=np.asarray(layout["coef_b"], np.float64),
+                                coef_inv=np.asarray(layout["coef_inv"], np.float64)), rtol=0, atol=1e-9 * np.abs(y).max())
+    scale_g = np.abs(sens_g).max()
+    scale_gain = np.abs(sens_gain).max()
+    for k in rng.choice(len(g), 5, replace=False):
+        fd = _fd(f, layout, drive, gain, c, "g", k, 1e-6)
+        assert abs(fd - sens_g[k]) <= 1e-6 * scale_g + 1e-5 * abs(fd)
+    for j in rng.choice(len(gain), 5, replace=False):
+        fd = _fd(f, layout, drive, gain, c, "gain", j, 1e-6)
+        assert abs(fd - sens_gain[j]) <= 1e-6 * scale_gain + 1e-5 * abs(fd)
+
+
+@pytest.mark.gpu
+def test_checkpoint_interval_does_not_change_the_gradient():
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    sc, layout = _layout()
+    drive = sc.drive[:250]
+    gpu = HipWaveSolver(layout, precision=64, no_faces=True)
+    c = np.random.default_rng(6).normal(size=(gpu.probes, len(drive)))
+    a = gpu.gradient(drive, c, every=7)
+    b = gpu.gradient(drive, c, every=1000)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.gpu
+def test_float_adjoint_agrees_with_double():
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    sc, layout = _layout()
+    drive = sc.drive[:300]
+    c = np.random.default_rng(7).normal(size=(4, len(drive)))
+    _, g64, n64 = HipWaveSolver(layout, precision=64, no_faces=True).gradient(drive, c)
+    _, g32, n32 = HipWaveSolver(layout, precision=32, no_faces=True).gradient(drive, c)
+    assert np.linalg.norm(g32 - g64) <= 1e-4 * np.linalg.norm(g64)
+    assert np.linalg.norm(n32 - n64) <= 1e-4 * np.linalg.norm(n64)
