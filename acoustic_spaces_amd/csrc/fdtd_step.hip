@@ -85,13 +85,13 @@ struct StepArgs {
     const int32_t *__restrict__ link_offset;
     const float *__restrict__ link_weight;
     const float *__restrict__ coef_a;
-    const float *__restrict__ coef_b;
-    const float *__restrict__ coef_inv;
+    const T *__restrict__ coef_b;      // T: double overrides for precision 64 (as_layout)
+    const T *__restrict__ coef_inv;
     const float *__restrict__ source;
     const float *__restrict__ state_weight;
     const float *__restrict__ state_now;
     const float *__restrict__ state_before;
-    const float *__restrict__ state_gain;
+    const T *__restrict__ state_gain;
     const float *__restrict__ signal;  // device, indexed by step
     // face specials, stepped in the tile pass (see face_special)
     const uint32_t *__restrict__ face_tile;    // per tile block: its first record (and one past the last)
@@ -353,7 +353,8 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
     if (k >= s.specials) return;
     const int i = static_cast<int>(s.sp_cell[k]);
     const int nl = s.sp_links[k], ns = s.sp_states[k];
-    const float ca = s.coef_a[k], cb = s.coef_b[k], ci = s.coef_inv[k], src = s.source[k];
+    const float ca = s.coef_a[k], src = s.source[k];
+    const T cb = s.coef_b[k], ci = s.coef_inv[k];
     const float sig = s.signal[s.step];
     const uint32_t lb = s.link_base[block] + t;
     const uint32_t sb = s.state_base[block] + t;
@@ -645,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(const StepArgs<T> s) {
 // on the cell of weight dJ/dy(probe, n)), one thread per distinct tap cell.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void inject_kernel(T *__restrict__ q, const uint32_t *__restrict__ cell,
-                                                       const float *__restrict__ ci, const uint32_t *__restrict__ start,
+                                                       const double *__restrict__ ci, const uint32_t *__restrict__ start,
                                                        const uint32_t *__restrict__ probe,
                                                        const double *__restrict__ weight,
                                                        const double *__restrict__ g, unsigned long long stride,
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(kBlock) void inject_kernel(T *__restrict__ q, const
     if (u >= count) return;
     double sum = 0.0;
     for (uint32_t t = start[u]; t < start[u + 1]; ++t) sum += weight[t] * g[probe[t] * stride + n];
-    q[cell[u]] += static_cast<T>(static_cast<double>(ci[u]) * sum);
+    q[cell[u]] += static_cast<T>(ci[u] * sum);
 }
 
 // The gradient's sums: each general special's D(n) Q(n) and each of its states' D(n) W(n).
@@ -704,8 +705,9 @@ struct as_fdtd {
     uint32_t *sp_cell = nullptr, *link_base = nullptr, *state_base = nullptr;
     uint8_t *sp_links = nullptr, *sp_states = nullptr;
     int32_t *link_offset = nullptr;
-    float *link_weight = nullptr, *coef_a = nullptr, *coef_b = nullptr, *coef_inv = nullptr, *source = nullptr;
-    float *state_weight = nullptr, *state_now = nullptr, *state_before = nullptr, *state_gain = nullptr;
+    float *link_weight = nullptr, *coef_a = nullptr, *source = nullptr;
+    void *coef_b = nullptr, *coef_inv = nullptr, *state_gain = nullptr;  // float or double, as the precision
+    float *state_weight = nullptr, *state_now = nullptr, *state_before = nullptr;
     size_t state_slots = 0;   // the states' arrays' length (general slabs, then face states)
     uint32_t face_specials = 0;
     uint32_t *face_tile = nullptr;
@@ -721,13 +723,12 @@ struct as_fdtd {
     uint32_t taps_unique = 0;
     uint32_t *tap_cell = nullptr, *tap_start = nullptr, *tap_probe = nullptr;
     double *tap_weight = nullptr;
-    float *tap_ci = nullptr;
     std::vector<uint64_t> sorted_to_special;  // general special (sorted) -> layout index
     std::vector<uint64_t> state_slot;         // layout state -> its slot
     std::vector<uint32_t> probe_start_host;
     std::vector<uint32_t> probe_cell_host;
     std::vector<double> probe_weight_host;
-    std::vector<float> coef_inv_host;         // per layout special
+    std::vector<double> coef_inv_host;        // per layout special
     std::vector<uint64_t> special_cell_host;  // padded cell per layout special
     uint32_t *layer_cell = nullptr;
     uint64_t *layer_at = nullptr;
@@ -767,6 +768,13 @@ U *upload(as_fdtd *h, const std::vector<U> &v) {
     h->allocations.push_back(p);
     check(hipMemcpy(p, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice), "hipMemcpy");
     return static_cast<U *>(p);
+}
+
+// A host array of doubles on the device as float (precision 32) or double (64).
+void *upload_as(as_fdtd *h, const std::vector<double> &v, int precision) {
+    if (precision == 64) return upload(h, v);
+    std::vector<float> f(v.begin(), v.end());
+    return upload(h, f);
 }
 
 void *zeros(as_fdtd *h, size_t bytes) {
@@ -851,6 +859,13 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     std::vector<uint8_t> face(K, 0);
     std::vector<uint8_t> seen(K ? n_pad : 0, 0);
     const float axial = L.weights[0];
+    // Double coefficients (precision 64 only): every special then stepped by its own blocks.
+    const bool exact = L.coef_b64 != nullptr || L.coef_inv64 != nullptr || L.branch_gain64 != nullptr;
+    if (exact && (precision != 64 || L.coef_b64 == nullptr || L.coef_inv64 == nullptr ||
+                  (L.branches > 0 && L.branch_gain64 == nullptr))) {
+        refuse("double coefficients need precision 64 and all of coef_b64, coef_inv64 and branch_gain64");
+    }
+    if (exact) h->no_faces = 1;
     for (uint64_t k = 0; k < K; ++k) {
         const int64_t i = L.special[k];
         if (i < 0 || static_cast<uint64_t>(i) >= n) refuse("a special cell lies outside the grid");
@@ -955,7 +970,8 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     const uint64_t blocks = (G + kBlock - 1) / kBlock;
     std::vector<uint32_t> sp_cell(G), link_base(blocks), state_base(blocks);
     std::vector<uint8_t> sp_links(G), sp_states(G);
-    std::vector<float> coef_a(G), coef_b(G), coef_inv(G), source(G);
+    std::vector<float> coef_a(G), source(G);
+    std::vector<double> coef_b(G), coef_inv(G);
     uint64_t link_slots = 0, state_slots = 0;
     for (uint64_t B = 0; B < blocks; ++B) {
         uint64_t wl = 0, ws = 0;
@@ -973,16 +989,16 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     }
     std::vector<int32_t> link_offset(link_slots, 0);
     std::vector<float> link_weight(link_slots, 0.0f);
-    std::vector<float> state_weight(state_slots, 0.0f), now(state_slots, 0.0f), before(state_slots, 0.0f),
-        gain(state_slots, 0.0f);
+    std::vector<float> state_weight(state_slots, 0.0f), now(state_slots, 0.0f), before(state_slots, 0.0f);
+    std::vector<double> gain(state_slots, 0.0);
     for (uint64_t k = 0; k < G; ++k) {
         const uint64_t src = order[k], B = k / kBlock, t = k % kBlock;
         sp_cell[k] = static_cast<uint32_t>(pad(static_cast<uint64_t>(L.special[src])));
         sp_links[k] = static_cast<uint8_t>(links_of(src));
         sp_states[k] = static_cast<uint8_t>(states_of(src));
         coef_a[k] = L.coef_a[src];
-        coef_b[k] = L.coef_b[src];
-        coef_inv[k] = L.coef_inv[src];
+        coef_b[k] = exact ? L.coef_b64[src] : L.coef_b[src];
+        coef_inv[k] = exact ? L.coef_inv64[src] : L.coef_inv[src];
         source[k] = L.source[src];
         for (uint64_t u = 0; u < links_of(src); ++u) {
             const uint64_t l = L.link_start[src] + u;
@@ -997,14 +1013,15 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             state_weight[slot] = L.state_weight[j];
             now[slot] = L.branch_now[br];
             before[slot] = L.branch_before[br];
-            gain[slot] = L.branch_gain[br];
+            gain[slot] = exact ? L.branch_gain64[br] : L.branch_gain[br];
         }
     }
     h->specials = static_cast<uint32_t>(G);
     // Where each layout special and state went, for the adjoint's sensitivities.
     h->sorted_to_special = order;
     h->state_slot.assign(L.states, ~0ull);
-    h->coef_inv_host.assign(L.coef_inv, L.coef_inv + K);
+    h->coef_inv_host.resize(K);
+    for (uint64_t k = 0; k < K; ++k) h->coef_inv_host[k] = exact ? L.coef_inv64[k] : L.coef_inv[k];
     h->special_cell_host.resize(K);
     for (uint64_t k = 0; k < K; ++k) h->special_cell_host[k] = pad(static_cast<uint64_t>(L.special[k]));
     for (uint64_t k = 0; k < G; ++k) {
@@ -1127,13 +1144,13 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->link_offset = upload(h, link_offset);
     h->link_weight = upload(h, link_weight);
     h->coef_a = upload(h, coef_a);
-    h->coef_b = upload(h, coef_b);
-    h->coef_inv = upload(h, coef_inv);
+    h->coef_b = upload_as(h, coef_b, precision);
+    h->coef_inv = upload_as(h, coef_inv, precision);
     h->source = upload(h, source);
     h->state_weight = upload(h, state_weight);
     h->state_now = upload(h, now);
     h->state_before = upload(h, before);
-    h->state_gain = upload(h, gain);
+    h->state_gain = upload_as(h, gain, precision);
     h->face_tile = upload(h, face_tile);
     {
         std::vector<uint32_t> records(4 * face_cell.size());
@@ -1202,13 +1219,13 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.link_offset = h->link_offset;
     s.link_weight = h->link_weight;
     s.coef_a = h->coef_a;
-    s.coef_b = h->coef_b;
-    s.coef_inv = h->coef_inv;
+    s.coef_b = static_cast<const T *>(h->coef_b);
+    s.coef_inv = static_cast<const T *>(h->coef_inv);
     s.source = h->source;
     s.state_weight = h->state_weight;
     s.state_now = h->state_now;
     s.state_before = h->state_before;
-    s.state_gain = h->state_gain;
+    s.state_gain = static_cast<const T *>(h->state_gain);
     s.face_tile = h->face_tile;
     s.face_record = h->face_record;
     s.face_state_kind = h->face_state_kind;
@@ -1528,16 +1545,16 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
     std::stable_sort(taps.begin(), taps.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
     std::vector<uint32_t> tcell, tstart{0}, tprobe;
     std::vector<double> tweight;
-    std::vector<float> tci;
-    std::vector<std::pair<uint64_t, float>> ci_of;
+    std::vector<double> tci;
+    std::vector<std::pair<uint64_t, double>> ci_of;
     for (size_t k = 0; k < h->special_cell_host.size(); ++k) ci_of.push_back({h->special_cell_host[k], h->coef_inv_host[k]});
     std::sort(ci_of.begin(), ci_of.end());
     for (size_t t = 0; t < taps.size(); ++t) {
         if (t == 0 || taps[t].first != taps[t - 1].first) {
             if (t) tstart.push_back(static_cast<uint32_t>(tprobe.size()));
             tcell.push_back(taps[t].first);
-            const auto it = std::lower_bound(ci_of.begin(), ci_of.end(), std::make_pair<uint64_t, float>(taps[t].first, -1.0f));
-            tci.push_back(it != ci_of.end() && it->first == taps[t].first ? it->second : 1.0f);
+            const auto it = std::lower_bound(ci_of.begin(), ci_of.end(), std::make_pair<uint64_t, double>(taps[t].first, -1.0));
+            tci.push_back(it != ci_of.end() && it->first == taps[t].first ? it->second : 1.0);
         }
         tprobe.push_back(taps[t].second.first);
         tweight.push_back(taps[t].second.second);
@@ -1552,7 +1569,7 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
     };
     uint32_t *d_tcell = put(tcell), *d_tstart = put(tstart), *d_tprobe = put(tprobe);
     double *d_tweight = put(tweight);
-    float *d_tci = put(tci);
+    double *d_tci = put(tci);
 
     // The forward run, saving the state before each segment.
     StepArgs<T> s = step_args<T>(h);

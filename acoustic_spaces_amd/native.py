@@ -34,7 +34,7 @@ class Layout(ctypes.Structure):
         ("branches", U64), ("branch_now", P), ("branch_before", P), ("branch_gain", P),
         ("layer_cells", U64), ("layer_cell", P), ("keep", P * 6), ("push", P * 6),
         ("probes", U64), ("probe_start", P), ("taps", U64), ("probe_cells", P),
-        ("probe_weights", P),
+        ("probe_weights", P), ("coef_b64", P), ("coef_inv64", P), ("branch_gain64", P),
     ]
 
 
@@ -174,4 +174,12 @@ def pack_layout(layout: dict):
                 raise ValueError("the matched layer's coefficients do not cover the grid")
             s.keep[2 * a + j] = keep.ctypes.data if len(keep) else None
             s.push[2 * a + j] = push.ctypes.data if len(push) else None
+    # Optional double coefficients (precision 64): coef_b64, coef_inv64, branch_gain64.
+    for name, count in (("coef_b64", s.specials), ("coef_inv64", s.specials), ("branch_gain64", s.branches)):
+        if name in layout:
+            arr = np.ascontiguousarray(layout[name], dtype=np.float64)
+            if len(arr) != count:
+                raise ValueError(f"{name} has {len(arr)} values, not {count}")
+            A[name] = arr
+            setattr(s, name, arr.ctypes.data if len(arr) else None)
     return s, A

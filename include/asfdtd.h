@@ -74,6 +74,11 @@ typedef struct as_layout {
     uint64_t taps;
     const uint64_t *probe_cells;   /* [taps] */
     const double *probe_weights;   /* [taps] */
+    /* Optional, with precision 64 only: coef_b, coef_inv (specials) and branch_gain
+     * (branches) in double, for a scheme whose coefficients are not rounded to float (an
+     * FP64 reference, finite differences); NULL: the floats above. Given, every special is
+     * stepped by its own blocks. */
+    const double *coef_b64, *coef_inv64, *branch_gain64;
 } as_layout;
 
 typedef struct as_fdtd as_fdtd;

@@ -1,0 +1,81 @@
+"""Fitting wall admittance: the model's closed forms on the CPU; on the GPU, the loss's
+gradient through torch.autograd (the adjoint) against central finite differences, h = 1e-3,
+of the double-precision solve with unrounded coefficients: within 1e-3 relative for the
+double adjoint; within 1e-2 for the float one, whose forward's rounding the loss's
+logarithmic decay curves amplify near their -40 dB floor."""
+
+import math
+
+import numpy as np
+import pytest
+
+from acoustic_spaces_amd import scenes
+
+
+def _scene(steps=700):
+    sc = scenes.shoebox(size=(1.9, 1.4, 1.1), cell=0.1, seed=3, length_s=steps / 3430.0 + 0.01, receivers=3)
+    return sc
+
+
+def test_absorption_of_a_constant_admittance_is_the_closed_form():
+    from acoustic_spaces_amd.fit import WallModel
+
+    m = WallModel(_scene(), steps=50)
+    beta = np.linspace(0.05, 0.3, m.walls)
+    theta = np.concatenate([beta, np.full(m.walls, 1e-12)])
+    a = m.absorption(theta)
+    expect = 1 - ((1 - beta) / (1 + beta)) ** 2
+    assert np.allclose(a[:, 0], expect, rtol=1e-9) and np.allclose(a[:, 1], expect, rtol=1e-9)
+    # theta_for inverts absorption at the two bands.
+    known = np.concatenate([np.linspace(0.02, 0.2, m.walls), np.linspace(0.3, 0.05, m.walls)])
+    lo, hi = m.absorption(known).T
+    t = m.theta_for(lo, hi)
+    assert np.allclose(m.absorption(t), np.stack([lo, hi], 1), atol=1e-9)
+    assert np.allclose(t, known, rtol=1e-6)
+
+
+def test_coefficients_follow_theta():
+    from acoustic_spaces_amd.fit import WallModel
+
+    m = WallModel(_scene(), steps=50)
+    theta = np.concatenate([np.full(m.walls, 0.1), np.full(m.walls, 0.05)])
+    L = m.coefficients(theta)
+    g = m.W @ (theta[: m.walls] + theta[m.walls:] * m.gamma)
+    assert np.array_equal(L["coef_b"], (1 - g).astype(np.float32))
+    assert np.allclose(L["branch_gain"], 0.05 * m.gamma)
+    # before = 2 w_c T / (1 + w_c T): the centred relaxation branch (fdtd.hpp:391-411).
+    wcT = 2 * math.pi * 177.0 / m.scene.rate
+    assert np.allclose(L["branch_before"], 2 * wcT / (1 + wcT))
+
+
+@pytest.mark.gpu
+def test_loss_gradient_equals_finite_differences():
+    import torch
+
+    from acoustic_spaces_amd.fit import WallModel, decay_loss, make_function
+
+    sc = _scene()
+    rate = sc.rate
+    truth = WallModel(sc, precision=64, every=64)
+    theta_true = np.concatenate([np.linspace(0.03, 0.15, 6), np.linspace(0.25, 0.05, 6)])
+    target = torch.from_numpy(truth.forward(theta_true))
+    theta0 = theta_true * np.array([1.3, 0.8, 1.1, 0.9, 1.2, 0.7] * 2)
+    WaveSolve = make_function()
+    for precision in (64, 32):
+        model = WallModel(sc, precision=precision, every=64)
+        t = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+        loss = decay_loss(WaveSolve.apply(t, model), target, rate)
+        loss.backward()
+        grad = t.grad.numpy()
+        fd = np.zeros_like(theta0)
+        h = 1e-3
+        for p in range(len(theta0)):
+            up, dn = theta0.copy(), theta0.copy()
+            up[p] += h
+            dn[p] -= h
+            lu = decay_loss(torch.from_numpy(truth.forward(up)), target, rate).item()
+            ld = decay_loss(torch.from_numpy(truth.forward(dn)), target, rate).item()
+            fd[p] = (lu - ld) / (2 * h)
+        rel = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
+        print(f"precision {precision}: |grad - fd| / |fd| = {rel:.2e}")
+        assert rel < (1e-3 if precision == 64 else 1e-2)
