@@ -180,10 +180,13 @@ def band_gains(n: int, rate: float, centre: float, order: int = 3):
     return hp * lp
 
 
-def decay_loss(y, target, rate: float, bands=(125.0, 250.0), early_s: float = 0.05, floor_db: float = -40.0):
+def decay_loss(y, target, rate: float, bands=(125.0, 250.0), early_s: float = 0.05, floor_db: float = -40.0,
+               early_weight: float = 100.0):
     """L2 on the band-filtered Schroeder decay curves (dB, down to floor_db of the target's
-    start) plus L2 on the first early_s seconds of pressure (relative to the target's
-    energy there): y and target (probes, steps) torch tensors."""
+    start, averaged over the samples kept) plus early_weight times the L2 on the first
+    early_s seconds of pressure (relative to the target's energy there): y and target
+    (probes, steps) torch tensors. The decay curves fix the walls' absorption together; the
+    early reflections tell the walls apart."""
     torch = _torch()
     n = y.shape[-1]
     total = y.new_zeros(())
@@ -200,5 +203,5 @@ def decay_loss(y, target, rate: float, bands=(125.0, 250.0), early_s: float = 0.
         total = total + torch.sum(mask * (cy - ct) ** 2) / torch.sum(mask)
     k = int(early_s * rate)
     scale = torch.sum(target[..., :k] ** 2) + 1e-30
-    total = total + torch.sum((y[..., :k] - target[..., :k]) ** 2) / scale
+    total = total + early_weight * torch.sum((y[..., :k] - target[..., :k]) ** 2) / scale
     return total
