@@ -95,6 +95,7 @@ struct StepArgs {
     const float *__restrict__ signal;  // device, indexed by step
     // face specials, stepped in the tile pass (see face_special)
     const uint32_t *__restrict__ face_tile;    // per tile block: its first record (and one past the last)
+    uint32_t face_blocks, face_count;          // or: all records by blocks of their own
     const uint4 *__restrict__ face_record;     // cell, links | states << 6 | coefficients << 10,
                                                // where its states start, its first four states' kinds
     const uint16_t *__restrict__ face_state_kind;  // per state: its row of face_branch
@@ -127,8 +128,8 @@ struct StepArgs {
 // is special_cell's, in its order. A tile's face specials are stepped after its column
 // pass, by all its lanes at once, while the tile's pressures are still in the caches.
 template <typename T>
-__device__ __forceinline__ void face_special(const StepArgs<T> &s, uint32_t r) {
-    const uint4 rec = s.face_record[r];  // cell, links | states << 6 | coefficients << 10, states' start, kinds
+__device__ __forceinline__ void face_special(const StepArgs<T> &s, const uint4 rec) {
+    // rec: cell, links | states << 6 | coefficients << 10, states' start, kinds
     const int i = static_cast<int>(rec.x);
     const uint32_t desc = rec.y;
     const uint32_t sbase = rec.z;
@@ -195,6 +196,11 @@ __device__ __forceinline__ void face_special(const StepArgs<T> &s, uint32_t r) {
         const T term = static_cast<T>(k.x) * s.wb[q] - static_cast<T>(k.y) * s.wa[q];
         s.wa[q] = s.wa[q] + (rise * static_cast<T>(k.z) + term);
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void face_special(const StepArgs<T> &s, uint32_t r) {
+    face_special<T>(s, s.face_record[r]);
 }
 
 // The 7-point column of one lane: planes z0 to z1 - 1 of cell (x, y), in chunks of kUnroll
@@ -524,7 +530,8 @@ __device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t blo
 }
 
 // One step in one launch: special blocks first (their gathers are the longest chains of
-// loads, so they start first), then interior tiles, layer cells and the probes.
+// loads, so they start first), then face specials by blocks of their own (if so chosen),
+// interior tiles, layer cells and the probes.
 template <typename T, bool WIDE, bool SCALED>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
     uint32_t block = blockIdx.x;
@@ -533,6 +540,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
         return;
     }
     block -= s.special_blocks;
+    if (block < s.face_blocks) {
+        const uint32_t r = block * kBlock + threadIdx.x;
+        if (r < s.face_count) face_special<T>(s, r);
+        return;
+    }
+    block -= s.face_blocks;
     if (block < s.tile_blocks) {
         interior_tile<T, WIDE, SCALED>(s, block);
         return;
@@ -551,6 +564,12 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void special_kernel(const StepArgs<T> s) {
     special_cell<T>(s, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void face_kernel(const StepArgs<T> s) {
+    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r < s.face_count) face_special<T>(s, r);
 }
 
 template <typename T, bool WIDE, bool SCALED>
@@ -717,6 +736,8 @@ struct as_fdtd {
     int split = 0;            // 1: four launches a step instead of one
     int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order
     int no_faces = 0;         // 1: every special is general (the adjoint's handles)
+    int face_blocks = 0;      // 1: face specials by blocks of their own, not in their tiles
+    uint32_t *face_tile_none = nullptr;  // a face_tile of empty ranges, for face_blocks
     // The adjoint (see as_fdtd_gradient): its fields, accumulators, taps and checkpoints.
     void *q0 = nullptr, *q1 = nullptr, *v0 = nullptr, *v1 = nullptr, *history = nullptr;
     double *acc_g = nullptr, *acc_gain = nullptr;
@@ -808,6 +829,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (tuning && tuning->split > 0) h->split = 1;
     if (tuning && tuning->tile_order > 0) h->tile_order = 1;
     if (tuning && tuning->no_faces > 0) h->no_faces = 1;
+    if (tuning && tuning->face_blocks > 0) h->face_blocks = 1;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
     const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);
     const uint64_t n_pad = px * ny * nz;
@@ -1152,6 +1174,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->state_before = upload(h, before);
     h->state_gain = upload_as(h, gain, precision);
     h->face_tile = upload(h, face_tile);
+    h->face_tile_none = upload(h, std::vector<uint32_t>(face_tile.size(), 0));
     {
         std::vector<uint32_t> records(4 * face_cell.size());
         for (size_t r = 0; r < face_cell.size(); ++r) {
@@ -1226,7 +1249,10 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.state_now = h->state_now;
     s.state_before = h->state_before;
     s.state_gain = static_cast<const T *>(h->state_gain);
-    s.face_tile = h->face_tile;
+    s.face_tile = h->face_blocks ? h->face_tile_none : h->face_tile;
+    s.face_count = h->face_blocks ? h->face_specials : 0;
+    s.face_blocks = (s.face_count + kBlock - 1) / kBlock;
+    s.face_blocks = (s.face_blocks + kXcds - 1) / kXcds * kXcds;
     s.face_record = h->face_record;
     s.face_state_kind = h->face_state_kind;
     s.face_coef = h->face_coef;
@@ -1261,15 +1287,17 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, unsigned long long 
     s.step = k;
     if (h->split) {
         if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
+        if (s.face_blocks) hipLaunchKernelGGL((face_kernel<T>), dim3(s.face_blocks), dim3(kBlock), 0, stream, s);
         if (s.tile_blocks) {
+            const dim3 grid(s.tile_blocks), block(kBlock);
             if (h->wide && h->scaled) {
-                hipLaunchKernelGGL((interior_kernel<T, true, true>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+                hipLaunchKernelGGL((interior_kernel<T, true, true>), grid, block, 0, stream, s);
             } else if (h->wide) {
-                hipLaunchKernelGGL((interior_kernel<T, true, false>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+                hipLaunchKernelGGL((interior_kernel<T, true, false>), grid, block, 0, stream, s);
             } else if (h->scaled) {
-                hipLaunchKernelGGL((interior_kernel<T, false, true>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+                hipLaunchKernelGGL((interior_kernel<T, false, true>), grid, block, 0, stream, s);
             } else {
-                hipLaunchKernelGGL((interior_kernel<T, false, false>), dim3(s.tile_blocks), dim3(kBlock), 0, stream, s);
+                hipLaunchKernelGGL((interior_kernel<T, false, false>), grid, block, 0, stream, s);
             }
         }
         if (s.layer_blocks) hipLaunchKernelGGL((layer_kernel<T>), dim3(s.layer_blocks), dim3(kBlock), 0, stream, s);
@@ -1277,22 +1305,22 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, unsigned long long 
         if (probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
         return;
     }
-    (void)blocks;
+    const dim3 grid(blocks), block(kBlock);
     if (h->wide && h->scaled) {
-        hipLaunchKernelGGL((step_kernel<T, true, true>), dim3(blocks), dim3(kBlock), 0, stream, s);
+        hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, 0, stream, s);
     } else if (h->wide) {
-        hipLaunchKernelGGL((step_kernel<T, true, false>), dim3(blocks), dim3(kBlock), 0, stream, s);
+        hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, 0, stream, s);
     } else if (h->scaled) {
-        hipLaunchKernelGGL((step_kernel<T, false, true>), dim3(blocks), dim3(kBlock), 0, stream, s);
+        hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, 0, stream, s);
     } else {
-        hipLaunchKernelGGL((step_kernel<T, false, false>), dim3(blocks), dim3(kBlock), 0, stream, s);
+        hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, 0, stream, s);
     }
 }
 
 template <typename T>
 uint32_t block_count(const as_fdtd *h, const StepArgs<T> &s) {
     const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
-    return s.tile_blocks + s.special_blocks + s.layer_blocks + probe_blocks;
+    return s.tile_blocks + s.special_blocks + s.face_blocks + s.layer_blocks + probe_blocks;
 }
 
 // Two graphs of graph_steps steps each, one starting on an even step and one on an odd, on

@@ -92,6 +92,8 @@ typedef struct as_tuning {
     int split;         /* 1: one launch per kind of cell (specials, interior, layer, probes) */
     int tile_order;    /* 0: interior tiles grouped by die (XCD), 1: in grid order */
     int no_faces;      /* 1: step every special cell by its own blocks (as the adjoint needs) */
+    int face_blocks;   /* 1: step the face specials (axial links of the interior's weight) by
+                          blocks of their own instead of in their tiles */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the

@@ -73,6 +73,24 @@ def test_launch_shape_does_not_change_results(reference_available, tile_z, row_a
     assert np.array_equal(gpu.run(scene.drive[:200]), cpu.run(scene.drive[:200]))
 
 
+@pytest.mark.parametrize("face_blocks,split", [(False, True), (True, False), (True, True)])
+def test_kernel_variants_equal_the_oracle(reference_available, face_blocks, split):
+    """Face specials in their tiles or by their own blocks, one launch a step or four: the
+    same fields, on a room with branch states and a scene with a matched layer."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    for scene in (cases.with_branches(cases.small_box()), cases.open_block()):
+        layout = _layout(scene)
+        print(f"variant face_blocks={face_blocks} split={split} shape {scene.grid.shape}", flush=True)
+        gpu = HipWaveSolver(layout, face_blocks=face_blocks, split=split)
+        cpu = OracleSolver(layout)
+        d = scene.drive[:230]
+        assert np.array_equal(gpu.run(d[:117]), cpu.run(d[:117]))
+        assert np.array_equal(gpu.run(d[117:]), cpu.run(d[117:]))
+        assert np.array_equal(gpu.field(), cpu.field())
+
+
 def test_shoebox_full_grid(reference_available):
     """The 6 x 4 x 3 m shoebox at 5 cm (125 x 85 x 65 cells, 576,000 air cells), 400 steps."""
     from acoustic_spaces_amd import scenes
