@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -35,6 +36,7 @@
 #include <map>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "asfdtd.h"
@@ -496,27 +498,31 @@ __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
 }
 
 // The probes (fdtd.hpp:3525-3534), one wavefront each: its lanes read the probe's cells
-// and form the products w * p in double at once, into the block's shared memory; then one
-// lane sums them in the order of the cells, as the reference sums them. Every thread of
-// the block reaches both barriers.
+// of `field` and form the products w * p in double at once, into `products` (kBlock
+// doubles of shared memory); then one lane sums them in the order of the cells, as the
+// reference sums them, into out[probe * stride + index]. Every thread of the block reaches
+// both barriers.
 template <typename T>
-__device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t block) {
-    __shared__ double products[kBlock];
+__device__ __forceinline__ void probe_sums(uint32_t probes, const uint32_t *__restrict__ probe_start,
+                                           const uint32_t *__restrict__ probe_cell,
+                                           const double *__restrict__ probe_weight, const T *__restrict__ field,
+                                           double *__restrict__ out, unsigned long long stride,
+                                           unsigned long long index, uint32_t block, double *products) {
     const uint32_t wave = threadIdx.x / kTileX, lane = threadIdx.x % kTileX;
     const uint32_t p = block * (kBlock / kTileX) + wave;
-    const bool active = p < s.probes;
-    const uint32_t k0 = active ? s.probe_start[p] : 0, k1 = active ? s.probe_start[p + 1] : 0;
+    const bool active = p < probes;
+    const uint32_t k0 = active ? probe_start[p] : 0, k1 = active ? probe_start[p + 1] : 0;
     // Every wave of the block runs the same number of rounds (the block's longest probe).
     uint32_t most = k1 - k0;
     for (int w = 0; w < kBlock / kTileX; ++w) {
         const uint32_t q = block * (kBlock / kTileX) + static_cast<uint32_t>(w);
-        if (q < s.probes) most = max(most, s.probe_start[q + 1] - s.probe_start[q]);
+        if (q < probes) most = max(most, probe_start[q + 1] - probe_start[q]);
     }
     double value = 0.0;
     for (uint32_t base = 0; base < most; base += kTileX) {
         const uint32_t k = k0 + base + lane;
         double product = 0.0;
-        if (k < k1) product = s.probe_weight[k] * static_cast<double>(s.b[s.probe_cell[k]]);
+        if (k < k1) product = probe_weight[k] * static_cast<double>(field[probe_cell[k]]);
         products[threadIdx.x] = product;
         __syncthreads();
         if (lane == 0 && k0 + base < k1) {
@@ -526,7 +532,14 @@ __device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t blo
         }
         __syncthreads();
     }
-    if (active && lane == 0) s.out[p * s.out_stride + s.step] = value;
+    if (active && lane == 0) out[p * stride + index] = value;
+}
+
+template <typename T>
+__device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t block) {
+    __shared__ double products[kBlock];
+    probe_sums<T>(s.probes, s.probe_start, s.probe_cell, s.probe_weight, s.b, s.out, s.out_stride, s.step, block,
+                  products);
 }
 
 // One step in one launch: special blocks first (their gathers are the longest chains of
@@ -694,6 +707,259 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(const StepArgs<T> s,
     }
 }
 
+// ---- Two steps a launch ----------------------------------------------------------------
+//
+// Steps k and k + 1 of a grid whose cells are interior, face specials or solid (no matched
+// layer, no general specials, one speed of sound, 7 points), in float. A block owns a box
+// of 64 x kPairY x kPairZ cells. It copies p(k) on the box and two cells round it into
+// LDS, by direct global-to-LDS loads (no registers, all in flight at once); steps the box
+// and one cell round it to p(k + 1) in LDS, with p(k - 1) and the classes read from memory
+// and the face specials of that region from the box's records; then steps the box itself
+// to p(k + 2), writing p(k + 1) and p(k + 2) of its cells and its own face specials' branch
+// velocities w(k + 1) and w(k + 2). p(k - 1), p(k), w(k - 1) and w(k) are read by every
+// block (the rings overlap) and written by none: the outputs go to the other two fields
+// and state arrays of four. Each cell's arithmetic is the one-step kernels', in their
+// order, so the fields are the same bit for bit. The same launch's probe blocks record
+// p(k - 1) and p(k), the fields it reads; the step after a run of such launches is
+// recorded by a launch of the probes alone.
+constexpr int kPairY = 8, kPairZ = 8;
+constexpr int kPX = kTileX + 4, kPY = kPairY + 4, kPZ = kPairZ + 4;  // p(k): two cells round
+constexpr int kQX = kTileX + 2, kQY = kPairY + 2, kQZ = kPairZ + 2;  // p(k + 1): one cell round
+constexpr int kPXY = kPX * kPY, kQXY = kQX * kQY;
+constexpr int kPCells = kPXY * kPZ;
+constexpr int kPAlloc = (kPCells + kTileX - 1) / kTileX * kTileX;  // whole wavefronts of loads
+constexpr int kQCells = kQXY * kQZ;
+constexpr int kOCells = kTileX * kPairY * kPairZ;
+constexpr int kQPer = (kQCells + kBlock - 1) / kBlock;
+constexpr int kOPer = (kOCells + kBlock - 1) / kBlock;
+static_assert(kQX < 256 && kQY < 256 && kQZ < 256, "a record's box coordinates are bytes");
+
+struct PairArgs {
+    const float *__restrict__ prev;  // p(k - 1)
+    const float *__restrict__ cur;   // p(k)
+    float *__restrict__ out1;        // p(k + 1)
+    float *__restrict__ out2;        // p(k + 2)
+    const float *__restrict__ wprev, *__restrict__ wcur;  // w(k - 1), w(k)
+    float *__restrict__ wout1, *__restrict__ wout2;       // w(k + 1), w(k + 2)
+    const float *__restrict__ zero;  // a float 0 in device memory, read for cells off the grid
+    const uint8_t *__restrict__ types;
+    const uint32_t *__restrict__ boxes;  // per box block: its box, x fastest, or kNoTile
+    uint32_t box_blocks, boxes_x, boxes_y;
+    const uint32_t *__restrict__ box_faces;  // per box block: first record, first not its own, end
+    // A record: its cell's coordinates in the p(k + 1) box (x | y << 8 | z << 16), then as
+    // a tile's face record: links | states << 6 | coefficients << 10, states' start, kinds.
+    const uint4 *__restrict__ pair_record;
+    const uint16_t *__restrict__ face_state_kind;
+    const float4 *__restrict__ face_coef;
+    const float4 *__restrict__ face_branch;
+    int ny, nz, px, sxy;
+    float axial, centre;
+    const float *__restrict__ signal;  // indexed by step
+    unsigned long long step;           // k: signal[k], signal[k + 1]; records out[k - 1], out[k]
+    int record_prev;                   // 0: leave out[k - 1] (a step before this run)
+    uint32_t probes;
+    const uint32_t *__restrict__ probe_start;
+    const uint32_t *__restrict__ probe_cell;
+    const double *__restrict__ probe_weight;
+    double *__restrict__ out;
+    unsigned long long out_stride;
+};
+
+// A face special's step (face_special's arithmetic, in its order) from its pressure, its
+// previous pressure, its six axial neighbours (-x, +x, -y, +y, -z, +z; those not linked
+// unread) and its states' velocities wa (the older) and wb; with wn, its states' new
+// velocities are written there.
+__device__ __forceinline__ float pair_face(const PairArgs &s, const uint4 rec, float here, float previous,
+                                           const float (&nb)[6], const float *__restrict__ wa,
+                                           const float *__restrict__ wb, float *__restrict__ wn, float sig) {
+    const uint32_t desc = rec.y, sbase = rec.z;
+    const uint32_t links = desc & 63u, ns = (desc >> 6) & 15u, ci_index = desc >> 10;
+    const float4 c = s.face_coef[ci_index];  // coef_a, coef_b, coef_inv, source
+    float sum = 0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        if (links & (1u << d)) sum += s.axial * nb[d];
+    }
+    constexpr int kChunk = 4;
+    float4 kind[kChunk];
+    float va[kChunk], vb[kChunk], own[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        if (static_cast<uint32_t>(u) < ns) {
+            uint32_t kid = (rec.w >> (8 * u)) & 0xFFu;
+            if (kid == 0xFFu) kid = s.face_state_kind[sbase + u];
+            kind[u] = s.face_branch[kid];  // now, before, gain, weight
+            va[u] = wa[sbase + u];
+            vb[u] = wb[sbase + u];
+        }
+    }
+    float flow = 0;
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        if (static_cast<uint32_t>(u) < ns) {
+            own[u] = kind[u].x * vb[u] - kind[u].y * va[u];
+            flow += kind[u].w * own[u];
+        }
+    }
+    for (uint32_t j = kChunk; j < ns; ++j) {
+        const uint32_t q = sbase + j;
+        const float4 k = s.face_branch[s.face_state_kind[q]];
+        const float term = k.x * wb[q] - k.y * wa[q];
+        flow += k.w * term;
+    }
+    float acc = c.x * here;
+    acc = acc + sum;
+    acc = acc - c.y * previous;
+    acc = acc - flow;
+    acc = acc + c.w * sig;
+    const float next = acc * c.z;
+    if (wn != nullptr) {
+        const float rise = next - previous;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            if (static_cast<uint32_t>(u) < ns) wn[sbase + u] = va[u] + (rise * kind[u].z + own[u]);
+        }
+        for (uint32_t j = kChunk; j < ns; ++j) {
+            const uint32_t q = sbase + j;
+            const float4 k = s.face_branch[s.face_state_kind[q]];
+            const float term = k.x * wb[q] - k.y * wa[q];
+            wn[q] = wa[q] + (rise * k.z + term);
+        }
+    }
+    return next;
+}
+
+__global__ __launch_bounds__(kBlock) void pair_kernel(const PairArgs s) {
+    __shared__ __align__(16) float P[kPAlloc];  // p(k) on the box and two cells round
+    __shared__ float Q[kQCells];                // p(k + 1) on the box and one cell round
+    __shared__ uint8_t C[kQCells];              // the classes of Q's cells (0 off the grid)
+    const uint32_t t = threadIdx.x;
+    if (blockIdx.x >= s.box_blocks) {
+        const uint32_t block = blockIdx.x - s.box_blocks;
+        double *products = reinterpret_cast<double *>(P);
+        if (s.record_prev) {
+            probe_sums<float>(s.probes, s.probe_start, s.probe_cell, s.probe_weight, s.prev, s.out, s.out_stride,
+                              s.step - 1, block, products);
+        }
+        probe_sums<float>(s.probes, s.probe_start, s.probe_cell, s.probe_weight, s.cur, s.out, s.out_stride, s.step,
+                          block, products);
+        return;
+    }
+    const uint32_t entry = s.boxes[blockIdx.x];
+    if (entry == kNoTile) return;
+    const int bx = static_cast<int>(entry % s.boxes_x);
+    const int rest = static_cast<int>(entry / s.boxes_x);
+    const int by = rest % static_cast<int>(s.boxes_y), bz = rest / static_cast<int>(s.boxes_y);
+    const int x0 = bx * kTileX, y0 = 1 + by * kPairY, z0 = 1 + bz * kPairZ;
+    const int px = s.px, sxy = s.sxy, ny = s.ny, nz = s.nz;
+    const int wave = static_cast<int>(t / kTileX), lane = static_cast<int>(t % kTileX);
+
+    // p(k) into P, a wavefront's 64 consecutive cells a load; cells off the grid read 0.
+    // (Cells past kPCells, in the last load's tail, land in P's padding.)
+    for (int f0 = wave * kTileX; f0 < kPAlloc; f0 += kBlock) {
+        const int f = f0 + lane;
+        const int ex = f % kPX, ey = (f / kPX) % kPY, ez = f / kPXY;
+        const int x = x0 - 2 + ex, y = y0 - 2 + ey, z = z0 - 2 + ez;
+        const bool in = f < kPCells && x >= 0 && x < px && y >= 0 && y < ny && z >= 0 && z < nz;
+        const float *src = in ? s.cur + (x + y * px + z * sxy) : s.zero;
+        __builtin_amdgcn_global_load_lds(src, P + f0, 4, 0, 0);
+    }
+    // Meanwhile, this thread's cells of the Q box: their classes and p(k - 1).
+    float pre[kQPer];
+    uint32_t cls[kQPer];
+#pragma unroll
+    for (int j = 0; j < kQPer; ++j) {
+        const int q = static_cast<int>(t) + j * kBlock;
+        pre[j] = 0.0f;
+        cls[j] = 0;
+        const int qx = q % kQX, qy = (q / kQX) % kQY, qz = q / kQXY;
+        const int x = x0 - 1 + qx, y = y0 - 1 + qy, z = z0 - 1 + qz;
+        if (q < kQCells && x >= 0 && x < px && y >= 0 && y < ny && z >= 0 && z < nz) {
+            const int g = x + y * px + z * sxy;
+            cls[j] = s.types[g];
+            pre[j] = s.prev[g];
+        }
+    }
+    // And this thread's first face record.
+    const uint32_t r0 = s.box_faces[3 * blockIdx.x] + t, r_own = s.box_faces[3 * blockIdx.x + 1],
+                   r_end = s.box_faces[3 * blockIdx.x + 2];
+    uint4 first = make_uint4(0u, 0u, 0u, 0u);
+    if (r0 < r_end) first = s.pair_record[r0];
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    // Step k on the Q box: interior cells, solid cells (0), then the face specials.
+#pragma unroll
+    for (int j = 0; j < kQPer; ++j) {
+        const int q = static_cast<int>(t) + j * kBlock;
+        if (q < kQCells) {
+            C[q] = static_cast<uint8_t>(cls[j]);
+            if (cls[j] == kClassInterior) {
+                const int qx = q % kQX, qy = (q / kQX) % kQY, qz = q / kQXY;
+                const int c = (qx + 1) + (qy + 1) * kPX + (qz + 1) * kPXY;
+                // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
+                float sum = P[c - 1] + P[c + 1];
+                sum = sum + P[c - kPX];
+                sum = sum + P[c + kPX];
+                sum = sum + P[c - kPXY];
+                sum = sum + P[c + kPXY];
+                const float next = s.centre * P[c] + s.axial * sum;
+                Q[q] = next - pre[j];
+            } else if (cls[j] != kClassFace) {
+                Q[q] = 0.0f;
+            }
+        }
+    }
+    const float sig0 = s.signal[s.step];
+    for (uint32_t r = r0; r < r_end; r += kBlock) {
+        const uint4 rec = r == r0 ? first : s.pair_record[r];
+        const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu),
+                  qz = static_cast<int>((rec.x >> 16) & 0xFFu);
+        const int g = (x0 - 1 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + qz) * sxy;
+        const int c = (qx + 1) + (qy + 1) * kPX + (qz + 1) * kPXY;
+        const float nb[6] = {P[c - 1], P[c + 1], P[c - kPX], P[c + kPX], P[c - kPXY], P[c + kPXY]};
+        const bool own = r < r_own;
+        const float next = pair_face(s, rec, P[c], s.prev[g], nb, s.wprev, s.wcur, own ? s.wout1 : nullptr, sig0);
+        Q[qx + qy * kQX + qz * kQXY] = next;
+        if (own) s.out1[g] = next;
+    }
+    __syncthreads();
+
+    // Step k + 1 on the box: its interior cells, then its own face specials (the same
+    // records, so the same threads, as above: each reads back the w(k + 1) it wrote).
+#pragma unroll
+    for (int j = 0; j < kOPer; ++j) {
+        const int o = static_cast<int>(t) + j * kBlock;
+        if (o < kOCells) {
+            const int ox = o % kTileX, oy = (o / kTileX) % kPairY, oz = o / (kTileX * kPairY);
+            const int q = (ox + 1) + (oy + 1) * kQX + (oz + 1) * kQXY;
+            if (C[q] == kClassInterior) {
+                const int c = (ox + 2) + (oy + 2) * kPX + (oz + 2) * kPXY;
+                float sum = Q[q - 1] + Q[q + 1];
+                sum = sum + Q[q - kQX];
+                sum = sum + Q[q + kQX];
+                sum = sum + Q[q - kQXY];
+                sum = sum + Q[q + kQXY];
+                const float next = s.centre * Q[q] + s.axial * sum;
+                const int g = (x0 + ox) + (y0 + oy) * px + (z0 + oz) * sxy;
+                s.out1[g] = Q[q];
+                s.out2[g] = next - P[c];
+            }
+        }
+    }
+    const float sig1 = s.signal[s.step + 1];
+    for (uint32_t r = r0; r < r_own; r += kBlock) {
+        const uint4 rec = r == r0 ? first : s.pair_record[r];
+        const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu),
+                  qz = static_cast<int>((rec.x >> 16) & 0xFFu);
+        const int g = (x0 - 1 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + qz) * sxy;
+        const int q = qx + qy * kQX + qz * kQXY;
+        const int c = (qx + 1) + (qy + 1) * kPX + (qz + 1) * kPXY;
+        const float nb[6] = {Q[q - 1], Q[q + 1], Q[q - kQX], Q[q + kQX], Q[q - kQXY], Q[q + kQXY]};
+        s.out2[g] = pair_face(s, rec, Q[q], P[c], nb, s.wcur, s.wout1, s.wout2, sig1);
+    }
+}
+
 thread_local std::string g_create_error;
 
 struct DeviceBuffer {
@@ -718,6 +984,15 @@ struct as_fdtd {
     std::vector<void *> allocations;
     // device arrays
     void *u0 = nullptr, *u1 = nullptr, *w0 = nullptr, *w1 = nullptr, *layer_state = nullptr;
+    // The fields and branch velocities by slot: 0 and 1 always, 2 and 3 for two steps a
+    // launch. p(k - 1) is in slot `prev` and p(k) in prev ^ 1 (the same for the velocities).
+    void *field[4] = {}, *velocity[4] = {};
+    int prev = 0;
+    int pairs = 0;  // 1: two steps a launch (see pair_kernel)
+    uint32_t box_blocks = 0, boxes_x = 0, boxes_y = 0;
+    uint32_t *boxes = nullptr, *box_faces = nullptr;
+    uint4 *pair_record = nullptr;
+    float *zero = nullptr;
     uint8_t *types = nullptr;
     float *scale = nullptr;
     uint32_t *tiles = nullptr;
@@ -764,7 +1039,8 @@ struct as_fdtd {
     size_t out_cap = 0;
     // graphs of graph_steps steps (see ensure_graphs)
     uint32_t graph_steps = 64;
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};  // by the slot of p(k - 1) at start
+    int graph_end[4] = {0, 0, 0, 0};                                        // and the slot after
     float *sig_stage = nullptr;
     double *out_stage = nullptr;
     std::string error;
@@ -830,6 +1106,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (tuning && tuning->tile_order > 0) h->tile_order = 1;
     if (tuning && tuning->no_faces > 0) h->no_faces = 1;
     if (tuning && tuning->face_blocks > 0) h->face_blocks = 1;
+    const bool pairs_on = tuning && tuning->pairs > 0;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
     const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);
     const uint64_t n_pad = px * ny * nz;
@@ -1078,6 +1355,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     };
     std::vector<uint32_t> face_tile(static_cast<size_t>(h->tile_count) + 1, 0), face_cell, face_desc, face_state, face_kinds;
     std::vector<uint16_t> face_state_kind;
+    std::vector<size_t> face_of(K, ~size_t{0});  // special -> its record
     uint64_t face_slots = state_slots;
     for (uint32_t blk = 0; blk < h->tile_count; ++blk) {
         face_tile[blk] = static_cast<uint32_t>(face_desc.size());
@@ -1100,6 +1378,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
                     face_cell.push_back(static_cast<uint32_t>(p));
                     face_desc.push_back(mask | static_cast<uint32_t>(states_of(k)) << 6 | ci << 10);
                     face_state.push_back(static_cast<uint32_t>(face_slots));
+                    face_of[k] = face_desc.size() - 1;
                     uint32_t packed = 0;
                     for (uint64_t j = L.state_start[k]; j < L.state_start[k + 1]; ++j) {
                         const int32_t br = L.state_branch[j];
@@ -1122,6 +1401,80 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     std::vector<uint16_t> kinds(face_slots, 0);
     std::copy(face_state_kind.begin(), face_state_kind.end(), kinds.begin() + static_cast<ptrdiff_t>(state_slots));
     h->state_slots = face_slots;
+
+    // Two steps a launch (pair_kernel): a grid of interior cells, face specials and solid
+    // cells, in float, rows of whole wavefronts.
+    h->pairs = pairs_on && precision == 32 && !h->wide && !h->scaled && L.layer_cells == 0 && G == 0 &&
+                       px % kTileX == 0 ? 1 : 0;
+    std::vector<uint32_t> boxes, box_faces;
+    std::vector<uint32_t> pair_records;
+    if (h->pairs) {
+        h->boxes_x = static_cast<uint32_t>(px / kTileX);
+        h->boxes_y = static_cast<uint32_t>((ny - 2 + kPairY - 1) / kPairY);
+        const uint32_t boxes_z = static_cast<uint32_t>((nz - 2 + kPairZ - 1) / kPairZ);
+        const uint64_t box_total = static_cast<uint64_t>(h->boxes_x) * h->boxes_y * boxes_z;
+        if (box_total >= kNoTile) refuse("too many boxes");
+        // Boxes with a cell to step.
+        std::vector<uint8_t> active(box_total, 0);
+        for (uint64_t z = 1; z + 1 < nz; ++z)
+            for (uint64_t y = 1; y + 1 < ny; ++y)
+                for (uint64_t x = 0; x < nx; ++x) {
+                    const uint8_t c = types[x + px * (y + ny * z)];
+                    if (c == kClassInterior || c == kClassFace) {
+                        active[x / kTileX + h->boxes_x * ((y - 1) / kPairY + h->boxes_y * ((z - 1) / kPairZ))] = 1;
+                    }
+                }
+        std::vector<uint32_t> list;
+        for (uint64_t b = 0; b < box_total; ++b)
+            if (active[b]) list.push_back(static_cast<uint32_t>(b));
+        // By die, as the tiles.
+        const size_t per = (list.size() + kXcds - 1) / kXcds;
+        boxes.assign(per * kXcds, kNoTile);
+        if (h->tile_order == 0) {
+            for (size_t r = 0; r < boxes.size(); ++r) {
+                const size_t idx = (r % kXcds) * per + r / kXcds;
+                if (idx < list.size()) boxes[r] = list[idx];
+            }
+        } else {
+            std::copy(list.begin(), list.end(), boxes.begin());
+        }
+        std::vector<uint32_t> block_of(box_total, kNoTile);
+        for (size_t r = 0; r < boxes.size(); ++r)
+            if (boxes[r] != kNoTile) block_of[boxes[r]] = static_cast<uint32_t>(r);
+        // Each face special goes to every box whose ring of one cell holds it: its own
+        // box's records first, then the others'.
+        std::vector<std::vector<std::array<uint32_t, 4>>> own(boxes.size()), ring(boxes.size());
+        for (const auto &bc : by_cell) {
+            const uint64_t cell = bc.first, k = bc.second;
+            const size_t r = face_of[k];
+            const int64_t x = static_cast<int64_t>(cell % px), y = static_cast<int64_t>((cell / px) % ny),
+                          z = static_cast<int64_t>(cell / (px * ny));
+            for (int64_t bz = (z - 2) / kPairZ; bz <= z / kPairZ; ++bz)
+                for (int64_t by = (y - 2) / kPairY; by <= y / kPairY; ++by)
+                    for (int64_t bx = (x - 1) / kTileX; bx <= (x + 1) / kTileX; ++bx) {
+                        if (bx < 0 || by < 0 || bz < 0 || bx >= h->boxes_x || by >= h->boxes_y || bz >= boxes_z) continue;
+                        const int64_t x0 = bx * kTileX, y0 = 1 + by * kPairY, z0 = 1 + bz * kPairZ;
+                        const int64_t qx = x - (x0 - 1), qy = y - (y0 - 1), qz = z - (z0 - 1);
+                        if (qx < 0 || qx >= kQX || qy < 0 || qy >= kQY || qz < 0 || qz >= kQZ) continue;
+                        const uint32_t blk = block_of[static_cast<uint64_t>(bx) + h->boxes_x * (static_cast<uint64_t>(by) + h->boxes_y * static_cast<uint64_t>(bz))];
+                        if (blk == kNoTile) continue;
+                        const std::array<uint32_t, 4> rec{static_cast<uint32_t>(qx | qy << 8 | qz << 16), face_desc[r],
+                                                          face_state[r], face_kinds[r]};
+                        const bool mine = qx >= 1 && qx <= kTileX && qy >= 1 && qy <= kPairY && qz >= 1 && qz <= kPairZ;
+                        (mine ? own : ring)[blk].push_back(rec);
+                    }
+        }
+        box_faces.assign(3 * boxes.size(), 0);
+        for (size_t b = 0; b < boxes.size(); ++b) {
+            box_faces[3 * b] = static_cast<uint32_t>(pair_records.size() / 4);
+            for (const auto &rec : own[b]) pair_records.insert(pair_records.end(), rec.begin(), rec.end());
+            box_faces[3 * b + 1] = static_cast<uint32_t>(pair_records.size() / 4);
+            for (const auto &rec : ring[b]) pair_records.insert(pair_records.end(), rec.begin(), rec.end());
+            box_faces[3 * b + 2] = static_cast<uint32_t>(pair_records.size() / 4);
+            if (pair_records.size() / 4 >= (1ull << 31)) throw Failure{AS_ERANGE, "too many face records"};
+        }
+        h->box_blocks = static_cast<uint32_t>(boxes.size());
+    }
 
     // Matched layer.
     std::vector<uint32_t> layer_cell(L.layer_cells);
@@ -1208,6 +1561,20 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->u1 = zeros(h, n_pad * e);
     h->w0 = zeros(h, face_slots * e);
     h->w1 = zeros(h, face_slots * e);
+    h->field[0] = h->u0;
+    h->field[1] = h->u1;
+    h->velocity[0] = h->w0;
+    h->velocity[1] = h->w1;
+    if (h->pairs) {
+        for (int k = 2; k < 4; ++k) {
+            h->field[k] = zeros(h, n_pad * e);
+            h->velocity[k] = zeros(h, face_slots * e);
+        }
+        h->boxes = upload(h, boxes);
+        h->box_faces = upload(h, box_faces);
+        h->pair_record = reinterpret_cast<uint4 *>(upload(h, pair_records));
+        h->zero = static_cast<float *>(zeros(h, sizeof(float)));
+    }
     h->layer_state = zeros(h, 9 * L.layer_cells * e);
 }
 
@@ -1274,16 +1641,14 @@ StepArgs<T> step_args(const as_fdtd *h) {
     return s;
 }
 
+// One step from the fields in slots prev (p(k - 1), overwritten by p(k + 1)) and prev ^ 1
+// (p(k)): the reference's swaps (fdtd.hpp:883-885, 900-903) are the slots' exchange.
 template <typename T>
-void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, unsigned long long index, unsigned long long k,
-                hipStream_t stream) {
-    T *u0 = static_cast<T *>(h->u0), *u1 = static_cast<T *>(h->u1);
-    T *w0 = static_cast<T *>(h->w0), *w1 = static_cast<T *>(h->w1);
-    const bool even = (index & 1ull) == 0;  // the reference's swaps (fdtd.hpp:883-885, 900-903)
-    s.a = even ? u0 : u1;
-    s.b = even ? u1 : u0;
-    s.wa = even ? w0 : w1;
-    s.wb = even ? w1 : w0;
+void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned long long k, hipStream_t stream) {
+    s.a = static_cast<T *>(h->field[prev]);
+    s.b = static_cast<T *>(h->field[prev ^ 1]);
+    s.wa = static_cast<T *>(h->velocity[prev]);
+    s.wb = static_cast<T *>(h->velocity[prev ^ 1]);
     s.step = k;
     if (h->split) {
         if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
@@ -1323,31 +1688,103 @@ uint32_t block_count(const as_fdtd *h, const StepArgs<T> &s) {
     return s.tile_blocks + s.special_blocks + s.face_blocks + s.layer_blocks + probe_blocks;
 }
 
-// Two graphs of graph_steps steps each, one starting on an even step and one on an odd, on
-// staging buffers for the drive and the probes' output: a run replays them, copying its
-// drive in and the probes out between replays, so the host launches one graph where it
-// would launch graph_steps kernels.
+// The two-step launch's arguments, its probes writing where the one-step launches' do.
+PairArgs pair_args(const as_fdtd *h, const StepArgs<float> &s) {
+    PairArgs a{};
+    a.zero = h->zero;
+    a.types = h->types;
+    a.boxes = h->boxes;
+    a.box_blocks = h->box_blocks;
+    a.boxes_x = h->boxes_x;
+    a.boxes_y = h->boxes_y;
+    a.box_faces = h->box_faces;
+    a.pair_record = h->pair_record;
+    a.face_state_kind = h->face_state_kind;
+    a.face_coef = h->face_coef;
+    a.face_branch = h->face_branch;
+    a.ny = h->ny;
+    a.nz = h->nz;
+    a.px = h->px;
+    a.sxy = static_cast<int>(h->sxy);
+    a.axial = static_cast<float>(h->axial);
+    a.centre = static_cast<float>(h->centre);
+    a.signal = s.signal;
+    a.probes = s.probes;
+    a.probe_start = s.probe_start;
+    a.probe_cell = s.probe_cell;
+    a.probe_weight = s.probe_weight;
+    a.out = s.out;
+    a.out_stride = s.out_stride;
+    return a;
+}
+
+// Steps k0 to k1 - 1 from slot `prev`, by two-step launches where the handle has them
+// (their probes record each launch's p(k - 1) but the first's, and a launch of the probes
+// alone the last p), the rest one step a launch. Returns the slot of p(k1 - 1).
 template <typename T>
-void ensure_graphs(as_fdtd *h) {
-    if (h->graph_exec[0] != nullptr) return;
-    const uint32_t G = h->graph_steps;
-    check(hipMalloc(reinterpret_cast<void **>(&h->sig_stage), G * sizeof(float)), "hipMalloc");
-    check(hipMemset(h->sig_stage, 0, G * sizeof(float)), "hipMemset");
-    const size_t out_len = static_cast<size_t>(std::max<uint32_t>(h->probes, 1)) * G;
-    check(hipMalloc(reinterpret_cast<void **>(&h->out_stage), out_len * sizeof(double)), "hipMalloc");
-    for (int parity = 0; parity < 2; ++parity) {
-        StepArgs<T> s = step_args<T>(h);
-        s.signal = h->sig_stage;
-        s.out = h->out_stage;
-        s.out_stride = G;
-        const uint32_t blocks = block_count(h, s);
-        hipGraph_t graph = nullptr;
-        check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        for (uint32_t k = 0; k < G; ++k) launch_one<T>(h, s, blocks, static_cast<unsigned long long>(parity) + k, k, h->stream);
-        check(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
-        check(hipGraphInstantiate(&h->graph_exec[parity], graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        check(hipGraphDestroy(graph), "hipGraphDestroy");
+int enqueue_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t k0, size_t k1, hipStream_t stream) {
+    size_t k = k0;
+    if constexpr (std::is_same<T, float>::value) {
+        if (h->pairs && k + 2 <= k1) {
+            PairArgs a = pair_args(h, s);
+            const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
+            for (; k + 2 <= k1; k += 2) {
+                const int next = prev < 2 ? 2 : 0;  // p(k + 1), then p(k + 2) in next ^ 1
+                a.prev = static_cast<const float *>(h->field[prev]);
+                a.cur = static_cast<const float *>(h->field[prev ^ 1]);
+                a.out1 = static_cast<float *>(h->field[next]);
+                a.out2 = static_cast<float *>(h->field[next ^ 1]);
+                a.wprev = static_cast<const float *>(h->velocity[prev]);
+                a.wcur = static_cast<const float *>(h->velocity[prev ^ 1]);
+                a.wout1 = static_cast<float *>(h->velocity[next]);
+                a.wout2 = static_cast<float *>(h->velocity[next ^ 1]);
+                a.step = k;
+                a.record_prev = k > k0 ? 1 : 0;
+                if (h->box_blocks + probe_blocks) {
+                    hipLaunchKernelGGL(pair_kernel, dim3(h->box_blocks + probe_blocks), dim3(kBlock), 0, stream, a);
+                }
+                prev = next;
+            }
+            if (probe_blocks) {
+                // p(k - 1) is the last launch's p(k + 1), in slot prev.
+                s.b = static_cast<const T *>(h->field[prev]);
+                s.step = k - 1;
+                hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
+            }
+        }
     }
+    for (; k < k1; ++k) {
+        launch_one<T>(h, s, blocks, prev, k, stream);
+        prev ^= 1;
+    }
+    return prev;
+}
+
+// Graphs of graph_steps steps, one for each slot the run may start from (built at need),
+// on staging buffers for the drive and the probes' output: a run replays them, copying
+// its drive in and the probes out between replays, so the host launches one graph where
+// it would launch graph_steps kernels.
+template <typename T>
+void ensure_graph(as_fdtd *h, int prev) {
+    if (h->graph_exec[prev] != nullptr) return;
+    const uint32_t G = h->graph_steps;
+    if (h->sig_stage == nullptr) {
+        check(hipMalloc(reinterpret_cast<void **>(&h->sig_stage), G * sizeof(float)), "hipMalloc");
+        check(hipMemset(h->sig_stage, 0, G * sizeof(float)), "hipMemset");
+        const size_t out_len = static_cast<size_t>(std::max<uint32_t>(h->probes, 1)) * G;
+        check(hipMalloc(reinterpret_cast<void **>(&h->out_stage), out_len * sizeof(double)), "hipMalloc");
+    }
+    StepArgs<T> s = step_args<T>(h);
+    s.signal = h->sig_stage;
+    s.out = h->out_stage;
+    s.out_stride = G;
+    const uint32_t blocks = block_count(h, s);
+    hipGraph_t graph = nullptr;
+    check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    h->graph_end[prev] = enqueue_steps<T>(h, s, blocks, prev, 0, G, h->stream);
+    check(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&h->graph_exec[prev], graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    check(hipGraphDestroy(graph), "hipGraphDestroy");
 }
 
 template <typename T>
@@ -1356,13 +1793,15 @@ void launch_steps(as_fdtd *h, const float *d_signal, size_t steps, double *d_out
     const uint32_t blocks = block_count(h, s);
     if (blocks == 0) return;
     size_t k = 0;
+    int prev = h->prev;
     const uint32_t G = h->graph_steps;
     if (G > 0 && steps >= G) {
-        ensure_graphs<T>(h);
         for (; k + G <= steps; k += G) {
+            ensure_graph<T>(h, prev);
             check(hipMemcpyAsync(h->sig_stage, d_signal + k, G * sizeof(float), hipMemcpyDeviceToDevice, h->stream),
                   "hipMemcpyAsync");
-            check(hipGraphLaunch(h->graph_exec[(h->done + k) & 1ull], h->stream), "hipGraphLaunch");
+            check(hipGraphLaunch(h->graph_exec[prev], h->stream), "hipGraphLaunch");
+            prev = h->graph_end[prev];
             if (h->probes > 0) {
                 check(hipMemcpy2DAsync(d_out + k, steps * sizeof(double), h->out_stage, G * sizeof(double),
                                        G * sizeof(double), h->probes, hipMemcpyDeviceToDevice, h->stream),
@@ -1373,8 +1812,9 @@ void launch_steps(as_fdtd *h, const float *d_signal, size_t steps, double *d_out
     s.signal = d_signal;
     s.out = d_out;
     s.out_stride = steps;
-    for (; k < steps; ++k) launch_one<T>(h, s, blocks, h->done + k, k, h->stream);
+    prev = enqueue_steps<T>(h, s, blocks, prev, k, steps, h->stream);
     check(hipGetLastError(), "step launch");
+    h->prev = prev;
     h->done += steps;
 }
 
@@ -1479,14 +1919,17 @@ int as_fdtd_reset(as_fdtd *h) {
     if (h == nullptr) return AS_EINVAL;
     try {
         const size_t e = h->precision == 64 ? sizeof(double) : sizeof(float);
-        check(hipMemsetAsync(h->u0, 0, h->n_pad * e, h->stream), "hipMemsetAsync");
-        check(hipMemsetAsync(h->u1, 0, h->n_pad * e, h->stream), "hipMemsetAsync");
-        check(hipMemsetAsync(h->w0, 0, std::max<size_t>(h->state_slots * e, 256), h->stream), "hipMemsetAsync");
-        check(hipMemsetAsync(h->w1, 0, std::max<size_t>(h->state_slots * e, 256), h->stream), "hipMemsetAsync");
+        for (int k = 0; k < 4; ++k) {
+            if (h->field[k] == nullptr) continue;
+            check(hipMemsetAsync(h->field[k], 0, h->n_pad * e, h->stream), "hipMemsetAsync");
+            check(hipMemsetAsync(h->velocity[k], 0, std::max<size_t>(h->state_slots * e, 256), h->stream),
+                  "hipMemsetAsync");
+        }
         check(hipMemsetAsync(h->layer_state, 0, std::max<size_t>(9ull * h->layer_cells * e, 256), h->stream),
               "hipMemsetAsync");
         check(hipStreamSynchronize(h->stream), "reset");
         h->done = 0;
+        h->prev = 0;
         return AS_OK;
     } catch (const Failure &f) {
         return fail(h, f);
@@ -1496,8 +1939,7 @@ int as_fdtd_reset(as_fdtd *h) {
 int as_fdtd_field(as_fdtd *h, double *out) {
     if (h == nullptr || out == nullptr) return AS_EINVAL;
     try {
-        const bool even = (h->done & 1ull) == 0;
-        const void *b = even ? h->u1 : h->u0;
+        const void *b = h->field[h->prev ^ 1];
         const size_t e = h->precision == 64 ? sizeof(double) : sizeof(float);
         std::vector<unsigned char> host(h->n_pad * e);
         check(hipMemcpyAsync(host.data(), b, host.size(), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
@@ -1631,7 +2073,7 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
     for (size_t seg = 0; seg < segments; ++seg) {
         save(seg);
         const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
-        for (size_t k = k0; k < k1; ++k) launch_one<T>(h, s, blocks, k, k, h->stream);
+        for (size_t k = k0; k < k1; ++k) launch_one<T>(h, s, blocks, static_cast<int>(k & 1u), k, h->stream);
     }
     check(hipGetLastError(), "forward");
     if (out != nullptr && P > 0) {
@@ -1657,7 +2099,7 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
         restore(seg);
         for (size_t k = k0; k < k1; ++k) {
             r.signal = d_signal + k0;
-            launch_one<T>(h, r, rblocks, k, k - k0, h->stream);
+            launch_one<T>(h, r, rblocks, static_cast<int>(k & 1u), k - k0, h->stream);
         }
         // Adjoint steps n from min(k1, steps - 1) down to k0 + 1 (Q(steps) is 0: p(steps) is
         // read by no probe; Q(0) enters no gradient).

@@ -94,6 +94,9 @@ typedef struct as_tuning {
     int no_faces;      /* 1: step every special cell by its own blocks (as the adjoint needs) */
     int face_blocks;   /* 1: step the face specials (axial links of the interior's weight) by
                           blocks of their own instead of in their tiles */
+    int pairs;         /* 1: two steps a launch where that applies (precision 32, 7 points, one
+                          speed of sound, no matched layer, every special a face special, row
+                          pitch a multiple of 64); 0: one step a launch */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the

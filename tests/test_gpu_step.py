@@ -73,22 +73,44 @@ def test_launch_shape_does_not_change_results(reference_available, tile_z, row_a
     assert np.array_equal(gpu.run(scene.drive[:200]), cpu.run(scene.drive[:200]))
 
 
-@pytest.mark.parametrize("face_blocks,split", [(False, True), (True, False), (True, True)])
-def test_kernel_variants_equal_the_oracle(reference_available, face_blocks, split):
-    """Face specials in their tiles or by their own blocks, one launch a step or four: the
-    same fields, on a room with branch states and a scene with a matched layer."""
+@pytest.mark.parametrize("pairs,face_blocks,split", [(False, False, False), (False, False, True),
+                                                     (False, True, False), (False, True, True), (True, True, True)])
+def test_kernel_variants_equal_the_oracle(reference_available, pairs, face_blocks, split):
+    """Two steps a launch or one; face specials in their tiles or by their own blocks; one
+    launch a step or four: the same fields, on a room with branch states and a scene with a
+    matched layer (which keeps to one step a launch)."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
     for scene in (cases.with_branches(cases.small_box()), cases.open_block()):
         layout = _layout(scene)
-        print(f"variant face_blocks={face_blocks} split={split} shape {scene.grid.shape}", flush=True)
-        gpu = HipWaveSolver(layout, face_blocks=face_blocks, split=split)
+        print(f"variant pairs={pairs} face_blocks={face_blocks} split={split} shape {scene.grid.shape}", flush=True)
+        gpu = HipWaveSolver(layout, pairs=pairs, face_blocks=face_blocks, split=split)
         cpu = OracleSolver(layout)
         d = scene.drive[:230]
         assert np.array_equal(gpu.run(d[:117]), cpu.run(d[:117]))
         assert np.array_equal(gpu.run(d[117:]), cpu.run(d[117:]))
         assert np.array_equal(gpu.field(), cpu.field())
+
+
+@pytest.mark.parametrize("pieces", [(1, 2, 63, 64, 65, 1, 130), (200,), (3, 3, 3)])
+def test_two_steps_a_launch_in_any_pieces(reference_available, pieces):
+    """Runs of one step, two, odd and even counts, shorter and longer than a graph: the
+    probes of every step and the fields match the oracle whatever the split into calls."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    scene = cases.with_branches(cases.small_box())
+    layout = _layout(scene)
+    print(f"pairs pieces {pieces}", flush=True)
+    gpu = HipWaveSolver(layout, pairs=True)
+    cpu = OracleSolver(layout)
+    start = 0
+    for n in pieces:
+        d = scene.drive[start:start + n]
+        assert np.array_equal(gpu.run(d), cpu.run(d)), (start, n)
+        start += n
+    assert np.array_equal(gpu.field(), cpu.field())
 
 
 def test_shoebox_full_grid(reference_available):
@@ -108,9 +130,11 @@ def test_shoebox_full_grid(reference_available):
     assert np.array_equal(gpu.run(d[201:]), cpu.run(d[201:]))
 
 
-def test_hall_steps_equal_the_oracle(reference_available):
+@pytest.mark.parametrize("pairs", [False, True])
+def test_hall_steps_equal_the_oracle(reference_available, pairs):
     """The hall (7.57M air cells; up to nine branch states on a cell where three walls
-    meet), laid out by this package's own set-up, 150 steps in two calls."""
+    meet), laid out by this package's own set-up, 150 steps in two calls, one or two steps
+    a launch."""
     from acoustic_spaces_amd import scenes
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from acoustic_spaces_amd.solver import WaveSolver
@@ -118,7 +142,8 @@ def test_hall_steps_equal_the_oracle(reference_available):
 
     scene = scenes.hall()[0]
     layout = WaveSolver.from_scene(scene).layout()
-    gpu = HipWaveSolver(layout)
+    print(f"hall pairs={pairs}", flush=True)
+    gpu = HipWaveSolver(layout, pairs=pairs)
     cpu = OracleSolver(layout)
     # Start where the pulse has reached the source's cells' neighbours: a drive delayed so
     # the first steps already carry it.
