@@ -1014,6 +1014,11 @@ struct as_fdtd {
     int face_blocks = 0;      // 1: face specials by blocks of their own, not in their tiles
     uint32_t *face_tile_none = nullptr;  // a face_tile of empty ranges, for face_blocks
     // The adjoint (see as_fdtd_gradient): its fields, accumulators, taps and checkpoints.
+    // The forward run it goes back through (as_fdtd_record): the drive on the device and
+    // per segment of rec_every steps the state before it (p(k - 1), p(k), w(k - 1), w(k)).
+    size_t rec_steps = 0, rec_every = 0;
+    char *rec_ckpt = nullptr;
+    float *rec_signal = nullptr;
     void *q0 = nullptr, *q1 = nullptr, *v0 = nullptr, *v1 = nullptr, *history = nullptr;
     double *acc_g = nullptr, *acc_gain = nullptr;
     uint32_t taps_unique = 0;
@@ -1788,7 +1793,7 @@ void ensure_graph(as_fdtd *h, int prev) {
 }
 
 template <typename T>
-void launch_steps(as_fdtd *h, const float *d_signal, size_t steps, double *d_out) {
+void launch_steps(as_fdtd *h, const float *d_signal, size_t steps, double *d_out, size_t out_stride) {
     StepArgs<T> s = step_args<T>(h);
     const uint32_t blocks = block_count(h, s);
     if (blocks == 0) return;
@@ -1803,7 +1808,7 @@ void launch_steps(as_fdtd *h, const float *d_signal, size_t steps, double *d_out
             check(hipGraphLaunch(h->graph_exec[prev], h->stream), "hipGraphLaunch");
             prev = h->graph_end[prev];
             if (h->probes > 0) {
-                check(hipMemcpy2DAsync(d_out + k, steps * sizeof(double), h->out_stage, G * sizeof(double),
+                check(hipMemcpy2DAsync(d_out + k, out_stride * sizeof(double), h->out_stage, G * sizeof(double),
                                        G * sizeof(double), h->probes, hipMemcpyDeviceToDevice, h->stream),
                       "hipMemcpy2DAsync");
             }
@@ -1811,7 +1816,7 @@ void launch_steps(as_fdtd *h, const float *d_signal, size_t steps, double *d_out
     }
     s.signal = d_signal;
     s.out = d_out;
-    s.out_stride = steps;
+    s.out_stride = out_stride;
     prev = enqueue_steps<T>(h, s, blocks, prev, k, steps, h->stream);
     check(hipGetLastError(), "step launch");
     h->prev = prev;
@@ -1857,9 +1862,9 @@ int as_fdtd_run_device(as_fdtd *h, const float *d_signal, size_t steps, double *
     try {
         if (steps == 0) return AS_OK;
         if (h->precision == 64) {
-            launch_steps<double>(h, d_signal, steps, d_out);
+            launch_steps<double>(h, d_signal, steps, d_out, steps);
         } else {
-            launch_steps<float>(h, d_signal, steps, d_out);
+            launch_steps<float>(h, d_signal, steps, d_out, steps);
         }
         return AS_OK;
     } catch (const Failure &f) {
@@ -1893,9 +1898,9 @@ int as_fdtd_run(as_fdtd *h, const float *signal, size_t steps, double *out) {
             check(hipMemcpyAsync(h->signal_buf, signal + done, m * sizeof(float), hipMemcpyHostToDevice, h->stream),
                   "hipMemcpyAsync");
             if (h->precision == 64) {
-                launch_steps<double>(h, h->signal_buf, m, h->out_buf);
+                launch_steps<double>(h, h->signal_buf, m, h->out_buf, m);
             } else {
-                launch_steps<float>(h, h->signal_buf, m, h->out_buf);
+                launch_steps<float>(h, h->signal_buf, m, h->out_buf, m);
             }
             chunk.resize(static_cast<size_t>(h->probes) * m);
             if (!chunk.empty()) {
@@ -1967,13 +1972,72 @@ int as_fdtd_field(as_fdtd *h, double *out) {
 
 namespace {
 
-template <typename T>
-void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout, double *out,
-              double *sens_g, double *sens_gain) {
+void free_record(as_fdtd *h) {
+    if (h->rec_ckpt) (void)hipFree(h->rec_ckpt);
+    if (h->rec_signal) (void)hipFree(h->rec_signal);
+    h->rec_ckpt = nullptr;
+    h->rec_signal = nullptr;
+    h->rec_steps = h->rec_every = 0;
+}
+
+void check_adjoint(const as_fdtd *h) {
     if (h->layer_cells > 0) refuse("the adjoint needs a closed grid: no matched layer");
     if (h->scaled) refuse("the adjoint needs one speed of sound for the whole grid");
     if (h->face_specials > 0) refuse("the adjoint needs a handle made with no_faces set");
+}
+
+// The forward half of the gradient: from rest, `steps` steps, the state saved before each
+// segment of `every` steps, the probes into out (host, probes x steps) if given.
+template <typename T>
+void record(as_fdtd *h, const float *signal, size_t steps, size_t every, double *out) {
+    check_adjoint(h);
     if (every == 0) refuse("checkpoints every 0 steps");
+    free_record(h);
+    const size_t e = sizeof(T), P = h->probes;
+    const size_t field = h->n_pad * e, states = h->state_slots * e;
+    const size_t segments = (steps + every - 1) / every;
+    const size_t per = 2 * field + 2 * states;
+    check(hipMalloc(reinterpret_cast<void **>(&h->rec_ckpt), std::max<size_t>(segments * per, 256)), "hipMalloc");
+    check(hipMalloc(reinterpret_cast<void **>(&h->rec_signal), steps * sizeof(float)), "hipMalloc");
+    double *d_out = nullptr;
+    check(hipMalloc(reinterpret_cast<void **>(&d_out), std::max<size_t>(P, 1) * steps * sizeof(double)), "hipMalloc");
+    struct Free {
+        double *p;
+        ~Free() { (void)hipFree(p); }
+    } release{d_out};
+    check(hipMemcpyAsync(h->rec_signal, signal, steps * sizeof(float), hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync");
+    const int code = as_fdtd_reset(h);
+    if (code != AS_OK) throw Failure{code, h->error};
+    for (size_t seg = 0; seg < segments; ++seg) {
+        char *base = h->rec_ckpt + seg * per;
+        // By role: p(k - 1), p(k), w(k - 1), w(k).
+        check(hipMemcpyAsync(base, h->field[h->prev], field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        check(hipMemcpyAsync(base + field, h->field[h->prev ^ 1], field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        if (states) {
+            check(hipMemcpyAsync(base + 2 * field, h->velocity[h->prev], states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+            check(hipMemcpyAsync(base + 2 * field + states, h->velocity[h->prev ^ 1], states, hipMemcpyDeviceToDevice,
+                                 h->stream),
+                  "ckpt");
+        }
+        const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
+        launch_steps<T>(h, h->rec_signal + k0, k1 - k0, d_out + k0, steps);
+    }
+    if (out != nullptr && P > 0) {
+        check(hipMemcpyAsync(out, d_out, P * steps * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
+    }
+    check(hipStreamSynchronize(h->stream), "record");
+    h->rec_steps = steps;
+    h->rec_every = every;
+}
+
+// The backward half: segment by segment from the last, the segment recomputed from its
+// checkpoint (keeping the specials' p(n + 1) - p(n - 1)), then the adjoint stepped back
+// through it; the sums of the gradient, into sens_g and sens_gain.
+template <typename T>
+void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gain) {
+    check_adjoint(h);
+    if (h->rec_ckpt == nullptr) refuse("no forward run recorded (as_fdtd_record)");
+    const size_t steps = h->rec_steps, every = h->rec_every;
     const size_t e = sizeof(T);
     const size_t G = h->specials, slots = h->state_slots, P = h->probes;
     const size_t segments = (steps + every - 1) / every;
@@ -1991,18 +2055,13 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
             for (void *p : v) (void)hipFree(p);
         }
     } release{scratch};
-    // Checkpoints: the state before each segment's first step.
-    const size_t field = h->n_pad * e, states = slots * e;
-    char *ckpt = static_cast<char *>(alloc(segments * (2 * field + 2 * states)));
+    const size_t field = h->n_pad * e, states = slots * e, per = 2 * field + 2 * states;
     T *history = static_cast<T *>(alloc(every * std::max<size_t>(G, 1) * e));
     T *q0 = static_cast<T *>(alloc(field)), *q1 = static_cast<T *>(alloc(field));
     T *v0 = static_cast<T *>(alloc(states)), *v1 = static_cast<T *>(alloc(states));
     double *acc_g = static_cast<double *>(alloc(std::max<size_t>(G, 1) * sizeof(double)));
     double *acc_gain = static_cast<double *>(alloc(std::max<size_t>(slots, 1) * sizeof(double)));
-    float *d_signal = static_cast<float *>(alloc(steps * sizeof(float)));
-    double *d_out = static_cast<double *>(alloc(std::max<size_t>(P, 1) * steps * sizeof(double)));
     double *d_g = static_cast<double *>(alloc(std::max<size_t>(P, 1) * steps * sizeof(double)));
-    check(hipMemcpyAsync(d_signal, signal, steps * sizeof(float), hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync");
     if (P > 0) {
         check(hipMemcpyAsync(d_g, dl_dout, P * steps * sizeof(double), hipMemcpyHostToDevice, h->stream),
               "hipMemcpyAsync");
@@ -2041,51 +2100,8 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
     double *d_tweight = put(tweight);
     double *d_tci = put(tci);
 
-    // The forward run, saving the state before each segment.
-    StepArgs<T> s = step_args<T>(h);
-    const uint32_t blocks = block_count(h, s);
-    T *u[2] = {static_cast<T *>(h->u0), static_cast<T *>(h->u1)};
-    T *w[2] = {static_cast<T *>(h->w0), static_cast<T *>(h->w1)};
-    auto save = [&](size_t seg) {
-        char *base = ckpt + seg * (2 * field + 2 * states);
-        check(hipMemcpyAsync(base, u[0], field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        check(hipMemcpyAsync(base + field, u[1], field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        if (states) {
-            check(hipMemcpyAsync(base + 2 * field, w[0], states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-            check(hipMemcpyAsync(base + 2 * field + states, w[1], states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        }
-    };
-    auto restore = [&](size_t seg) {
-        char *base = ckpt + seg * (2 * field + 2 * states);
-        check(hipMemcpyAsync(u[0], base, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        check(hipMemcpyAsync(u[1], base + field, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        if (states) {
-            check(hipMemcpyAsync(w[0], base + 2 * field, states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-            check(hipMemcpyAsync(w[1], base + 2 * field + states, states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        }
-    };
-    // The run starts from rest.
-    for (T *f : {u[0], u[1]}) check(hipMemsetAsync(f, 0, field, h->stream), "hipMemsetAsync");
-    for (T *f : {w[0], w[1]}) check(hipMemsetAsync(f, 0, std::max<size_t>(states, 256), h->stream), "hipMemsetAsync");
-    s.signal = d_signal;
-    s.out = d_out;
-    s.out_stride = steps;
-    for (size_t seg = 0; seg < segments; ++seg) {
-        save(seg);
-        const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
-        for (size_t k = k0; k < k1; ++k) launch_one<T>(h, s, blocks, static_cast<int>(k & 1u), k, h->stream);
-    }
-    check(hipGetLastError(), "forward");
-    if (out != nullptr && P > 0) {
-        check(hipMemcpyAsync(out, d_out, P * steps * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
-    }
-
-    // Backwards, segment by segment: recompute the segment from its checkpoint, keeping the
-    // specials' p(n+1) - p(n-1), then the adjoint steps n = (its last step + 1) down to its
-    // first step + 1.
     StepArgs<T> r = step_args<T>(h);  // recompute: no probes, history on
     r.probes = 0;
-    r.signal = d_signal;
     r.history = history;
     const uint32_t rblocks = block_count(h, r);
     StepArgs<T> a = step_args<T>(h);  // adjoint
@@ -2096,11 +2112,17 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
     unsigned long long m = 0;  // adjoint steps done: their parity says which field is which
     for (size_t seg = segments; seg-- > 0;) {
         const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
-        restore(seg);
-        for (size_t k = k0; k < k1; ++k) {
-            r.signal = d_signal + k0;
-            launch_one<T>(h, r, rblocks, static_cast<int>(k & 1u), k - k0, h->stream);
+        // The checkpoint into slots 0 (p(k0 - 1)) and 1 (p(k0)).
+        const char *base = h->rec_ckpt + seg * per;
+        check(hipMemcpyAsync(h->field[0], base, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        check(hipMemcpyAsync(h->field[1], base + field, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+        if (states) {
+            check(hipMemcpyAsync(h->velocity[0], base + 2 * field, states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+            check(hipMemcpyAsync(h->velocity[1], base + 2 * field + states, states, hipMemcpyDeviceToDevice, h->stream),
+                  "ckpt");
         }
+        r.signal = h->rec_signal + k0;
+        for (size_t k = k0; k < k1; ++k) launch_one<T>(h, r, rblocks, static_cast<int>((k - k0) & 1u), k - k0, h->stream);
         // Adjoint steps n from min(k1, steps - 1) down to k0 + 1 (Q(steps) is 0: p(steps) is
         // read by no probe; Q(0) enters no gradient).
         const size_t top = std::min(k1, steps - 1);
@@ -2135,8 +2157,6 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
     if (G) check(hipMemcpyAsync(g.data(), acc_g, G * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
     if (slots) check(hipMemcpyAsync(gain.data(), acc_gain, slots * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
     check(hipStreamSynchronize(h->stream), "adjoint");
-    // The state after the run is the forward run's end: restore it for the next call.
-    h->done = 0;
     if (sens_g != nullptr) {
         std::fill(sens_g, sens_g + h->special_cell_host.size(), 0.0);
         for (size_t k = 0; k < G; ++k) sens_g[h->sorted_to_special[k]] = -g[k];
@@ -2144,28 +2164,55 @@ void gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const
     if (sens_gain != nullptr) {
         for (size_t j = 0; j < h->state_slot.size(); ++j) sens_gain[j] = h->state_slot[j] == ~0ull ? 0.0 : gain[h->state_slot[j]];
     }
+    free_record(h);
 }
 
 }  // namespace
 
 extern "C" {
 
-int as_fdtd_gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout, double *out,
-                     double *sens_g, double *sens_gain) {
+int as_fdtd_record(as_fdtd *h, const float *signal, size_t steps, size_t every, double *out) {
     if (h == nullptr) return AS_EINVAL;
     try {
         if (steps < 2) refuse("the adjoint needs at least two steps");
-        if (signal == nullptr || (dl_dout == nullptr && h->probes > 0)) refuse("the gradient needs a drive and dJ/dy");
+        if (signal == nullptr) refuse("the record needs a drive");
         if (h->precision == 64) {
-            gradient<double>(h, signal, steps, every, dl_dout, out, sens_g, sens_gain);
+            record<double>(h, signal, steps, every, out);
         } else {
-            gradient<float>(h, signal, steps, every, dl_dout, out, sens_g, sens_gain);
+            record<float>(h, signal, steps, every, out);
+        }
+        return AS_OK;
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+int as_fdtd_adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gain) {
+    if (h == nullptr) return AS_EINVAL;
+    try {
+        if (dl_dout == nullptr && h->probes > 0) refuse("the adjoint needs dJ/dy");
+        if (h->precision == 64) {
+            adjoint<double>(h, dl_dout, sens_g, sens_gain);
+        } else {
+            adjoint<float>(h, dl_dout, sens_g, sens_gain);
         }
         // Leave the handle at rest, as after reset().
         return as_fdtd_reset(h);
     } catch (const Failure &f) {
         return fail(h, f);
     }
+}
+
+int as_fdtd_gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout, double *out,
+                     double *sens_g, double *sens_gain) {
+    if (h == nullptr) return AS_EINVAL;
+    if (dl_dout == nullptr && h->probes > 0) {
+        h->error = "the gradient needs a drive and dJ/dy";
+        return AS_EINVAL;
+    }
+    const int code = as_fdtd_record(h, signal, steps, every, out);
+    if (code != AS_OK) return code;
+    return as_fdtd_adjoint(h, dl_dout, sens_g, sens_gain);
 }
 
 }  // extern "C"
@@ -2192,6 +2239,7 @@ const char *as_fdtd_last_error(const as_fdtd *h) { return h ? h->error.c_str() :
 void as_fdtd_destroy(as_fdtd *h) {
     if (h == nullptr) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_record(h);
     for (void *p : h->allocations) (void)hipFree(p);
     if (h->signal_buf) (void)hipFree(h->signal_buf);
     if (h->out_buf) (void)hipFree(h->out_buf);

@@ -98,17 +98,32 @@ class WallModel:
         finally:
             stepper.close()
 
-    def gradient(self, theta, dl_dy):
-        """(y, dJ/dtheta) for dJ/dy (probes, steps), by the adjoint with checkpoints every
-        `every` steps."""
+    def record(self, theta):
+        """The forward half of :meth:`gradient`: (y, the stepper holding the run's
+        checkpoints, for :meth:`backward`)."""
         stepper = HipWaveSolver(self.coefficients(theta), precision=self.precision, no_faces=True)
         try:
-            y, sens_g, sens_gain = stepper.gradient(self.drive, dl_dy, every=self.every)
+            y = stepper.record(self.drive, every=self.every)
+        except Exception:
+            stepper.close()
+            raise
+        return y, stepper
+
+    def backward(self, stepper, dl_dy):
+        """dJ/dtheta for dJ/dy (probes, steps) from a :meth:`record` stepper (closed after)."""
+        try:
+            sens_g, sens_gain = stepper.adjoint(dl_dy)
         finally:
             stepper.close()
         per_wall = sens_g @ self.W
         gains = np.bincount(self.state_wall, sens_gain, minlength=self.walls)
-        return y, np.concatenate([per_wall, self.gamma * (per_wall + gains)])
+        return np.concatenate([per_wall, self.gamma * (per_wall + gains)])
+
+    def gradient(self, theta, dl_dy):
+        """(y, dJ/dtheta) for dJ/dy (probes, steps), by the adjoint with checkpoints every
+        `every` steps."""
+        y, stepper = self.record(theta)
+        return y, self.backward(stepper, dl_dy)
 
     def absorption(self, theta, bands=(125.0, 250.0)) -> np.ndarray:
         """Normal-incidence absorption 1 - |(1 - Y) / (1 + Y)|^2 of each wall at the bands'
@@ -156,13 +171,22 @@ def make_function():
         def forward(ctx, theta, model):
             ctx.model = model
             ctx.save_for_backward(theta)
-            y = model.forward(theta.detach().cpu().numpy())
+            ctx.stepper = None
+            if ctx.needs_input_grad[0]:
+                # The forward run keeps its checkpoints: the backward pass only recomputes
+                # each segment and steps the adjoint back through it.
+                y, ctx.stepper = model.record(theta.detach().cpu().numpy())
+            else:
+                y = model.forward(theta.detach().cpu().numpy())
             return torch.from_numpy(y)
 
         @staticmethod
         def backward(ctx, grad_y):
             (theta,) = ctx.saved_tensors
-            _, grad = ctx.model.gradient(theta.detach().cpu().numpy(), grad_y.detach().cpu().numpy())
+            stepper, ctx.stepper = ctx.stepper, None
+            if stepper is None:
+                _, stepper = ctx.model.record(theta.detach().cpu().numpy())
+            grad = ctx.model.backward(stepper, grad_y.detach().cpu().numpy())
             return torch.from_numpy(grad).to(theta.dtype), None
 
     return WaveSolve

@@ -90,6 +90,25 @@ class HipWaveSolver:
                                                y.ctypes.data, sens_g.ctypes.data, sens_gain.ctypes.data))
         return y, sens_g, sens_gain
 
+    def record(self, signal, every: int = 256) -> np.ndarray:
+        """The forward half of :meth:`gradient`: from rest, the probes' signals for
+        ``signal`` (probes, steps), with the checkpoints kept for :meth:`adjoint`."""
+        signal = np.ascontiguousarray(signal, dtype=np.float32).reshape(-1)
+        self._recorded = len(signal)
+        y = np.empty((self._probes, len(signal)), np.float64)
+        self._check(self._lib.as_fdtd_record(self._h, signal.ctypes.data, len(signal), int(every), y.ctypes.data))
+        return y
+
+    def adjoint(self, dl_dout):
+        """The backward half of :meth:`gradient`, after :meth:`record`: (dJ/dg (specials,),
+        dJ/dgain (states,)) for dJ/dy (probes, recorded steps). The solver is left at rest."""
+        steps = getattr(self, "_recorded", 0)
+        g = np.ascontiguousarray(dl_dout, dtype=np.float64).reshape(self._probes, steps)
+        sens_g = np.zeros(self._specials, np.float64)
+        sens_gain = np.zeros(self._states, np.float64)
+        self._check(self._lib.as_fdtd_adjoint(self._h, g.ctypes.data, sens_g.ctypes.data, sens_gain.ctypes.data))
+        return sens_g, sens_gain
+
     def reset(self) -> None:
         """Zeroes the fields and states (WaveSolver::reset, fdtd.hpp:906-915)."""
         self._check(self._lib.as_fdtd_reset(self._h))

@@ -53,7 +53,7 @@ _lib = None
 SYMBOLS = (
     "as_fdtd_create", "as_fdtd_create_error", "as_fdtd_run", "as_fdtd_run_device",
     "as_fdtd_reset", "as_fdtd_field", "as_fdtd_stream", "as_fdtd_info", "as_fdtd_last_error",
-    "as_fdtd_destroy", "as_fdtd_gradient",
+    "as_fdtd_destroy", "as_fdtd_gradient", "as_fdtd_record", "as_fdtd_adjoint",
 )
 
 #: The set-up's symbols (acoustic_spaces_amd.solver binds them).
@@ -94,6 +94,10 @@ def load():
     lib.as_fdtd_destroy.argtypes = [P]
     lib.as_fdtd_gradient.restype = ctypes.c_int
     lib.as_fdtd_gradient.argtypes = [P, P, SIZE, SIZE, P, P, P, P]
+    lib.as_fdtd_record.restype = ctypes.c_int
+    lib.as_fdtd_record.argtypes = [P, P, SIZE, SIZE, P]
+    lib.as_fdtd_adjoint.restype = ctypes.c_int
+    lib.as_fdtd_adjoint.argtypes = [P, P, P, P]
     _lib = lib
     return lib
 

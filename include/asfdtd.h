@@ -147,6 +147,14 @@ void as_fdtd_destroy(as_fdtd *h);
 int as_fdtd_gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout,
                      double *out, double *sens_g, double *sens_gain);
 
+/* as_fdtd_gradient in two halves, for a caller whose forward pass needs the signals
+ * before it knows dJ/dy (an autograd function): as_fdtd_record runs the forward run and
+ * keeps its checkpoints (and drive) in the handle; as_fdtd_adjoint then takes dJ/dy
+ * (probes x the recorded steps), writes dJ/dg and dJ/dgain, frees the record and leaves
+ * the handle at rest. No counterpart in the reference (see as_fdtd_gradient). */
+int as_fdtd_record(as_fdtd *h, const float *signal, size_t steps, size_t every, double *out);
+int as_fdtd_adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gain);
+
 /*
  * The set-up on walls on cell faces (the reference's WaveSolver constructor,
  * fdtd.hpp:360-690, bound at cpp/bindings/native_module.cpp:1238-1285): the arguments

@@ -60,3 +60,9 @@ def probes_near_source(seed=9):
     sc.probes = np.array([s + [1.3 * c, 0.4 * c, -0.7 * c], s + [-2.1 * c, 1.1 * c, 0.3 * c],
                           [0.03, 0.71, 0.55], [1.0, 0.05, 1.07]])
     return sc
+
+
+def coarse_hall(cell=0.4, length_s=0.4):
+    """The hall (tilted ceiling planes, splayed side walls) at 40 cm: 80 x 68 x 35 cells."""
+    sc = scenes.hall(cell=cell, length_s=length_s)[0]
+    return sc

@@ -107,3 +107,22 @@ def test_float_adjoint_agrees_with_double():
     _, g32, n32 = HipWaveSolver(layout, precision=32, no_faces=True).gradient(drive, c)
     assert np.linalg.norm(g32 - g64) <= 1e-4 * np.linalg.norm(g64)
     assert np.linalg.norm(n32 - n64) <= 1e-4 * np.linalg.norm(n64)
+
+
+@pytest.mark.gpu
+def test_record_then_adjoint_equals_gradient():
+    """The gradient's two halves, as an autograd function calls them (the forward run
+    keeping its checkpoints, then the backward pass), give the one call's results; the
+    handle is at rest after either."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    sc, layout = _layout()
+    drive = sc.drive[:260]
+    gpu = HipWaveSolver(layout, precision=32, no_faces=True)
+    c = np.random.default_rng(8).normal(size=(gpu.probes, len(drive)))
+    y, g, n = gpu.gradient(drive, c, every=64)
+    y2 = gpu.record(drive, every=64)
+    g2, n2 = gpu.adjoint(c)
+    assert np.array_equal(y, y2) and np.array_equal(g, g2) and np.array_equal(n, n2)
+    assert not np.any(gpu.field())
+    assert np.array_equal(gpu.run(drive[:50]), y[:, :50])

@@ -25,6 +25,10 @@ CASES = {
     "open_layer": lambda: (cases.open_block(), {}),
     "speed": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.small_box()),
     "speed19": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.small_box(stencil=19)),
+    # Fitted walls (the reference's cut cells, fdtd.hpp:970-1264): general specials with
+    # the walls' own weights, and diagonal links where the hall's planes are tilted.
+    "fitted_box": lambda: (cases.small_box(), {"fitted_walls": True}),
+    "fitted_hall": lambda: (cases.coarse_hall(), {"fitted_walls": True}),
 }
 
 
