@@ -1,4 +1,4 @@
-"""Many rooms in one launch a step: the batch's layouts merged into one grid.
+"""Many rooms in one launch a step: the batch's layouts merged into one grid (or a few).
 
 Rooms with one cell size and stencil (one time step) are stacked along z into one grid,
 each padded in x and y to the batch's largest room. Every room keeps its own outermost
@@ -112,27 +112,71 @@ def merge_layouts(layouts):
     return merged, placements
 
 
-class BatchedWaveSolver:
-    """Steps a batch of rooms (their solvers, or layouts) on the GPU, one launch a step for
-    all of them. ``run(signal)`` returns each room's probes (probes, steps), in order."""
+# Bytes a merged grid's cell costs on the device: two pressure fields and its class.
+BYTES_PER_CELL = {32: 9, 64: 17}
 
-    def __init__(self, rooms, precision: int = 32, **tuning):
+
+def plan_groups(layouts, group_bytes=None, precision: int = 32):
+    """The rooms (indices into ``layouts``) in groups to merge, each group's grid within
+    ``group_bytes`` of device memory (None: one group). Rooms go largest footprint (x y)
+    first, so that a group's rooms pad little to its largest; a room larger than the
+    budget is a group of its own. The MI355X's 256 MB Infinity Cache keeps a grid of a
+    fraction of it resident from one step to the next."""
+    order = sorted(range(len(layouts)), key=lambda i: -int(layouts[i]["shape"][0]) * int(layouts[i]["shape"][1]))
+    if group_bytes is None:
+        return [order]
+    per = BYTES_PER_CELL[precision]
+    groups, current, nx, ny, nz = [], [], 0, 0, 0
+    for i in order:
+        sx, sy, sz = (int(v) for v in layouts[i]["shape"])
+        gx, gy, gz = max(nx, sx), max(ny, sy), nz + sz
+        if current and gx * gy * gz * per > group_bytes:
+            groups.append(current)
+            current, gx, gy, gz = [], sx, sy, sz
+        current.append(i)
+        nx, ny, nz = gx, gy, gz
+    if current:
+        groups.append(current)
+    return groups
+
+
+class BatchedWaveSolver:
+    """Steps a batch of rooms (their solvers, or layouts) on the GPU: the rooms merged into
+    one grid, or into groups of grids of at most ``group_bytes`` each (see plan_groups),
+    one launch a step for each grid. ``run(signal)`` returns each room's probes (probes,
+    steps), in the order given."""
+
+    def __init__(self, rooms, precision: int = 32, group_bytes=None, **tuning):
         layouts = [r if isinstance(r, dict) else r.layout() for r in rooms]
-        merged, self.placements = merge_layouts(layouts)
-        self.cells = int(np.prod(merged["shape"]))
-        self.air_cells = int(np.count_nonzero(merged["types"]))
-        self.stepper = HipWaveSolver(merged, precision=precision, **tuning)
+        self.groups = []
+        self.cells = self.air_cells = 0
+        for idx in plan_groups(layouts, group_bytes, precision):
+            merged, placements = merge_layouts([layouts[i] for i in idx])
+            self.cells += int(np.prod(merged["shape"]))
+            self.air_cells += int(np.count_nonzero(merged["types"]))
+            self.groups.append((idx, placements, HipWaveSolver(merged, precision=precision, **tuning)))
+        self.rooms = len(layouts)
+
+    @property
+    def steppers(self) -> list:
+        return [g[2] for g in self.groups]
 
     @property
     def probes(self) -> int:
-        return self.stepper.probes
+        return sum(s.probes for s in self.steppers)
 
     def run(self, signal) -> list:
-        out = self.stepper.run(signal)
-        return [out[p.probes] for p in self.placements]
+        out = [None] * self.rooms
+        for idx, placements, stepper in self.groups:
+            y = stepper.run(signal)
+            for i, p in zip(idx, placements):
+                out[i] = y[p.probes]
+        return out
 
     def reset(self) -> None:
-        self.stepper.reset()
+        for s in self.steppers:
+            s.reset()
 
     def close(self) -> None:
-        self.stepper.close()
+        for s in self.steppers:
+            s.close()

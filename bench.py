@@ -206,10 +206,12 @@ def plain(scene, name, args, traffic_name=None):
 
 
 def rooms(args):
-    """64 rooms: one batched solve (every room in one launch a step, its launch shape from
-    the autotuner) against 64 plain solves one after another, the same steps each."""
+    """64 rooms: batched solves (the rooms merged into one grid, one launch a step, its
+    launch shape from the autotuner; and merged into groups of grids sized to stay in the
+    256 MB Infinity Cache, one launch a step each) against 64 plain solves one after
+    another, the same steps each. The value is the fastest batched layout's."""
     from acoustic_spaces_amd import scenes, tune
-    from acoustic_spaces_amd.batch import merge_layouts
+    from acoustic_spaces_amd.batch import BatchedWaveSolver, merge_layouts
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from acoustic_spaces_amd.solver import WaveSolver
 
@@ -223,24 +225,36 @@ def rooms(args):
     shape = tune.autotune(merged, steps=128)
     stepper = HipWaveSolver(merged, tile_z=shape["tile_z"], graph_steps=shape["graph_steps"])
     info = stepper.info()
-    ms_batch, out = time_steps(stepper, drive, W, K)
+    ms_one, out = time_steps(stepper, drive, W, K)
     stepper.close()
     if not np.all(np.isfinite(out)):
         raise SystemExit("non-finite probe output")
+    merged_shape = [int(v) for v in merged["shape"]]
+    del merged
+    grouped = {}
+    for mb in (384, 256, 192, 128):
+        b = BatchedWaveSolver(layouts, group_bytes=mb << 20)
+        grouped[mb] = (sum(time_steps(s, drive, W, K)[0] for s in b.steppers), len(b.groups))
+        b.close()
     ms_seq = 0.0
     for L in layouts:
         s = HipWaveSolver(L)
         ms, _ = time_steps(s, drive, W, K)
         ms_seq += ms
         s.close()
+    best_mb = min(grouped, key=lambda k: grouped[k][0])
+    ms_batch = min(ms_one, grouped[best_mb][0])
     air = int(sum(np.count_nonzero(L["types"]) for L in layouts))
     states = int(sum(len(L["state_branch"]) for L in layouts))
     per_step_s = ms_batch / 1e3 / K
     step_bytes = PER_AIR_CELL * air + PER_STATE * states
     config = {
-        "rooms": len(batch), "air_cells": air, "merged_grid": [int(v) for v in merged["shape"]],
+        "rooms": len(batch), "air_cells": air, "merged_grid": merged_shape,
         "setup_s": round(setup_s, 2), "tuned": {k: shape[k] for k in ("tile_z", "graph_steps", "us_per_step")},
-        "sequential_ms": round(ms_seq, 2), "batched_ms": round(ms_batch, 2),
+        "sequential_ms": round(ms_seq, 2), "one_grid_ms": round(ms_one, 2),
+        "grouped_ms": {f"{mb} MB": {"ms": round(v[0], 2), "groups": v[1]} for mb, v in grouped.items()},
+        "batched": "one grid" if ms_one <= grouped[best_mb][0] else f"groups of at most {best_mb} MB",
+        "batched_ms": round(ms_batch, 2),
         "batched_over_sequential_speedup": round(ms_seq / ms_batch, 3),
         "sequential_mcell_updates_per_s": round(air * K / (ms_seq / 1e3) / 1e6, 1),
         "special_cells": info["specials"],
@@ -248,7 +262,7 @@ def rooms(args):
     return line_for("FDTD Mcell-updates/s (64 rooms batched; vs 64 sequential plain solves)",
                     air / per_step_s / 1e6, K, W, ms_batch,
                     "64 shoeboxes U(3,12)xU(3,10)xU(2.5,5) m at 5 cm, Monte Carlo wall absorption, "
-                    "1 source and 4 receivers each, stacked into one grid", config,
+                    "1 source and 4 receivers each, merged into batched grids", config,
                     roofline_of(step_bytes, per_step_s, None, "step_kernel<float,false,false> over all rooms"))
 
 
@@ -258,8 +272,9 @@ def adjoint(args):
     each), a seeded true model, targets from the forward solve plus white noise at 40 dB
     SNR. Timed: one forward solve, and a gradient's two halves: the forward run keeping
     checkpoints every 256 steps, then the backward pass (each segment recomputed, the
-    adjoint stepped back through it); the value is the backward pass over the forward solve. Checked: the float and double adjoint's gradients of the
-    decay-curve loss against central differences (h = 1e-3) of the double solve. Then 50
+    adjoint stepped back through it); the value is the backward pass over the forward
+    solve. Checked: the float and double adjoint's gradients of the decay-curve loss
+    against central differences (h = 1e-3) of the double solve. Then 50
     L-BFGS iterations from a flat start, and the recovered absorption at 125 and 250 Hz."""
     import torch
 

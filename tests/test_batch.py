@@ -44,14 +44,33 @@ def test_batch_refuses_mixed_stencils():
         merge_layouts([a, b])
 
 
+def test_groups_cover_each_room_once_within_budget():
+    from acoustic_spaces_amd.batch import BYTES_PER_CELL, plan_groups
+
+    layouts = _layouts(_rooms(count=7, seed=23))
+    assert len(plan_groups(layouts)) == 1 and sorted(plan_groups(layouts)[0]) == list(range(len(layouts)))
+    budget = 2 * max(int(np.prod(L["shape"])) for L in layouts) * BYTES_PER_CELL[32]
+    groups = plan_groups(layouts, budget)
+    assert sorted(i for g in groups for i in g) == list(range(len(layouts)))
+    assert len(groups) > 1
+    for g in groups:
+        nx = max(int(layouts[i]["shape"][0]) for i in g)
+        ny = max(int(layouts[i]["shape"][1]) for i in g)
+        nz = sum(int(layouts[i]["shape"][2]) for i in g)
+        assert len(g) == 1 or nx * ny * nz * BYTES_PER_CELL[32] <= budget
+
+
 @pytest.mark.gpu
-def test_batched_gpu_equals_each_room_alone():
-    from acoustic_spaces_amd.batch import BatchedWaveSolver
+@pytest.mark.parametrize("grouped", [False, True])
+def test_batched_gpu_equals_each_room_alone(grouped):
+    from acoustic_spaces_amd.batch import BYTES_PER_CELL, BatchedWaveSolver
     from oracle.stepper import OracleSolver
 
     rooms = _rooms(count=6, seed=22)
     layouts = _layouts(rooms)
-    batch = BatchedWaveSolver(layouts)
+    budget = 2 * max(int(np.prod(L["shape"])) for L in layouts) * BYTES_PER_CELL[32] if grouped else None
+    batch = BatchedWaveSolver(layouts, group_bytes=budget)
+    print(f"batch grouped={grouped} groups={len(batch.groups)}", flush=True)
     drive = rooms[0].drive[:200]
     first = batch.run(drive[:77])
     second = batch.run(drive[77:])
