@@ -130,8 +130,8 @@ def time_steps(stepper, drive: np.ndarray, warmup: int, steps: int):
 
 def measured_traffic(name: str):
     """HBM bytes per step from the rocprofv3 FETCH_SIZE and WRITE_SIZE runs summarised in
-    profiles/<name>.json (FETCH_SIZE scaled by 2, WRITE_SIZE by 1, as a copy of known size
-    reads them), or None."""
+    profiles/<name>.json, each counter times the scale a copy of known size gives it
+    (FETCH_SIZE 2, WRITE_SIZE 1), or None."""
     path = os.path.join(ROOT, "profiles", f"{name}.json")
     if not os.path.exists(path):
         return None
