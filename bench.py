@@ -41,9 +41,31 @@ def algorithmic_bytes(info: dict, air_cells: int, layer_cells: int, states: int)
     return PER_AIR_CELL * (air_cells - layer_cells) + PER_LAYER_CELL * layer_cells + PER_STATE * states
 
 
-def cpu_baseline(scene, target_s: float = 12.0) -> dict:
+def octave_outputs(y: np.ndarray, rate: float) -> dict:
+    """Each receiver's 63, 125 and 250 Hz octave-band strength G (dB) and T30 (s), of its
+    signal high-passed as the reference's source signal is (the Gaussian drive carries air
+    into the closed room, which would never decay)."""
+    from acoustic_spaces_amd.analysis import octave_results
+    from acoustic_spaces_amd.wave import high_pass
+
+    rows = [octave_results(high_pass(r, rate), rate, bands=3) for r in y]
+    return {"bands_hz": rows[0]["bands_hz"], "g_db": [r["g_db"] for r in rows], "t30_s": [r["t30_s"] for r in rows]}
+
+
+def compare_outputs(a: dict, b: dict, ya: np.ndarray, yb: np.ndarray) -> dict:
+    """The largest differences between two runs' receivers: relative L2 of the signals,
+    G in dB and T30 in s."""
+    rel = [float(np.linalg.norm(p - q) / np.linalg.norm(q)) for p, q in zip(ya, yb)]
+    dg = np.abs(np.array(a["g_db"]) - np.array(b["g_db"]))
+    dt = np.abs(np.array(a["t30_s"]) - np.array(b["t30_s"]))
+    return {"max_rel_l2": max(rel), "max_abs_g_db": float(np.nanmax(dg)), "max_abs_t30_s": float(np.nanmax(dt))}
+
+
+def cpu_baseline(scene, target_s: float = 12.0, compare=None) -> dict:
     """The reference's CPU solver (oracle/_ref) on all host cores, on a bounded run of the
-    same workload; the oracle's one-thread port if the reference's build is absent."""
+    same workload; the oracle's one-thread port if the reference's build is absent. With
+    ``compare`` (the GPU's float probes over the scene's whole drive), the reference's own
+    run of the whole drive too, and the differences of its signals, band levels and T30."""
     cores = len(os.sched_getaffinity(0))
     from oracle import reference
 
@@ -66,14 +88,20 @@ def cpu_baseline(scene, target_s: float = 12.0) -> dict:
     t = time.perf_counter()
     solver.run(drive[8:8 + steps])
     elapsed = time.perf_counter() - t
-    return {
+    out = {
         "value": round(air * steps / elapsed / 1e6, 1),
         "unit": "Mcell-updates/s",
         "cores": int(used),
         "kind": kind,
-        "sample": f"{steps} steps of the same hall grid ({air:,} air cells) after 8, "
+        "sample": f"{steps} steps of the same grid ({air:,} air cells) after 8, "
                   f"{elapsed:.1f} s, threads={used}",
     }
+    if compare is not None:
+        solver.reset()
+        y = solver.run(drive[: compare.shape[1]])
+        out["gpu_f32_vs_this"] = compare_outputs(octave_outputs(compare, scene.rate), octave_outputs(y, scene.rate),
+                                                 compare, y)
+    return out
 
 
 def calibrate() -> int:
@@ -200,6 +228,15 @@ def plain(scene, name, args, traffic_name=None):
         "precision": "float (bit-identical to the reference's CPU scheme)",
     }
     traffic = measured_traffic(traffic_name) if traffic_name else None
+    stepper.close()
+    if args.outputs:
+        # The receivers' outputs over the whole drive, in float and in double.
+        y32 = HipWaveSolver(layout, precision=32).run(scene.drive)
+        y64 = HipWaveSolver(layout, precision=64).run(scene.drive)
+        o32, o64 = octave_outputs(y32, scene.rate), octave_outputs(y64, scene.rate)
+        config["outputs"] = {"steps": len(scene.drive), "f32": o32,
+                             "f32_vs_f64": compare_outputs(o32, o64, y32, y64)}
+        args.signals = y32
     return line_for(f"FDTD Mcell-updates/s ({name}, plain solver)", air / per_step_s / 1e6, K, W, ms, scene.name,
                     config, roofline_of(step_bytes, per_step_s, traffic,
                                         "step_kernel<float,false,false> (one fused launch per step)"))
@@ -391,6 +428,7 @@ def main(argv=None) -> int:
                     help="time the runtime's own device-to-device copy kernel instead (the bandwidth "
                          "reference and the rocprofv3 counter scale)")
     args = ap.parse_args(argv)
+    args.outputs, args.signals = False, None
     if args.calibrate:
         return calibrate()
     if args.gpus != 1:
@@ -413,6 +451,7 @@ def main(argv=None) -> int:
     elif args.workload == "shoebox":
         args.steps = args.steps or 5941
         scene = scenes.shoebox()
+        args.outputs = True
         line = plain(scene, "shoebox", args)
     elif args.workload == "outdoor":
         args.steps = args.steps or 2377
@@ -426,7 +465,7 @@ def main(argv=None) -> int:
         line = adjoint(args)
         scene = None
     if not args.no_cpu and scene is not None:
-        line["cpu_baseline"] = cpu_baseline(scene)
+        line["cpu_baseline"] = cpu_baseline(scene, compare=args.signals)
     print(json.dumps(line), flush=True)
     return 0
 

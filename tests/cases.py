@@ -66,3 +66,15 @@ def coarse_hall(cell=0.4, length_s=0.4):
     """The hall (tilted ceiling planes, splayed side walls) at 40 cm: 80 x 68 x 35 cells."""
     sc = scenes.hall(cell=cell, length_s=length_s)[0]
     return sc
+
+
+def listeners13(stencil=7, seed=31):
+    """A box whose probes are two listeners' 13 points each (the listener, one cell and
+    two cells either side along each axis), one listener a cell from a wall."""
+    from acoustic_spaces_amd.wave import probe_points
+
+    sc = small_box(stencil=stencil, seed=seed)
+    c = sc.grid.cell
+    listeners = [np.array([1.21, 0.73, 0.52]), np.array([0.5 * c + 0.013, 0.61, 0.44])]
+    sc.probes = np.vstack([probe_points(p, c) for p in listeners])
+    return sc

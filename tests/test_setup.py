@@ -43,6 +43,10 @@ SCENES = {
     "shoebox": lambda: (scenes.shoebox(), {}),
     "shoebox19": lambda: (scenes.shoebox(stencil=19, seed=4), {}),
     "near_source": lambda: (cases.probes_near_source(), {}),
+    # Thirteen points a listener, as the reference's wave_responses reads a listener
+    # (engine/wave.py:677-682), two listeners one a cell from a wall.
+    "listeners13": lambda: (cases.listeners13(), {"probes_per_listener": 13}),
+    "listeners13_19": lambda: (cases.listeners13(stencil=19), {"probes_per_listener": 13}),
 }
 
 
