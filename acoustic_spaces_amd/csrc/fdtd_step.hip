@@ -1103,7 +1103,14 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->nz = static_cast<int>(L.nz);
     int align = tuning && tuning->row_align > 0 ? tuning->row_align : 64;
     if (align != 1 && align != 16 && align != 32 && align != 64) refuse("the row alignment is 1, 16, 32 or 64 cells");
-    h->tile_z = tuning && tuning->tile_z > 0 ? tuning->tile_z : 8;
+    if (tuning && tuning->tile_z > 0) {
+        h->tile_z = tuning->tile_z;
+    } else {
+        // 8 planes a tile, or 4 where 8 would leave fewer than 2048 tiles: too few blocks of
+        // four wavefronts to keep the chip's 256 CUs' memory pipes full.
+        const uint64_t tiles8 = ((L.nx + kTileX - 1) / kTileX) * ((L.ny + kTileRows - 3) / kTileRows) * ((L.nz + 5) / 8);
+        h->tile_z = tiles8 < 2048 ? 4 : 8;
+    }
     if (tuning && tuning->tile_y > 0 && tuning->tile_y != kTileRows) refuse("tile_y is 4 (one wavefront a row)");
     if (tuning && tuning->graph_steps != 0) h->graph_steps = tuning->graph_steps < 0 ? 0u : static_cast<uint32_t>(tuning->graph_steps);
     if (h->graph_steps > 4096) refuse("at most 4096 steps a graph");

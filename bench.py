@@ -224,7 +224,7 @@ def plain(scene, name, args, traffic_name=None):
     config = {
         "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
         "special_cells": info["specials"], "grid": list(scene.grid.shape), "setup_s": round(setup_s, 2),
-        "graph_steps": args.graph_steps or 64, "tile_z": args.tile_z or 8,
+        "graph_steps": args.graph_steps or 64, "tile_z": args.tile_z or "8, or 4 below 2048 tiles of 8",
         "precision": "float (bit-identical to the reference's CPU scheme)",
     }
     traffic = measured_traffic(traffic_name) if traffic_name else None
