@@ -129,75 +129,110 @@ struct StepArgs {
 // each state an index into a small table of now, before, gain and weight). The arithmetic
 // is special_cell's, in its order. A tile's face specials are stepped after its column
 // pass, by all its lanes at once, while the tile's pressures are still in the caches.
+// A face special's loads from the fields and the states, by its record, all issued before
+// its arithmetic (face_finish). The small tables (coefficients, state kinds) are read in
+// face_finish.
+constexpr int kFaceChunk = 4;  // states held in registers; a special's others are read as it goes
 template <typename T>
-__device__ __forceinline__ void face_special(const StepArgs<T> &s, const uint4 rec) {
-    // rec: cell, links | states << 6 | coefficients << 10, states' start, kinds
+struct FaceLoads {
+    uint4 rec;  // cell, links | states << 6 | coefficients << 10, states' start, kinds
+    T here, previous, nb[6];
+    T wa[kFaceChunk], wb[kFaceChunk];
+};
+
+template <typename T>
+__device__ __forceinline__ void face_load(const StepArgs<T> &s, const uint4 rec, FaceLoads<T> &f) {
+    const int i = static_cast<int>(rec.x);
+    const uint32_t sbase = rec.z, ns = (rec.y >> 6) & 15u;
+    const int sy = s.px, sz = static_cast<int>(s.sxy);
+    const T *__restrict__ b = s.b;
+    f.rec = rec;
+    f.here = b[i];
+    f.previous = s.a[i];
+    f.nb[0] = b[i - 1];
+    f.nb[1] = b[i + 1];
+    f.nb[2] = b[i - sy];
+    f.nb[3] = b[i + sy];
+    f.nb[4] = b[i - sz];
+    f.nb[5] = b[i + sz];
+#pragma unroll
+    for (int u = 0; u < kFaceChunk; ++u) {
+        if (static_cast<uint32_t>(u) < ns) {
+            f.wa[u] = s.wa[sbase + u];
+            f.wb[u] = s.wb[sbase + u];
+        }
+    }
+}
+
+// The special's update (special_cell's arithmetic, in its order): its linked neighbours
+// summed in the order of the directions, its states' terms, the source, coef_inv last;
+// then its states' velocities.
+template <typename T>
+__device__ __forceinline__ void face_finish(const StepArgs<T> &s, const FaceLoads<T> &f) {
+    const uint4 rec = f.rec;
     const int i = static_cast<int>(rec.x);
     const uint32_t desc = rec.y;
     const uint32_t sbase = rec.z;
     const uint32_t links = desc & 63u, ns = (desc >> 6) & 15u, ci_index = desc >> 10;
     const float sig = s.signal[s.step];
-    const int sy = s.px, sz = static_cast<int>(s.sxy);
-    const T *__restrict__ b = s.b;
     const float4 c = s.face_coef[ci_index];  // coef_a, coef_b, coef_inv, source
-    const T here = b[i], previous = s.a[i];
-    const T xm = b[i - 1], xp = b[i + 1], ym = b[i - sy], yp = b[i + sy], below = b[i - sz], above = b[i + sz];
     const T w = s.axial;
     T sum = 0;
-    if (links & 1u) sum += w * xm;
-    if (links & 2u) sum += w * xp;
-    if (links & 4u) sum += w * ym;
-    if (links & 8u) sum += w * yp;
-    if (links & 16u) sum += w * below;
-    if (links & 32u) sum += w * above;
-    constexpr int kChunk = 4;
-    float4 kind[kChunk];
-    T wa[kChunk], wb[kChunk];
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
+    for (int d = 0; d < 6; ++d) {
+        if (links & (1u << d)) sum += w * f.nb[d];
+    }
+    float4 kind[kFaceChunk];
+#pragma unroll
+    for (int u = 0; u < kFaceChunk; ++u) {
         if (static_cast<uint32_t>(u) < ns) {
             // The first four states' kinds ride in the record (255: look it up).
             uint32_t kid = (rec.w >> (8 * u)) & 0xFFu;
             if (kid == 0xFFu) kid = s.face_state_kind[sbase + u];
             kind[u] = s.face_branch[kid];  // now, before, gain, weight
-            wa[u] = s.wa[sbase + u];
-            wb[u] = s.wb[sbase + u];
         }
     }
-    T own[kChunk];
+    T own[kFaceChunk];
     T flow = 0;
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
+    for (int u = 0; u < kFaceChunk; ++u) {
         if (static_cast<uint32_t>(u) < ns) {
-            own[u] = static_cast<T>(kind[u].x) * wb[u] - static_cast<T>(kind[u].y) * wa[u];
+            own[u] = static_cast<T>(kind[u].x) * f.wb[u] - static_cast<T>(kind[u].y) * f.wa[u];
             flow += static_cast<T>(kind[u].w) * own[u];
         }
     }
-    for (uint32_t j = kChunk; j < ns; ++j) {
+    for (uint32_t j = kFaceChunk; j < ns; ++j) {
         const uint32_t q = sbase + j;
         const float4 k = s.face_branch[s.face_state_kind[q]];
         const T term = static_cast<T>(k.x) * s.wb[q] - static_cast<T>(k.y) * s.wa[q];
         flow += static_cast<T>(k.w) * term;
     }
-    T acc = static_cast<T>(c.x) * here;
+    T acc = static_cast<T>(c.x) * f.here;
     acc = acc + sum;
-    acc = acc - static_cast<T>(c.y) * previous;
+    acc = acc - static_cast<T>(c.y) * f.previous;
     acc = acc - flow;
     acc = acc + static_cast<T>(c.w) * static_cast<T>(sig);
     const T next = acc * static_cast<T>(c.z);
     s.a[i] = next;
-    const T rise = next - previous;
+    const T rise = next - f.previous;
 #pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-        if (static_cast<uint32_t>(u) < ns) s.wa[sbase + u] = wa[u] + (rise * static_cast<T>(kind[u].z) + own[u]);
+    for (int u = 0; u < kFaceChunk; ++u) {
+        if (static_cast<uint32_t>(u) < ns) s.wa[sbase + u] = f.wa[u] + (rise * static_cast<T>(kind[u].z) + own[u]);
     }
-    for (uint32_t j = kChunk; j < ns; ++j) {
+    for (uint32_t j = kFaceChunk; j < ns; ++j) {
         // The term again: wa has not changed since it was taken.
         const uint32_t q = sbase + j;
         const float4 k = s.face_branch[s.face_state_kind[q]];
         const T term = static_cast<T>(k.x) * s.wb[q] - static_cast<T>(k.y) * s.wa[q];
         s.wa[q] = s.wa[q] + (rise * static_cast<T>(k.z) + term);
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void face_special(const StepArgs<T> &s, const uint4 rec) {
+    FaceLoads<T> f;
+    face_load<T>(s, rec, f);
+    face_finish<T>(s, f);
 }
 
 template <typename T>
@@ -293,7 +328,9 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
             }
         }
         if (!all) {
-            // The tile's face specials, by all the block's lanes.
+            // The tile's face specials, by all the block's lanes, after the column march.
+            // (Loading a lane's first one under the march holds some twenty more registers
+            // through it, for every tile: fewer wavefronts fit, and the step is slower.)
             const uint32_t r1 = s.face_tile[block + 1];
             for (uint32_t r = s.face_tile[block] + threadIdx.x; r < r1; r += kBlock) face_special<T>(s, r);
         }
@@ -543,8 +580,8 @@ __device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t blo
 }
 
 // One step in one launch: special blocks first (their gathers are the longest chains of
-// loads, so they start first), then face specials by blocks of their own (if so chosen),
-// interior tiles, layer cells and the probes.
+// loads, so they start first), then face specials by blocks of their own (unless they are
+// stepped in their tiles), interior tiles, layer cells and the probes.
 template <typename T, bool WIDE, bool SCALED>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
     uint32_t block = blockIdx.x;
@@ -1011,7 +1048,7 @@ struct as_fdtd {
     int split = 0;            // 1: four launches a step instead of one
     int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order
     int no_faces = 0;         // 1: every special is general (the adjoint's handles)
-    int face_blocks = 0;      // 1: face specials by blocks of their own, not in their tiles
+    int face_blocks = 0;      // 1: face specials by blocks of their own, 0: in their tiles
     uint32_t *face_tile_none = nullptr;  // a face_tile of empty ranges, for face_blocks
     // The adjoint (see as_fdtd_gradient): its fields, accumulators, taps and checkpoints.
     // The forward run it goes back through (as_fdtd_record): the drive on the device and
@@ -1103,21 +1140,19 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->nz = static_cast<int>(L.nz);
     int align = tuning && tuning->row_align > 0 ? tuning->row_align : 64;
     if (align != 1 && align != 16 && align != 32 && align != 64) refuse("the row alignment is 1, 16, 32 or 64 cells");
-    if (tuning && tuning->tile_z > 0) {
-        h->tile_z = tuning->tile_z;
-    } else {
-        // 8 planes a tile, or 4 where 8 would leave fewer than 2048 tiles: too few blocks of
-        // four wavefronts to keep the chip's 256 CUs' memory pipes full.
-        const uint64_t tiles8 = ((L.nx + kTileX - 1) / kTileX) * ((L.ny + kTileRows - 3) / kTileRows) * ((L.nz + 5) / 8);
-        h->tile_z = tiles8 < 2048 ? 4 : 8;
-    }
+    // A small grid (fewer than 2048 tiles of 8 planes: too few blocks of four wavefronts to
+    // keep the chip's 256 CUs' memory pipes full) takes tiles of 4 planes, each stepping its
+    // own face specials; a larger one tiles of 8 beside blocks of face specials.
+    const uint64_t tiles8 = ((L.nx + kTileX - 1) / kTileX) * ((L.ny + kTileRows - 3) / kTileRows) * ((L.nz + 5) / 8);
+    const bool small = tiles8 < 2048;
+    h->tile_z = tuning && tuning->tile_z > 0 ? tuning->tile_z : (small ? 4 : 8);
+    h->face_blocks = tuning && tuning->face_blocks != 0 ? (tuning->face_blocks > 0 ? 1 : 0) : (small ? 0 : 1);
     if (tuning && tuning->tile_y > 0 && tuning->tile_y != kTileRows) refuse("tile_y is 4 (one wavefront a row)");
     if (tuning && tuning->graph_steps != 0) h->graph_steps = tuning->graph_steps < 0 ? 0u : static_cast<uint32_t>(tuning->graph_steps);
     if (h->graph_steps > 4096) refuse("at most 4096 steps a graph");
     if (tuning && tuning->split > 0) h->split = 1;
     if (tuning && tuning->tile_order > 0) h->tile_order = 1;
     if (tuning && tuning->no_faces > 0) h->no_faces = 1;
-    if (tuning && tuning->face_blocks > 0) h->face_blocks = 1;
     const bool pairs_on = tuning && tuning->pairs > 0;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
     const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);

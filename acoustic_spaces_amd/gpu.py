@@ -24,17 +24,18 @@ class HipWaveSolver:
     scheme in double. ``tile_z``, ``row_align`` and ``graph_steps`` set the launch shape (0: the
     library's defaults; graph_steps < 0 launches every step from the host); ``split`` launches
     each kind of cell on its own instead of one launch per step; ``face_blocks`` steps the
-    face specials by blocks of their own instead of in their tiles; ``pairs`` takes two
-    steps a launch where that applies."""
+    face specials by blocks of their own (True) or in their tiles (False; None: the
+    library's choice); ``pairs`` takes two steps a launch where that applies."""
 
     def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
                  graph_steps: int = 0, split: bool = False, tile_order: int = 0,
-                 no_faces: bool = False, face_blocks: bool = False, pairs: bool = False) -> None:
+                 no_faces: bool = False, face_blocks=None, pairs: bool = False) -> None:
         lib = native.load()
         layout = solver if isinstance(solver, dict) else solver.layout()
         struct, keep = native.pack_layout(layout)
         tuning = native.Tuning(0, int(tile_z), int(row_align), int(graph_steps), 1 if split else 0,
-                               int(tile_order), 1 if no_faces else 0, 1 if face_blocks else 0,
+                               int(tile_order), 1 if no_faces else 0,
+                               0 if face_blocks is None else (1 if face_blocks else -1),
                                1 if pairs else 0)
         h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
         if not h:

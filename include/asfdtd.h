@@ -92,8 +92,10 @@ typedef struct as_tuning {
     int split;         /* 1: one launch per kind of cell (specials, interior, layer, probes) */
     int tile_order;    /* 0: interior tiles grouped by die (XCD), 1: in grid order */
     int no_faces;      /* 1: step every special cell by its own blocks (as the adjoint needs) */
-    int face_blocks;   /* 1: step the face specials (axial links of the interior's weight) by
-                          blocks of their own instead of in their tiles */
+    int face_blocks;   /* the face specials (axial links of the interior's weight): positive, by
+                          blocks of their own beside the tiles; negative, each tile's in the
+                          tile after its column march; 0, the library's choice (blocks of their
+                          own unless the grid is small) */
     int pairs;         /* 1: two steps a launch where that applies (precision 32, 7 points, one
                           speed of sound, no matched layer, every special a face special, row
                           pitch a multiple of 64); 0: one step a launch */
