@@ -120,6 +120,12 @@ struct StepArgs {
     unsigned long long out_stride;
     unsigned long long step;  // index into signal and out
     T *__restrict__ history;  // or null: the specials' p(n+1) - p(n-1), step-major (adjoint)
+    // The adjoint's gradient sums (as_fdtd_adjoint): with `delta` (D(n + 1) of each general
+    // special, or null), the step computing Q(n) adds D(n + 1) Q(n + 1) and D(n + 1) W(n + 1)
+    // (read as its b and wb) into acc_g and acc_gain.
+    const T *__restrict__ delta;
+    double *__restrict__ acc_g;
+    double *__restrict__ acc_gain;
 };
 
 // A face special (see the tiles' notes): a special cell whose links are axial and weigh
@@ -688,12 +694,16 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
             if (c + u < nl) sum += static_cast<T>(w[u]) * v[u];
         }
     }
+    // The previous adjoint step's sums: D(n + 1) Q(n + 1), D(n + 1) W(n + 1).
+    const double d = s.delta != nullptr ? static_cast<double>(s.delta[k]) : 0.0;
+    if (s.delta != nullptr) s.acc_g[k] += d * static_cast<double>(qb);
     T gsum = 0;
     for (int j = 0; j < ns; ++j) {
         const uint32_t q = sb + j * kBlock;
         const T now = s.state_now[q], before = s.state_before[q];
-        const T old = s.wa[q];
-        const T fresh = now * s.wb[q] + (static_cast<T>(1) - before) * old +
+        const T old = s.wa[q], wb = s.wb[q];
+        if (s.delta != nullptr) s.acc_gain[q] += d * static_cast<double>(wb);
+        const T fresh = now * wb + (static_cast<T>(1) - before) * old +
                         static_cast<T>(s.state_weight[q]) * (before * qa - now * qb);
         gsum += static_cast<T>(s.state_gain[q]) * (fresh - old);
         s.wa[q] = fresh;
@@ -2148,10 +2158,70 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     const uint32_t rblocks = block_count(h, r);
     StepArgs<T> a = step_args<T>(h);  // adjoint
     a.probes = 0;
+    a.acc_g = acc_g;
+    a.acc_gain = acc_gain;
     const uint32_t ablocks = a.special_blocks + a.tile_blocks;
     const uint32_t sblocks = (h->specials + kBlock - 1) / kBlock;
     const uint32_t iblocks = (unique + kBlock - 1) / kBlock;
-    unsigned long long m = 0;  // adjoint steps done: their parity says which field is which
+    // Segment k0 .. k1 - 1 recomputed from slots 0 (p(k0 - 1)) and 1 (p(k0)), its drive at sig.
+    auto recompute = [&](size_t count, const float *sig) {
+        r.signal = sig;
+        for (size_t k = 0; k < count; ++k) launch_one<T>(h, r, rblocks, static_cast<int>(k & 1u), k, h->stream);
+    };
+    // Adjoint steps n = k0 + top down to k0 + 1, each adding the previous step's sums
+    // (none before the first), dJ/dy of step n at g[probe * gstride + n - gbase]; then
+    // the last step's sums. m counts the adjoint steps before: its parity says which
+    // field is which.
+    auto backward = [&](size_t k0, size_t top, const double *g, size_t gstride, size_t gbase, unsigned long long m) {
+        for (size_t n = k0 + top; n >= k0 + 1; --n, ++m) {
+            const bool even = (m & 1ull) == 0;
+            a.a = even ? q0 : q1;
+            a.b = even ? q1 : q0;
+            a.wa = even ? v0 : v1;
+            a.wb = even ? v1 : v0;
+            a.delta = n < k0 + top ? history + (n - k0) * G : nullptr;
+            if (ablocks) {
+                if (h->wide) {
+                    hipLaunchKernelGGL((adjoint_kernel<T, true>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                } else {
+                    hipLaunchKernelGGL((adjoint_kernel<T, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                }
+            }
+            if (iblocks) {
+                hipLaunchKernelGGL((inject_kernel<T>), dim3(iblocks), dim3(kBlock), 0, h->stream, a.a, d_tcell, d_tci,
+                                   d_tstart, d_tprobe, d_tweight, g, static_cast<unsigned long long>(gstride),
+                                   static_cast<unsigned long long>(n - gbase), unique);
+            }
+        }
+        if (sblocks) {
+            hipLaunchKernelGGL((accumulate_kernel<T>), dim3(sblocks), dim3(kBlock), 0, h->stream, a, a.a, a.wa,
+                               history, acc_g, acc_gain);
+        }
+    };
+    // Whole segments (every steps, the adjoint's top at the segment's end) replay captured
+    // graphs: the recompute from a staged drive, the adjoint from staged dJ/dy, one graph
+    // for each parity of the adjoint steps before.
+    float *sig_stage = static_cast<float *>(alloc(every * sizeof(float)));
+    double *g_stage = static_cast<double *>(alloc(std::max<size_t>(P, 1) * (every + 1) * sizeof(double)));
+    std::vector<hipGraphExec_t> graphs(3, nullptr);  // recompute, adjoint from an even and an odd m
+    struct Graphs {
+        std::vector<hipGraphExec_t> &v;
+        ~Graphs() {
+            for (auto g : v)
+                if (g) (void)hipGraphExecDestroy(g);
+        }
+    } release_graphs{graphs};
+    auto capture = [&](int which, auto &&enqueue) {
+        if (graphs[which] != nullptr) return;
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        enqueue();
+        check(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&graphs[which], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        check(e, "hipGraphInstantiate");
+    };
+    unsigned long long m = 0;
     for (size_t seg = segments; seg-- > 0;) {
         const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
         // The checkpoint into slots 0 (p(k0 - 1)) and 1 (p(k0)).
@@ -2163,36 +2233,30 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
             check(hipMemcpyAsync(h->velocity[1], base + 2 * field + states, states, hipMemcpyDeviceToDevice, h->stream),
                   "ckpt");
         }
-        r.signal = h->rec_signal + k0;
-        for (size_t k = k0; k < k1; ++k) launch_one<T>(h, r, rblocks, static_cast<int>((k - k0) & 1u), k - k0, h->stream);
         // Adjoint steps n from min(k1, steps - 1) down to k0 + 1 (Q(steps) is 0: p(steps) is
         // read by no probe; Q(0) enters no gradient).
         const size_t top = std::min(k1, steps - 1);
-        for (size_t n = top; n >= k0 + 1 && n >= 1; --n) {
-            const bool even = (m & 1ull) == 0;
-            a.a = even ? q0 : q1;
-            a.b = even ? q1 : q0;
-            a.wa = even ? v0 : v1;
-            a.wb = even ? v1 : v0;
-            if (ablocks) {
-                if (h->wide) {
-                    hipLaunchKernelGGL((adjoint_kernel<T, true>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
-                } else {
-                    hipLaunchKernelGGL((adjoint_kernel<T, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
-                }
-            }
-            if (iblocks) {
-                hipLaunchKernelGGL((inject_kernel<T>), dim3(iblocks), dim3(kBlock), 0, h->stream, a.a, d_tcell, d_tci,
-                                   d_tstart, d_tprobe, d_tweight, d_g, static_cast<unsigned long long>(steps),
-                                   static_cast<unsigned long long>(n), unique);
-            }
-            if (sblocks) {
-                hipLaunchKernelGGL((accumulate_kernel<T>), dim3(sblocks), dim3(kBlock), 0, h->stream, a, a.a, a.wa,
-                                   history + (n - 1 - k0) * G, acc_g, acc_gain);
-            }
-            ++m;
-            if (n == 0) break;
+        const bool whole = k1 - k0 == every && top == k1 && h->graph_steps > 0;
+        if (!whole) {
+            recompute(k1 - k0, h->rec_signal + k0);
+            if (top > k0) backward(k0, top - k0, d_g, steps, 0, m);
+            m += top > k0 ? top - k0 : 0;
+            continue;
         }
+        check(hipMemcpyAsync(sig_stage, h->rec_signal + k0, every * sizeof(float), hipMemcpyDeviceToDevice, h->stream),
+              "hipMemcpyAsync");
+        capture(0, [&] { recompute(every, sig_stage); });
+        check(hipGraphLaunch(graphs[0], h->stream), "hipGraphLaunch");
+        if (P > 0) {
+            check(hipMemcpy2DAsync(g_stage, (every + 1) * sizeof(double), d_g + k0, steps * sizeof(double),
+                                   (every + 1) * sizeof(double), P, hipMemcpyDeviceToDevice, h->stream),
+                  "hipMemcpy2DAsync");
+        }
+        const int which = 1 + static_cast<int>(m & 1ull);
+        const unsigned long long start = m;
+        capture(which, [&] { backward(0, every, g_stage, every + 1, 0, start); });
+        check(hipGraphLaunch(graphs[which], h->stream), "hipGraphLaunch");
+        m += every;
     }
     check(hipGetLastError(), "adjoint");
     std::vector<double> g(G), gain(slots);

@@ -1,8 +1,9 @@
-"""The stepper's autotuner: tile depth and steps per graph, chosen by timing them.
+"""The stepper's autotuner: tile depth, steps per graph and where the face specials are
+stepped, chosen by timing them.
 
 For a layout it times each candidate launch shape (tile_z planes per tile, graph_steps
-steps per captured graph) over a short run on the GPU, with hipEvents on the stepper's
-stream, and keeps the fastest. Results are cached by the layout's signature (its grid,
+steps per captured graph, face specials by blocks of their own or in their tiles) over a
+short run on the GPU, with hipEvents on the stepper's stream, and keeps the fastest. Results are cached by the layout's signature (its grid,
 cell counts and stencil) in a JSON file, as the reference keeps its own device timings
 (engine/tuning.py:76-180); the reference tunes only the CPU/GPU choice, this tunes the
 kernels' shape.
@@ -16,7 +17,8 @@ from pathlib import Path
 
 import numpy as np
 
-CANDIDATES = tuple({"tile_z": tz, "graph_steps": gs} for tz in (4, 8, 12, 16) for gs in (32, 64, 128))
+CANDIDATES = tuple({"tile_z": tz, "graph_steps": gs, "face_blocks": fb}
+                   for tz in (4, 8) for gs in (32, 64) for fb in (False, True))
 
 
 def cache_path() -> Path:
@@ -54,9 +56,10 @@ def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int 
 
 def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: bool = True,
              precision: int = 32) -> dict:
-    """The fastest launch shape for the layout: {"tile_z", "graph_steps", "us_per_step",
-    "timings"}; from the cache when it holds this layout's signature."""
-    key = f"{signature(layout)}|p{precision}"
+    """The fastest launch shape for the layout: the candidate's keys (tile_z, graph_steps,
+    face_blocks), "us_per_step" and "timings"; from the cache when it holds this layout's
+    signature."""
+    key = f"{signature(layout)}|p{precision}|" + ",".join(sorted({k for c in candidates for k in c}))
     path = cache_path()
     cache = {}
     if use_cache and path.exists():
@@ -71,8 +74,7 @@ def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: b
         us = time_shape(layout, steps=max(steps, shape["graph_steps"]), precision=precision, **shape)
         timings.append({**shape, "us_per_step": round(us, 3)})
     best = min(timings, key=lambda t: t["us_per_step"])
-    result = {"tile_z": best["tile_z"], "graph_steps": best["graph_steps"], "us_per_step": best["us_per_step"],
-              "timings": timings}
+    result = {**best, "timings": timings}
     if use_cache:
         try:
             path.parent.mkdir(parents=True, exist_ok=True)

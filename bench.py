@@ -260,7 +260,8 @@ def rooms(args):
     drive = scenes.gaussian_drive(batch[0].rate, W + K)
     merged, placements = merge_layouts(layouts)
     shape = tune.autotune(merged, steps=128)
-    stepper = HipWaveSolver(merged, tile_z=shape["tile_z"], graph_steps=shape["graph_steps"])
+    launch = {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks")}
+    stepper = HipWaveSolver(merged, **launch)
     info = stepper.info()
     ms_one, out = time_steps(stepper, drive, W, K)
     stepper.close()
@@ -270,7 +271,7 @@ def rooms(args):
     del merged
     grouped = {}
     for mb in (384, 256, 192, 128):
-        b = BatchedWaveSolver(layouts, group_bytes=mb << 20)
+        b = BatchedWaveSolver(layouts, group_bytes=mb << 20, **launch)
         grouped[mb] = (sum(time_steps(s, drive, W, K)[0] for s in b.steppers), len(b.groups))
         b.close()
     ms_seq = 0.0
@@ -287,7 +288,7 @@ def rooms(args):
     step_bytes = PER_AIR_CELL * air + PER_STATE * states
     config = {
         "rooms": len(batch), "air_cells": air, "merged_grid": merged_shape,
-        "setup_s": round(setup_s, 2), "tuned": {k: shape[k] for k in ("tile_z", "graph_steps", "us_per_step")},
+        "setup_s": round(setup_s, 2), "tuned": {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks", "us_per_step")},
         "sequential_ms": round(ms_seq, 2), "one_grid_ms": round(ms_one, 2),
         "grouped_ms": {f"{mb} MB": {"ms": round(v[0], 2), "groups": v[1]} for mb, v in grouped.items()},
         "batched": "one grid" if ms_one <= grouped[best_mb][0] else f"groups of at most {best_mb} MB",
