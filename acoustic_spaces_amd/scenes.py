@@ -359,12 +359,14 @@ def delany_bazley_admittance(f, sigma=200e3):
 
 
 def outdoor(cell=0.25, seed=2, length_s=1.0, receivers=32, layer=16, layer_damping=0.5,
-            size=(200.0, 200.0, 40.0)) -> Scene:
+            size=(200.0, 200.0, 40.0), barrier=True) -> Scene:
     """The outdoor case: a ground block (its top at z = 0, 1 m thick) under open sky, a
-    3 m high, 30 m long, 1 m thick rigid box barrier at its centre, a 16-cell matched layer
-    round the grid. The grid spans ``size`` (200 x 200 x 40 m at 25 cm: 800 x 800 x 160
-    cells) with the block inside its layer and margin. The ground's admittance is a
-    constant and two relaxation branches fitted to Delany-Bazley at sigma 200 kPa s/m^2."""
+    3 m high, 30 m long, 1 m thick rigid box barrier at its centre (without it with
+    ``barrier`` False: the same source and receivers over the bare ground, for the
+    barrier's insertion loss), a 16-cell matched layer round the grid. The grid spans
+    ``size`` (200 x 200 x 40 m at 25 cm: 800 x 800 x 160 cells) with the block inside its
+    layer and margin. The ground's admittance is a constant and two relaxation branches
+    fitted to Delany-Bazley at sigma 200 kPa s/m^2."""
     rng = np.random.default_rng(seed)
     n = np.round(np.asarray(size) / cell).astype(int)
     pad = layer + 2
@@ -381,11 +383,12 @@ def outdoor(cell=0.25, seed=2, length_s=1.0, receivers=32, layer=16, layer_dampi
     ground.vertices = ground.vertices + np.array([0.0, 0.0, shift_z])
     ground_top = shift_z
     cx, cy = gx / 2, gy / 2
+    barrier_on = barrier
     barrier = box_mesh((1.0, 30.0, 3.0), low=(cx - 0.5, cy - 15.0, ground_top), inward=False)
     # The barrier's bottom lies on the ground: drop its floor face (two triangles).
     keep = barrier.face_material != 0
     barrier = Mesh(barrier.vertices, barrier.faces[keep], barrier.face_material[keep] - 1, barrier.names[1:])
-    mesh = merge(ground, barrier)
+    mesh = merge(ground, barrier) if barrier_on else ground
     nm = mesh.materials
     rate = C / (COURANT * cell)
     steps = int(round(length_s * rate))
@@ -412,7 +415,8 @@ def outdoor(cell=0.25, seed=2, length_s=1.0, receivers=32, layer=16, layer_dampi
     xs = np.minimum(xs, gx - 8.0)
     probes = np.column_stack([xs, np.full(receivers, cy), np.full(receivers, ground_top + 1.5)])
     return Scene(
-        name=f"outdoor {size[0]:g}x{size[1]:g}x{size[2]:g} m at {cell:g} m, layer {layer}",
+        name=f"outdoor {size[0]:g}x{size[1]:g}x{size[2]:g} m at {cell:g} m, layer {layer}"
+             + ("" if barrier_on else ", no barrier"),
         mesh=mesh, beta=beta, branches=np.vstack(branches), branch_material=np.concatenate(owners),
         grid=grid, source=source, probes=probes, rate=rate, steps=steps,
         drive=gaussian_drive(rate, steps), layer=layer, layer_damping=layer_damping,

@@ -61,6 +61,33 @@ def compare_outputs(a: dict, b: dict, ya: np.ndarray, yb: np.ndarray) -> dict:
     return {"max_rel_l2": max(rel), "max_abs_g_db": float(np.nanmax(dg)), "max_abs_t30_s": float(np.nanmax(dt))}
 
 
+def insertion_loss(scene, layout) -> dict:
+    """The barrier's insertion loss at each receiver in the 63 and 125 Hz octave bands: the
+    band energy over the bare ground against that with the barrier (dB), from the whole
+    drive in float; and its largest difference from the same in double."""
+    import dataclasses
+
+    from acoustic_spaces_amd import scenes as sc
+    from acoustic_spaces_amd.analysis import band_energy
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    bare = WaveSolver.from_scene(sc.outdoor(barrier=False)).layout()
+    slots = int(np.ceil(len(scene.drive) / scene.rate / 1e-3))
+
+    def levels(y):
+        return np.array([10 * np.log10(band_energy(r, scene.rate, 2, 1e-3, slots).sum(axis=1)) for r in y])
+
+    out = {}
+    for precision in (32, 64):
+        with_barrier = levels(HipWaveSolver(layout, precision=precision).run(scene.drive))
+        without = levels(HipWaveSolver(bare, precision=precision).run(scene.drive))
+        out[precision] = without - with_barrier
+    return {"bands_hz": [63, 125], "il_db_f32": np.round(out[32], 2).tolist(),
+            "max_abs_il_db_f32_vs_f64": float(np.abs(out[32] - out[64]).max()),
+            "receivers_m_behind_barrier": np.round(scene.probes[:, 0] - (scene.probes[:, 0].min() - 10.0), 1).tolist()}
+
+
 def cpu_baseline(scene, target_s: float = 12.0, compare=None) -> dict:
     """The reference's CPU solver (oracle/_ref) on all host cores, on a bounded run of the
     same workload; the oracle's one-thread port if the reference's build is absent. With
@@ -237,6 +264,8 @@ def plain(scene, name, args, traffic_name=None):
         config["outputs"] = {"steps": len(scene.drive), "f32": o32,
                              "f32_vs_f64": compare_outputs(o32, o64, y32, y64)}
         args.signals = y32
+    if args.insertion_loss:
+        config["insertion_loss"] = insertion_loss(scene, layout)
     return line_for(f"FDTD Mcell-updates/s ({name}, plain solver)", air / per_step_s / 1e6, K, W, ms, scene.name,
                     config, roofline_of(step_bytes, per_step_s, traffic,
                                         "step_kernel<float,false,false> (one fused launch per step)"))
@@ -423,13 +452,15 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=256)
     ap.add_argument("--workload", choices=("hall", "shoebox", "outdoor", "rooms", "adjoint"), default="hall")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-outputs", action="store_true",
+                    help="skip the hall's receiver outputs (band levels and T30, float against double)")
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
                     help="time the runtime's own device-to-device copy kernel instead (the bandwidth "
                          "reference and the rocprofv3 counter scale)")
     args = ap.parse_args(argv)
-    args.outputs, args.signals = False, None
+    args.outputs, args.signals, args.insertion_loss = False, None, False
     if args.calibrate:
         return calibrate()
     if args.gpus != 1:
@@ -443,6 +474,7 @@ def main(argv=None) -> int:
     if args.workload == "hall":
         args.steps = args.steps or 11882
         scene = scenes.hall()[0]
+        args.outputs = not args.no_outputs
         line = plain(scene, "hall", args, traffic_name="step_kernel_hall_traffic")
         line["config"]["workload"] = (
             "hall 30x20x12 m, ceiling of 6 tilted planes, side walls splayed 5 deg, 10 cm cells "
@@ -457,6 +489,7 @@ def main(argv=None) -> int:
     elif args.workload == "outdoor":
         args.steps = args.steps or 2377
         scene = scenes.outdoor()
+        args.insertion_loss = True
         line = plain(scene, "outdoor", args)
     elif args.workload == "rooms":
         args.steps = args.steps or 5941
