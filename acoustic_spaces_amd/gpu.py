@@ -104,9 +104,12 @@ class HipWaveSolver:
         """The backward half of :meth:`gradient`, after :meth:`record`: (dJ/dg (specials,),
         dJ/dgain (states,)) for dJ/dy (probes, recorded steps). The solver is left at rest."""
         steps = getattr(self, "_recorded", 0)
+        if not steps:
+            raise ValueError("no forward run recorded: call record() first")
         g = np.ascontiguousarray(dl_dout, dtype=np.float64).reshape(self._probes, steps)
         sens_g = np.zeros(self._specials, np.float64)
         sens_gain = np.zeros(self._states, np.float64)
+        self._recorded = 0
         self._check(self._lib.as_fdtd_adjoint(self._h, g.ctypes.data, sens_g.ctypes.data, sens_gain.ctypes.data))
         return sens_g, sens_gain
 

@@ -445,6 +445,18 @@ def adjoint(args):
     }
 
 
+def strict(value):
+    """``value`` with every NaN or infinity (an unreadable T30, say) as None, so that the
+    line is strict JSON."""
+    if isinstance(value, float):
+        return value if np.isfinite(value) else None
+    if isinstance(value, dict):
+        return {k: strict(v) for k, v in value.items()}
+    if isinstance(value, (list, tuple)):
+        return [strict(v) for v in value]
+    return value
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,7 +512,7 @@ def main(argv=None) -> int:
         scene = None
     if not args.no_cpu and scene is not None:
         line["cpu_baseline"] = cpu_baseline(scene, compare=args.signals)
-    print(json.dumps(line), flush=True)
+    print(json.dumps(strict(line), allow_nan=False), flush=True)
     return 0
 
 
