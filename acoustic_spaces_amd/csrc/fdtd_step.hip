@@ -106,7 +106,7 @@ struct StepArgs {
     // matched layer
     uint32_t layer_cells, layer_blocks;
     const uint32_t *__restrict__ layer_cell;
-    const uint64_t *__restrict__ layer_at;  // x | y << 21 | z << 42
+    uint32_t px_magic, px_shift, ny_magic, ny_shift;  // division by px and by ny (see divide)
     T *__restrict__ layer_state;            // 9 planes of layer_cells
     const float *__restrict__ layer_scale;  // or null
     const float *__restrict__ keep[6];
@@ -502,14 +502,24 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
     }
 }
 
+// n / d for n < 2^31 by a multiply: magic = floor(2^32 (2^shift - d) / d) + 1 with
+// shift = ceil(log2 d) (division by invariant integers, Granlund and Montgomery 1994; made
+// by divisor_of on the host).
+__device__ __forceinline__ uint32_t divide(uint32_t n, uint32_t magic, uint32_t shift) {
+    const uint64_t t = __umulhi(n, magic);
+    return static_cast<uint32_t>((t + n) >> shift);
+}
+
 // A matched-layer cell (fdtd.hpp:2242-2266): its six face velocities, then the three
 // parts of its pressure, from the current field into the next.
 template <typename T>
 __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
-    const long long i = s.layer_cell[c];
-    const uint64_t at = s.layer_at[c];
-    const uint32_t coord[3] = {static_cast<uint32_t>(at & 0x1FFFFF), static_cast<uint32_t>((at >> 21) & 0x1FFFFF),
-                               static_cast<uint32_t>(at >> 42)};
+    const uint32_t cell = s.layer_cell[c];
+    const long long i = cell;
+    // The cell's coordinates from its padded index (no table of them to read).
+    const uint32_t yz = divide(cell, s.px_magic, s.px_shift);
+    const uint32_t z = divide(yz, s.ny_magic, s.ny_shift);
+    const uint32_t coord[3] = {cell - yz * static_cast<uint32_t>(s.px), yz - z * static_cast<uint32_t>(s.ny), z};
     const long long stride[3] = {1, s.px, s.sxy};
     const uint32_t L = s.layer_cells;
     T *__restrict__ st = s.layer_state;
@@ -1079,7 +1089,7 @@ struct as_fdtd {
     std::vector<double> coef_inv_host;        // per layout special
     std::vector<uint64_t> special_cell_host;  // padded cell per layout special
     uint32_t *layer_cell = nullptr;
-    uint64_t *layer_at = nullptr;
+    uint32_t px_magic = 0, px_shift = 0, ny_magic = 0, ny_shift = 0;
     float *layer_scale = nullptr;
     float *keep[6] = {}, *push[6] = {};
     uint32_t *probe_start = nullptr, *probe_cell = nullptr;
@@ -1135,6 +1145,14 @@ void *zeros(as_fdtd *h, size_t bytes) {
 }
 
 void refuse(const std::string &what) { throw Failure{AS_EINVAL, what}; }
+
+// The multiplier and shift with which divide() divides by d (1 <= d < 2^31).
+void divisor_of(uint32_t d, uint32_t *magic, uint32_t *shift) {
+    uint32_t sh = 0;
+    while ((1ull << sh) < d) ++sh;
+    *shift = sh;
+    *magic = static_cast<uint32_t>(((1ull << 32) * ((1ull << sh) - d)) / d + 1);
+}
 
 void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tuning) {
     if (precision != 32 && precision != 64) refuse("precision is 32 or 64");
@@ -1535,7 +1553,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
 
     // Matched layer.
     std::vector<uint32_t> layer_cell(L.layer_cells);
-    std::vector<uint64_t> layer_at(L.layer_cells);
+
     for (uint64_t c = 0; c < L.layer_cells; ++c) {
         const int64_t i = L.layer_cell[c];
         if (i < 0 || static_cast<uint64_t>(i) >= n) refuse("a layer cell lies outside the grid");
@@ -1545,7 +1563,6 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             refuse("a layer cell lies on the grid's edge");
         }
         layer_cell[c] = static_cast<uint32_t>(pad(static_cast<uint64_t>(i)));
-        layer_at[c] = x | (y << 21) | (z << 42);
     }
     h->layer_cells = static_cast<uint32_t>(L.layer_cells);
 
@@ -1599,7 +1616,8 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->face_coef = reinterpret_cast<float4 *>(upload(h, face_coef));
     h->face_branch = reinterpret_cast<float4 *>(upload(h, face_branch));
     h->layer_cell = upload(h, layer_cell);
-    h->layer_at = upload(h, layer_at);
+    divisor_of(static_cast<uint32_t>(px), &h->px_magic, &h->px_shift);
+    divisor_of(static_cast<uint32_t>(ny), &h->ny_magic, &h->ny_shift);
     if (L.layer_scale != nullptr && L.layer_cells > 0) {
         h->layer_scale = upload(h, std::vector<float>(L.layer_scale, L.layer_scale + L.layer_cells));
     }
@@ -1684,7 +1702,10 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.layer_cells = h->layer_cells;
     s.layer_blocks = (h->layer_cells + kBlock - 1) / kBlock;
     s.layer_cell = h->layer_cell;
-    s.layer_at = h->layer_at;
+    s.px_magic = h->px_magic;
+    s.px_shift = h->px_shift;
+    s.ny_magic = h->ny_magic;
+    s.ny_shift = h->ny_shift;
     s.layer_state = static_cast<T *>(h->layer_state);
     s.layer_scale = h->layer_scale;
     for (int k = 0; k < 6; ++k) {
