@@ -112,7 +112,7 @@ struct StepArgs {
     const float *__restrict__ keep[6];
     const float *__restrict__ push[6];
     // probes
-    uint32_t probes;
+    uint32_t probes, probe_blocks;  // probe_blocks: whole multiples of the dies, ahead of the rest
     const uint32_t *__restrict__ probe_start;
     const uint32_t *__restrict__ probe_cell;
     const double *__restrict__ probe_weight;
@@ -595,12 +595,18 @@ __device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t blo
                   products);
 }
 
-// One step in one launch: special blocks first (their gathers are the longest chains of
-// loads, so they start first), then face specials by blocks of their own (unless they are
-// stepped in their tiles), interior tiles, layer cells and the probes.
+// One step in one launch: the probes first (a serial sum each, read from the current
+// field, which nothing writes), then special blocks (their gathers are the longest chains
+// of loads), face specials by blocks of their own (unless they are stepped in their
+// tiles), interior tiles and layer cells.
 template <typename T, bool WIDE, bool SCALED>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
     uint32_t block = blockIdx.x;
+    if (block < s.probe_blocks) {
+        record_probes<T>(s, block);
+        return;
+    }
+    block -= s.probe_blocks;
     if (block < s.special_blocks) {
         special_cell<T>(s, block);
         return;
@@ -622,8 +628,6 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
         if (c < s.layer_cells) layer_cell<T>(s, c);
         return;
     }
-    block -= s.layer_blocks;
-    record_probes<T>(s, block);
 }
 
 // The same work as four launches, one per kind of cell (the split launch shape).
@@ -1713,6 +1717,8 @@ StepArgs<T> step_args(const as_fdtd *h) {
         s.push[k] = h->push[k];
     }
     s.probes = h->probes;
+    s.probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
+    s.probe_blocks = (s.probe_blocks + kXcds - 1) / kXcds * kXcds;
     s.probe_start = h->probe_start;
     s.probe_cell = h->probe_cell;
     s.probe_weight = h->probe_weight;
@@ -1762,8 +1768,8 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned 
 
 template <typename T>
 uint32_t block_count(const as_fdtd *h, const StepArgs<T> &s) {
-    const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
-    return s.tile_blocks + s.special_blocks + s.face_blocks + s.layer_blocks + probe_blocks;
+    (void)h;
+    return s.probe_blocks + s.special_blocks + s.face_blocks + s.tile_blocks + s.layer_blocks;
 }
 
 // The two-step launch's arguments, its probes writing where the one-step launches' do.
@@ -2175,10 +2181,12 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
 
     StepArgs<T> r = step_args<T>(h);  // recompute: no probes, history on
     r.probes = 0;
+    r.probe_blocks = 0;
     r.history = history;
     const uint32_t rblocks = block_count(h, r);
     StepArgs<T> a = step_args<T>(h);  // adjoint
     a.probes = 0;
+    a.probe_blocks = 0;
     a.acc_g = acc_g;
     a.acc_gain = acc_gain;
     const uint32_t ablocks = a.special_blocks + a.tile_blocks;
