@@ -768,33 +768,28 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(const StepArgs<T> s,
     }
 }
 
-// ---- Two steps a launch ----------------------------------------------------------------
+// ---- Two steps a launch: `march_kernel` (opt-in) ----------------------------------------
 //
 // Steps k and k + 1 of a grid whose cells are interior, face specials or solid (no matched
-// layer, no general specials, one speed of sound, 7 points), in float. A block owns a box
-// of 64 x kPairY x kPairZ cells. It copies p(k) on the box and two cells round it into
-// LDS, by direct global-to-LDS loads (no registers, all in flight at once); steps the box
-// and one cell round it to p(k + 1) in LDS, with p(k - 1) and the classes read from memory
-// and the face specials of that region from the box's records; then steps the box itself
-// to p(k + 2), writing p(k + 1) and p(k + 2) of its cells and its own face specials' branch
-// velocities w(k + 1) and w(k + 2). p(k - 1), p(k), w(k - 1) and w(k) are read by every
-// block (the rings overlap) and written by none: the outputs go to the other two fields
-// and state arrays of four. Each cell's arithmetic is the one-step kernels', in their
-// order, so the fields are the same bit for bit. The same launch's probe blocks record
-// p(k - 1) and p(k), the fields it reads; the step after a run of such launches is
-// recorded by a launch of the probes alone.
-constexpr int kPairY = 8, kPairZ = 8;
-constexpr int kPX = kTileX + 4, kPY = kPairY + 4, kPZ = kPairZ + 4;  // p(k): two cells round
-constexpr int kQX = kTileX + 2, kQY = kPairY + 2, kQZ = kPairZ + 2;  // p(k + 1): one cell round
-constexpr int kPXY = kPX * kPY, kQXY = kQX * kQY;
-constexpr int kPCells = kPXY * kPZ;
-constexpr int kPAlloc = (kPCells + kTileX - 1) / kTileX * kTileX;  // whole wavefronts of loads
-constexpr int kQCells = kQXY * kQZ;
-constexpr int kOCells = kTileX * kPairY * kPairZ;
-constexpr int kQPer = (kQCells + kBlock - 1) / kBlock;
-constexpr int kOPer = (kOCells + kBlock - 1) / kBlock;
-static_assert(kQX < 256 && kQY < 256 && kQZ < 256, "a record's box coordinates are bytes");
-
+// layer, no general specials, one speed of sound, 7 points), in float. Each block owns a
+// column of 60 x 8 x march_z cells and marches it plane by plane, keeping only a few
+// planes in LDS:
+// - p(k) of the column and two cells round it, a plane of 64 x 12 cells (the block's
+//   wavefronts load whole rows of 64, a row's lanes 0 to 63 the columns x0 - 2 to x0 + 61);
+//   four planes in LDS, loaded by direct global-to-LDS loads one plane ahead;
+// - p(k + 1) of the column and one cell round it (lanes 1 to 62, 10 rows), three planes;
+// - the classes of those cells, two planes.
+// Plane j of p(k + 1) (z = z0 - 1 + j) is stepped from p(k) planes j - 1, j, j + 1 (in LDS),
+// p(k - 1) and the classes (read a plane ahead into registers) and plane j's face specials
+// from the block's records; then plane j - 1 of the owned cells (lanes 2 to 61, rows 1 to 8)
+// is stepped to p(k + 2) from p(k + 1) planes j - 2, j - 1, j, and p(k + 1) and p(k + 2) of
+// its cells are written, with its own face specials' branch velocities w(k + 1) and
+// w(k + 2). Two barriers a plane. p(k - 1), p(k), w(k - 1) and w(k) are read by every block
+// (the rings overlap) and written by none: the outputs go to the other two fields and state
+// arrays of four. Each cell's arithmetic is the one-step kernels', in their order, so the
+// fields are the same bit for bit. The same launch's probe blocks record p(k - 1) and p(k),
+// the fields it reads; the step after a run of such launches is recorded by a launch of the
+// probes alone.
 struct PairArgs {
     const float *__restrict__ prev;  // p(k - 1)
     const float *__restrict__ cur;   // p(k)
@@ -804,11 +799,11 @@ struct PairArgs {
     float *__restrict__ wout1, *__restrict__ wout2;       // w(k + 1), w(k + 2)
     const float *__restrict__ zero;  // a float 0 in device memory, read for cells off the grid
     const uint8_t *__restrict__ types;
-    const uint32_t *__restrict__ boxes;  // per box block: its box, x fastest, or kNoTile
+    const uint32_t *__restrict__ boxes;  // per box block: its column, x fastest, or kNoTile
     uint32_t box_blocks, boxes_x, boxes_y;
-    const uint32_t *__restrict__ box_faces;  // per box block: first record, first not its own, end
-    // A record: its cell's coordinates in the p(k + 1) box (x | y << 8 | z << 16), then as
-    // a tile's face record: links | states << 6 | coefficients << 10, states' start, kinds.
+    // A record: its cell's lane, row and plane in the column's p(k + 1) region
+    // (lane | row << 8 | plane << 16), then as a tile's face record: links | states << 6 |
+    // coefficients << 10, states' start, kinds.
     const uint4 *__restrict__ pair_record;
     const uint16_t *__restrict__ face_state_kind;
     const float4 *__restrict__ face_coef;
@@ -824,6 +819,12 @@ struct PairArgs {
     const double *__restrict__ probe_weight;
     double *__restrict__ out;
     unsigned long long out_stride;
+    // Planes a column marches through, and per box block its records' bounds by plane
+    // (plane_stride entries a block: per plane its first record and its first ring record,
+    // then one past the last plane's records).
+    int march_z;
+    const uint32_t *__restrict__ plane_faces;
+    uint32_t plane_stride;
 };
 
 // A face special's step (face_special's arithmetic, in its order) from its pressure, its
@@ -890,10 +891,20 @@ __device__ __forceinline__ float pair_face(const PairArgs &s, const uint4 rec, f
     return next;
 }
 
-__global__ __launch_bounds__(kBlock) void pair_kernel(const PairArgs s) {
-    __shared__ __align__(16) float P[kPAlloc];  // p(k) on the box and two cells round
-    __shared__ float Q[kQCells];                // p(k + 1) on the box and one cell round
-    __shared__ uint8_t C[kQCells];              // the classes of Q's cells (0 off the grid)
+constexpr int kMarchX = kTileX - 4;            // columns a box owns (lanes 2 to 61)
+constexpr int kMarchY = 8;                     // rows a box owns
+constexpr int kMarchPRows = kMarchY + 4;       // p(k): two rows round
+constexpr int kMarchQRows = kMarchY + 2;       // p(k + 1): one row round
+constexpr int kMarchP = kMarchPRows * kTileX;  // a plane of p(k) in LDS
+constexpr int kMarchQ = kMarchQRows * kTileX;  // a plane of p(k + 1) (or classes) in LDS
+constexpr int kMarchRowsPerWave = (kMarchPRows + kBlock / kTileX - 1) / (kBlock / kTileX);
+static_assert(kMarchPRows % (kBlock / kTileX) == 0, "each wavefront loads whole rows of p(k)");
+static_assert(kMarchY % (kBlock / kTileX) == 0, "each wavefront steps whole owned rows");
+
+__global__ __launch_bounds__(kBlock) void march_kernel(const PairArgs s) {
+    __shared__ __align__(16) float P[4 * kMarchP];
+    __shared__ float Q[3 * kMarchQ];
+    __shared__ uint8_t C[2 * kMarchQ];
     const uint32_t t = threadIdx.x;
     if (blockIdx.x >= s.box_blocks) {
         const uint32_t block = blockIdx.x - s.box_blocks;
@@ -911,113 +922,140 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(const PairArgs s) {
     const int bx = static_cast<int>(entry % s.boxes_x);
     const int rest = static_cast<int>(entry / s.boxes_x);
     const int by = rest % static_cast<int>(s.boxes_y), bz = rest / static_cast<int>(s.boxes_y);
-    const int x0 = bx * kTileX, y0 = 1 + by * kPairY, z0 = 1 + bz * kPairZ;
     const int px = s.px, sxy = s.sxy, ny = s.ny, nz = s.nz;
+    const int x0 = bx * kMarchX, y0 = 1 + by * kMarchY, z0 = 1 + bz * s.march_z;
+    const int planes = min(s.march_z, nz - 1 - z0);  // owned planes z0 .. z0 + planes - 1
     const int wave = static_cast<int>(t / kTileX), lane = static_cast<int>(t % kTileX);
+    const int gx = x0 - 2 + lane;  // this lane's column
+    const bool x_in = gx >= 0 && gx < px;
+    const bool q_lane = lane >= 1 && lane <= kTileX - 2;
+    const bool o_lane = lane >= 2 && lane <= kTileX - 3;
 
-    // p(k) into P, a wavefront's 64 consecutive cells a load; cells off the grid read 0.
-    // (Cells past kPCells, in the last load's tail, land in P's padding.)
-    for (int f0 = wave * kTileX; f0 < kPAlloc; f0 += kBlock) {
-        const int f = f0 + lane;
-        const int ex = f % kPX, ey = (f / kPX) % kPY, ez = f / kPXY;
-        const int x = x0 - 2 + ex, y = y0 - 2 + ey, z = z0 - 2 + ez;
-        const bool in = f < kPCells && x >= 0 && x < px && y >= 0 && y < ny && z >= 0 && z < nz;
-        const float *src = in ? s.cur + (x + y * px + z * sxy) : s.zero;
-        __builtin_amdgcn_global_load_lds(src, P + f0, 4, 0, 0);
-    }
-    // Meanwhile, this thread's cells of the Q box: their classes and p(k - 1).
-    float pre[kQPer];
-    uint32_t cls[kQPer];
+    // p(k) plane i (z = z0 - 2 + i) into its LDS slot; cells off the grid read 0.
+    const auto load_p = [&](int i) {
+        const int z = z0 - 2 + i;
+        float *slot = P + (i & 3) * kMarchP;
 #pragma unroll
-    for (int j = 0; j < kQPer; ++j) {
-        const int q = static_cast<int>(t) + j * kBlock;
-        pre[j] = 0.0f;
-        cls[j] = 0;
-        const int qx = q % kQX, qy = (q / kQX) % kQY, qz = q / kQXY;
-        const int x = x0 - 1 + qx, y = y0 - 1 + qy, z = z0 - 1 + qz;
-        if (q < kQCells && x >= 0 && x < px && y >= 0 && y < ny && z >= 0 && z < nz) {
-            const int g = x + y * px + z * sxy;
-            cls[j] = s.types[g];
-            pre[j] = s.prev[g];
+        for (int m = 0; m < kMarchRowsPerWave; ++m) {
+            const int row = wave + m * (kBlock / kTileX);
+            const int y = y0 - 2 + row;
+            const bool in = x_in && y >= 0 && y < ny && z >= 0 && z < nz;
+            const float *src = in ? s.cur + (gx + y * px + z * sxy) : s.zero;
+            __builtin_amdgcn_global_load_lds(src, slot + row * kTileX, 4, 0, 0);
         }
-    }
-    // And this thread's first face record.
-    const uint32_t r0 = s.box_faces[3 * blockIdx.x] + t, r_own = s.box_faces[3 * blockIdx.x + 1],
-                   r_end = s.box_faces[3 * blockIdx.x + 2];
-    uint4 first = make_uint4(0u, 0u, 0u, 0u);
-    if (r0 < r_end) first = s.pair_record[r0];
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-
-    // Step k on the Q box: interior cells, solid cells (0), then the face specials.
+    };
+    // p(k - 1) and the classes of this thread's cells of p(k + 1) plane j: rows wave,
+    // wave + 4, wave + 8 (those below 10) of its lane.
+    float pre[kMarchRowsPerWave], pre_next[kMarchRowsPerWave];
+    uint32_t cls[kMarchRowsPerWave], cls_next[kMarchRowsPerWave];
+    const auto load_q = [&](int j, float (&pr)[kMarchRowsPerWave], uint32_t (&cl)[kMarchRowsPerWave]) {
+        const int z = z0 - 1 + j;
 #pragma unroll
-    for (int j = 0; j < kQPer; ++j) {
-        const int q = static_cast<int>(t) + j * kBlock;
-        if (q < kQCells) {
-            C[q] = static_cast<uint8_t>(cls[j]);
-            if (cls[j] == kClassInterior) {
-                const int qx = q % kQX, qy = (q / kQX) % kQY, qz = q / kQXY;
-                const int c = (qx + 1) + (qy + 1) * kPX + (qz + 1) * kPXY;
-                // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
-                float sum = P[c - 1] + P[c + 1];
-                sum = sum + P[c - kPX];
-                sum = sum + P[c + kPX];
-                sum = sum + P[c - kPXY];
-                sum = sum + P[c + kPXY];
-                const float next = s.centre * P[c] + s.axial * sum;
-                Q[q] = next - pre[j];
-            } else if (cls[j] != kClassFace) {
-                Q[q] = 0.0f;
+        for (int m = 0; m < kMarchRowsPerWave; ++m) {
+            const int r = wave + m * (kBlock / kTileX);
+            const int y = y0 - 1 + r;
+            pr[m] = 0.0f;
+            cl[m] = 0;
+            if (q_lane && r < kMarchQRows && x_in && y < ny && z < nz) {
+                const int g = gx + y * px + z * sxy;
+                pr[m] = s.prev[g];
+                cl[m] = s.types[g];
             }
         }
-    }
-    const float sig0 = s.signal[s.step];
-    for (uint32_t r = r0; r < r_end; r += kBlock) {
-        const uint4 rec = r == r0 ? first : s.pair_record[r];
-        const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu),
-                  qz = static_cast<int>((rec.x >> 16) & 0xFFu);
-        const int g = (x0 - 1 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + qz) * sxy;
-        const int c = (qx + 1) + (qy + 1) * kPX + (qz + 1) * kPXY;
-        const float nb[6] = {P[c - 1], P[c + 1], P[c - kPX], P[c + kPX], P[c - kPXY], P[c + kPXY]};
-        const bool own = r < r_own;
-        const float next = pair_face(s, rec, P[c], s.prev[g], nb, s.wprev, s.wcur, own ? s.wout1 : nullptr, sig0);
-        Q[qx + qy * kQX + qz * kQXY] = next;
-        if (own) s.out1[g] = next;
-    }
-    __syncthreads();
-
-    // Step k + 1 on the box: its interior cells, then its own face specials (the same
-    // records, so the same threads, as above: each reads back the w(k + 1) it wrote).
+    };
+    load_p(0);
+    load_p(1);
+    load_p(2);
+    load_q(0, pre, cls);
+    const uint32_t *pf = s.plane_faces + static_cast<size_t>(blockIdx.x) * s.plane_stride;
+    const float sig0 = s.signal[s.step], sig1 = s.signal[s.step + 1];
+    for (int j = 0; j <= planes + 1; ++j) {
+        // p(k) planes j .. j + 2 have landed, and no thread still reads the slot of plane
+        // j - 1, which plane j + 3 takes next.
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (j <= planes) load_p(j + 3);
+        if (j <= planes) load_q(j + 1, pre_next, cls_next);
+        const float *Pm = P + (j & 3) * kMarchP, *Pc = P + ((j + 1) & 3) * kMarchP,
+                    *Pp = P + ((j + 2) & 3) * kMarchP;
+        float *Qj = Q + (j % 3) * kMarchQ;
+        uint8_t *Cj = C + (j & 1) * kMarchQ;
+        // Step k on plane j: interior cells, solid cells (0), then the face specials.
 #pragma unroll
-    for (int j = 0; j < kOPer; ++j) {
-        const int o = static_cast<int>(t) + j * kBlock;
-        if (o < kOCells) {
-            const int ox = o % kTileX, oy = (o / kTileX) % kPairY, oz = o / (kTileX * kPairY);
-            const int q = (ox + 1) + (oy + 1) * kQX + (oz + 1) * kQXY;
-            if (C[q] == kClassInterior) {
-                const int c = (ox + 2) + (oy + 2) * kPX + (oz + 2) * kPXY;
-                float sum = Q[q - 1] + Q[q + 1];
-                sum = sum + Q[q - kQX];
-                sum = sum + Q[q + kQX];
-                sum = sum + Q[q - kQXY];
-                sum = sum + Q[q + kQXY];
-                const float next = s.centre * Q[q] + s.axial * sum;
-                const int g = (x0 + ox) + (y0 + oy) * px + (z0 + oz) * sxy;
-                s.out1[g] = Q[q];
-                s.out2[g] = next - P[c];
+        for (int m = 0; m < kMarchRowsPerWave; ++m) {
+            const int r = wave + m * (kBlock / kTileX);
+            if (r < kMarchQRows) {
+                const int q = r * kTileX + lane;
+                const int c = q + kTileX;  // the same cell in the p(k) plane
+                if (cls[m] == kClassInterior) {
+                    // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
+                    float sum = Pc[c - 1] + Pc[c + 1];
+                    sum = sum + Pc[c - kTileX];
+                    sum = sum + Pc[c + kTileX];
+                    sum = sum + Pm[c];
+                    sum = sum + Pp[c];
+                    const float next = s.centre * Pc[c] + s.axial * sum;
+                    Qj[q] = next - pre[m];
+                } else if (cls[m] != kClassFace) {
+                    Qj[q] = 0.0f;
+                }
+                Cj[q] = static_cast<uint8_t>(cls[m]);
             }
         }
-    }
-    const float sig1 = s.signal[s.step + 1];
-    for (uint32_t r = r0; r < r_own; r += kBlock) {
-        const uint4 rec = r == r0 ? first : s.pair_record[r];
-        const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu),
-                  qz = static_cast<int>((rec.x >> 16) & 0xFFu);
-        const int g = (x0 - 1 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + qz) * sxy;
-        const int q = qx + qy * kQX + qz * kQXY;
-        const int c = (qx + 1) + (qy + 1) * kPX + (qz + 1) * kPXY;
-        const float nb[6] = {Q[q - 1], Q[q + 1], Q[q - kQX], Q[q + kQX], Q[q - kQXY], Q[q + kQXY]};
-        s.out2[g] = pair_face(s, rec, Q[q], P[c], nb, s.wcur, s.wout1, s.wout2, sig1);
+        {
+            const uint32_t r_own = pf[2 * j + 1], r_end = pf[2 * j + 2];
+            for (uint32_t r = pf[2 * j] + t; r < r_end; r += kBlock) {
+                const uint4 rec = s.pair_record[r];
+                const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu);
+                const int q = qy * kTileX + qx, c = q + kTileX;
+                const int g = (x0 - 2 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + j) * sxy;
+                const float nb[6] = {Pc[c - 1], Pc[c + 1], Pc[c - kTileX], Pc[c + kTileX], Pm[c], Pp[c]};
+                const bool own = r < r_own;
+                const float next = pair_face(s, rec, Pc[c], s.prev[g], nb, s.wprev, s.wcur, own ? s.wout1 : nullptr, sig0);
+                Qj[q] = next;
+                if (own) s.out1[g] = next;
+            }
+        }
+        __syncthreads();
+        if (j >= 2) {
+            // Step k + 1 on the owned cells of plane j - 1, then its own face specials (the
+            // same records, so the same threads, as its step k: each reads back the
+            // w(k + 1) it wrote).
+            const int jo = j - 1;
+            const float *Qm = Q + ((jo + 2) % 3) * kMarchQ, *Qc = Q + (jo % 3) * kMarchQ, *Qp = Qj;
+            const uint8_t *Cc = C + (jo & 1) * kMarchQ;
+            const float *Po = P + (j & 3) * kMarchP;  // p(k) plane of the same z
+#pragma unroll
+            for (int m = 0; m < kMarchY / (kBlock / kTileX); ++m) {
+                const int r = 1 + wave + m * (kBlock / kTileX);
+                const int q = r * kTileX + lane;
+                if (o_lane && Cc[q] == kClassInterior) {
+                    float sum = Qc[q - 1] + Qc[q + 1];
+                    sum = sum + Qc[q - kTileX];
+                    sum = sum + Qc[q + kTileX];
+                    sum = sum + Qm[q];
+                    sum = sum + Qp[q];
+                    const float next = s.centre * Qc[q] + s.axial * sum;
+                    const int g = gx + (y0 - 1 + r) * px + (z0 - 1 + jo) * sxy;
+                    s.out1[g] = Qc[q];
+                    s.out2[g] = next - Po[q + kTileX];
+                }
+            }
+            const uint32_t r_own = pf[2 * jo + 1];
+            for (uint32_t r = pf[2 * jo] + t; r < r_own; r += kBlock) {
+                const uint4 rec = s.pair_record[r];
+                const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu);
+                const int q = qy * kTileX + qx;
+                const int g = (x0 - 2 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + jo) * sxy;
+                const float nb[6] = {Qc[q - 1], Qc[q + 1], Qc[q - kTileX], Qc[q + kTileX], Qm[q], Qp[q]};
+                s.out2[g] = pair_face(s, rec, Qc[q], Po[q + kTileX], nb, s.wcur, s.wout1, s.wout2, sig1);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < kMarchRowsPerWave; ++m) {
+            pre[m] = pre_next[m];
+            cls[m] = cls_next[m];
+        }
     }
 }
 
@@ -1049,10 +1087,13 @@ struct as_fdtd {
     // launch. p(k - 1) is in slot `prev` and p(k) in prev ^ 1 (the same for the velocities).
     void *field[4] = {}, *velocity[4] = {};
     int prev = 0;
-    int pairs = 0;  // 1: two steps a launch (see pair_kernel)
+    int pairs = 0;  // 1: two steps a launch (see march_kernel)
     uint32_t box_blocks = 0, boxes_x = 0, boxes_y = 0;
-    uint32_t *boxes = nullptr, *box_faces = nullptr;
+    uint32_t *boxes = nullptr;
     uint4 *pair_record = nullptr;
+    int march_z = 8;          // planes a march_kernel column owns
+    uint32_t *plane_faces = nullptr;
+    uint32_t plane_stride = 0;
     float *zero = nullptr;
     uint8_t *types = nullptr;
     float *scale = nullptr;
@@ -1070,7 +1111,7 @@ struct as_fdtd {
     uint16_t *face_state_kind = nullptr;
     float4 *face_coef = nullptr, *face_branch = nullptr;
     int split = 0;            // 1: four launches a step instead of one
-    int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order
+    int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order, 2: by die, columns first
     int no_faces = 0;         // 1: every special is general (the adjoint's handles)
     int face_blocks = 0;      // 1: face specials by blocks of their own, 0: in their tiles
     uint32_t *face_tile_none = nullptr;  // a face_tile of empty ranges, for face_blocks
@@ -1183,7 +1224,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (tuning && tuning->graph_steps != 0) h->graph_steps = tuning->graph_steps < 0 ? 0u : static_cast<uint32_t>(tuning->graph_steps);
     if (h->graph_steps > 4096) refuse("at most 4096 steps a graph");
     if (tuning && tuning->split > 0) h->split = 1;
-    if (tuning && tuning->tile_order > 0) h->tile_order = 1;
+    if (tuning && tuning->tile_order > 0) h->tile_order = tuning->tile_order == 2 ? 2 : 1;
     if (tuning && tuning->no_faces > 0) h->no_faces = 1;
     const bool pairs_on = tuning && tuning->pairs > 0;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
@@ -1318,7 +1359,17 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     // with its own L2: so that a die's tiles are neighbours, block r takes tile
     // (r mod 8) * per + r / 8 of the list (x fastest, then y, then z): die d steps the d-th
     // slab of planes, and only the slabs' boundary planes are read by two dies.
-    if (h->tile_order == 0 && !tiles.empty()) {
+    if (h->tile_order == 2) {
+        // Columns first: z fastest, then x, then y, so a tile's neighbours above and below
+        // run beside it on its die and their shared planes stay in that die's L2.
+        const auto column_key = [&](uint32_t entry) {
+            const uint32_t id = entry & ~kAllInterior;
+            const uint32_t tx = id % h->tiles_x, ty = (id / h->tiles_x) % h->tiles_y, tz = id / (h->tiles_x * h->tiles_y);
+            return (static_cast<uint64_t>(ty) * h->tiles_x + tx) * tiles_z + tz;
+        };
+        std::stable_sort(tiles.begin(), tiles.end(), [&](uint32_t a, uint32_t b) { return column_key(a) < column_key(b); });
+    }
+    if (h->tile_order != 1 && !tiles.empty()) {
         const size_t per = (tiles.size() + kXcds - 1) / kXcds;
         std::vector<uint32_t> ordered(per * kXcds, kNoTile);
         for (size_t r = 0; r < ordered.size(); ++r) {
@@ -1481,77 +1532,93 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     std::copy(face_state_kind.begin(), face_state_kind.end(), kinds.begin() + static_cast<ptrdiff_t>(state_slots));
     h->state_slots = face_slots;
 
-    // Two steps a launch (pair_kernel): a grid of interior cells, face specials and solid
+    // Two steps a launch (march_kernel): a grid of interior cells, face specials and solid
     // cells, in float, rows of whole wavefronts.
     h->pairs = pairs_on && precision == 32 && !h->wide && !h->scaled && L.layer_cells == 0 && G == 0 &&
                        px % kTileX == 0 ? 1 : 0;
-    std::vector<uint32_t> boxes, box_faces;
+    std::vector<uint32_t> boxes, plane_faces;
     std::vector<uint32_t> pair_records;
     if (h->pairs) {
-        h->boxes_x = static_cast<uint32_t>(px / kTileX);
-        h->boxes_y = static_cast<uint32_t>((ny - 2 + kPairY - 1) / kPairY);
-        const uint32_t boxes_z = static_cast<uint32_t>((nz - 2 + kPairZ - 1) / kPairZ);
+        // Columns of 60 x 8 x march_z cells (march_kernel), by die as the tiles; each face
+        // special goes to every column whose p(k + 1) region (one cell round) holds it, by
+        // plane, the column's own records of a plane first.
+        const int Z = tuning && tuning->tile_z > 0 ? tuning->tile_z : 8;
+        if (Z > 200) refuse("a column marches at most 200 planes");
+        h->march_z = Z;
+        h->boxes_x = static_cast<uint32_t>((nx + kMarchX - 1) / kMarchX);
+        h->boxes_y = static_cast<uint32_t>((ny - 2 + kMarchY - 1) / kMarchY);
+        const uint32_t boxes_z = static_cast<uint32_t>((nz - 2 + Z - 1) / Z);
         const uint64_t box_total = static_cast<uint64_t>(h->boxes_x) * h->boxes_y * boxes_z;
         if (box_total >= kNoTile) refuse("too many boxes");
-        // Boxes with a cell to step.
         std::vector<uint8_t> active(box_total, 0);
         for (uint64_t z = 1; z + 1 < nz; ++z)
             for (uint64_t y = 1; y + 1 < ny; ++y)
                 for (uint64_t x = 0; x < nx; ++x) {
                     const uint8_t c = types[x + px * (y + ny * z)];
                     if (c == kClassInterior || c == kClassFace) {
-                        active[x / kTileX + h->boxes_x * ((y - 1) / kPairY + h->boxes_y * ((z - 1) / kPairZ))] = 1;
+                        active[x / kMarchX + h->boxes_x * ((y - 1) / kMarchY + h->boxes_y * ((z - 1) / Z))] = 1;
                     }
                 }
         std::vector<uint32_t> list;
         for (uint64_t b = 0; b < box_total; ++b)
             if (active[b]) list.push_back(static_cast<uint32_t>(b));
-        // By die, as the tiles.
         const size_t per = (list.size() + kXcds - 1) / kXcds;
         boxes.assign(per * kXcds, kNoTile);
-        if (h->tile_order == 0) {
-            for (size_t r = 0; r < boxes.size(); ++r) {
-                const size_t idx = (r % kXcds) * per + r / kXcds;
-                if (idx < list.size()) boxes[r] = list[idx];
-            }
-        } else {
-            std::copy(list.begin(), list.end(), boxes.begin());
+        for (size_t r = 0; r < boxes.size(); ++r) {
+            const size_t idx = h->tile_order != 1 ? (r % kXcds) * per + r / kXcds : r;
+            if (idx < list.size()) boxes[r] = list[idx];
         }
         std::vector<uint32_t> block_of(box_total, kNoTile);
         for (size_t r = 0; r < boxes.size(); ++r)
             if (boxes[r] != kNoTile) block_of[boxes[r]] = static_cast<uint32_t>(r);
-        // Each face special goes to every box whose ring of one cell holds it: its own
-        // box's records first, then the others'.
-        std::vector<std::vector<std::array<uint32_t, 4>>> own(boxes.size()), ring(boxes.size());
+        struct Placed {
+            uint32_t block, plane, ring;
+            std::array<uint32_t, 4> rec;
+        };
+        std::vector<Placed> placed;
         for (const auto &bc : by_cell) {
             const uint64_t cell = bc.first, k = bc.second;
             const size_t r = face_of[k];
             const int64_t x = static_cast<int64_t>(cell % px), y = static_cast<int64_t>((cell / px) % ny),
                           z = static_cast<int64_t>(cell / (px * ny));
-            for (int64_t bz = (z - 2) / kPairZ; bz <= z / kPairZ; ++bz)
-                for (int64_t by = (y - 2) / kPairY; by <= y / kPairY; ++by)
-                    for (int64_t bx = (x - 1) / kTileX; bx <= (x + 1) / kTileX; ++bx) {
+            for (int64_t bz = (z - 1) / Z - 1; bz <= (z - 1) / Z + 1; ++bz)
+                for (int64_t by = (y - 1) / kMarchY - 1; by <= (y - 1) / kMarchY + 1; ++by)
+                    for (int64_t bx = x / kMarchX - 1; bx <= x / kMarchX + 1; ++bx) {
                         if (bx < 0 || by < 0 || bz < 0 || bx >= h->boxes_x || by >= h->boxes_y || bz >= boxes_z) continue;
-                        const int64_t x0 = bx * kTileX, y0 = 1 + by * kPairY, z0 = 1 + bz * kPairZ;
-                        const int64_t qx = x - (x0 - 1), qy = y - (y0 - 1), qz = z - (z0 - 1);
-                        if (qx < 0 || qx >= kQX || qy < 0 || qy >= kQY || qz < 0 || qz >= kQZ) continue;
+                        const int64_t x0 = bx * kMarchX, y0 = 1 + by * kMarchY, z0 = 1 + bz * Z;
+                        const int64_t planes = std::min<int64_t>(Z, static_cast<int64_t>(nz) - 1 - z0);
+                        const int64_t qx = x - (x0 - 2), qy = y - (y0 - 1), j = z - (z0 - 1);
+                        if (qx < 1 || qx > kTileX - 2 || qy < 0 || qy >= kMarchQRows || j < 0 || j > planes + 1) continue;
                         const uint32_t blk = block_of[static_cast<uint64_t>(bx) + h->boxes_x * (static_cast<uint64_t>(by) + h->boxes_y * static_cast<uint64_t>(bz))];
                         if (blk == kNoTile) continue;
-                        const std::array<uint32_t, 4> rec{static_cast<uint32_t>(qx | qy << 8 | qz << 16), face_desc[r],
-                                                          face_state[r], face_kinds[r]};
-                        const bool mine = qx >= 1 && qx <= kTileX && qy >= 1 && qy <= kPairY && qz >= 1 && qz <= kPairZ;
-                        (mine ? own : ring)[blk].push_back(rec);
+                        const bool mine = qx >= 2 && qx <= kTileX - 3 && qy >= 1 && qy <= kMarchY && j >= 1 && j <= planes;
+                        placed.push_back({blk, static_cast<uint32_t>(j), mine ? 0u : 1u,
+                                          {static_cast<uint32_t>(qx | qy << 8 | j << 16), face_desc[r], face_state[r],
+                                           face_kinds[r]}});
                     }
         }
-        box_faces.assign(3 * boxes.size(), 0);
+        std::stable_sort(placed.begin(), placed.end(), [](const Placed &a, const Placed &b) {
+            return a.block != b.block ? a.block < b.block : a.plane != b.plane ? a.plane < b.plane : a.ring < b.ring;
+        });
+        if (placed.size() >= (1ull << 31)) throw Failure{AS_ERANGE, "too many face records"};
+        h->plane_stride = static_cast<uint32_t>(2 * (Z + 2) + 1);
+        plane_faces.assign(boxes.size() * h->plane_stride, 0);
+        size_t at = 0;
         for (size_t b = 0; b < boxes.size(); ++b) {
-            box_faces[3 * b] = static_cast<uint32_t>(pair_records.size() / 4);
-            for (const auto &rec : own[b]) pair_records.insert(pair_records.end(), rec.begin(), rec.end());
-            box_faces[3 * b + 1] = static_cast<uint32_t>(pair_records.size() / 4);
-            for (const auto &rec : ring[b]) pair_records.insert(pair_records.end(), rec.begin(), rec.end());
-            box_faces[3 * b + 2] = static_cast<uint32_t>(pair_records.size() / 4);
-            if (pair_records.size() / 4 >= (1ull << 31)) throw Failure{AS_ERANGE, "too many face records"};
+            uint32_t *row = plane_faces.data() + b * h->plane_stride;
+            for (int j = 0; j < Z + 2; ++j) {
+                row[2 * j] = static_cast<uint32_t>(at);
+                while (at < placed.size() && placed[at].block == b && placed[at].plane == static_cast<uint32_t>(j) &&
+                       placed[at].ring == 0)
+                    ++at;
+                row[2 * j + 1] = static_cast<uint32_t>(at);
+                while (at < placed.size() && placed[at].block == b && placed[at].plane == static_cast<uint32_t>(j)) ++at;
+            }
+            row[2 * (Z + 2)] = static_cast<uint32_t>(at);
         }
+        if (at != placed.size()) throw Failure{AS_EINVAL, "internal: a face record was not placed"};
+        pair_records.reserve(4 * placed.size());
+        for (const auto &pl : placed) pair_records.insert(pair_records.end(), pl.rec.begin(), pl.rec.end());
         h->box_blocks = static_cast<uint32_t>(boxes.size());
     }
 
@@ -1650,7 +1717,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             h->velocity[k] = zeros(h, face_slots * e);
         }
         h->boxes = upload(h, boxes);
-        h->box_faces = upload(h, box_faces);
+        h->plane_faces = upload(h, plane_faces);
         h->pair_record = reinterpret_cast<uint4 *>(upload(h, pair_records));
         h->zero = static_cast<float *>(zeros(h, sizeof(float)));
     }
@@ -1781,7 +1848,6 @@ PairArgs pair_args(const as_fdtd *h, const StepArgs<float> &s) {
     a.box_blocks = h->box_blocks;
     a.boxes_x = h->boxes_x;
     a.boxes_y = h->boxes_y;
-    a.box_faces = h->box_faces;
     a.pair_record = h->pair_record;
     a.face_state_kind = h->face_state_kind;
     a.face_coef = h->face_coef;
@@ -1799,6 +1865,9 @@ PairArgs pair_args(const as_fdtd *h, const StepArgs<float> &s) {
     a.probe_weight = s.probe_weight;
     a.out = s.out;
     a.out_stride = s.out_stride;
+    a.march_z = h->march_z;
+    a.plane_faces = h->plane_faces;
+    a.plane_stride = h->plane_stride;
     return a;
 }
 
@@ -1825,7 +1894,7 @@ int enqueue_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t k
                 a.step = k;
                 a.record_prev = k > k0 ? 1 : 0;
                 if (h->box_blocks + probe_blocks) {
-                    hipLaunchKernelGGL(pair_kernel, dim3(h->box_blocks + probe_blocks), dim3(kBlock), 0, stream, a);
+                    hipLaunchKernelGGL(march_kernel, dim3(h->box_blocks + probe_blocks), dim3(kBlock), 0, stream, a);
                 }
                 prev = next;
             }

@@ -86,11 +86,12 @@ typedef struct as_fdtd as_fdtd;
 /* Launch shape of the step kernels (0 = the library's default for the field). */
 typedef struct as_tuning {
     int tile_y;        /* rows of a tile (the tile is 64 cells along x) */
-    int tile_z;        /* planes a tile marches through */
+    int tile_z;        /* planes a tile marches through (with pairs, also a column's: default 8) */
     int row_align;     /* row pitch in floats: a multiple of this (1, 16, 32, 64) */
     int graph_steps;   /* steps a captured hipGraph replays (0 = 64; negative: no graphs) */
     int split;         /* 1: one launch per kind of cell (specials, interior, layer, probes) */
-    int tile_order;    /* 0: interior tiles grouped by die (XCD), 1: in grid order */
+    int tile_order;    /* 0: interior tiles grouped by die (XCD), 1: in grid order, 2: grouped by
+                          die, each die's in columns (z fastest, then x, then y) */
     int no_faces;      /* 1: step every special cell by its own blocks (as the adjoint needs) */
     int face_blocks;   /* the face specials (axial links of the interior's weight): positive, by
                           blocks of their own beside the tiles; negative, each tile's in the

@@ -65,14 +65,14 @@ def test_reset_starts_over(reference_available):
     assert np.array_equal(first, again)
 
 
-@pytest.mark.parametrize("tile_z,row_align", [(4, 1), (16, 16), (64, 64)])
-def test_launch_shape_does_not_change_results(reference_available, tile_z, row_align):
+@pytest.mark.parametrize("tile_z,row_align,tile_order", [(4, 1, 0), (16, 16, 0), (64, 64, 0), (4, 64, 1), (8, 64, 2)])
+def test_launch_shape_does_not_change_results(reference_available, tile_z, row_align, tile_order):
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
     scene = cases.small_box(stencil=7)
     layout = _layout(scene)
-    gpu = HipWaveSolver(layout, tile_z=tile_z, row_align=row_align)
+    gpu = HipWaveSolver(layout, tile_z=tile_z, row_align=row_align, tile_order=tile_order)
     cpu = OracleSolver(layout)
     assert np.array_equal(gpu.run(scene.drive[:200]), cpu.run(scene.drive[:200]))
 
@@ -97,17 +97,19 @@ def test_kernel_variants_equal_the_oracle(reference_available, pairs, face_block
         assert np.array_equal(gpu.field(), cpu.field())
 
 
-@pytest.mark.parametrize("pieces", [(1, 2, 63, 64, 65, 1, 130), (200,), (3, 3, 3)])
-def test_two_steps_a_launch_in_any_pieces(reference_available, pieces):
+@pytest.mark.parametrize("pieces,planes", [((1, 2, 63, 64, 65, 1, 130), 0), ((200,), 0), ((3, 3, 3), 0),
+                                           ((120,), 1), ((120,), 3), ((120,), 64)])
+def test_two_steps_a_launch_in_any_pieces(reference_available, pieces, planes):
     """Runs of one step, two, odd and even counts, shorter and longer than a graph: the
-    probes of every step and the fields match the oracle whatever the split into calls."""
+    probes of every step and the fields match the oracle whatever the split into calls.
+    The marching columns of 1, 3 or 64 planes (more than the grid has) give the same."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
     scene = cases.with_branches(cases.small_box())
     layout = _layout(scene)
-    print(f"pairs pieces {pieces}", flush=True)
-    gpu = HipWaveSolver(layout, pairs=True)
+    print(f"pairs pieces {pieces} planes {planes}", flush=True)
+    gpu = HipWaveSolver(layout, pairs=True, tile_z=planes)
     cpu = OracleSolver(layout)
     start = 0
     for n in pieces:
