@@ -53,6 +53,9 @@ constexpr uint32_t kNoTile = 0xFFFFFFFFu;    // a block with no tile
 // Cell classes on the device: the layout's 0 solid, 1 interior, 2 special (here: a general
 // special, stepped by its own blocks), 3 layer; and 4, a face special (stepped in its tile).
 constexpr uint8_t kClassInterior = 1, kClassFace = 4;
+// In the adjoint's own classes only: an interior cell that a receiver reads (see inj).
+constexpr uint8_t kClassTap = 5;
+constexpr uint32_t kNoTap = 0xFFFFFFFFu;
 
 constexpr int kSteps[18][3] = {
     {-1, 0, 0},  {1, 0, 0},  {0, -1, 0}, {0, 1, 0},  {0, 0, -1}, {0, 0, 1},
@@ -126,6 +129,12 @@ struct StepArgs {
     const T *__restrict__ delta;
     double *__restrict__ acc_g;
     double *__restrict__ acc_gain;
+    // The adjoint's receivers: the step computing Q(n) adds inj[u], dJ/dy of step n through
+    // tap u's weights (tap_values), where it writes a tap cell: tap_of[cell] at cells of class
+    // kClassTap, sp_tap[k] (or kNoTap) at special k.
+    const uint32_t *__restrict__ tap_of;
+    const uint32_t *__restrict__ sp_tap;
+    const T *__restrict__ inj;
 };
 
 // A face special (see the tiles' notes): a special cell whose links are axial and weigh
@@ -253,7 +262,7 @@ __device__ __forceinline__ void face_special(const StepArgs<T> &s, uint32_t r) {
 // twice a plane. Each address lies in the padded grid whatever the cell's class
 // (x < nx <= px, 1 <= y <= ny - 2, 1 <= z <= nz - 2), so the loads need no guard; only
 // interior cells are written.
-template <typename T, bool SCALED, bool ALL>
+template <typename T, bool SCALED, bool ALL, bool TAPS = false>
 __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int z0, int z1) {
     constexpr int kUnroll = 4;
     const int sy = s.px, sz = static_cast<int>(s.sxy);
@@ -280,7 +289,7 @@ __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            if (u < count && (ALL || cls[u] == kClassInterior)) {
+            if (u < count && (ALL || cls[u] == kClassInterior || (TAPS && cls[u] == kClassTap))) {
                 // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
                 T sum = xm[u] + xp[u];
                 sum = sum + ym[u];
@@ -296,7 +305,11 @@ __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int
                 } else {
                     next = s.centre * col[u + 1] + s.axial * sum;
                 }
-                s.a[i + u * sz] = next - prev[u];
+                T q = next - prev[u];
+                if constexpr (TAPS) {
+                    if (cls[u] == kClassTap) q = q + s.inj[s.tap_of[i + u * sz]];
+                }
+                s.a[i + u * sz] = q;
             }
         }
         if (count == kUnroll) {
@@ -306,7 +319,7 @@ __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int
     }
 }
 
-template <typename T, bool WIDE, bool SCALED>
+template <typename T, bool WIDE, bool SCALED, bool TAPS = false>
 __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t block) {
     const uint32_t entry = s.tiles[block];
     if (entry == kNoTile) return;
@@ -330,7 +343,7 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
             if (all) {
                 interior_column<T, SCALED, true>(s, i, z0, z1);
             } else {
-                interior_column<T, SCALED, false>(s, i, z0, z1);
+                interior_column<T, SCALED, false, TAPS>(s, i, z0, z1);
             }
         }
         if (!all) {
@@ -347,7 +360,8 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
     T here = b[i];
     for (int z = z0; z < z1; ++z, i += sz) {
         const T above = b[i + sz];
-        if (s.types[i] == kClassInterior) {
+        const uint8_t cls = s.types[i];
+        if (cls == kClassInterior || (TAPS && cls == kClassTap)) {
             // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
             T sum = b[i - 1] + b[i + 1];
             sum = sum + b[i - sy];
@@ -382,7 +396,11 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
                     next = next + s.diagonal * edges;
                 }
             }
-            s.a[i] = next - s.a[i];
+            T q = next - s.a[i];
+            if constexpr (TAPS) {
+                if (cls == kClassTap) q = q + s.inj[s.tap_of[i]];
+            }
+            s.a[i] = q;
         }
         below = here;
         here = above;
@@ -722,7 +740,12 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
         gsum += static_cast<T>(s.state_gain[q]) * (fresh - old);
         s.wa[q] = fresh;
     }
-    s.a[i] = ci * (ca * qb + sum - cb * qa + gsum);
+    T q = ci * (ca * qb + sum - cb * qa + gsum);
+    if (s.sp_tap != nullptr) {
+        const uint32_t u = s.sp_tap[k];
+        if (u != kNoTap) q = q + s.inj[u];
+    }
+    s.a[i] = q;
 }
 
 template <typename T, bool WIDE>
@@ -732,23 +755,24 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(const StepArgs<T> s) {
         adjoint_special<T>(s, block);
         return;
     }
-    interior_tile<T, WIDE, false>(s, block - s.special_blocks);
+    interior_tile<T, WIDE, false, true>(s, block - s.special_blocks);
 }
 
 // The probes' sensitivities into the adjoint field: Q[cell] += coef_inv (sum over the taps
 // on the cell of weight dJ/dy(probe, n)), one thread per distinct tap cell.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void inject_kernel(T *__restrict__ q, const uint32_t *__restrict__ cell,
-                                                       const double *__restrict__ ci, const uint32_t *__restrict__ start,
-                                                       const uint32_t *__restrict__ probe,
-                                                       const double *__restrict__ weight,
-                                                       const double *__restrict__ g, unsigned long long stride,
-                                                       unsigned long long n, uint32_t count) {
-    const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
-    if (u >= count) return;
+__global__ __launch_bounds__(kBlock) void tap_values(T *__restrict__ inj, const double *__restrict__ ci,
+                                                    const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ probe,
+                                                    const double *__restrict__ weight,
+                                                    const double *__restrict__ g, unsigned long long stride,
+                                                    unsigned long long first, uint32_t taps, uint32_t rows) {
+    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    if (id >= taps * rows) return;
+    const uint32_t u = id % taps, row = id / taps;
     double sum = 0.0;
-    for (uint32_t t = start[u]; t < start[u + 1]; ++t) sum += weight[t] * g[probe[t] * stride + n];
-    q[cell[u]] += static_cast<T>(ci[u] * sum);
+    for (uint32_t t = start[u]; t < start[u + 1]; ++t) sum += weight[t] * g[probe[t] * stride + first + row];
+    inj[id] = static_cast<T>(ci[u] * sum);
 }
 
 // The gradient's sums: each general special's D(n) Q(n) and each of its states' D(n) W(n).
@@ -2244,9 +2268,55 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
         if (!v.empty()) check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync");
         return p;
     };
-    uint32_t *d_tcell = put(tcell), *d_tstart = put(tstart), *d_tprobe = put(tprobe);
+    uint32_t *d_tstart = put(tstart), *d_tprobe = put(tprobe);
     double *d_tweight = put(tweight);
     double *d_tci = put(tci);
+    // Where each tap cell's dJ/dy enters: a special adds it in its own update (sp_tap); an
+    // interior cell becomes kClassTap in the adjoint's own copy of the classes, and its tile
+    // is no longer taken as all interior, so the column march adds it (tap_of). Solid cells
+    // are read by no update: their taps are left out.
+    std::vector<uint8_t> types_host(h->n_pad);
+    check(hipMemcpy(types_host.data(), h->types, h->n_pad, hipMemcpyDeviceToHost), "hipMemcpy");
+    std::vector<uint32_t> tiles_host(h->tile_count);
+    if (h->tile_count) {
+        check(hipMemcpy(tiles_host.data(), h->tiles, h->tile_count * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    std::vector<std::pair<uint64_t, uint32_t>> special_at;  // (cell, sorted special)
+    for (size_t k = 0; k < G; ++k) special_at.push_back({h->special_cell_host[h->sorted_to_special[k]], static_cast<uint32_t>(k)});
+    std::sort(special_at.begin(), special_at.end());
+    std::vector<uint32_t> sp_tap(G, kNoTap), tap_tiles;
+    std::vector<std::pair<uint32_t, uint32_t>> interior_taps;  // (cell, tap)
+    for (uint32_t u = 0; u < unique; ++u) {
+        const uint64_t c = tcell[u];
+        const auto it = std::lower_bound(special_at.begin(), special_at.end(), std::make_pair(c, uint32_t{0}));
+        if (it != special_at.end() && it->first == c) {
+            sp_tap[it->second] = u;
+        } else if (types_host[c] == kClassInterior) {
+            types_host[c] = kClassTap;
+            interior_taps.push_back({static_cast<uint32_t>(c), u});
+            const uint64_t x = c % static_cast<uint64_t>(h->px), y = (c / static_cast<uint64_t>(h->px)) % static_cast<uint64_t>(h->ny),
+                           z = c / static_cast<uint64_t>(h->sxy);
+            tap_tiles.push_back(static_cast<uint32_t>(x / kTileX + h->tiles_x * ((y - 1) / kTileRows + h->tiles_y * ((z - 1) / h->tile_z))));
+        }
+    }
+    std::sort(tap_tiles.begin(), tap_tiles.end());
+    for (auto &entry : tiles_host) {
+        if (entry != kNoTile && std::binary_search(tap_tiles.begin(), tap_tiles.end(), entry & ~kAllInterior)) entry &= ~kAllInterior;
+    }
+    uint8_t *d_types = nullptr;
+    uint32_t *d_tiles = nullptr, *d_tap_of = nullptr;
+    if (!interior_taps.empty()) {
+        d_types = put(types_host);
+        d_tiles = put(tiles_host);
+        d_tap_of = static_cast<uint32_t *>(alloc(h->n_pad * sizeof(uint32_t)));
+        std::vector<uint32_t> tap_of(h->n_pad, 0);  // read at the tap cells only
+        for (const auto &ct : interior_taps) tap_of[ct.first] = ct.second;
+        check(hipMemcpyAsync(d_tap_of, tap_of.data(), h->n_pad * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(h->stream), "adjoint");  // before tap_of goes
+    }
+    uint32_t *d_sp_tap = G ? put(sp_tap) : nullptr;
+    T *inj = static_cast<T *>(alloc((every + 1) * std::max<uint32_t>(unique, 1) * e));
 
     StepArgs<T> r = step_args<T>(h);  // recompute: no probes, history on
     r.probes = 0;
@@ -2258,9 +2328,15 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     a.probe_blocks = 0;
     a.acc_g = acc_g;
     a.acc_gain = acc_gain;
+    if (d_types != nullptr) {
+        a.types = d_types;
+        a.tiles = d_tiles;
+        a.tap_of = d_tap_of;
+    }
+    a.sp_tap = d_sp_tap;
     const uint32_t ablocks = a.special_blocks + a.tile_blocks;
     const uint32_t sblocks = (h->specials + kBlock - 1) / kBlock;
-    const uint32_t iblocks = (unique + kBlock - 1) / kBlock;
+
     // Segment k0 .. k1 - 1 recomputed from slots 0 (p(k0 - 1)) and 1 (p(k0)), its drive at sig.
     auto recompute = [&](size_t count, const float *sig) {
         r.signal = sig;
@@ -2271,6 +2347,14 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     // the last step's sums. m counts the adjoint steps before: its parity says which
     // field is which.
     auto backward = [&](size_t k0, size_t top, const double *g, size_t gstride, size_t gbase, unsigned long long m) {
+        // The segment's dJ/dy through the taps' weights, for n = k0 + 1 .. k0 + top (row
+        // n - k0 - 1 of inj).
+        if (unique > 0 && top > 0) {
+            const uint32_t rows = static_cast<uint32_t>(top), count = unique * rows;
+            hipLaunchKernelGGL((tap_values<T>), dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream, inj, d_tci,
+                               d_tstart, d_tprobe, d_tweight, g, static_cast<unsigned long long>(gstride),
+                               static_cast<unsigned long long>(k0 + 1 - gbase), unique, rows);
+        }
         for (size_t n = k0 + top; n >= k0 + 1; --n, ++m) {
             const bool even = (m & 1ull) == 0;
             a.a = even ? q0 : q1;
@@ -2278,17 +2362,13 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
             a.wa = even ? v0 : v1;
             a.wb = even ? v1 : v0;
             a.delta = n < k0 + top ? history + (n - k0) * G : nullptr;
+            a.inj = inj + (n - k0 - 1) * unique;
             if (ablocks) {
                 if (h->wide) {
                     hipLaunchKernelGGL((adjoint_kernel<T, true>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
                 } else {
                     hipLaunchKernelGGL((adjoint_kernel<T, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
                 }
-            }
-            if (iblocks) {
-                hipLaunchKernelGGL((inject_kernel<T>), dim3(iblocks), dim3(kBlock), 0, h->stream, a.a, d_tcell, d_tci,
-                                   d_tstart, d_tprobe, d_tweight, g, static_cast<unsigned long long>(gstride),
-                                   static_cast<unsigned long long>(n - gbase), unique);
             }
         }
         if (sblocks) {
