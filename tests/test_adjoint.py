@@ -13,10 +13,10 @@ import pytest
 import cases
 
 
-def _layout(stencil=7):
+def _layout(stencil=7, scene=None):
     from acoustic_spaces_amd.solver import WaveSolver
 
-    sc = cases.small_box(stencil=stencil)
+    sc = scene if scene is not None else cases.small_box(stencil=stencil)
     return sc, WaveSolver.from_scene(sc).layout()
 
 
@@ -56,12 +56,15 @@ def _fd(f, layout, drive, gain, c, which, index, h):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("stencil", [7, 19])
-def test_gradient_equals_finite_differences(stencil):
+@pytest.mark.parametrize("stencil,listeners", [(7, False), (19, False), (7, True)])
+def test_gradient_equals_finite_differences(stencil, listeners):
+    """On the small box's probes (every tap an interior cell), and on two listeners' 13
+    points each, one listener a cell from a wall: taps on special cells and interior cells,
+    cells shared by several probes' taps. The specials checked include the tap cells."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.adjoint import Forward
 
-    sc, layout = _layout(stencil)
+    sc, layout = _layout(stencil, cases.listeners13() if listeners else None)
     drive = sc.drive[:300]
     rng = np.random.default_rng(5)
     gpu = HipWaveSolver(layout, precision=64, no_faces=True)
@@ -74,7 +77,12 @@ def test_gradient_equals_finite_differences(stencil):
                                 coef_inv=np.asarray(layout["coef_inv"], np.float64)), rtol=0, atol=1e-9 * np.abs(y).max())
     scale_g = np.abs(sens_g).max()
     scale_gain = np.abs(sens_gain).max()
-    for k in rng.choice(len(g), 5, replace=False):
+    specials = rng.choice(len(g), 5, replace=False)
+    tapped = np.flatnonzero(np.isin(np.asarray(layout["special"]), np.asarray(layout["probe_cells"])))
+    if listeners:
+        assert len(tapped) > 0
+        specials = np.concatenate([specials, rng.choice(tapped, 3, replace=False)])
+    for k in specials:
         fd = _fd(f, layout, drive, gain, c, "g", k, 1e-6)
         assert abs(fd - sens_g[k]) <= 1e-6 * scale_g + 1e-5 * abs(fd)
     for j in rng.choice(len(gain), 5, replace=False):
