@@ -1139,17 +1139,12 @@ struct as_fdtd {
     int no_faces = 0;         // 1: every special is general (the adjoint's handles)
     int face_blocks = 0;      // 1: face specials by blocks of their own, 0: in their tiles
     uint32_t *face_tile_none = nullptr;  // a face_tile of empty ranges, for face_blocks
-    // The adjoint (see as_fdtd_gradient): its fields, accumulators, taps and checkpoints.
-    // The forward run it goes back through (as_fdtd_record): the drive on the device and
-    // per segment of rec_every steps the state before it (p(k - 1), p(k), w(k - 1), w(k)).
+    // The forward run the adjoint goes back through (as_fdtd_record): the drive on the
+    // device and per segment of rec_every steps the state before it (p(k - 1), p(k),
+    // w(k - 1), w(k)).
     size_t rec_steps = 0, rec_every = 0;
     char *rec_ckpt = nullptr;
     float *rec_signal = nullptr;
-    void *q0 = nullptr, *q1 = nullptr, *v0 = nullptr, *v1 = nullptr, *history = nullptr;
-    double *acc_g = nullptr, *acc_gain = nullptr;
-    uint32_t taps_unique = 0;
-    uint32_t *tap_cell = nullptr, *tap_start = nullptr, *tap_probe = nullptr;
-    double *tap_weight = nullptr;
     std::vector<uint64_t> sorted_to_special;  // general special (sorted) -> layout index
     std::vector<uint64_t> state_slot;         // layout state -> its slot
     std::vector<uint32_t> probe_start_host;
