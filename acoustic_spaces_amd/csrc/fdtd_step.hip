@@ -62,6 +62,31 @@ constexpr int kSteps[18][3] = {
     {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0},  {-1, 0, -1}, {1, 0, -1},
     {-1, 0, 1},  {1, 0, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1}};
 
+// A general special's links come in chunks of kLinkChunk; a chunk's directions are packed
+// into one word, five bits each, in the links' order.
+constexpr int kLinkChunk = 6;
+
+// Direction k's steps as (dx + 1) | (dy + 1) << 2 | (dz + 1) << 4, six bits a direction:
+// directions 0 to 9 in one word, 10 to 17 in another.
+constexpr uint64_t step_codes(int first, int count) {
+    uint64_t w = 0;
+    for (int d = 0; d < count; ++d) {
+        const int k = first + d;
+        const uint64_t code = static_cast<uint64_t>((kSteps[k][0] + 1) | (kSteps[k][1] + 1) << 2 | (kSteps[k][2] + 1) << 4);
+        w |= code << (6 * d);
+    }
+    return w;
+}
+constexpr uint64_t kStepsLow = step_codes(0, 10), kStepsHigh = step_codes(10, 8);
+
+// The cell offset of link direction d (0 to 17) on a grid of row pitch px and plane sxy.
+__device__ __forceinline__ int step_offset(uint32_t d, int px, int sxy) {
+    const uint64_t word = d < 10 ? kStepsLow : kStepsHigh;
+    const uint32_t code = static_cast<uint32_t>(word >> (6 * (d < 10 ? d : d - 10))) & 63u;
+    return (static_cast<int>(code & 3u) - 1) + (static_cast<int>((code >> 2) & 3u) - 1) * px +
+           (static_cast<int>((code >> 4) & 3u) - 1) * sxy;
+}
+
 // Everything one step's launch reads, by value (kernel arguments).
 template <typename T>
 struct StepArgs {
@@ -87,16 +112,17 @@ struct StepArgs {
     const uint8_t *__restrict__ sp_states;  // branch states of each special
     const uint32_t *__restrict__ link_base; // per block: its link slab
     const uint32_t *__restrict__ state_base;
-    const int32_t *__restrict__ link_offset;
+    const uint32_t *__restrict__ link_dirs;  // per chunk of links (see kLinkChunk), at link_base / kLinkChunk
     const float *__restrict__ link_weight;
     const float *__restrict__ coef_a;
     const T *__restrict__ coef_b;      // T: double overrides for precision 64 (as_layout)
     const T *__restrict__ coef_inv;
     const float *__restrict__ source;
     const float *__restrict__ state_weight;
-    const float *__restrict__ state_now;
-    const float *__restrict__ state_before;
-    const T *__restrict__ state_gain;
+    const uint16_t *__restrict__ state_branch;  // per state slot: its branch
+    const float *__restrict__ branch_now;        // per branch (fdtd.hpp:391-411)
+    const float *__restrict__ branch_before;
+    const T *__restrict__ branch_gain;
     const float *__restrict__ signal;  // device, indexed by step
     // face specials, stepped in the tile pass (see face_special)
     const uint32_t *__restrict__ face_tile;    // per tile block: its first record (and one past the last)
@@ -416,7 +442,9 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
 // links and states past the first, rare, follow one by one.
 template <typename T>
 __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t block) {
-    constexpr int kLinkChunk = 6, kStateChunk = 4;
+    // States taken with the first loads (the rest one by one, in the same order); three in
+    // double keep the step within five wavefronts a SIMD.
+    constexpr int kStateChunk = sizeof(T) == 8 ? 3 : 4;
     const uint32_t t = threadIdx.x;
     const uint32_t k = block * kBlock + t;
     if (k >= s.specials) return;
@@ -426,28 +454,38 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
     const T cb = s.coef_b[k], ci = s.coef_inv[k];
     const float sig = s.signal[s.step];
     const uint32_t lb = s.link_base[block] + t;
+    const uint32_t db = s.link_base[block] / kLinkChunk + t;
     const uint32_t sb = s.state_base[block] + t;
+    const int sy = s.px, sz = static_cast<int>(s.sxy);
     const T *__restrict__ b = s.b;
     int32_t off[kLinkChunk];
     float w[kLinkChunk];
+    const uint32_t dirs = nl > 0 ? s.link_dirs[db] : 0u;
 #pragma unroll
     for (int u = 0; u < kLinkChunk; ++u) {
         if (u < nl) {
-            off[u] = s.link_offset[lb + u * kBlock];
+            off[u] = step_offset((dirs >> (5 * u)) & 31u, sy, sz);
             w[u] = s.link_weight[lb + u * kBlock];
         }
     }
     float sw[kStateChunk], sn[kStateChunk], sbf[kStateChunk];
     T swa[kStateChunk], swb[kStateChunk];
+    uint32_t kind[kStateChunk];
 #pragma unroll
     for (int u = 0; u < kStateChunk; ++u) {
         if (u < ns) {
             const uint32_t q = sb + u * kBlock;
             sw[u] = s.state_weight[q];
-            sn[u] = s.state_now[q];
-            sbf[u] = s.state_before[q];
+            kind[u] = s.state_branch[q];
             swa[u] = s.wa[q];
             swb[u] = s.wb[q];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kStateChunk; ++u) {
+        if (u < ns) {
+            sn[u] = s.branch_now[kind[u]];
+            sbf[u] = s.branch_before[kind[u]];
         }
     }
     const T here = b[i];
@@ -463,10 +501,11 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
         if (u < nl) sum += static_cast<T>(w[u]) * v[u];
     }
     for (int c = kLinkChunk; c < nl; c += kLinkChunk) {
+        const uint32_t more = s.link_dirs[db + (c / kLinkChunk) * kBlock];
 #pragma unroll
         for (int u = 0; u < kLinkChunk; ++u) {
             if (c + u < nl) {
-                off[u] = s.link_offset[lb + (c + u) * kBlock];
+                off[u] = step_offset((more >> (5 * u)) & 31u, sy, sz);
                 w[u] = s.link_weight[lb + (c + u) * kBlock];
             }
         }
@@ -490,7 +529,8 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
     }
     for (int c = kStateChunk; c < ns; ++c) {
         const uint32_t q = sb + c * kBlock;
-        const T term = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
+        const uint32_t kd = s.state_branch[q];
+        const T term = static_cast<T>(s.branch_now[kd]) * s.wb[q] - static_cast<T>(s.branch_before[kd]) * s.wa[q];
         flow += static_cast<T>(s.state_weight[q]) * term;
     }
     T acc = static_cast<T>(ca) * here;
@@ -508,14 +548,15 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
         for (int u = 0; u < kStateChunk; ++u) {
             if (u < ns) {
                 const uint32_t q = sb + u * kBlock;
-                s.wa[q] = swa[u] + (rise * static_cast<T>(s.state_gain[q]) + own[u]);
+                s.wa[q] = swa[u] + (rise * static_cast<T>(s.branch_gain[kind[u]]) + own[u]);
             }
         }
         for (int c = kStateChunk; c < ns; ++c) {
             // The term again: wa has not changed since it was taken.
             const uint32_t q = sb + c * kBlock;
-            const T term = static_cast<T>(s.state_now[q]) * s.wb[q] - static_cast<T>(s.state_before[q]) * s.wa[q];
-            s.wa[q] = s.wa[q] + (rise * static_cast<T>(s.state_gain[q]) + term);
+            const uint32_t kd = s.state_branch[q];
+            const T term = static_cast<T>(s.branch_now[kd]) * s.wb[q] - static_cast<T>(s.branch_before[kd]) * s.wa[q];
+            s.wa[q] = s.wa[q] + (rise * static_cast<T>(s.branch_gain[kd]) + term);
         }
     }
 }
@@ -694,7 +735,6 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const StepArgs<T> s) {
 
 template <typename T>
 __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t block) {
-    constexpr int kLinkChunk = 6;
     const uint32_t t = threadIdx.x;
     const uint32_t k = block * kBlock + t;
     if (k >= s.specials) return;
@@ -702,7 +742,9 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
     const int nl = s.sp_links[k], ns = s.sp_states[k];
     const T ca = s.coef_a[k], cb = s.coef_b[k], ci = s.coef_inv[k];
     const uint32_t lb = s.link_base[block] + t;
+    const uint32_t db = s.link_base[block] / kLinkChunk + t;
     const uint32_t sb = s.state_base[block] + t;
+    const int sy = s.px, sz = static_cast<int>(s.sxy);
     const T *__restrict__ b = s.b;
     const T qb = b[i], qa = s.a[i];
     T sum = 0;
@@ -710,10 +752,11 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
         int32_t off[kLinkChunk];
         float w[kLinkChunk];
         T v[kLinkChunk];
+        const uint32_t dirs = s.link_dirs[db + (c / kLinkChunk) * kBlock];
 #pragma unroll
         for (int u = 0; u < kLinkChunk; ++u) {
             if (c + u < nl) {
-                off[u] = s.link_offset[lb + (c + u) * kBlock];
+                off[u] = step_offset((dirs >> (5 * u)) & 31u, sy, sz);
                 w[u] = s.link_weight[lb + (c + u) * kBlock];
             }
         }
@@ -732,12 +775,13 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
     T gsum = 0;
     for (int j = 0; j < ns; ++j) {
         const uint32_t q = sb + j * kBlock;
-        const T now = s.state_now[q], before = s.state_before[q];
+        const uint32_t kd = s.state_branch[q];
+        const T now = s.branch_now[kd], before = s.branch_before[kd];
         const T old = s.wa[q], wb = s.wb[q];
         if (s.delta != nullptr) s.acc_gain[q] += d * static_cast<double>(wb);
         const T fresh = now * wb + (static_cast<T>(1) - before) * old +
                         static_cast<T>(s.state_weight[q]) * (before * qa - now * qb);
-        gsum += static_cast<T>(s.state_gain[q]) * (fresh - old);
+        gsum += static_cast<T>(s.branch_gain[kd]) * (fresh - old);
         s.wa[q] = fresh;
     }
     T q = ci * (ca * qb + sum - cb * qa + gsum);
@@ -1124,10 +1168,11 @@ struct as_fdtd {
     uint32_t *tiles = nullptr;
     uint32_t *sp_cell = nullptr, *link_base = nullptr, *state_base = nullptr;
     uint8_t *sp_links = nullptr, *sp_states = nullptr;
-    int32_t *link_offset = nullptr;
+    uint32_t *link_dirs = nullptr;
     float *link_weight = nullptr, *coef_a = nullptr, *source = nullptr;
-    void *coef_b = nullptr, *coef_inv = nullptr, *state_gain = nullptr;  // float or double, as the precision
-    float *state_weight = nullptr, *state_now = nullptr, *state_before = nullptr;
+    void *coef_b = nullptr, *coef_inv = nullptr, *branch_gain = nullptr;  // float or double, as the precision
+    float *state_weight = nullptr, *branch_now = nullptr, *branch_before = nullptr;
+    uint16_t *state_branch = nullptr;
     size_t state_slots = 0;   // the states' arrays' length (general slabs, then face states)
     uint32_t face_specials = 0;
     uint32_t *face_tile = nullptr;
@@ -1402,12 +1447,9 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     // General specials: sorted by their counts of links and states (a special's update reads
     // only its own cell, links and states, so their order is free), in blocks of kBlock;
     // each block's links and states in slabs of (its largest count) x kBlock, slot-major.
-    // The links are offsets on the padded grid; each state carries its branch's coefficients.
-    std::vector<int32_t> offset(18);
-    for (int k = 0; k < 18; ++k) {
-        offset[static_cast<size_t>(k)] = static_cast<int32_t>(kSteps[k][0] + kSteps[k][1] * static_cast<long long>(px) +
-                                                              kSteps[k][2] * static_cast<long long>(px * ny));
-    }
+    // A block's links are its chunks' packed directions and the links' weights, slot-major;
+    // each state names its branch, whose coefficients sit in small tables.
+    if (L.branches > 0xFFFFu) refuse("more than 65535 branches");
     std::vector<uint64_t> order;
     for (uint64_t k = 0; k < K; ++k)
         if (!face[k]) order.push_back(k);
@@ -1427,6 +1469,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             wl = std::max(wl, links_of(order[B * kBlock + t]));
             ws = std::max(ws, states_of(order[B * kBlock + t]));
         }
+        wl = (wl + kLinkChunk - 1) / kLinkChunk * kLinkChunk;  // whole chunks: link_base / kLinkChunk
         link_base[B] = static_cast<uint32_t>(link_slots);
         state_base[B] = static_cast<uint32_t>(state_slots);
         link_slots += wl * kBlock;
@@ -1435,10 +1478,10 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             throw Failure{AS_ERANGE, "too many links or states for 32-bit indices"};
         }
     }
-    std::vector<int32_t> link_offset(link_slots, 0);
+    std::vector<uint32_t> link_dirs(link_slots / kLinkChunk, 0);
     std::vector<float> link_weight(link_slots, 0.0f);
-    std::vector<float> state_weight(state_slots, 0.0f), now(state_slots, 0.0f), before(state_slots, 0.0f);
-    std::vector<double> gain(state_slots, 0.0);
+    std::vector<float> state_weight(state_slots, 0.0f);
+    std::vector<uint16_t> state_branch(state_slots, 0);
     for (uint64_t k = 0; k < G; ++k) {
         const uint64_t src = order[k], B = k / kBlock, t = k % kBlock;
         sp_cell[k] = static_cast<uint32_t>(pad(static_cast<uint64_t>(L.special[src])));
@@ -1451,17 +1494,15 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         for (uint64_t u = 0; u < links_of(src); ++u) {
             const uint64_t l = L.link_start[src] + u;
             const uint64_t slot = link_base[B] + u * kBlock + t;
-            link_offset[slot] = offset[L.link_direction[l]];
+            link_dirs[link_base[B] / kLinkChunk + (u / kLinkChunk) * kBlock + t] |=
+                static_cast<uint32_t>(L.link_direction[l]) << (5 * (u % kLinkChunk));
             link_weight[slot] = L.link_weight[l];
         }
         for (uint64_t u = 0; u < states_of(src); ++u) {
             const uint64_t j = L.state_start[src] + u;
-            const int32_t br = L.state_branch[j];
             const uint64_t slot = state_base[B] + u * kBlock + t;
             state_weight[slot] = L.state_weight[j];
-            now[slot] = L.branch_now[br];
-            before[slot] = L.branch_before[br];
-            gain[slot] = exact ? L.branch_gain64[br] : L.branch_gain[br];
+            state_branch[slot] = static_cast<uint16_t>(L.state_branch[j]);
         }
     }
     h->specials = static_cast<uint32_t>(G);
@@ -1680,16 +1721,22 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->sp_states = upload(h, sp_states);
     h->link_base = upload(h, link_base);
     h->state_base = upload(h, state_base);
-    h->link_offset = upload(h, link_offset);
+    h->link_dirs = upload(h, link_dirs);
     h->link_weight = upload(h, link_weight);
     h->coef_a = upload(h, coef_a);
     h->coef_b = upload_as(h, coef_b, precision);
     h->coef_inv = upload_as(h, coef_inv, precision);
     h->source = upload(h, source);
     h->state_weight = upload(h, state_weight);
-    h->state_now = upload(h, now);
-    h->state_before = upload(h, before);
-    h->state_gain = upload_as(h, gain, precision);
+    h->state_branch = upload(h, state_branch);
+    {
+        std::vector<float> now(L.branch_now, L.branch_now + L.branches), before(L.branch_before, L.branch_before + L.branches);
+        std::vector<double> gain(L.branches);
+        for (uint64_t br = 0; br < L.branches; ++br) gain[br] = exact ? L.branch_gain64[br] : L.branch_gain[br];
+        h->branch_now = upload(h, now);
+        h->branch_before = upload(h, before);
+        h->branch_gain = upload_as(h, gain, precision);
+    }
     h->face_tile = upload(h, face_tile);
     h->face_tile_none = upload(h, std::vector<uint32_t>(face_tile.size(), 0));
     {
@@ -1771,16 +1818,17 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.sp_states = h->sp_states;
     s.link_base = h->link_base;
     s.state_base = h->state_base;
-    s.link_offset = h->link_offset;
+    s.link_dirs = h->link_dirs;
     s.link_weight = h->link_weight;
     s.coef_a = h->coef_a;
     s.coef_b = static_cast<const T *>(h->coef_b);
     s.coef_inv = static_cast<const T *>(h->coef_inv);
     s.source = h->source;
     s.state_weight = h->state_weight;
-    s.state_now = h->state_now;
-    s.state_before = h->state_before;
-    s.state_gain = static_cast<const T *>(h->state_gain);
+    s.state_branch = h->state_branch;
+    s.branch_now = h->branch_now;
+    s.branch_before = h->branch_before;
+    s.branch_gain = static_cast<const T *>(h->branch_gain);
     s.face_tile = h->face_blocks ? h->face_tile_none : h->face_tile;
     s.face_count = h->face_blocks ? h->face_specials : 0;
     s.face_blocks = (s.face_count + kBlock - 1) / kBlock;

@@ -136,6 +136,24 @@ def test_shoebox_full_grid(reference_available):
     assert np.array_equal(gpu.run(d[201:]), cpu.run(d[201:]))
 
 
+def test_full_fitted_hall_equals_the_reference(reference_available):
+    """The hall at its full 10 cm grid (7.57M air cells) with fitted walls, laid out by the
+    reference's own set-up (its cut cells: 486,785 general specials, diagonal links where
+    the planes are tilted): 150 steps in two calls equal the reference's own run."""
+    from acoustic_spaces_amd import scenes
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.reference import ReferenceSolver
+
+    scene = scenes.hall()[0]
+    ref = ReferenceSolver.from_scene(scene, threads=16, fitted_walls=True)
+    layout = ref.layout()
+    print(f"fitted hall: {len(layout['special'])} specials", flush=True)
+    gpu = HipWaveSolver(layout)
+    d = scene.drive[40:190]
+    assert np.array_equal(gpu.run(d[:75]), ref.run(d[:75]))
+    assert np.array_equal(gpu.run(d[75:]), ref.run(d[75:]))
+
+
 @pytest.mark.parametrize("pairs", [False, True])
 def test_hall_steps_equal_the_oracle(reference_available, pairs):
     """The hall (7.57M air cells; up to nine branch states on a cell where three walls
