@@ -465,7 +465,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", choices=("hall", "shoebox", "outdoor", "rooms", "adjoint"), default="hall")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-outputs", action="store_true",
-                    help="skip the hall's receiver outputs (band levels and T30, float against double)")
+                    help="skip the receiver outputs (the hall's and shoebox's band levels and T30, "
+                         "float against double; the outdoor barrier's insertion loss)")
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--calibrate", action="store_true",
@@ -496,12 +497,12 @@ def main(argv=None) -> int:
     elif args.workload == "shoebox":
         args.steps = args.steps or 5941
         scene = scenes.shoebox()
-        args.outputs = True
+        args.outputs = not args.no_outputs
         line = plain(scene, "shoebox", args)
     elif args.workload == "outdoor":
         args.steps = args.steps or 2377
         scene = scenes.outdoor()
-        args.insertion_loss = True
+        args.insertion_loss = not args.no_outputs
         line = plain(scene, "outdoor", args)
     elif args.workload == "rooms":
         args.steps = args.steps or 5941
