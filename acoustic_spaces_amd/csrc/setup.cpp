@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <new>
 #include <stdexcept>
@@ -155,8 +156,12 @@ public:
     // triangle bins for crossing queries
     size_t bin = 8, bx = 0, by = 0, bz = 0;
     std::vector<uint32_t> bin_start, bin_faces;
-    mutable std::vector<uint32_t> stamp;
-    mutable uint32_t stamp_now = 0;
+    // Which faces a query has met (each thread's own: the bins list a face more than once).
+    struct Stamps {
+        std::vector<uint32_t> mark;
+        uint32_t now = 0;
+    };
+    mutable Stamps serial;
 
     V3 centre_at(long long x, long long y, long long z) const {
         return {origin.x + (static_cast<double>(x) + 0.5) * cell, origin.y + (static_cast<double>(y) + 0.5) * cell,
@@ -171,25 +176,48 @@ public:
         const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
         return x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz;
     }
-    size_t depth(size_t i) const {
-        const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
-        return std::min({x, y, z, nx - 1 - x, ny - 1 - y, nz - 1 - z});
-    }
     double cell_scale(size_t i) const { return scale.empty() ? 1.0 : static_cast<double>(scale[i]); }
     bool is_cut(size_t i, int k) const {
         if (bit[k] >= 0) return (cut[i] >> bit[k]) & 1u;
         return (cut[neighbour(i, k)] >> bit[opposite(k)]) & 1u;
     }
     bool blocked(size_t i, int k) const { return edge(neighbour(i, k)) || is_cut(i, k); }
-
-    void reach_sides(size_t i, size_t within) {
-        const size_t at[3] = {i % nx, (i / nx) % ny, i / sz};
-        const size_t size[3] = {nx, ny, nz};
-        for (size_t a = 0; a < 3; ++a) {
-            if (at[a] <= within) open_sides[2 * a] = true;
-            if (at[a] + 1 + within >= size[a]) open_sides[2 * a + 1] = true;
-        }
+    // edge(neighbour(i, k)) for cell i at (x, y, z), without dividing. A step past the
+    // grid lands, in neighbour()'s indexing, on a cell of the far edge: on the edge too.
+    bool neighbour_on_edge(size_t x, size_t y, size_t z, int k) const {
+        const long long to[3] = {static_cast<long long>(x) + kSteps[k][0], static_cast<long long>(y) + kSteps[k][1],
+                                 static_cast<long long>(z) + kSteps[k][2]};
+        const long long size[3] = {static_cast<long long>(nx), static_cast<long long>(ny), static_cast<long long>(nz)};
+        for (int a = 0; a < 3; ++a)
+            if (to[a] <= 0 || to[a] >= size[a] - 1) return true;
+        return false;
     }
+    // blocked(i, k) for cell i at (x, y, z), without dividing.
+    bool blocked_at(size_t i, int k, size_t x, size_t y, size_t z) const {
+        return neighbour_on_edge(x, y, z, k) || is_cut(i, k);
+    }
+
+    // Runs work(t, z_lo, z_hi) for slabs of planes t = 0 .. T - 1 on T threads (T returned
+    // by slabs()); the first slab's exception, by order of slabs, is thrown again.
+    size_t slabs() const { return std::max<size_t>(1, std::min<size_t>(static_cast<size_t>(threads), nz / 2)); }
+    template <typename Work>
+    void in_slabs(size_t T, Work work) const {
+        std::vector<std::exception_ptr> errors(T);
+        auto run = [&](size_t t) {
+            try {
+                work(t, nz * t / T, nz * (t + 1) / T);
+            } catch (...) {
+                errors[t] = std::current_exception();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < T; ++t) pool.emplace_back(run, t);
+        run(0);
+        for (auto &th : pool) th.join();
+        for (const auto &e : errors)
+            if (e) std::rethrow_exception(e);
+    }
+
 
     // --- geometry -------------------------------------------------------------------
 
@@ -224,7 +252,7 @@ public:
     }
 
     template <typename Visit>
-    void each_face_near_segment(const V3 &a, const V3 &b, Visit visit) const {
+    void each_face_near_segment(const V3 &a, const V3 &b, Stamps &st, Visit visit) const {
         const double lo[3] = {std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z)};
         const double hi[3] = {std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)};
         const size_t dims[3] = {bx, by, bz};
@@ -236,9 +264,13 @@ public:
             from[k] = static_cast<size_t>(std::clamp(f, 0.0, static_cast<double>(dims[k] - 1)));
             to[k] = static_cast<size_t>(std::clamp(l, 0.0, static_cast<double>(dims[k] - 1)));
         }
-        if (++stamp_now == 0) {
-            std::fill(stamp.begin(), stamp.end(), 0u);
-            stamp_now = 1;
+        if (st.mark.size() != tri.size()) {
+            st.mark.assign(tri.size(), 0u);
+            st.now = 0;
+        }
+        if (++st.now == 0) {
+            std::fill(st.mark.begin(), st.mark.end(), 0u);
+            st.now = 1;
         }
         for (size_t z = from[2]; z <= to[2]; ++z)
             for (size_t y = from[1]; y <= to[1]; ++y)
@@ -246,8 +278,8 @@ public:
                     const size_t c = x + bx * (y + by * z);
                     for (uint32_t q = bin_start[c]; q < bin_start[c + 1]; ++q) {
                         const uint32_t f = bin_faces[q];
-                        if (stamp[f] == stamp_now) continue;
-                        stamp[f] = stamp_now;
+                        if (st.mark[f] == st.now) continue;
+                        st.mark[f] = st.now;
                         visit(f);
                     }
                 }
@@ -255,7 +287,7 @@ public:
 
     // The face nearest a that the segment from a to b crosses, or -1 (the reference's
     // crossing, fdtd.hpp:3082-3257, where no tie touches it).
-    int64_t crossing(const V3 &a, const V3 &b) const {
+    int64_t crossing(const V3 &a, const V3 &b, Stamps &st) const {
         Hit firm;
         struct Loose {
             double t;
@@ -264,7 +296,7 @@ public:
         Loose loose[16];
         int nloose = 0;
         bool twice = false;
-        each_face_near_segment(a, b, [&](uint32_t f) {
+        each_face_near_segment(a, b, st, [&](uint32_t f) {
             double t = 0;
             const int r = meet(tri[f], a, b, t);
             if (r == 1) {
@@ -293,11 +325,12 @@ public:
         return firm.face;
     }
 
+    int64_t crossing(const V3 &a, const V3 &b) const { return crossing(a, b, serial); }
     bool sees(const V3 &p, const V3 &q) const { return crossing(p, q) < 0; }
 
     // Whether p lies on a face: refused (the reference's leans are not taken).
     void require_off_faces(const V3 &p, const char *what) const {
-        each_face_near_segment(p, p, [&](uint32_t f) {
+        each_face_near_segment(p, p, serial, [&](uint32_t f) {
             const Triangle &t = tri[f];
             const double s = dot(t.normal, p - t.v0);
             if (std::abs(s) > guard) return;
@@ -376,7 +409,7 @@ public:
                         }
             }
         }
-        stamp.assign(tri.size(), 0u);
+        serial.mark.assign(tri.size(), 0u);
     }
 
     // Cut links (fdtd.hpp:446-477): every face against the + half's links of the cells
@@ -591,16 +624,44 @@ public:
         return out;
     }
 
+    // Every cell the queue's cells (already air) reach by links not blocked; the set does
+    // not depend on the order. By spans along x: a span is marked whole (each end extended
+    // through open links to cells not yet air), then every link of its cells off the x axis
+    // is followed, each cell it reaches opening a span of its own. The grid is read row by
+    // row, with one stack entry a span.
     void flood(std::vector<size_t> &queue) {
-        while (!queue.empty()) {
-            const size_t i = queue.back();
-            queue.pop_back();
-            for (int k = 0; k < directions; ++k) {
-                if (blocked(i, k)) continue;
-                const size_t j = neighbour(i, k);
-                if (type[j] == kSolid) {
+        struct Span {
+            size_t row, x0, x1;  // row = i / nx; cells x0 .. x1 of it
+        };
+        std::vector<Span> spans;
+        // Marks the span through air cell i (already marked) and pushes it.
+        const auto open_span = [&](size_t i) {
+            const size_t row = i / nx, x = i - row * nx, y = row % ny, z = row / ny;
+            size_t x0 = x, x1 = x;
+            while (!blocked_at(row * nx + x0, 0, x0, y, z) && type[row * nx + x0 - 1] == kSolid) {
+                type[row * nx + x0 - 1] = kInterior;
+                --x0;
+            }
+            while (!blocked_at(row * nx + x1, 1, x1, y, z) && type[row * nx + x1 + 1] == kSolid) {
+                type[row * nx + x1 + 1] = kInterior;
+                ++x1;
+            }
+            spans.push_back({row, x0, x1});
+        };
+        for (const size_t i : queue) open_span(i);
+        queue.clear();
+        while (!spans.empty()) {
+            const Span sp = spans.back();
+            spans.pop_back();
+            const size_t y = sp.row % ny, z = sp.row / ny;
+            for (int k = 2; k < directions; ++k) {
+                for (size_t x = sp.x0; x <= sp.x1; ++x) {
+                    const size_t i = sp.row * nx + x;
+                    if (blocked_at(i, k, x, y, z)) continue;
+                    const size_t j = neighbour(i, k);
+                    if (type[j] != kSolid) continue;
                     type[j] = kInterior;
-                    queue.push_back(j);
+                    open_span(j);
                 }
             }
         }
@@ -649,23 +710,45 @@ public:
         }
     }
 
-    size_t add_special(size_t i, const double *w, double g) {
-        type[i] = kSpecial;
-        special.push_back(static_cast<int64_t>(i));
+    // The specials of a slab of planes, as the layout's arrays (starts from 0).
+    struct Specials {
+        std::vector<int64_t> special;
+        std::vector<uint64_t> link_start{0};
+        std::vector<uint8_t> link_direction;
+        std::vector<float> link_weight;
+        std::vector<float> coef_a, coef_b, coef_inv, source;
+        std::vector<uint64_t> state_start{0};
+        std::vector<int32_t> state_branch;
+        std::vector<float> state_weight;
+        size_t air = 0;
+        bool open = false;
+        std::array<bool, 6> sides{};
+    };
+
+    // Special i's links (the weights w it has, lowered), coefficients and the start of its
+    // states (already added) into `out`, whose layout arrays it extends.
+    template <typename Out>
+    void put_special(Out &out, size_t i, const double *w, double g) const {
+        out.special.push_back(static_cast<int64_t>(i));
         const double s = cell_scale(i), r = std::sqrt(s);
         double sum = 0.0;
         for (int k = 0; k < directions; ++k) {
             if (!(w[k] > 0.0)) continue;
             sum += w[k];
-            link_direction.push_back(static_cast<uint8_t>(k));
-            link_weight.push_back(lowered(s * w[k]));
+            out.link_direction.push_back(static_cast<uint8_t>(k));
+            out.link_weight.push_back(lowered(s * w[k]));
         }
-        link_start.push_back(link_direction.size());
-        coef_a.push_back(static_cast<float>(2.0 - s * sum * (1.0 - kMargin)));
-        coef_b.push_back(static_cast<float>(1.0 - r * g));
-        coef_inv.push_back(static_cast<float>(1.0 / (1.0 + r * g)));
-        source.push_back(0.0f);
-        state_start.push_back(state_branch.size());
+        out.link_start.push_back(out.link_direction.size());
+        out.coef_a.push_back(static_cast<float>(2.0 - s * sum * (1.0 - kMargin)));
+        out.coef_b.push_back(static_cast<float>(1.0 - r * g));
+        out.coef_inv.push_back(static_cast<float>(1.0 / (1.0 + r * g)));
+        out.source.push_back(0.0f);
+        out.state_start.push_back(out.state_branch.size());
+    }
+
+    size_t add_special(size_t i, const double *w, double g) {
+        type[i] = kSpecial;
+        put_special(*this, i, w, g);
         return special.size() - 1;
     }
 
@@ -739,19 +822,42 @@ public:
         if (queue.empty()) throw std::invalid_argument("the source sees no cell of the wave grid");
         flood(queue);
 
-        // The matched layer's air.
-        for (size_t i = 0; i < n && layer > 0; ++i) {
-            if (type[i] == kSolid || depth(i) > layer) continue;
-            for (int k = 0; k < 6; ++k) {
-                if (!edge(neighbour(i, k)) && is_cut(i, k)) {
-                    throw std::invalid_argument("the model reaches into the matched layer");
-                }
+        // The matched layer's air, slab by slab of planes on the threads (each cell's class
+        // its own; the cells listed in order).
+        if (layer > 0) {
+            const size_t T = slabs();
+            std::vector<std::vector<int64_t>> found(T);
+            std::vector<std::array<bool, 6>> sides(T);
+            in_slabs(T, [&](size_t t, size_t z0, size_t z1) {
+                std::array<bool, 6> reach{};
+                const size_t size[3] = {nx, ny, nz};
+                for (size_t z = z0; z < z1; ++z)
+                    for (size_t y = 0; y < ny; ++y)
+                        for (size_t x = 0; x < nx; ++x) {
+                            const size_t i = x + y * sy + z * sz;
+                            if (type[i] == kSolid) continue;
+                            if (std::min({x, y, z, nx - 1 - x, ny - 1 - y, nz - 1 - z}) > layer) continue;
+                            const size_t at[3] = {x, y, z};
+                            for (int k = 0; k < 6; ++k) {
+                                if (!neighbour_on_edge(x, y, z, k) && is_cut(i, k)) {
+                                    throw std::invalid_argument("the model reaches into the matched layer");
+                                }
+                            }
+                            type[i] = kLayer;
+                            found[t].push_back(static_cast<int64_t>(i));
+                            for (size_t a = 0; a < 3; ++a) {
+                                if (at[a] <= layer) reach[2 * a] = true;
+                                if (at[a] + 1 + layer >= size[a]) reach[2 * a + 1] = true;
+                            }
+                        }
+                sides[t] = reach;
+            });
+            for (size_t t = 0; t < T; ++t) {
+                layer_cells.insert(layer_cells.end(), found[t].begin(), found[t].end());
+                air += found[t].size();
+                if (!found[t].empty()) open = true;
+                for (int k = 0; k < 6; ++k) open_sides[static_cast<size_t>(k)] = open_sides[static_cast<size_t>(k)] || sides[t][static_cast<size_t>(k)];
             }
-            type[i] = kLayer;
-            layer_cells.push_back(static_cast<int64_t>(i));
-            ++air;
-            open = true;
-            reach_sides(i, layer);
         }
         if (!layer_cells.empty()) {
             for (int a = 0; a < 3; ++a) layer_coefficients(a);
@@ -760,24 +866,45 @@ public:
             }
         }
 
-        // Boundary cells: marks 1 and 2, then weights, materials and states.
+        // Boundary cells: marks 1 and 2, then weights, materials and states. Each pass runs
+        // on the threads by slabs of planes; the second writes its marks apart (a cell can
+        // take mark 2 from several neighbours), and the specials of each slab are joined in
+        // the order of the slabs, so of the cells, as one pass in order would list them.
+        const size_t T = slabs();
         std::vector<uint8_t> mark(n, 0);
-        for (size_t i = 0; i < n; ++i) {
-            if (type[i] != kInterior) continue;
-            for (int k = 0; k < directions && !mark[i]; ++k) mark[i] = blocked(i, k);
-        }
-        std::array<double, 18> wi{}, wj{};
-        for (size_t i = 0; i < n; ++i) {
-            if (mark[i] != 1) continue;
-            raw_weights(i, wi.data());
-            for (int k = 0; k < directions; ++k) {
-                const size_t j = neighbour(i, k);
-                if (wi[k] != exact[k] && type[j] == kInterior && !mark[j]) mark[j] = 2;
+        in_slabs(T, [&](size_t, size_t z0, size_t z1) {
+            for (size_t z = z0; z < z1; ++z)
+                for (size_t y = 0; y < ny; ++y)
+                    for (size_t x = 0; x < nx; ++x) {
+                        const size_t i = x + y * sy + z * sz;
+                        if (type[i] != kInterior) continue;
+                        for (int k = 0; k < directions && !mark[i]; ++k) mark[i] = blocked_at(i, k, x, y, z);
+                    }
+        });
+        std::vector<uint8_t> second(n, 0);
+        in_slabs(T, [&](size_t, size_t z0, size_t z1) {
+            std::array<double, 18> wi{};
+            for (size_t i = z0 * sz; i < z1 * sz; ++i) {
+                if (mark[i] != 1) continue;
+                raw_weights(i, wi.data());
+                for (int k = 0; k < directions; ++k) {
+                    const size_t j = neighbour(i, k);
+                    if (wi[k] != exact[k] && type[j] == kInterior && !mark[j]) {
+                        __atomic_store_n(&second[j], static_cast<uint8_t>(1), __ATOMIC_RELAXED);
+                    }
+                }
             }
-        }
-        for (size_t i = 0; i < n; ++i) {
+        });
+        for (size_t i = 0; i < n; ++i)
+            if (second[i]) mark[i] = 2;
+        std::vector<Specials> parts(T);
+        in_slabs(T, [&](size_t t, size_t z0, size_t z1) {
+          Specials &out = parts[t];
+          Stamps stamps;
+          std::array<double, 18> wi{}, wj{};
+          for (size_t i = z0 * sz; i < z1 * sz; ++i) {
             if (type[i] != kInterior) continue;
-            ++air;
+            ++out.air;
             if (!mark[i]) continue;
             raw_weights(i, wi.data());
             std::array<double, 18> w{};
@@ -789,12 +916,17 @@ public:
                 const size_t j = neighbour(i, k);
                 if (edge(j)) {
                     beta_sum += bw * edge_admittance;
-                    open = true;
-                    reach_sides(j, 0);
+                    out.open = true;
+                    const size_t at[3] = {j % nx, (j / nx) % ny, j / sz};
+                    const size_t size[3] = {nx, ny, nz};
+                    for (size_t a = 0; a < 3; ++a) {
+                        if (at[a] == 0) out.sides[2 * a] = true;
+                        if (at[a] + 1 >= size[a]) out.sides[2 * a + 1] = true;
+                    }
                     return;
                 }
-                int64_t face = crossing(centre_of(i), centre_of(j));
-                if (face < 0) face = crossing(centre_of(j), centre_of(i));
+                int64_t face = crossing(centre_of(i), centre_of(j), stamps);
+                if (face < 0) face = crossing(centre_of(j), centre_of(i), stamps);
                 if (face < 0) return;
                 const auto m = static_cast<int32_t>(face_material[static_cast<size_t>(face)]);
                 beta_sum += bw * beta[static_cast<size_t>(m)];
@@ -827,11 +959,30 @@ public:
             for (size_t kind = 0; kind < kinds; ++kind) {
                 for (const int32_t jb : material_branches[static_cast<size_t>(crossed[kind])]) {
                     g += crossed_weight[kind] * (0.5 / branch_a[static_cast<size_t>(jb)]);
-                    state_branch.push_back(jb);
-                    state_weight.push_back(static_cast<float>(r * crossed_weight[kind]));
+                    out.state_branch.push_back(jb);
+                    out.state_weight.push_back(static_cast<float>(r * crossed_weight[kind]));
                 }
             }
-            add_special(i, w.data(), g);
+            put_special(out, i, w.data(), g);
+          }
+        });
+        for (const Specials &part : parts) {
+            const uint64_t links0 = link_direction.size(), states0 = state_branch.size();
+            for (const int64_t i : part.special) type[static_cast<size_t>(i)] = kSpecial;
+            special.insert(special.end(), part.special.begin(), part.special.end());
+            for (size_t q = 1; q < part.link_start.size(); ++q) link_start.push_back(links0 + part.link_start[q]);
+            for (size_t q = 1; q < part.state_start.size(); ++q) state_start.push_back(states0 + part.state_start[q]);
+            link_direction.insert(link_direction.end(), part.link_direction.begin(), part.link_direction.end());
+            link_weight.insert(link_weight.end(), part.link_weight.begin(), part.link_weight.end());
+            coef_a.insert(coef_a.end(), part.coef_a.begin(), part.coef_a.end());
+            coef_b.insert(coef_b.end(), part.coef_b.begin(), part.coef_b.end());
+            coef_inv.insert(coef_inv.end(), part.coef_inv.begin(), part.coef_inv.end());
+            source.insert(source.end(), part.source.begin(), part.source.end());
+            state_branch.insert(state_branch.end(), part.state_branch.begin(), part.state_branch.end());
+            state_weight.insert(state_weight.end(), part.state_weight.begin(), part.state_weight.end());
+            air += part.air;
+            open = open || part.open;
+            for (int k = 0; k < 6; ++k) open_sides[static_cast<size_t>(k)] = open_sides[static_cast<size_t>(k)] || part.sides[static_cast<size_t>(k)];
         }
         boundary_cells = special.size();
 

@@ -105,3 +105,17 @@ def test_argument_errors_match_the_reference():
     sc.beta = -sc.beta
     with pytest.raises(ValueError, match="negative admittance"):
         WaveSolver.from_scene(sc)
+
+
+@pytest.mark.parametrize("name", ["open_layer", "shoebox19", "listeners13"])
+def test_thread_count_does_not_change_the_layout(reference_available, name):
+    """The set-up's passes over the grid run on slabs of planes, one a thread: on 1, 3, 7
+    and 16 threads the layout equals the reference's, the flood of the air, the layer, the
+    marks and the specials' order included."""
+    from acoustic_spaces_amd.solver import WaveSolver
+    from oracle.reference import ReferenceSolver
+
+    scene, kw = SCENES[name]()
+    ref = ReferenceSolver.from_scene(scene, threads=4, **kw)
+    for threads in (1, 3, 7, 16):
+        _same(ref, WaveSolver.from_scene(scene, threads=threads, **kw))
