@@ -3,9 +3,9 @@
 // The scheme is the reference's (cpp/include/acoustic_spaces/fdtd.hpp:17-75, run() at
 // 786-904): a two-level pressure leapfrog, p+ = (2 - G) p + sum_k g_k p_k - p-, over the
 // grid's interior cells; the special cells (boundary, fitted-wall and source cells) by
-// their CSR links, admittance-filter branch states and source; the split-field matched
-// layer; and the probes, read before the update. Every update keeps the reference's
-// operations and their order, compiled without contraction (-ffp-contract=off) and with
+// their links, admittance-filter branch states and source; the split-field matched layer;
+// and the probes, read before the update. Every update keeps the reference's operations
+// and their order, compiled without contraction (-ffp-contract=off) and with
 // single-precision denormals flushed (-fgpu-flush-denormals-to-zero), as the reference
 // runs with FTZ/DAZ (fdtd.hpp:314-336): in float the fields equal the reference's bit for
 // bit. The probes are summed in double, as the reference's record() (fdtd.hpp:3525-3534).
@@ -13,16 +13,18 @@
 // Layout in HBM. The two pressure fields are float (or double) arrays of nx x ny x nz
 // cells, x fastest, each row padded to a pitch of px cells (a multiple of the row
 // alignment, 64 floats = 256 bytes by default), so that a wavefront's 64 lanes read one
-// aligned row segment. The cell classes are one byte per cell. The update of the
-// interior runs over tiles of 64 x TY cells of a plane, marching TZ planes along z with
-// the planes below and above the current one held in registers; only tiles holding
-// interior cells are launched (a list made at set-up). Special cells, their links and
-// branch states are CSR arrays with the links' neighbour offsets precomputed for the
-// padded grid; the matched layer's nine state values are nine planes of its cells
-// (structure of arrays).
+// aligned row segment. The cell classes are one byte per cell. The interior is stepped by
+// tiles of 64 x 4 cells of a plane, each marching tile_z planes along z; only tiles
+// holding interior cells or face specials are launched, in a list made at set-up and
+// ordered by die. A special whose links are axial and weigh as the interior's (a face
+// special) is a 16-byte record with its coefficients and branches in small tables; the
+// other specials sit in slabs, blocks of 256 sorted by their counts of links and states,
+// slot-major, each chunk of six links' directions packed into one word. The matched
+// layer's nine state values are nine planes of its cells (structure of arrays).
 //
-// Work of one step, by block: interior tiles, then specials, then layer cells, then the
-// probes. The four sets write disjoint cells of the next field and read only the current
+// Work of one step, by block: the probes, the general specials, the face specials (by
+// blocks of their own, or each tile's after its column march), the interior tiles, the
+// layer cells. They write disjoint cells of the next field and read only the current
 // field (and their own cells of the previous one), so one launch carries them all.
 
 #include <hip/hip_runtime.h>
@@ -51,7 +53,8 @@ constexpr uint32_t kXcds = 8;  // MI355X's dies: blocks go to them round robin b
 constexpr uint32_t kAllInterior = 1u << 31;  // a tile's flag: every cell of it is interior
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;    // a block with no tile
 // Cell classes on the device: the layout's 0 solid, 1 interior, 2 special (here: a general
-// special, stepped by its own blocks), 3 layer; and 4, a face special (stepped in its tile).
+// special, stepped by its own blocks), 3 layer; and 4, a face special (stepped in its
+// tile, or by blocks of face specials).
 constexpr uint8_t kClassInterior = 1, kClassFace = 4;
 // In the adjoint's own classes only: an interior cell that a receiver reads (see inj).
 constexpr uint8_t kClassTap = 5;
@@ -124,7 +127,7 @@ struct StepArgs {
     const float *__restrict__ branch_before;
     const T *__restrict__ branch_gain;
     const float *__restrict__ signal;  // device, indexed by step
-    // face specials, stepped in the tile pass (see face_special)
+    // face specials (see face_special): each tile's after its column march, or
     const uint32_t *__restrict__ face_tile;    // per tile block: its first record (and one past the last)
     uint32_t face_blocks, face_count;          // or: all records by blocks of their own
     const uint4 *__restrict__ face_record;     // cell, links | states << 6 | coefficients << 10,
@@ -169,7 +172,8 @@ struct StepArgs {
 // source) and its branch states (their count, and where they start in the states' arrays;
 // each state an index into a small table of now, before, gain and weight). The arithmetic
 // is special_cell's, in its order. A tile's face specials are stepped after its column
-// pass, by all its lanes at once, while the tile's pressures are still in the caches.
+// pass, by all its lanes at once, while the tile's pressures are still in the caches; or
+// (on grids of many tiles, where that is faster) all of them by blocks of their own.
 // A face special's loads from the fields and the states, by its record, all issued before
 // its arithmetic (face_finish). The small tables (coefficients, state kinds) are read in
 // face_finish.
@@ -437,9 +441,10 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
 // terms, the source, coef_inv applied last; then its states' velocities. The links and
 // states sit in the block's slabs (slot-major, so a wavefront's loads of a slot coalesce).
 // The loads come in two waves: first everything indexed by the special itself (its cell,
-// coefficients, first chunk of link offsets and weights, first chunk of states), then
-// everything indexed by its cell (its pressures and its links' far ends); chunks of
-// links and states past the first, rare, follow one by one.
+// coefficients, first chunk of link directions and weights, first chunk of states), then
+// everything indexed by its cell and branches (its pressures, its links' far ends, its
+// branches' coefficients); chunks of links and states past the first, rare, follow one
+// by one.
 template <typename T>
 __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t block) {
     // States taken with the first loads (the rest one by one, in the same order); three in
