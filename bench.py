@@ -88,11 +88,12 @@ def insertion_loss(scene, layout) -> dict:
             "receivers_m_behind_barrier": np.round(scene.probes[:, 0] - (scene.probes[:, 0].min() - 10.0), 1).tolist()}
 
 
-def cpu_baseline(scene, target_s: float = 12.0, compare=None) -> dict:
+def cpu_baseline(scene, target_s: float = 12.0, compare=None, others=()) -> dict:
     """The reference's CPU solver (oracle/_ref) on all host cores, on a bounded run of the
     same workload; the oracle's one-thread port if the reference's build is absent. With
     ``compare`` (the GPU's float probes over the scene's whole drive), the reference's own
-    run of the whole drive too, and the differences of its signals, band levels and T30."""
+    run of the whole drive too, and the differences of its signals, band levels and T30;
+    the same for each (scene, GPU probes) of ``others`` (the hall's second source)."""
     cores = len(os.sched_getaffinity(0))
     from oracle import reference
 
@@ -128,6 +129,13 @@ def cpu_baseline(scene, target_s: float = 12.0, compare=None) -> dict:
         y = solver.run(drive[: compare.shape[1]])
         out["gpu_f32_vs_this"] = compare_outputs(octave_outputs(compare, scene.rate), octave_outputs(y, scene.rate),
                                                  compare, y)
+    for k, (other, gpu) in enumerate(others):
+        if kind != "reference":
+            break
+        ref = reference.ReferenceSolver.from_scene(other, threads=cores)
+        y = ref.run(other.drive[: gpu.shape[1]])
+        out[f"gpu_f32_vs_this_source{k + 2}"] = compare_outputs(octave_outputs(gpu, other.rate),
+                                                                 octave_outputs(y, other.rate), gpu, y)
     return out
 
 
@@ -269,6 +277,20 @@ def plain(scene, name, args, traffic_name=None):
     return line_for(f"FDTD Mcell-updates/s ({name}, plain solver)", air / per_step_s / 1e6, K, W, ms, scene.name,
                     config, roofline_of(step_bytes, per_step_s, traffic,
                                         "step_kernel<float,false,false> (one fused launch per step)"))
+
+
+def source_outputs(scene, args) -> dict:
+    """The receivers' outputs of ``scene`` over its whole drive, in float and in double (the
+    float probes kept in args.others for the CPU leg's comparison)."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    layout = WaveSolver.from_scene(scene).layout()
+    y32 = HipWaveSolver(layout, precision=32).run(scene.drive)
+    y64 = HipWaveSolver(layout, precision=64).run(scene.drive)
+    o32, o64 = octave_outputs(y32, scene.rate), octave_outputs(y64, scene.rate)
+    args.others.append((scene, y32))
+    return {"steps": len(scene.drive), "f32": o32, "f32_vs_f64": compare_outputs(o32, o64, y32, y64)}
 
 
 def rooms(args):
@@ -473,7 +495,7 @@ def main(argv=None) -> int:
                     help="time the runtime's own device-to-device copy kernel instead (the bandwidth "
                          "reference and the rocprofv3 counter scale)")
     args = ap.parse_args(argv)
-    args.outputs, args.signals, args.insertion_loss = False, None, False
+    args.outputs, args.signals, args.insertion_loss, args.others = False, None, False, []
     if args.calibrate:
         return calibrate()
     if args.gpus != 1:
@@ -486,14 +508,18 @@ def main(argv=None) -> int:
 
     if args.workload == "hall":
         args.steps = args.steps or 11882
-        scene = scenes.hall()[0]
+        sources = scenes.hall()
+        scene = sources[0]
         args.outputs = not args.no_outputs
         line = plain(scene, "hall", args, traffic_name="step_kernel_hall_traffic")
+        if args.outputs:
+            # The second source's run (the config's two runs), its outputs as the first's.
+            line["config"]["outputs_source2"] = source_outputs(sources[1], args)
         line["config"]["workload"] = (
             "hall 30x20x12 m, ceiling of 6 tilted planes, side walls splayed 5 deg, 10 cm cells "
             f"{scene.grid.shape[0]}x{scene.grid.shape[1]}x{scene.grid.shape[2]}, "
             f"{line['config']['air_cells']:,} air cells, walls on cell faces, 3 relaxation branches per "
-            "material, 1 source, 16 receivers")
+            "material, 2 sources (the timed steps from the first; the outputs of both), 16 receivers")
     elif args.workload == "shoebox":
         args.steps = args.steps or 5941
         scene = scenes.shoebox()
@@ -512,7 +538,7 @@ def main(argv=None) -> int:
         line = adjoint(args)
         scene = None
     if not args.no_cpu and scene is not None:
-        line["cpu_baseline"] = cpu_baseline(scene, compare=args.signals)
+        line["cpu_baseline"] = cpu_baseline(scene, compare=args.signals, others=args.others)
     print(json.dumps(strict(line), allow_nan=False), flush=True)
     return 0
 
