@@ -171,7 +171,8 @@ def calibrate() -> int:
 def time_steps(stepper, drive: np.ndarray, warmup: int, steps: int):
     """Warm-up, then exactly ``steps`` steps between synchronisations, timed with hipEvents
     on the stepper's own stream; the drive and the probes' output in device memory.
-    Returns (milliseconds, the probes' output of the timed steps)."""
+    Returns (milliseconds, the probes' output of the warm-up and timed steps, (probes,
+    warmup + steps))."""
     from acoustic_spaces_amd import hip
 
     need = warmup + steps
@@ -188,7 +189,11 @@ def time_steps(stepper, drive: np.ndarray, warmup: int, steps: int):
     stop.record(stream)
     hip.synchronize()
     ms = start.elapsed_ms(stop)
-    return ms, d_out.to_host()
+    # Each call wrote its own (probes, its steps) block: the warm-up's, then the timed one's.
+    flat = d_out.to_host().reshape(-1)
+    warm = flat[: probes * warmup].reshape(probes, warmup)
+    timed = flat[probes * warmup: probes * need].reshape(probes, steps)
+    return ms, np.concatenate([warm, timed], axis=1)
 
 
 def measured_traffic(name: str):
@@ -320,17 +325,37 @@ def rooms(args):
         raise SystemExit("non-finite probe output")
     merged_shape = [int(v) for v in merged["shape"]]
     del merged
-    grouped = {}
+    grouped, batched_out = {}, {}
     for mb in (384, 256, 192, 128):
         b = BatchedWaveSolver(layouts, group_bytes=mb << 20, **launch)
-        grouped[mb] = (sum(time_steps(s, drive, W, K)[0] for s in b.steppers), len(b.groups))
+        total, rooms_out = 0.0, [None] * len(layouts)
+        for idx, placements, s in b.groups:
+            ms, y = time_steps(s, drive, W, K)
+            total += ms
+            for i, p in zip(idx, placements):
+                rooms_out[i] = y[p.probes]
+        grouped[mb] = (total, len(b.groups))
+        batched_out[mb] = rooms_out
         b.close()
-    ms_seq = 0.0
+    ms_seq, plain_out = 0.0, []
     for L in layouts:
         s = HipWaveSolver(L)
-        ms, _ = time_steps(s, drive, W, K)
+        ms, y = time_steps(s, drive, W, K)
         ms_seq += ms
+        plain_out.append(y)
         s.close()
+    # Each room's receivers, batched in every group size, against its plain solve.
+    same = {f"{mb} MB": all(np.array_equal(a, b) for a, b in zip(batched_out[mb], plain_out)) for mb in batched_out}
+    # The study's result: each band's T30 and G over the rooms' receivers.
+    bands = [octave_outputs(y, batch[0].rate) for y in plain_out]
+    study = {}
+    for q, key in ((0, "t30_s"), (1, "g_db")):
+        for b_i, hz in enumerate(bands[0]["bands_hz"]):
+            v = np.array([row[b_i] for o in bands for row in o[key]], float)
+            v = v[np.isfinite(v)]
+            study[f"{key}_{hz}hz"] = {"receivers": int(v.size), "p10": round(float(np.percentile(v, 10)), 4),
+                                      "median": round(float(np.median(v)), 4),
+                                      "p90": round(float(np.percentile(v, 90)), 4)}
     best_mb = min(grouped, key=lambda k: grouped[k][0])
     ms_batch = min(ms_one, grouped[best_mb][0])
     air = int(sum(np.count_nonzero(L["types"]) for L in layouts))
@@ -347,6 +372,9 @@ def rooms(args):
         "batched_over_sequential_speedup": round(ms_seq / ms_batch, 3),
         "sequential_mcell_updates_per_s": round(air * K / (ms_seq / 1e3) / 1e6, 1),
         "special_cells": info["specials"],
+        "batched_equals_plain_bitwise": same,
+        "drive": f"Gaussian (sigma 0.6 ms), {W + K} steps; G is against this drive's, not a calibrated source's",
+        "study": study,
     }
     return line_for("FDTD Mcell-updates/s (64 rooms batched; vs 64 sequential plain solves)",
                     air / per_step_s / 1e6, K, W, ms_batch,
