@@ -93,10 +93,16 @@ __device__ __forceinline__ int step_offset(uint32_t d, int px, int sxy) {
 // Everything one step's launch reads, by value (kernel arguments).
 template <typename T>
 struct StepArgs {
-    T *__restrict__ a;        // previous pressure, overwritten by the next
-    const T *__restrict__ b;  // current pressure
-    T *__restrict__ wa;       // branch velocities, the same way
+    // The previous pressure (read at a cell by that cell's update only), the current one,
+    // and where the next goes: a's slot (one step a launch) or another (see pair_kernel).
+    // No restrict on a and o, which may be one array; each cell's update reads its a
+    // before it writes its o.
+    const T *a;
+    const T *__restrict__ b;
+    T *o;
+    const T *wa;  // the branch velocities, the same way
     const T *__restrict__ wb;
+    T *wo;
     // interior
     const uint8_t *__restrict__ types;
     const float *__restrict__ scale;  // padded, or null
@@ -164,6 +170,10 @@ struct StepArgs {
     const uint32_t *__restrict__ tap_of;
     const uint32_t *__restrict__ sp_tap;
     const T *__restrict__ inj;
+    // Interior cells stepped one by one: the second launch of two steps' (see pair_kernel)
+    // interior cells that are not deep.
+    uint32_t cell_blocks, cell_count;
+    const uint32_t *__restrict__ cells;
 };
 
 // A face special (see the tiles' notes): a special cell whose links are axial and weigh
@@ -258,18 +268,18 @@ __device__ __forceinline__ void face_finish(const StepArgs<T> &s, const FaceLoad
     acc = acc - flow;
     acc = acc + static_cast<T>(c.w) * static_cast<T>(sig);
     const T next = acc * static_cast<T>(c.z);
-    s.a[i] = next;
+    s.o[i] = next;
     const T rise = next - f.previous;
 #pragma unroll
     for (int u = 0; u < kFaceChunk; ++u) {
-        if (static_cast<uint32_t>(u) < ns) s.wa[sbase + u] = f.wa[u] + (rise * static_cast<T>(kind[u].z) + own[u]);
+        if (static_cast<uint32_t>(u) < ns) s.wo[sbase + u] = f.wa[u] + (rise * static_cast<T>(kind[u].z) + own[u]);
     }
     for (uint32_t j = kFaceChunk; j < ns; ++j) {
         // The term again: wa has not changed since it was taken.
         const uint32_t q = sbase + j;
         const float4 k = s.face_branch[s.face_state_kind[q]];
         const T term = static_cast<T>(k.x) * s.wb[q] - static_cast<T>(k.y) * s.wa[q];
-        s.wa[q] = s.wa[q] + (rise * static_cast<T>(k.z) + term);
+        s.wo[q] = s.wa[q] + (rise * static_cast<T>(k.z) + term);
     }
 }
 
@@ -339,7 +349,7 @@ __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int
                 if constexpr (TAPS) {
                     if (cls[u] == kClassTap) q = q + s.inj[s.tap_of[i + u * sz]];
                 }
-                s.a[i + u * sz] = q;
+                s.o[i + u * sz] = q;
             }
         }
         if (count == kUnroll) {
@@ -430,7 +440,7 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
             if constexpr (TAPS) {
                 if (cls == kClassTap) q = q + s.inj[s.tap_of[i]];
             }
-            s.a[i] = q;
+            s.o[i] = q;
         }
         below = here;
         here = above;
@@ -544,7 +554,7 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
     acc = acc - flow;
     acc = acc + static_cast<T>(src) * static_cast<T>(sig);
     const T next = acc * static_cast<T>(ci);
-    s.a[i] = next;
+    s.o[i] = next;
     // For the adjoint's gradient: each special's p(n+1) - p(n-1) of this step.
     if (s.history != nullptr) s.history[s.step * s.specials + k] = next - previous;
     if (ns > 0) {
@@ -553,7 +563,7 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
         for (int u = 0; u < kStateChunk; ++u) {
             if (u < ns) {
                 const uint32_t q = sb + u * kBlock;
-                s.wa[q] = swa[u] + (rise * static_cast<T>(s.branch_gain[kind[u]]) + own[u]);
+                s.wo[q] = swa[u] + (rise * static_cast<T>(s.branch_gain[kind[u]]) + own[u]);
             }
         }
         for (int c = kStateChunk; c < ns; ++c) {
@@ -561,7 +571,7 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
             const uint32_t q = sb + c * kBlock;
             const uint32_t kd = s.state_branch[q];
             const T term = static_cast<T>(s.branch_now[kd]) * s.wb[q] - static_cast<T>(s.branch_before[kd]) * s.wa[q];
-            s.wa[q] = s.wa[q] + (rise * static_cast<T>(s.branch_gain[kd]) + term);
+            s.wo[q] = s.wa[q] + (rise * static_cast<T>(s.branch_gain[kd]) + term);
         }
     }
 }
@@ -611,7 +621,7 @@ __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
         st[axis * static_cast<size_t>(L) + c] = part;
         total += part;
     }
-    s.a[i] = total;
+    s.o[i] = total;
 }
 
 // The probes (fdtd.hpp:3525-3534), one wavefront each: its lanes read the probe's cells
@@ -659,10 +669,24 @@ __device__ __forceinline__ void record_probes(const StepArgs<T> &s, uint32_t blo
                   products);
 }
 
+// An interior cell of the list (7 points, one speed of sound): interior_column's update.
+template <typename T>
+__device__ __forceinline__ void interior_cell(const StepArgs<T> &s, int i) {
+    const int sy = s.px, sz = static_cast<int>(s.sxy);
+    const T *__restrict__ b = s.b;
+    T sum = b[i - 1] + b[i + 1];
+    sum = sum + b[i - sy];
+    sum = sum + b[i + sy];
+    sum = sum + b[i - sz];
+    sum = sum + b[i + sz];
+    const T next = s.centre * b[i] + s.axial * sum;
+    s.o[i] = next - s.a[i];
+}
+
 // One step in one launch: the probes first (a serial sum each, read from the current
 // field, which nothing writes), then special blocks (their gathers are the longest chains
 // of loads), face specials by blocks of their own (unless they are stepped in their
-// tiles), interior tiles and layer cells.
+// tiles), interior tiles, layer cells and listed interior cells.
 template <typename T, bool WIDE, bool SCALED>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
     uint32_t block = blockIdx.x;
@@ -691,6 +715,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
         const uint32_t c = block * kBlock + threadIdx.x;
         if (c < s.layer_cells) layer_cell<T>(s, c);
         return;
+    }
+    block -= s.layer_blocks;
+    if constexpr (!WIDE && !SCALED) {
+        if (block < s.cell_blocks) {
+            const uint32_t c = block * kBlock + threadIdx.x;
+            if (c < s.cell_count) interior_cell<T>(s, static_cast<int>(s.cells[c]));
+        }
     }
 }
 
@@ -787,14 +818,14 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
         const T fresh = now * wb + (static_cast<T>(1) - before) * old +
                         static_cast<T>(s.state_weight[q]) * (before * qa - now * qb);
         gsum += static_cast<T>(s.branch_gain[kd]) * (fresh - old);
-        s.wa[q] = fresh;
+        s.wo[q] = fresh;
     }
     T q = ci * (ca * qb + sum - cb * qa + gsum);
     if (s.sp_tap != nullptr) {
         const uint32_t u = s.sp_tap[k];
         if (u != kNoTap) q = q + s.inj[u];
     }
-    s.a[i] = q;
+    s.o[i] = q;
 }
 
 template <typename T, bool WIDE>
@@ -841,294 +872,203 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(const StepArgs<T> s,
     }
 }
 
-// ---- Two steps a launch: `march_kernel` (opt-in) ----------------------------------------
+// ---- Two steps a launch: `pair_kernel` --------------------------------------------------
 //
-// Steps k and k + 1 of a grid whose cells are interior, face specials or solid (no matched
-// layer, no general specials, one speed of sound, 7 points), in float. Each block owns a
-// column of 60 x 8 x march_z cells and marches it plane by plane, keeping only a few
-// planes in LDS:
-// - p(k) of the column and two cells round it, a plane of 64 x 12 cells (the block's
-//   wavefronts load whole rows of 64, a row's lanes 0 to 63 the columns x0 - 2 to x0 + 61);
-//   four planes in LDS, loaded by direct global-to-LDS loads one plane ahead;
-// - p(k + 1) of the column and one cell round it (lanes 1 to 62, 10 rows), three planes;
-// - the classes of those cells, two planes.
-// Plane j of p(k + 1) (z = z0 - 1 + j) is stepped from p(k) planes j - 1, j, j + 1 (in LDS),
-// p(k - 1) and the classes (read a plane ahead into registers) and plane j's face specials
-// from the block's records; then plane j - 1 of the owned cells (lanes 2 to 61, rows 1 to 8)
-// is stepped to p(k + 2) from p(k + 1) planes j - 2, j - 1, j, and p(k + 1) and p(k + 2) of
-// its cells are written, with its own face specials' branch velocities w(k + 1) and
-// w(k + 2). Two barriers a plane. p(k - 1), p(k), w(k - 1) and w(k) are read by every block
-// (the rings overlap) and written by none: the outputs go to the other two fields and state
-// arrays of four. Each cell's arithmetic is the one-step kernels', in their order, so the
-// fields are the same bit for bit. The same launch's probe blocks record p(k - 1) and p(k),
-// the fields it reads; the step after a run of such launches is recorded by a launch of the
-// probes alone.
-struct PairArgs {
-    const float *__restrict__ prev;  // p(k - 1)
-    const float *__restrict__ cur;   // p(k)
-    float *__restrict__ out1;        // p(k + 1)
-    float *__restrict__ out2;        // p(k + 2)
-    const float *__restrict__ wprev, *__restrict__ wcur;  // w(k - 1), w(k)
-    float *__restrict__ wout1, *__restrict__ wout2;       // w(k + 1), w(k + 2)
-    const float *__restrict__ zero;  // a float 0 in device memory, read for cells off the grid
-    const uint8_t *__restrict__ types;
-    const uint32_t *__restrict__ boxes;  // per box block: its column, x fastest, or kNoTile
-    uint32_t box_blocks, boxes_x, boxes_y;
-    // A record: its cell's lane, row and plane in the column's p(k + 1) region
-    // (lane | row << 8 | plane << 16), then as a tile's face record: links | states << 6 |
-    // coefficients << 10, states' start, kinds.
-    const uint4 *__restrict__ pair_record;
-    const uint16_t *__restrict__ face_state_kind;
-    const float4 *__restrict__ face_coef;
-    const float4 *__restrict__ face_branch;
-    int ny, nz, px, sxy;
-    float axial, centre;
-    const float *__restrict__ signal;  // indexed by step
-    unsigned long long step;           // k: signal[k], signal[k + 1]; records out[k - 1], out[k]
-    int record_prev;                   // 0: leave out[k - 1] (a step before this run)
-    uint32_t probes;
-    const uint32_t *__restrict__ probe_start;
-    const uint32_t *__restrict__ probe_cell;
-    const double *__restrict__ probe_weight;
-    double *__restrict__ out;
-    unsigned long long out_stride;
-    // Planes a column marches through, and per box block its records' bounds by plane
-    // (plane_stride entries a block: per plane its first record and its first ring record,
-    // then one past the last plane's records).
-    int march_z;
-    const uint32_t *__restrict__ plane_faces;
-    uint32_t plane_stride;
+// Steps k and k + 1 as two launches instead of two steps of the one-step kernel (float, 7
+// points, one speed of sound). The first, pair_kernel, steps every special and layer cell
+// once (p(k + 1)) and, by boxes, every interior cell twice: p(k + 1) of all of them and
+// p(k + 2) of the deep ones, interior cells whose six neighbours are interior too. The
+// second launch (step_kernel, its tiles replaced by a list of cells) steps the specials,
+// the layer and the other interior cells from p(k + 1) to p(k + 2). The fields take four
+// slots: p(k - 1) and p(k) are read by the first launch and written by neither, p(k + 1)
+// and p(k + 2) go to the other two.
+//
+// A box is one wavefront's: 60 columns (lanes 2 to 61 of a row of 64: x0 to x0 + 59) by 4
+// rows (y0 to y0 + 3) by kPairZ planes, marched along z one plane at a time with every
+// value in registers:
+// - p(k) of 8 rows (y0 - 2 to y0 + 5) on three planes, z - 1, z and z + 1;
+// - p(k + 1) of 6 rows (y0 - 1 to y0 + 4, lanes 1 to 62 valid) on three planes: plane z is
+//   stepped from p(k) and p(k - 1); then plane z - 1 of the owned rows is stepped to
+//   p(k + 2) from planes z - 2 to z of p(k + 1) and its own p(k).
+// The x neighbours come from the next lanes (DPP row shifts across the wavefront), the y
+// and z neighbours from registers. Values of cells that are not interior (or beyond the
+// grid's edges, whose loads are clamped to it) are computed and never used: a deep cell's
+// neighbours are interior. Each owned cell's p(k + 1) is written if it is interior, its
+// p(k + 2) if it is deep. The last box of a column starts low enough to march whole
+// planes, overlapping the one below it, which writes the same values there. Every update
+// is the interior's arithmetic in its order, so the fields equal one step a launch's bit
+// for bit.
+constexpr int kBoxX = kTileX - 4;  // columns a box owns
+constexpr int kBoxRows = 4;        // rows a box owns (a row group)
+constexpr uint32_t kBoxWaves = kBlock / kTileX;
+
+struct BoxArgs {
+    const float *prev;  // p(k - 1)
+    const float *cur;   // p(k)
+    float *out1;        // p(k + 1)
+    float *out2;        // p(k + 2)
+    // Per plane, row group (rows 1 + 4 g to 4 + 4 g) and x: the four rows' bytes, bit 0 set
+    // where the cell is interior, bit 1 where it is deep.
+    const uint32_t *bits;
+    const uint32_t *boxes;  // per wavefront of the box blocks: g + groups_y (bx + boxes_x bz), or kNoTile
+    uint32_t box_blocks, boxes_x, groups_y;
+    int ny, nz, px;
+    uint32_t plane_bytes, row_bytes, field_bytes, bits_bytes;
+    float centre, axial;
 };
 
-// A face special's step (face_special's arithmetic, in its order) from its pressure, its
-// previous pressure, its six axial neighbours (-x, +x, -y, +y, -z, +z; those not linked
-// unread) and its states' velocities wa (the older) and wb; with wn, its states' new
-// velocities are written there.
-__device__ __forceinline__ float pair_face(const PairArgs &s, const uint4 rec, float here, float previous,
-                                           const float (&nb)[6], const float *__restrict__ wa,
-                                           const float *__restrict__ wb, float *__restrict__ wn, float sig) {
-    const uint32_t desc = rec.y, sbase = rec.z;
-    const uint32_t links = desc & 63u, ns = (desc >> 6) & 15u, ci_index = desc >> 10;
-    const float4 c = s.face_coef[ci_index];  // coef_a, coef_b, coef_inv, source
-    float sum = 0;
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-        if (links & (1u << d)) sum += s.axial * nb[d];
-    }
-    constexpr int kChunk = 4;
-    float4 kind[kChunk];
-    float va[kChunk], vb[kChunk], own[kChunk];
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-        if (static_cast<uint32_t>(u) < ns) {
-            uint32_t kid = (rec.w >> (8 * u)) & 0xFFu;
-            if (kid == 0xFFu) kid = s.face_state_kind[sbase + u];
-            kind[u] = s.face_branch[kid];  // now, before, gain, weight
-            va[u] = wa[sbase + u];
-            vb[u] = wb[sbase + u];
-        }
-    }
-    float flow = 0;
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-        if (static_cast<uint32_t>(u) < ns) {
-            own[u] = kind[u].x * vb[u] - kind[u].y * va[u];
-            flow += kind[u].w * own[u];
-        }
-    }
-    for (uint32_t j = kChunk; j < ns; ++j) {
-        const uint32_t q = sbase + j;
-        const float4 k = s.face_branch[s.face_state_kind[q]];
-        const float term = k.x * wb[q] - k.y * wa[q];
-        flow += k.w * term;
-    }
-    float acc = c.x * here;
-    acc = acc + sum;
-    acc = acc - c.y * previous;
-    acc = acc - flow;
-    acc = acc + c.w * sig;
-    const float next = acc * c.z;
-    if (wn != nullptr) {
-        const float rise = next - previous;
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            if (static_cast<uint32_t>(u) < ns) wn[sbase + u] = va[u] + (rise * kind[u].z + own[u]);
-        }
-        for (uint32_t j = kChunk; j < ns; ++j) {
-            const uint32_t q = sbase + j;
-            const float4 k = s.face_branch[s.face_state_kind[q]];
-            const float term = k.x * wb[q] - k.y * wa[q];
-            wn[q] = wa[q] + (rise * k.z + term);
-        }
-    }
-    return next;
+// Raw buffer access: a wavefront-uniform byte offset (soff) and the lane's (voff), against
+// a resource whose range is the array's, so that an address past it reads 0 instead of
+// faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), static_cast<short>(0), static_cast<int>(bytes),
+                                            0x00020000);
+}
+__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buffer_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+// Lane l's value of v from lane l - 1 (shr) or l + 1 (shl) of the wavefront; lanes 0 and
+// 63 get 0. Every lane must be active.
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
 
-constexpr int kMarchX = kTileX - 4;            // columns a box owns (lanes 2 to 61)
-constexpr int kMarchY = 8;                     // rows a box owns
-constexpr int kMarchPRows = kMarchY + 4;       // p(k): two rows round
-constexpr int kMarchQRows = kMarchY + 2;       // p(k + 1): one row round
-constexpr int kMarchP = kMarchPRows * kTileX;  // a plane of p(k) in LDS
-constexpr int kMarchQ = kMarchQRows * kTileX;  // a plane of p(k + 1) (or classes) in LDS
-constexpr int kMarchRowsPerWave = (kMarchPRows + kBlock / kTileX - 1) / (kBlock / kTileX);
-static_assert(kMarchPRows % (kBlock / kTileX) == 0, "each wavefront loads whole rows of p(k)");
-static_assert(kMarchY % (kBlock / kTileX) == 0, "each wavefront steps whole owned rows");
-
-__global__ __launch_bounds__(kBlock) void march_kernel(const PairArgs s) {
-    __shared__ __align__(16) float P[4 * kMarchP];
-    __shared__ float Q[3 * kMarchQ];
-    __shared__ uint8_t C[2 * kMarchQ];
-    const uint32_t t = threadIdx.x;
-    if (blockIdx.x >= s.box_blocks) {
-        const uint32_t block = blockIdx.x - s.box_blocks;
-        double *products = reinterpret_cast<double *>(P);
-        if (s.record_prev) {
-            probe_sums<float>(s.probes, s.probe_start, s.probe_cell, s.probe_weight, s.prev, s.out, s.out_stride,
-                              s.step - 1, block, products);
+template <int kPairZ>
+__device__ __forceinline__ void pair_box(const BoxArgs &s, uint32_t block) {
+    const int lane = static_cast<int>(threadIdx.x % kTileX);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
+    const uint32_t id = s.boxes[block * kBoxWaves + wave];
+    if (id == kNoTile) return;
+    const uint32_t g = id % s.groups_y, rest = id / s.groups_y;
+    const int bx = static_cast<int>(rest % s.boxes_x), bz = static_cast<int>(rest / s.boxes_x);
+    const int x = bx * kBoxX - 2 + lane;
+    const uint32_t xo = static_cast<uint32_t>(min(max(x, 0), s.px - 1)) * 4u;
+    const int y0 = 1 + kBoxRows * static_cast<int>(g);
+    const int z0 = min(1 + bz * kPairZ, s.nz - 1 - kPairZ);
+    const bool owned = lane >= 2 && lane < kTileX - 2;
+    const auto rc = buffer_of(s.cur, s.field_bytes), rp = buffer_of(s.prev, s.field_bytes);
+    const auto r1 = buffer_of(s.out1, s.field_bytes), r2 = buffer_of(s.out2, s.field_bytes);
+    const auto rb = buffer_of(s.bits, s.bits_bytes);
+    uint32_t yo[kBoxRows + 4];  // rows y0 - 2 .. y0 + 5, clamped to the grid
+#pragma unroll
+    for (int q = 0; q < kBoxRows + 4; ++q) yo[q] = static_cast<uint32_t>(min(max(y0 - 2 + q, 0), s.ny - 1)) * s.row_bytes;
+    const auto plane = [&](int z) { return static_cast<uint32_t>(min(max(z, 0), s.nz - 1)) * s.plane_bytes; };
+    const uint32_t bits_row = g * static_cast<uint32_t>(s.px) * 4u;
+    const uint32_t bits_plane = s.groups_y * static_cast<uint32_t>(s.px) * 4u;
+    const float c = s.centre, w = s.axial;
+    float P0[kBoxRows + 4], P1[kBoxRows + 4];  // p(k), planes z - 1 and z
+    float Q0[kBoxRows + 2], Q1[kBoxRows + 2];  // p(k + 1), planes z - 2 and z - 1
+#pragma unroll
+    for (int q = 0; q < kBoxRows + 4; ++q) {
+        P0[q] = buffer_load(rc, xo, plane(z0 - 2) + yo[q]);
+        P1[q] = buffer_load(rc, xo, plane(z0 - 1) + yo[q]);
+    }
+#pragma unroll
+    for (int r = 0; r < kBoxRows + 2; ++r) {
+        Q0[r] = 0.0f;
+        Q1[r] = 0.0f;
+    }
+    uint32_t bits_before = 0;
+#pragma unroll
+    for (int t = 0; t <= kPairZ + 1; ++t) {
+        const int z = z0 - 1 + t;  // the plane of p(k + 1) this pass steps
+        float P2[kBoxRows + 4], A[kBoxRows + 2];
+        const uint32_t pz = plane(z), pn = plane(z + 1);
+#pragma unroll
+        for (int q = 0; q < kBoxRows + 4; ++q) P2[q] = buffer_load(rc, xo, pn + yo[q]);
+#pragma unroll
+        for (int r = 0; r < kBoxRows + 2; ++r) A[r] = buffer_load(rp, xo, pz + yo[r + 1]);
+        const uint32_t bits =
+            __float_as_uint(buffer_load(rb, xo, static_cast<uint32_t>(min(max(z, 0), s.nz - 1)) * bits_plane + bits_row));
+        // p(k + 1) of plane z, rows y0 - 1 .. y0 + 4: the interior update (fdtd.hpp:240-288)
+        // in its order, -x, +x, -y, +y, -z, +z.
+        float N[kBoxRows + 2];
+#pragma unroll
+        for (int r = 0; r < kBoxRows + 2; ++r) {
+            const int q = r + 1;
+            float sum = from_left(P1[q]) + from_right(P1[q]);
+            sum = sum + P1[q - 1];
+            sum = sum + P1[q + 1];
+            sum = sum + P0[q];
+            sum = sum + P2[q];
+            const float next = c * P1[q] + w * sum;
+            N[r] = next - A[r];
         }
-        probe_sums<float>(s.probes, s.probe_start, s.probe_cell, s.probe_weight, s.cur, s.out, s.out_stride, s.step,
-                          block, products);
+        // p(k + 2) of plane z - 1, the owned rows (on every lane: the shifts read them all).
+        float V[kBoxRows];
+#pragma unroll
+        for (int r = 1; r <= kBoxRows; ++r) {
+            float sum = from_left(Q1[r]) + from_right(Q1[r]);
+            sum = sum + Q1[r - 1];
+            sum = sum + Q1[r + 1];
+            sum = sum + Q0[r];
+            sum = sum + N[r];
+            const float next = c * Q1[r] + w * sum;
+            V[r - 1] = next - P0[r + 1];
+        }
+        if (owned && t >= 1 && t <= kPairZ) {
+#pragma unroll
+            for (int r = 0; r < kBoxRows; ++r) {
+                if ((bits >> (8 * r)) & 1u) buffer_store(N[r + 1], r1, xo, pz + yo[r + 2]);
+            }
+        }
+        if (owned && t >= 2) {
+            const uint32_t pm = plane(z - 1);
+#pragma unroll
+            for (int r = 0; r < kBoxRows; ++r) {
+                if ((bits_before >> (8 * r)) & 2u) buffer_store(V[r], r2, xo, pm + yo[r + 2]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kBoxRows + 4; ++q) {
+            P0[q] = P1[q];
+            P1[q] = P2[q];
+        }
+#pragma unroll
+        for (int r = 0; r < kBoxRows + 2; ++r) {
+            Q0[r] = Q1[r];
+            Q1[r] = N[r];
+        }
+        bits_before = bits;
+    }
+}
+
+// The first launch of two steps: probes (of p(k)), general specials, face specials, the
+// interior's boxes, layer cells. Every role but the boxes is one step's (p(k + 1) into s.o).
+template <int kPairZ>
+__global__ __launch_bounds__(kBlock) void pair_kernel(const StepArgs<float> s, const BoxArgs x) {
+    uint32_t block = blockIdx.x;
+    if (block < s.probe_blocks) {
+        record_probes<float>(s, block);
         return;
     }
-    const uint32_t entry = s.boxes[blockIdx.x];
-    if (entry == kNoTile) return;
-    const int bx = static_cast<int>(entry % s.boxes_x);
-    const int rest = static_cast<int>(entry / s.boxes_x);
-    const int by = rest % static_cast<int>(s.boxes_y), bz = rest / static_cast<int>(s.boxes_y);
-    const int px = s.px, sxy = s.sxy, ny = s.ny, nz = s.nz;
-    const int x0 = bx * kMarchX, y0 = 1 + by * kMarchY, z0 = 1 + bz * s.march_z;
-    const int planes = min(s.march_z, nz - 1 - z0);  // owned planes z0 .. z0 + planes - 1
-    const int wave = static_cast<int>(t / kTileX), lane = static_cast<int>(t % kTileX);
-    const int gx = x0 - 2 + lane;  // this lane's column
-    const bool x_in = gx >= 0 && gx < px;
-    const bool q_lane = lane >= 1 && lane <= kTileX - 2;
-    const bool o_lane = lane >= 2 && lane <= kTileX - 3;
-
-    // p(k) plane i (z = z0 - 2 + i) into its LDS slot; cells off the grid read 0.
-    const auto load_p = [&](int i) {
-        const int z = z0 - 2 + i;
-        float *slot = P + (i & 3) * kMarchP;
-#pragma unroll
-        for (int m = 0; m < kMarchRowsPerWave; ++m) {
-            const int row = wave + m * (kBlock / kTileX);
-            const int y = y0 - 2 + row;
-            const bool in = x_in && y >= 0 && y < ny && z >= 0 && z < nz;
-            const float *src = in ? s.cur + (gx + y * px + z * sxy) : s.zero;
-            __builtin_amdgcn_global_load_lds(src, slot + row * kTileX, 4, 0, 0);
-        }
-    };
-    // p(k - 1) and the classes of this thread's cells of p(k + 1) plane j: rows wave,
-    // wave + 4, wave + 8 (those below 10) of its lane.
-    float pre[kMarchRowsPerWave], pre_next[kMarchRowsPerWave];
-    uint32_t cls[kMarchRowsPerWave], cls_next[kMarchRowsPerWave];
-    const auto load_q = [&](int j, float (&pr)[kMarchRowsPerWave], uint32_t (&cl)[kMarchRowsPerWave]) {
-        const int z = z0 - 1 + j;
-#pragma unroll
-        for (int m = 0; m < kMarchRowsPerWave; ++m) {
-            const int r = wave + m * (kBlock / kTileX);
-            const int y = y0 - 1 + r;
-            pr[m] = 0.0f;
-            cl[m] = 0;
-            if (q_lane && r < kMarchQRows && x_in && y < ny && z < nz) {
-                const int g = gx + y * px + z * sxy;
-                pr[m] = s.prev[g];
-                cl[m] = s.types[g];
-            }
-        }
-    };
-    load_p(0);
-    load_p(1);
-    load_p(2);
-    load_q(0, pre, cls);
-    const uint32_t *pf = s.plane_faces + static_cast<size_t>(blockIdx.x) * s.plane_stride;
-    const float sig0 = s.signal[s.step], sig1 = s.signal[s.step + 1];
-    for (int j = 0; j <= planes + 1; ++j) {
-        // p(k) planes j .. j + 2 have landed, and no thread still reads the slot of plane
-        // j - 1, which plane j + 3 takes next.
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (j <= planes) load_p(j + 3);
-        if (j <= planes) load_q(j + 1, pre_next, cls_next);
-        const float *Pm = P + (j & 3) * kMarchP, *Pc = P + ((j + 1) & 3) * kMarchP,
-                    *Pp = P + ((j + 2) & 3) * kMarchP;
-        float *Qj = Q + (j % 3) * kMarchQ;
-        uint8_t *Cj = C + (j & 1) * kMarchQ;
-        // Step k on plane j: interior cells, solid cells (0), then the face specials.
-#pragma unroll
-        for (int m = 0; m < kMarchRowsPerWave; ++m) {
-            const int r = wave + m * (kBlock / kTileX);
-            if (r < kMarchQRows) {
-                const int q = r * kTileX + lane;
-                const int c = q + kTileX;  // the same cell in the p(k) plane
-                if (cls[m] == kClassInterior) {
-                    // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
-                    float sum = Pc[c - 1] + Pc[c + 1];
-                    sum = sum + Pc[c - kTileX];
-                    sum = sum + Pc[c + kTileX];
-                    sum = sum + Pm[c];
-                    sum = sum + Pp[c];
-                    const float next = s.centre * Pc[c] + s.axial * sum;
-                    Qj[q] = next - pre[m];
-                } else if (cls[m] != kClassFace) {
-                    Qj[q] = 0.0f;
-                }
-                Cj[q] = static_cast<uint8_t>(cls[m]);
-            }
-        }
-        {
-            const uint32_t r_own = pf[2 * j + 1], r_end = pf[2 * j + 2];
-            for (uint32_t r = pf[2 * j] + t; r < r_end; r += kBlock) {
-                const uint4 rec = s.pair_record[r];
-                const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu);
-                const int q = qy * kTileX + qx, c = q + kTileX;
-                const int g = (x0 - 2 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + j) * sxy;
-                const float nb[6] = {Pc[c - 1], Pc[c + 1], Pc[c - kTileX], Pc[c + kTileX], Pm[c], Pp[c]};
-                const bool own = r < r_own;
-                const float next = pair_face(s, rec, Pc[c], s.prev[g], nb, s.wprev, s.wcur, own ? s.wout1 : nullptr, sig0);
-                Qj[q] = next;
-                if (own) s.out1[g] = next;
-            }
-        }
-        __syncthreads();
-        if (j >= 2) {
-            // Step k + 1 on the owned cells of plane j - 1, then its own face specials (the
-            // same records, so the same threads, as its step k: each reads back the
-            // w(k + 1) it wrote).
-            const int jo = j - 1;
-            const float *Qm = Q + ((jo + 2) % 3) * kMarchQ, *Qc = Q + (jo % 3) * kMarchQ, *Qp = Qj;
-            const uint8_t *Cc = C + (jo & 1) * kMarchQ;
-            const float *Po = P + (j & 3) * kMarchP;  // p(k) plane of the same z
-#pragma unroll
-            for (int m = 0; m < kMarchY / (kBlock / kTileX); ++m) {
-                const int r = 1 + wave + m * (kBlock / kTileX);
-                const int q = r * kTileX + lane;
-                if (o_lane && Cc[q] == kClassInterior) {
-                    float sum = Qc[q - 1] + Qc[q + 1];
-                    sum = sum + Qc[q - kTileX];
-                    sum = sum + Qc[q + kTileX];
-                    sum = sum + Qm[q];
-                    sum = sum + Qp[q];
-                    const float next = s.centre * Qc[q] + s.axial * sum;
-                    const int g = gx + (y0 - 1 + r) * px + (z0 - 1 + jo) * sxy;
-                    s.out1[g] = Qc[q];
-                    s.out2[g] = next - Po[q + kTileX];
-                }
-            }
-            const uint32_t r_own = pf[2 * jo + 1];
-            for (uint32_t r = pf[2 * jo] + t; r < r_own; r += kBlock) {
-                const uint4 rec = s.pair_record[r];
-                const int qx = static_cast<int>(rec.x & 0xFFu), qy = static_cast<int>((rec.x >> 8) & 0xFFu);
-                const int q = qy * kTileX + qx;
-                const int g = (x0 - 2 + qx) + (y0 - 1 + qy) * px + (z0 - 1 + jo) * sxy;
-                const float nb[6] = {Qc[q - 1], Qc[q + 1], Qc[q - kTileX], Qc[q + kTileX], Qm[q], Qp[q]};
-                s.out2[g] = pair_face(s, rec, Qc[q], Po[q + kTileX], nb, s.wcur, s.wout1, s.wout2, sig1);
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < kMarchRowsPerWave; ++m) {
-            pre[m] = pre_next[m];
-            cls[m] = cls_next[m];
-        }
+    block -= s.probe_blocks;
+    if (block < s.special_blocks) {
+        special_cell<float>(s, block);
+        return;
+    }
+    block -= s.special_blocks;
+    if (block < s.face_blocks) {
+        const uint32_t r = block * kBlock + threadIdx.x;
+        if (r < s.face_count) face_special<float>(s, r);
+        return;
+    }
+    block -= s.face_blocks;
+    if (block < x.box_blocks) {
+        pair_box<kPairZ>(x, block);
+        return;
+    }
+    block -= x.box_blocks;
+    if (block < s.layer_blocks) {
+        const uint32_t c = block * kBlock + threadIdx.x;
+        if (c < s.layer_cells) layer_cell<float>(s, c);
+        return;
     }
 }
 
@@ -1160,14 +1100,10 @@ struct as_fdtd {
     // launch. p(k - 1) is in slot `prev` and p(k) in prev ^ 1 (the same for the velocities).
     void *field[4] = {}, *velocity[4] = {};
     int prev = 0;
-    int pairs = 0;  // 1: two steps a launch (see march_kernel)
-    uint32_t box_blocks = 0, boxes_x = 0, boxes_y = 0;
-    uint32_t *boxes = nullptr;
-    uint4 *pair_record = nullptr;
-    int march_z = 8;          // planes a march_kernel column owns
-    uint32_t *plane_faces = nullptr;
-    uint32_t plane_stride = 0;
-    float *zero = nullptr;
+    int pairs = 0;      // 1: two steps a launch (see pair_kernel)
+    int pair_z = 16;    // planes a box marches
+    uint32_t box_blocks = 0, boxes_x = 0, groups_y = 0, cell_count = 0;
+    uint32_t *boxes = nullptr, *box_bits = nullptr, *cells = nullptr;
     uint8_t *types = nullptr;
     float *scale = nullptr;
     uint32_t *tiles = nullptr;
@@ -1597,94 +1533,74 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     std::copy(face_state_kind.begin(), face_state_kind.end(), kinds.begin() + static_cast<ptrdiff_t>(state_slots));
     h->state_slots = face_slots;
 
-    // Two steps a launch (march_kernel): a grid of interior cells, face specials and solid
-    // cells, in float, rows of whole wavefronts.
-    h->pairs = pairs_on && precision == 32 && !h->wide && !h->scaled && L.layer_cells == 0 && G == 0 &&
-                       px % kTileX == 0 ? 1 : 0;
-    std::vector<uint32_t> boxes, plane_faces;
-    std::vector<uint32_t> pair_records;
+    // Two steps a launch (pair_kernel): float, 7 points, one speed of sound, and arrays
+    // within a buffer resource's 32-bit byte range.
+    const uint64_t groups = (ny - 2 + kBoxRows - 1) / kBoxRows;
+    h->pairs = pairs_on && precision == 32 && !h->wide && !h->scaled && nz >= 10 && n_pad * sizeof(float) < (1ull << 32) &&
+                       nz * groups * px * sizeof(uint32_t) < (1ull << 32)
+                   ? 1
+                   : 0;
+    std::vector<uint32_t> boxes, box_bits, cells;
     if (h->pairs) {
-        // Columns of 60 x 8 x march_z cells (march_kernel), by die as the tiles; each face
-        // special goes to every column whose p(k + 1) region (one cell round) holds it, by
-        // plane, the column's own records of a plane first.
-        const int Z = tuning && tuning->tile_z > 0 ? tuning->tile_z : 8;
-        if (Z > 200) refuse("a column marches at most 200 planes");
-        h->march_z = Z;
-        h->boxes_x = static_cast<uint32_t>((nx + kMarchX - 1) / kMarchX);
-        h->boxes_y = static_cast<uint32_t>((ny - 2 + kMarchY - 1) / kMarchY);
-        const uint32_t boxes_z = static_cast<uint32_t>((nz - 2 + Z - 1) / Z);
-        const uint64_t box_total = static_cast<uint64_t>(h->boxes_x) * h->boxes_y * boxes_z;
-        if (box_total >= kNoTile) refuse("too many boxes");
-        std::vector<uint8_t> active(box_total, 0);
-        for (uint64_t z = 1; z + 1 < nz; ++z)
-            for (uint64_t y = 1; y + 1 < ny; ++y)
-                for (uint64_t x = 0; x < nx; ++x) {
-                    const uint8_t c = types[x + px * (y + ny * z)];
-                    if (c == kClassInterior || c == kClassFace) {
-                        active[x / kMarchX + h->boxes_x * ((y - 1) / kMarchY + h->boxes_y * ((z - 1) / Z))] = 1;
-                    }
-                }
-        std::vector<uint32_t> list;
-        for (uint64_t b = 0; b < box_total; ++b)
-            if (active[b]) list.push_back(static_cast<uint32_t>(b));
-        const size_t per = (list.size() + kXcds - 1) / kXcds;
-        boxes.assign(per * kXcds, kNoTile);
-        for (size_t r = 0; r < boxes.size(); ++r) {
-            const size_t idx = h->tile_order != 1 ? (r % kXcds) * per + r / kXcds : r;
-            if (idx < list.size()) boxes[r] = list[idx];
-        }
-        std::vector<uint32_t> block_of(box_total, kNoTile);
-        for (size_t r = 0; r < boxes.size(); ++r)
-            if (boxes[r] != kNoTile) block_of[boxes[r]] = static_cast<uint32_t>(r);
-        struct Placed {
-            uint32_t block, plane, ring;
-            std::array<uint32_t, 4> rec;
+        const uint64_t boxes_x = (nx + kBoxX - 1) / kBoxX;
+        const auto count_boxes = [&](uint64_t Z) { return boxes_x * groups * ((nz - 2 + Z - 1) / Z); };
+        // Boxes of 16 planes where there are enough of them to fill the chip, else of 8; or
+        // as asked (pairs 8 or 16), where the grid has the planes.
+        int Z = nz - 2 >= 16 && count_boxes(16) >= 2048 ? 16 : 8;
+        if ((tuning->pairs == 8 || tuning->pairs == 16) && nz - 2 >= static_cast<uint64_t>(tuning->pairs)) Z = tuning->pairs;
+        const uint64_t boxes_z = (nz - 2 + Z - 1) / Z;
+        if (count_boxes(Z) >= kNoTile) refuse("too many boxes");
+        h->pair_z = Z;
+        h->boxes_x = static_cast<uint32_t>(boxes_x);
+        h->groups_y = static_cast<uint32_t>(groups);
+        // Each interior cell's bits (interior; deep: its six neighbours interior too); the
+        // interior cells that are not deep go to the list the second launch steps.
+        box_bits.assign(nz * groups * px, 0);
+        std::vector<uint8_t> active(count_boxes(Z), 0);
+        const auto inner = [&](uint64_t x, uint64_t y, uint64_t z) {
+            return types[x + px * (y + ny * z)] == kClassInterior;
         };
-        std::vector<Placed> placed;
-        for (const auto &bc : by_cell) {
-            const uint64_t cell = bc.first, k = bc.second;
-            const size_t r = face_of[k];
-            const int64_t x = static_cast<int64_t>(cell % px), y = static_cast<int64_t>((cell / px) % ny),
-                          z = static_cast<int64_t>(cell / (px * ny));
-            for (int64_t bz = (z - 1) / Z - 1; bz <= (z - 1) / Z + 1; ++bz)
-                for (int64_t by = (y - 1) / kMarchY - 1; by <= (y - 1) / kMarchY + 1; ++by)
-                    for (int64_t bx = x / kMarchX - 1; bx <= x / kMarchX + 1; ++bx) {
-                        if (bx < 0 || by < 0 || bz < 0 || bx >= h->boxes_x || by >= h->boxes_y || bz >= boxes_z) continue;
-                        const int64_t x0 = bx * kMarchX, y0 = 1 + by * kMarchY, z0 = 1 + bz * Z;
-                        const int64_t planes = std::min<int64_t>(Z, static_cast<int64_t>(nz) - 1 - z0);
-                        const int64_t qx = x - (x0 - 2), qy = y - (y0 - 1), j = z - (z0 - 1);
-                        if (qx < 1 || qx > kTileX - 2 || qy < 0 || qy >= kMarchQRows || j < 0 || j > planes + 1) continue;
-                        const uint32_t blk = block_of[static_cast<uint64_t>(bx) + h->boxes_x * (static_cast<uint64_t>(by) + h->boxes_y * static_cast<uint64_t>(bz))];
-                        if (blk == kNoTile) continue;
-                        const bool mine = qx >= 2 && qx <= kTileX - 3 && qy >= 1 && qy <= kMarchY && j >= 1 && j <= planes;
-                        placed.push_back({blk, static_cast<uint32_t>(j), mine ? 0u : 1u,
-                                          {static_cast<uint32_t>(qx | qy << 8 | j << 16), face_desc[r], face_state[r],
-                                           face_kinds[r]}});
+        bool edge = false;  // an interior cell on the grid's edge: one step a launch only
+        for (uint64_t z = 0; z < nz && !edge; ++z)
+            for (uint64_t y = 0; y < ny && !edge; ++y)
+                for (uint64_t x = 0; x < nx; ++x) {
+                    if (!inner(x, y, z)) continue;
+                    if (x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz) {
+                        edge = true;
+                        break;
                     }
-        }
-        std::stable_sort(placed.begin(), placed.end(), [](const Placed &a, const Placed &b) {
-            return a.block != b.block ? a.block < b.block : a.plane != b.plane ? a.plane < b.plane : a.ring < b.ring;
-        });
-        if (placed.size() >= (1ull << 31)) throw Failure{AS_ERANGE, "too many face records"};
-        h->plane_stride = static_cast<uint32_t>(2 * (Z + 2) + 1);
-        plane_faces.assign(boxes.size() * h->plane_stride, 0);
-        size_t at = 0;
-        for (size_t b = 0; b < boxes.size(); ++b) {
-            uint32_t *row = plane_faces.data() + b * h->plane_stride;
-            for (int j = 0; j < Z + 2; ++j) {
-                row[2 * j] = static_cast<uint32_t>(at);
-                while (at < placed.size() && placed[at].block == b && placed[at].plane == static_cast<uint32_t>(j) &&
-                       placed[at].ring == 0)
-                    ++at;
-                row[2 * j + 1] = static_cast<uint32_t>(at);
-                while (at < placed.size() && placed[at].block == b && placed[at].plane == static_cast<uint32_t>(j)) ++at;
+                    const bool deep = inner(x - 1, y, z) && inner(x + 1, y, z) && inner(x, y - 1, z) &&
+                                      inner(x, y + 1, z) && inner(x, y, z - 1) && inner(x, y, z + 1);
+                    const uint64_t g = (y - 1) / kBoxRows, r = (y - 1) % kBoxRows;
+                    box_bits[(z * groups + g) * px + x] |= (deep ? 3u : 1u) << (8 * r);
+                    if (!deep) cells.push_back(static_cast<uint32_t>(x + px * (y + ny * z)));
+                    const uint64_t bz = std::min<uint64_t>((z - 1) / static_cast<uint64_t>(Z), boxes_z - 1);
+                    active[g + groups * (x / kBoxX + boxes_x * bz)] = 1;
+                }
+        if (edge) {
+            h->pairs = 0;
+            box_bits.clear();
+            cells.clear();
+        } else {
+            // The boxes in order (row groups fastest, then x, then z), kBoxWaves a block; block
+            // r takes chunk (r mod 8) * per + r / 8, so that each die's blocks are neighbours
+            // (see the tiles).
+            std::vector<uint32_t> list;
+            for (uint64_t b = 0; b < active.size(); ++b)
+                if (active[b]) list.push_back(static_cast<uint32_t>(b));
+            const size_t chunks = (list.size() + kBoxWaves - 1) / kBoxWaves;
+            const size_t per = (chunks + kXcds - 1) / kXcds;
+            boxes.assign(per * kXcds * kBoxWaves, kNoTile);
+            for (size_t r = 0; r < per * kXcds; ++r) {
+                const size_t chunk = h->tile_order != 1 ? (r % kXcds) * per + r / kXcds : r;
+                for (uint32_t w = 0; w < kBoxWaves; ++w) {
+                    const size_t at = chunk * kBoxWaves + w;
+                    if (at < list.size()) boxes[r * kBoxWaves + w] = list[at];
+                }
             }
-            row[2 * (Z + 2)] = static_cast<uint32_t>(at);
+            h->box_blocks = static_cast<uint32_t>(per * kXcds);
+            h->cell_count = static_cast<uint32_t>(cells.size());
         }
-        if (at != placed.size()) throw Failure{AS_EINVAL, "internal: a face record was not placed"};
-        pair_records.reserve(4 * placed.size());
-        for (const auto &pl : placed) pair_records.insert(pair_records.end(), pl.rec.begin(), pl.rec.end());
-        h->box_blocks = static_cast<uint32_t>(boxes.size());
     }
 
     // Matched layer.
@@ -1788,9 +1704,8 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             h->velocity[k] = zeros(h, face_slots * e);
         }
         h->boxes = upload(h, boxes);
-        h->plane_faces = upload(h, plane_faces);
-        h->pair_record = reinterpret_cast<uint4 *>(upload(h, pair_records));
-        h->zero = static_cast<float *>(zeros(h, sizeof(float)));
+        h->box_bits = upload(h, box_bits);
+        h->cells = upload(h, cells);
     }
     h->layer_state = zeros(h, 9 * L.layer_cells * e);
 }
@@ -1868,10 +1783,12 @@ StepArgs<T> step_args(const as_fdtd *h) {
 // (p(k)): the reference's swaps (fdtd.hpp:883-885, 900-903) are the slots' exchange.
 template <typename T>
 void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned long long k, hipStream_t stream) {
-    s.a = static_cast<T *>(h->field[prev]);
-    s.b = static_cast<T *>(h->field[prev ^ 1]);
-    s.wa = static_cast<T *>(h->velocity[prev]);
-    s.wb = static_cast<T *>(h->velocity[prev ^ 1]);
+    s.a = static_cast<const T *>(h->field[prev]);
+    s.o = static_cast<T *>(h->field[prev]);  // in place
+    s.b = static_cast<const T *>(h->field[prev ^ 1]);
+    s.wa = static_cast<const T *>(h->velocity[prev]);
+    s.wo = static_cast<T *>(h->velocity[prev]);
+    s.wb = static_cast<const T *>(h->velocity[prev ^ 1]);
     s.step = k;
     if (h->split) {
         if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
@@ -1908,73 +1825,94 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned 
 template <typename T>
 uint32_t block_count(const as_fdtd *h, const StepArgs<T> &s) {
     (void)h;
-    return s.probe_blocks + s.special_blocks + s.face_blocks + s.tile_blocks + s.layer_blocks;
+    return s.probe_blocks + s.special_blocks + s.face_blocks + s.tile_blocks + s.layer_blocks + s.cell_blocks;
 }
 
-// The two-step launch's arguments, its probes writing where the one-step launches' do.
-PairArgs pair_args(const as_fdtd *h, const StepArgs<float> &s) {
-    PairArgs a{};
-    a.zero = h->zero;
-    a.types = h->types;
-    a.boxes = h->boxes;
-    a.box_blocks = h->box_blocks;
-    a.boxes_x = h->boxes_x;
-    a.boxes_y = h->boxes_y;
-    a.pair_record = h->pair_record;
-    a.face_state_kind = h->face_state_kind;
-    a.face_coef = h->face_coef;
-    a.face_branch = h->face_branch;
-    a.ny = h->ny;
-    a.nz = h->nz;
-    a.px = h->px;
-    a.sxy = static_cast<int>(h->sxy);
-    a.axial = static_cast<float>(h->axial);
-    a.centre = static_cast<float>(h->centre);
-    a.signal = s.signal;
-    a.probes = s.probes;
-    a.probe_start = s.probe_start;
-    a.probe_cell = s.probe_cell;
-    a.probe_weight = s.probe_weight;
-    a.out = s.out;
-    a.out_stride = s.out_stride;
-    a.march_z = h->march_z;
-    a.plane_faces = h->plane_faces;
-    a.plane_stride = h->plane_stride;
-    return a;
+// The boxes' arguments (pair_kernel), their fields set per launch.
+BoxArgs box_args(const as_fdtd *h) {
+    BoxArgs x{};
+    x.bits = h->box_bits;
+    x.boxes = h->boxes;
+    x.box_blocks = h->box_blocks;
+    x.boxes_x = h->boxes_x;
+    x.groups_y = h->groups_y;
+    x.ny = h->ny;
+    x.nz = h->nz;
+    x.px = h->px;
+    x.row_bytes = static_cast<uint32_t>(h->px) * 4u;
+    x.plane_bytes = static_cast<uint32_t>(h->sxy) * 4u;
+    x.field_bytes = static_cast<uint32_t>(h->n_pad * 4u);
+    x.bits_bytes = static_cast<uint32_t>(static_cast<uint64_t>(h->nz) * h->groups_y * h->px * 4u);
+    x.centre = static_cast<float>(h->centre);
+    x.axial = static_cast<float>(h->axial);
+    return x;
 }
 
-// Steps k0 to k1 - 1 from slot `prev`, by two-step launches where the handle has them
-// (their probes record each launch's p(k - 1) but the first's, and a launch of the probes
-// alone the last p), the rest one step a launch. Returns the slot of p(k1 - 1).
+// Steps k0 to k1 - 1 from slot `prev`: two steps by two launches where the handle has them
+// (see pair_kernel), the rest one step a launch. Returns the slot of p(k1 - 1).
 template <typename T>
 int enqueue_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t k0, size_t k1, hipStream_t stream) {
     size_t k = k0;
     if constexpr (std::is_same<T, float>::value) {
         if (h->pairs && k + 2 <= k1) {
-            PairArgs a = pair_args(h, s);
-            const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
+            BoxArgs x = box_args(h);
+            // Both launches step the face specials by their own blocks; the second steps the
+            // listed interior cells where one step a launch has its tiles.
+            StepArgs<float> first = s, second = s;
+            for (StepArgs<float> *t : {&first, &second}) {
+                t->tile_blocks = 0;
+                t->face_tile = h->face_tile_none;
+                t->face_count = h->face_specials;
+                t->face_blocks = (h->face_specials + kBlock - 1) / kBlock;
+                t->face_blocks = (t->face_blocks + kXcds - 1) / kXcds * kXcds;
+            }
+            second.cells = h->cells;
+            second.cell_count = h->cell_count;
+            second.cell_blocks = (h->cell_count + kBlock - 1) / kBlock;
+            if ((x.box_blocks > 0 && (x.boxes == nullptr || x.bits == nullptr)) ||
+                (second.cell_blocks > 0 && second.cells == nullptr) ||
+                (first.face_blocks > 0 && first.face_record == nullptr)) {
+                throw Failure{AS_EINVAL, "internal: a two-step launch lacks its arrays"};
+            }
+            const uint32_t first_blocks =
+                first.probe_blocks + first.special_blocks + first.face_blocks + x.box_blocks + first.layer_blocks;
+            const uint32_t second_blocks = block_count(h, second);
             for (; k + 2 <= k1; k += 2) {
-                const int next = prev < 2 ? 2 : 0;  // p(k + 1), then p(k + 2) in next ^ 1
-                a.prev = static_cast<const float *>(h->field[prev]);
-                a.cur = static_cast<const float *>(h->field[prev ^ 1]);
-                a.out1 = static_cast<float *>(h->field[next]);
-                a.out2 = static_cast<float *>(h->field[next ^ 1]);
-                a.wprev = static_cast<const float *>(h->velocity[prev]);
-                a.wcur = static_cast<const float *>(h->velocity[prev ^ 1]);
-                a.wout1 = static_cast<float *>(h->velocity[next]);
-                a.wout2 = static_cast<float *>(h->velocity[next ^ 1]);
-                a.step = k;
-                a.record_prev = k > k0 ? 1 : 0;
-                if (h->box_blocks + probe_blocks) {
-                    hipLaunchKernelGGL(march_kernel, dim3(h->box_blocks + probe_blocks), dim3(kBlock), 0, stream, a);
+                const int next = prev < 2 ? 2 : 0;  // p(k + 1) into slot next, p(k + 2) into next ^ 1
+                float *f[4] = {static_cast<float *>(h->field[prev]), static_cast<float *>(h->field[prev ^ 1]),
+                               static_cast<float *>(h->field[next]), static_cast<float *>(h->field[next ^ 1])};
+                float *v[4] = {static_cast<float *>(h->velocity[prev]), static_cast<float *>(h->velocity[prev ^ 1]),
+                               static_cast<float *>(h->velocity[next]), static_cast<float *>(h->velocity[next ^ 1])};
+                first.a = f[0];
+                first.b = f[1];
+                first.o = f[2];
+                first.wa = v[0];
+                first.wb = v[1];
+                first.wo = v[2];
+                first.step = k;
+                x.prev = f[0];
+                x.cur = f[1];
+                x.out1 = f[2];
+                x.out2 = f[3];
+                if (first_blocks) {
+                    if (h->pair_z == 16) {
+                        hipLaunchKernelGGL((pair_kernel<16>), dim3(first_blocks), dim3(kBlock), 0, stream, first, x);
+                    } else {
+                        hipLaunchKernelGGL((pair_kernel<8>), dim3(first_blocks), dim3(kBlock), 0, stream, first, x);
+                    }
+                }
+                second.a = f[1];
+                second.b = f[2];
+                second.o = f[3];
+                second.wa = v[1];
+                second.wb = v[2];
+                second.wo = v[3];
+                second.step = k + 1;
+                if (second_blocks) {
+                    hipLaunchKernelGGL((step_kernel<float, false, false>), dim3(second_blocks), dim3(kBlock), 0, stream,
+                                       second);
                 }
                 prev = next;
-            }
-            if (probe_blocks) {
-                // p(k - 1) is the last launch's p(k + 1), in slot prev.
-                s.b = static_cast<const T *>(h->field[prev]);
-                s.step = k - 1;
-                hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
             }
         }
     }
@@ -2406,8 +2344,10 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
         for (size_t n = k0 + top; n >= k0 + 1; --n, ++m) {
             const bool even = (m & 1ull) == 0;
             a.a = even ? q0 : q1;
+            a.o = even ? q0 : q1;  // in place
             a.b = even ? q1 : q0;
             a.wa = even ? v0 : v1;
+            a.wo = even ? v0 : v1;
             a.wb = even ? v1 : v0;
             a.delta = n < k0 + top ? history + (n - k0) * G : nullptr;
             a.inj = inj + (n - k0 - 1) * unique;
@@ -2563,6 +2503,8 @@ int as_fdtd_info(const as_fdtd *h, uint64_t *out) {
     out[5] = h->tile_count;
     out[6] = static_cast<uint64_t>(h->px);
     out[7] = h->done;
+    out[8] = h->pairs ? static_cast<uint64_t>(h->pair_z) : 0u;
+    out[9] = h->pairs ? h->cell_count : 0u;
     return AS_OK;
 }
 

@@ -25,7 +25,8 @@ class HipWaveSolver:
     library's defaults; graph_steps < 0 launches every step from the host); ``split`` launches
     each kind of cell on its own instead of one launch per step; ``face_blocks`` steps the
     face specials by blocks of their own (True) or in their tiles (False; None: the
-    library's choice); ``pairs`` takes two steps a launch where that applies."""
+    library's choice); ``pairs`` takes two steps by two launches where that applies (True,
+    or 8 or 16 for the planes a box marches)."""
 
     def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
                  graph_steps: int = 0, split: bool = False, tile_order: int = 0,
@@ -36,7 +37,7 @@ class HipWaveSolver:
         tuning = native.Tuning(0, int(tile_z), int(row_align), int(graph_steps), 1 if split else 0,
                                int(tile_order), 1 if no_faces else 0,
                                0 if face_blocks is None else (1 if face_blocks else -1),
-                               1 if pairs else 0)
+                               int(pairs) if not isinstance(pairs, bool) and int(pairs) > 1 else (1 if pairs else 0))
         h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
         if not h:
             message = lib.as_fdtd_create_error().decode()
@@ -129,10 +130,10 @@ class HipWaveSolver:
         return int(self._lib.as_fdtd_stream(self._h) or 0)
 
     def info(self) -> dict:
-        out = np.zeros(8, np.uint64)
+        out = np.zeros(10, np.uint64)
         self._check(self._lib.as_fdtd_info(self._h, out.ctypes.data))
         keys = ("interior_cells", "specials", "states", "layer_cells", "probes", "tiles",
-                "row_pitch", "steps_done")
+                "row_pitch", "steps_done", "pair_planes", "pair_cells")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self) -> None:

@@ -86,7 +86,7 @@ typedef struct as_fdtd as_fdtd;
 /* Launch shape of the step kernels (0 = the library's default for the field). */
 typedef struct as_tuning {
     int tile_y;        /* rows of a tile (the tile is 64 cells along x) */
-    int tile_z;        /* planes a tile marches through (with pairs, also a column's: default 8) */
+    int tile_z;        /* planes a tile marches through */
     int row_align;     /* row pitch in floats: a multiple of this (1, 16, 32, 64) */
     int graph_steps;   /* steps a captured hipGraph replays (0 = 64; negative: no graphs) */
     int split;         /* 1: one launch per kind of cell (specials, interior, layer, probes) */
@@ -97,9 +97,11 @@ typedef struct as_tuning {
                           blocks of their own beside the tiles; negative, each tile's in the
                           tile after its column march; 0, the library's choice (blocks of their
                           own unless the grid is small) */
-    int pairs;         /* 1: two steps a launch where that applies (precision 32, 7 points, one
-                          speed of sound, no matched layer, every special a face special, row
-                          pitch a multiple of 64); 0: one step a launch */
+    int pairs;         /* 1: two steps by two launches where that applies (precision 32, 7 points,
+                          one speed of sound): the first steps every interior cell twice where
+                          its neighbours are interior, by boxes of 60 x 4 cells marching 16 planes
+                          (8 on small grids; 8 or 16 here ask for that many); 0: one step a
+                          launch */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the
@@ -127,8 +129,10 @@ int as_fdtd_field(as_fdtd *h, double *out);
 /* The handle's HIP stream (a hipStream_t), for the caller's events. */
 void *as_fdtd_stream(as_fdtd *h);
 
-/* Counts: [0] cells stepped as interior, [1] specials, [2] states, [3] layer cells,
- * [4] probes, [5] active tiles, [6] row pitch, [7] steps done. */
+/* Counts (out holds 10): [0] cells stepped as interior, [1] specials, [2] states, [3] layer
+ * cells, [4] probes, [5] active tiles, [6] row pitch, [7] steps done, [8] planes a box of
+ * two steps marches (0: one step a launch), [9] interior cells the second launch of two
+ * steps takes one by one. */
 int as_fdtd_info(const as_fdtd *h, uint64_t *out);
 
 const char *as_fdtd_last_error(const as_fdtd *h);

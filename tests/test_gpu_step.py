@@ -80,9 +80,9 @@ def test_launch_shape_does_not_change_results(reference_available, tile_z, row_a
 @pytest.mark.parametrize("pairs,face_blocks,split", [(False, False, False), (False, False, True),
                                                      (False, True, False), (False, True, True), (True, True, True)])
 def test_kernel_variants_equal_the_oracle(reference_available, pairs, face_blocks, split):
-    """Two steps a launch or one; face specials in their tiles or by their own blocks; one
-    launch a step or four: the same fields, on a room with branch states and a scene with a
-    matched layer (which keeps to one step a launch)."""
+    """Two steps by two launches or one step a launch; face specials in their tiles or by
+    their own blocks; one launch a step or four: the same fields, on a room with branch
+    states and on a scene with a matched layer."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
@@ -90,6 +90,7 @@ def test_kernel_variants_equal_the_oracle(reference_available, pairs, face_block
         layout = _layout(scene)
         print(f"variant pairs={pairs} face_blocks={face_blocks} split={split} shape {scene.grid.shape}", flush=True)
         gpu = HipWaveSolver(layout, pairs=pairs, face_blocks=face_blocks, split=split)
+        assert (gpu.info()["pair_planes"] > 0) == pairs
         cpu = OracleSolver(layout)
         d = scene.drive[:230]
         assert np.array_equal(gpu.run(d[:117]), cpu.run(d[:117]))
@@ -97,19 +98,20 @@ def test_kernel_variants_equal_the_oracle(reference_available, pairs, face_block
         assert np.array_equal(gpu.field(), cpu.field())
 
 
-@pytest.mark.parametrize("pieces,planes", [((1, 2, 63, 64, 65, 1, 130), 0), ((200,), 0), ((3, 3, 3), 0),
+@pytest.mark.parametrize("pieces,tile_z", [((1, 2, 63, 64, 65, 1, 130), 0), ((200,), 0), ((3, 3, 3), 0),
                                            ((120,), 1), ((120,), 3), ((120,), 64)])
-def test_two_steps_a_launch_in_any_pieces(reference_available, pieces, planes):
+def test_two_steps_a_launch_in_any_pieces(reference_available, pieces, tile_z):
     """Runs of one step, two, odd and even counts, shorter and longer than a graph: the
-    probes of every step and the fields match the oracle whatever the split into calls.
-    The marching columns of 1, 3 or 64 planes (more than the grid has) give the same."""
+    probes of every step and the fields match the oracle whatever the split into calls
+    (the odd steps one step a launch, by tiles of 1, 3 or 64 planes)."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
     scene = cases.with_branches(cases.small_box())
     layout = _layout(scene)
-    print(f"pairs pieces {pieces} planes {planes}", flush=True)
-    gpu = HipWaveSolver(layout, pairs=True, tile_z=planes)
+    print(f"pairs pieces {pieces} tile_z {tile_z}", flush=True)
+    gpu = HipWaveSolver(layout, pairs=True, tile_z=tile_z)
+    assert gpu.info()["pair_planes"] == 8
     cpu = OracleSolver(layout)
     start = 0
     for n in pieces:
@@ -117,6 +119,28 @@ def test_two_steps_a_launch_in_any_pieces(reference_available, pieces, planes):
         assert np.array_equal(gpu.run(d), cpu.run(d)), (start, n)
         start += n
     assert np.array_equal(gpu.field(), cpu.field())
+
+
+@pytest.mark.parametrize("planes", [8, 16])
+@pytest.mark.parametrize("no_faces", [False, True])
+def test_two_launches_on_general_specials_and_the_layer(reference_available, planes, no_faces):
+    """Two steps by two launches with every special general (no_faces) or face specials by
+    their own blocks, on the matched-layer scene (boxes of 8 or 16 planes) and on a room
+    with branch states (8 planes): the probes and fields equal the oracle's."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    for scene in (cases.open_block(), cases.with_branches(cases.small_box())):
+        layout = _layout(scene)
+        print(f"two launches planes {planes} no_faces {no_faces} shape {scene.grid.shape}", flush=True)
+        gpu = HipWaveSolver(layout, pairs=planes, no_faces=no_faces)
+        expect = planes if scene.grid.shape[2] - 2 >= planes else 8
+        assert gpu.info()["pair_planes"] == expect
+        cpu = OracleSolver(layout)
+        d = scene.drive[:161]
+        assert np.array_equal(gpu.run(d[:80]), cpu.run(d[:80]))
+        assert np.array_equal(gpu.run(d[80:]), cpu.run(d[80:]))
+        assert np.array_equal(gpu.field(), cpu.field())
 
 
 def test_shoebox_full_grid(reference_available):
@@ -168,6 +192,7 @@ def test_hall_steps_equal_the_oracle(reference_available, pairs):
     layout = WaveSolver.from_scene(scene).layout()
     print(f"hall pairs={pairs}", flush=True)
     gpu = HipWaveSolver(layout, pairs=pairs)
+    assert gpu.info()["pair_planes"] == (16 if pairs else 0)
     cpu = OracleSolver(layout)
     # Start where the pulse has reached the source's cells' neighbours: a drive delayed so
     # the first steps already carry it.
