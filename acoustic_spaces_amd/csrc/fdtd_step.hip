@@ -1101,7 +1101,7 @@ struct as_fdtd {
     void *field[4] = {}, *velocity[4] = {};
     int prev = 0;
     int pairs = 0;      // 1: two steps a launch (see pair_kernel)
-    int pair_z = 16;    // planes a box marches
+    int pair_z = 8;     // planes a box marches
     uint32_t box_blocks = 0, boxes_x = 0, groups_y = 0, cell_count = 0;
     uint32_t *boxes = nullptr, *box_bits = nullptr, *cells = nullptr;
     uint8_t *types = nullptr;
@@ -1544,9 +1544,9 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (h->pairs) {
         const uint64_t boxes_x = (nx + kBoxX - 1) / kBoxX;
         const auto count_boxes = [&](uint64_t Z) { return boxes_x * groups * ((nz - 2 + Z - 1) / Z); };
-        // Boxes of 16 planes where there are enough of them to fill the chip, else of 8; or
-        // as asked (pairs 8 or 16), where the grid has the planes.
-        int Z = nz - 2 >= 16 && count_boxes(16) >= 2048 ? 16 : 8;
+        // Boxes of 8 planes (on the hall, 16 planes' boxes are slower: fewer wavefronts to
+        // hide a plane's loads), or 16 as asked (pairs 16) where the grid has the planes.
+        int Z = 8;
         if ((tuning->pairs == 8 || tuning->pairs == 16) && nz - 2 >= static_cast<uint64_t>(tuning->pairs)) Z = tuning->pairs;
         const uint64_t boxes_z = (nz - 2 + Z - 1) / Z;
         if (count_boxes(Z) >= kNoTile) refuse("too many boxes");

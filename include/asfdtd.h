@@ -99,9 +99,8 @@ typedef struct as_tuning {
                           own unless the grid is small) */
     int pairs;         /* 1: two steps by two launches where that applies (precision 32, 7 points,
                           one speed of sound): the first steps every interior cell twice where
-                          its neighbours are interior, by boxes of 60 x 4 cells marching 16 planes
-                          (8 on small grids; 8 or 16 here ask for that many); 0: one step a
-                          launch */
+                          its neighbours are interior, by boxes of 60 x 4 cells marching 8
+                          planes (16 here asks for 16); 0: one step a launch */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the

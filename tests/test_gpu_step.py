@@ -192,7 +192,7 @@ def test_hall_steps_equal_the_oracle(reference_available, pairs):
     layout = WaveSolver.from_scene(scene).layout()
     print(f"hall pairs={pairs}", flush=True)
     gpu = HipWaveSolver(layout, pairs=pairs)
-    assert gpu.info()["pair_planes"] == (16 if pairs else 0)
+    assert gpu.info()["pair_planes"] == (8 if pairs else 0)
     cpu = OracleSolver(layout)
     # Start where the pulse has reached the source's cells' neighbours: a drive delayed so
     # the first steps already carry it.
