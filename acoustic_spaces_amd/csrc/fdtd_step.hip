@@ -140,7 +140,8 @@ struct StepArgs {
                                                // where its states start, its first four states' kinds
     const uint16_t *__restrict__ face_state_kind;  // per state: its row of face_branch
     const float4 *__restrict__ face_coef;      // coef_a, coef_b, coef_inv, source
-    const float4 *__restrict__ face_branch;    // now, before, gain, weight
+    const float4 *__restrict__ face_branch;    // now, before, gain, weight (row 255: zeros, unless face_lookup)
+    int face_lookup;                           // 1: the table has more than 255 rows (see face_load)
     // matched layer
     uint32_t layer_cells, layer_blocks;
     const uint32_t *__restrict__ layer_cell;
@@ -184,18 +185,22 @@ struct StepArgs {
 // is special_cell's, in its order. A tile's face specials are stepped after its column
 // pass, by all its lanes at once, while the tile's pressures are still in the caches; or
 // (on grids of many tiles, where that is faster) all of them by blocks of their own.
-// A face special's loads from the fields and the states, by its record, all issued before
-// its arithmetic (face_finish). The small tables (coefficients, state kinds) are read in
-// face_finish.
+// A face special's loads by its record, all issued before its arithmetic (face_finish):
+// the fields, its first states' velocities, and its rows of the small tables (coefficients,
+// state kinds). The first four states' kinds ride in the record. Only where the kinds'
+// table has more than 255 rows (face_lookup) can a kind of 255 there mean "look it up in
+// face_state_kind", and only then does a lane wait for a look-up before reading the rows.
 constexpr int kFaceChunk = 4;  // states held in registers; a special's others are read as it goes
 template <typename T>
 struct FaceLoads {
     uint4 rec;  // cell, links | states << 6 | coefficients << 10, states' start, kinds
     T here, previous, nb[6];
     T wa[kFaceChunk], wb[kFaceChunk];
+    float4 coef;              // coef_a, coef_b, coef_inv, source
+    float4 kind[kFaceChunk];  // now, before, gain, weight
 };
 
-template <typename T>
+template <typename T, bool LOOKUP>
 __device__ __forceinline__ void face_load(const StepArgs<T> &s, const uint4 rec, FaceLoads<T> &f) {
     const int i = static_cast<int>(rec.x);
     const uint32_t sbase = rec.z, ns = (rec.y >> 6) & 15u;
@@ -217,6 +222,22 @@ __device__ __forceinline__ void face_load(const StepArgs<T> &s, const uint4 rec,
             f.wb[u] = s.wb[sbase + u];
         }
     }
+    f.coef = s.face_coef[rec.y >> 10];
+    if constexpr (!LOOKUP) {
+        // Every kind fits its byte; a state slot the special lacks reads row 255, a row of
+        // zeros, and leaves it unused.
+#pragma unroll
+        for (int u = 0; u < kFaceChunk; ++u) f.kind[u] = s.face_branch[(rec.w >> (8 * u)) & 0xFFu];
+    } else {
+#pragma unroll
+        for (int u = 0; u < kFaceChunk; ++u) {
+            if (static_cast<uint32_t>(u) < ns) {
+                uint32_t kid = (rec.w >> (8 * u)) & 0xFFu;
+                if (kid == 0xFFu) kid = s.face_state_kind[sbase + u];
+                f.kind[u] = s.face_branch[kid];
+            }
+        }
+    }
 }
 
 // The special's update (special_cell's arithmetic, in its order): its linked neighbours
@@ -228,25 +249,16 @@ __device__ __forceinline__ void face_finish(const StepArgs<T> &s, const FaceLoad
     const int i = static_cast<int>(rec.x);
     const uint32_t desc = rec.y;
     const uint32_t sbase = rec.z;
-    const uint32_t links = desc & 63u, ns = (desc >> 6) & 15u, ci_index = desc >> 10;
+    const uint32_t links = desc & 63u, ns = (desc >> 6) & 15u;
     const float sig = s.signal[s.step];
-    const float4 c = s.face_coef[ci_index];  // coef_a, coef_b, coef_inv, source
+    const float4 c = f.coef;  // coef_a, coef_b, coef_inv, source
     const T w = s.axial;
     T sum = 0;
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         if (links & (1u << d)) sum += w * f.nb[d];
     }
-    float4 kind[kFaceChunk];
-#pragma unroll
-    for (int u = 0; u < kFaceChunk; ++u) {
-        if (static_cast<uint32_t>(u) < ns) {
-            // The first four states' kinds ride in the record (255: look it up).
-            uint32_t kid = (rec.w >> (8 * u)) & 0xFFu;
-            if (kid == 0xFFu) kid = s.face_state_kind[sbase + u];
-            kind[u] = s.face_branch[kid];  // now, before, gain, weight
-        }
-    }
+    const float4 *kind = f.kind;  // now, before, gain, weight
     T own[kFaceChunk];
     T flow = 0;
 #pragma unroll
@@ -283,16 +295,28 @@ __device__ __forceinline__ void face_finish(const StepArgs<T> &s, const FaceLoad
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void face_special(const StepArgs<T> &s, const uint4 rec) {
+template <typename T, bool LOOKUP>
+__device__ __forceinline__ void face_special(const StepArgs<T> &s, uint32_t r) {
     FaceLoads<T> f;
-    face_load<T>(s, rec, f);
+    face_load<T, LOOKUP>(s, s.face_record[r], f);
     face_finish<T>(s, f);
 }
 
+// The face specials r0, r0 + stride, ... below r1, by the look-up variant where the handle
+// needs it (a choice uniform over the launch).
 template <typename T>
-__device__ __forceinline__ void face_special(const StepArgs<T> &s, uint32_t r) {
-    face_special<T>(s, s.face_record[r]);
+__device__ __forceinline__ void face_specials(const StepArgs<T> &s, uint32_t r0, uint32_t r1, uint32_t stride) {
+    if (s.face_lookup) {
+        for (uint32_t r = r0; r < r1; r += stride) face_special<T, true>(s, r);
+    } else {
+        for (uint32_t r = r0; r < r1; r += stride) face_special<T, false>(s, r);
+    }
+}
+
+// Face special r alone, if it is one.
+template <typename T>
+__device__ __forceinline__ void face_special_at(const StepArgs<T> &s, uint32_t r) {
+    if (r < s.face_count) face_specials<T>(s, r, r + 1, 1);
 }
 
 // The 7-point column of one lane: planes z0 to z1 - 1 of cell (x, y), in chunks of kUnroll
@@ -391,7 +415,7 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
             // (Loading a lane's first one under the march holds some twenty more registers
             // through it, for every tile: fewer wavefronts fit, and the step is slower.)
             const uint32_t r1 = s.face_tile[block + 1];
-            for (uint32_t r = s.face_tile[block] + threadIdx.x; r < r1; r += kBlock) face_special<T>(s, r);
+            face_specials<T>(s, s.face_tile[block] + threadIdx.x, r1, kBlock);
         }
         return;
     }
@@ -702,7 +726,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
     block -= s.special_blocks;
     if (block < s.face_blocks) {
         const uint32_t r = block * kBlock + threadIdx.x;
-        if (r < s.face_count) face_special<T>(s, r);
+        face_special_at<T>(s, r);
         return;
     }
     block -= s.face_blocks;
@@ -734,7 +758,7 @@ __global__ __launch_bounds__(kBlock) void special_kernel(const StepArgs<T> s) {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void face_kernel(const StepArgs<T> s) {
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r < s.face_count) face_special<T>(s, r);
+    face_special_at<T>(s, r);
 }
 
 template <typename T, bool WIDE, bool SCALED>
@@ -1056,7 +1080,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(const StepArgs<float> s, c
     block -= s.special_blocks;
     if (block < s.face_blocks) {
         const uint32_t r = block * kBlock + threadIdx.x;
-        if (r < s.face_count) face_special<float>(s, r);
+        face_special_at<float>(s, r);
         return;
     }
     block -= s.face_blocks;
@@ -1120,6 +1144,7 @@ struct as_fdtd {
     uint4 *face_record = nullptr;
     uint16_t *face_state_kind = nullptr;
     float4 *face_coef = nullptr, *face_branch = nullptr;
+    int face_lookup = 0;  // the face kinds' table has more than 255 rows
     int split = 0;            // 1: four launches a step instead of one
     int tile_order = 0;       // 0: tiles by die (see build), 1: in grid order, 2: by die, columns first
     int no_faces = 0;         // 1: every special is general (the adjoint's handles)
@@ -1672,6 +1697,10 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     }
     h->face_state_kind = upload(h, kinds);
     h->face_coef = reinterpret_cast<float4 *>(upload(h, face_coef));
+    // Kinds that all fit a byte read row 255 for the state slots a special lacks: pad the
+    // table to 256 rows of which the last is zeros. A larger table keeps the look-ups.
+    h->face_lookup = face_branch.size() / 4 >= 255 ? 1 : 0;
+    if (!h->face_lookup && !face_desc.empty()) face_branch.resize(256 * 4, 0.0f);
     h->face_branch = reinterpret_cast<float4 *>(upload(h, face_branch));
     h->layer_cell = upload(h, layer_cell);
     divisor_of(static_cast<uint32_t>(px), &h->px_magic, &h->px_shift);
@@ -1757,6 +1786,7 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.face_state_kind = h->face_state_kind;
     s.face_coef = h->face_coef;
     s.face_branch = h->face_branch;
+    s.face_lookup = h->face_lookup;
     s.layer_cells = h->layer_cells;
     s.layer_blocks = (h->layer_cells + kBlock - 1) / kBlock;
     s.layer_cell = h->layer_cell;

@@ -143,6 +143,29 @@ def test_two_launches_on_general_specials_and_the_layer(reference_available, pla
         assert np.array_equal(gpu.field(), cpu.field())
 
 
+@pytest.mark.parametrize("face_blocks,pairs", [(True, False), (False, False), (True, True)])
+def test_face_kinds_beyond_a_byte(reference_available, face_blocks, pairs):
+    """Every branch state its own weight, so that the face specials' table of state kinds
+    has more than 255 rows and their records look kinds up: by face blocks, in their tiles
+    and in two steps by two launches, the fields still equal the oracle's."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    scene = cases.with_branches(cases.small_box())
+    layout = dict(_layout(scene))
+    weight = np.asarray(layout["state_weight"], np.float32)
+    assert len(weight) > 300
+    rng = np.random.default_rng(11)
+    layout["state_weight"] = (weight * rng.uniform(0.95, 1.05, len(weight))).astype(np.float32)
+    print(f"face kinds beyond a byte: face_blocks={face_blocks} pairs={pairs}", flush=True)
+    gpu = HipWaveSolver(layout, face_blocks=face_blocks, pairs=pairs)
+    cpu = OracleSolver(layout)
+    d = scene.drive[:141]
+    assert np.array_equal(gpu.run(d[:70]), cpu.run(d[:70]))
+    assert np.array_equal(gpu.run(d[70:]), cpu.run(d[70:]))
+    assert np.array_equal(gpu.field(), cpu.field())
+
+
 def test_shoebox_full_grid(reference_available):
     """The 6 x 4 x 3 m shoebox at 5 cm (125 x 85 x 65 cells, 576,000 air cells), 400 steps."""
     from acoustic_spaces_amd import scenes
