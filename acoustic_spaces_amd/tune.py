@@ -33,14 +33,21 @@ def signature(layout: dict) -> str:
             f"ly{len(layout['layer_cells'])}|s{int(layout['stencil'][0])}|sc{int(layout['scaled'][0])}")
 
 
-def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int = 32, **shape) -> float:
-    """Microseconds per step of the layout under one launch shape."""
+def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int = 32, drive=None,
+               **shape) -> float:
+    """Microseconds per step of the layout under one launch shape, from rest, driven by
+    ``drive`` (its first warmup + steps samples) or by zeros. (On zero fields the same
+    steps run a few per cent faster than on live ones; compare shapes on one drive.)"""
     from acoustic_spaces_amd import hip
     from acoustic_spaces_amd.gpu import HipWaveSolver
 
+    signal = np.zeros(warmup + steps, np.float32)
+    if drive is not None:
+        d = np.asarray(drive, np.float32).reshape(-1)[: warmup + steps]
+        signal[: len(d)] = d
     stepper = HipWaveSolver(layout, precision=precision, **shape)
     try:
-        drive = hip.DeviceArray(np.zeros(warmup + steps, np.float32))
+        drive = hip.DeviceArray(signal)
         out = hip.DeviceArray(np.zeros((max(stepper.probes, 1), warmup + steps), np.float64))
         stepper.run_device(drive.ptr, warmup, out.ptr)
         hip.synchronize()
