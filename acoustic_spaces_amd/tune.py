@@ -18,7 +18,7 @@ from pathlib import Path
 import numpy as np
 
 CANDIDATES = tuple({"tile_z": tz, "graph_steps": gs, "face_blocks": fb}
-                   for tz in (4, 8) for gs in (32, 64) for fb in (False, True))
+                   for tz in (4, 8) for gs in (32, 64, 128) for fb in (False, True))
 
 
 def cache_path() -> Path:
@@ -62,11 +62,13 @@ def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int 
 
 
 def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: bool = True,
-             precision: int = 32) -> dict:
+             precision: int = 32, drive=None) -> dict:
     """The fastest launch shape for the layout: the candidate's keys (tile_z, graph_steps,
     face_blocks), "us_per_step" and "timings"; from the cache when it holds this layout's
-    signature."""
-    key = f"{signature(layout)}|p{precision}|" + ",".join(sorted({k for c in candidates for k in c}))
+    signature. With ``drive`` each candidate is timed on live fields (after 256 steps of it),
+    which can rank shapes otherwise than zero fields do."""
+    live = "|live" if drive is not None else ""
+    key = f"{signature(layout)}|p{precision}{live}|" + ",".join(sorted({k for c in candidates for k in c}))
     path = cache_path()
     cache = {}
     if use_cache and path.exists():
@@ -78,7 +80,8 @@ def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: b
             return cache[key]
     timings = []
     for shape in candidates:
-        us = time_shape(layout, steps=max(steps, shape["graph_steps"]), precision=precision, **shape)
+        us = time_shape(layout, steps=max(steps, shape["graph_steps"]), warmup=64 if drive is None else 256,
+                        precision=precision, drive=drive, **shape)
         timings.append({**shape, "us_per_step": round(us, 3)})
     best = min(timings, key=lambda t: t["us_per_step"])
     result = {**best, "timings": timings}

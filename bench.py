@@ -5,7 +5,8 @@ A step is one time step of the FDTD scheme over the whole grid of the hall workl
 10 cm cells (305 x 257 x 125 cells, 7,568,516 air cells), walls on cell faces with a
 constant admittance and three first-order (relaxation) admittance branches per material,
 one source, 16 receivers; 2 s of response is 11,882 steps. The grid is laid out by this
-package's set-up (csrc/setup.cpp) and stepped by the HIP kernels (csrc/fdtd_step.hip);
+package's set-up (csrc/setup.cpp) and stepped by the HIP kernels (csrc/fdtd_step.hip),
+in the launch shape the autotuner finds fastest on the scene's drive (part of the set-up);
 the drive and the probes' output stay in device memory over the timed region.
 
 Printed: one JSON line with Mcell-updates/s (air cells x steps / s), the step kernel's
@@ -244,6 +245,7 @@ def roofline_of(step_bytes, per_step_s, traffic, kernel):
 def plain(scene, name, args, traffic_name=None):
     """One plain solve of ``scene``: set-up, then K timed steps."""
     from acoustic_spaces_amd import scenes as sc
+    from acoustic_spaces_amd import tune
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from acoustic_spaces_amd.solver import WaveSolver
 
@@ -253,7 +255,16 @@ def plain(scene, name, args, traffic_name=None):
     solver = WaveSolver.from_scene(scene)
     layout = solver.layout()
     setup_s = time.perf_counter() - t
-    stepper = HipWaveSolver(layout, graph_steps=args.graph_steps, tile_z=args.tile_z)
+    if args.tile_z or args.graph_steps or args.face_blocks >= 0:
+        launch = {"tile_z": args.tile_z, "graph_steps": args.graph_steps,
+                  "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks)}
+        launch_from = "flags (0 or null: the library's default)"
+    else:
+        # Set-up, outside the timed region: the launch shape timed on the scene's drive.
+        shape = tune.autotune(layout, steps=512, drive=drive)
+        launch = {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks")}
+        launch_from = "the autotuner, each candidate timed on the scene's drive (live fields)"
+    stepper = HipWaveSolver(layout, **launch)
     info = stepper.info()
     ms, out = time_steps(stepper, drive, W, K)
     if not np.all(np.isfinite(out)):
@@ -264,7 +275,7 @@ def plain(scene, name, args, traffic_name=None):
     config = {
         "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
         "special_cells": info["specials"], "grid": list(scene.grid.shape), "setup_s": round(setup_s, 2),
-        "graph_steps": args.graph_steps or 64, "tile_z": args.tile_z or "8, or 4 below 2048 tiles of 8",
+        "launch": launch, "launch_from": launch_from,
         "precision": "float (bit-identical to the reference's CPU scheme)",
     }
     traffic = measured_traffic(traffic_name) if traffic_name else None
@@ -315,7 +326,7 @@ def rooms(args):
     setup_s = time.perf_counter() - t
     drive = scenes.gaussian_drive(batch[0].rate, W + K)
     merged, placements = merge_layouts(layouts)
-    shape = tune.autotune(merged, steps=128)
+    shape = tune.autotune(merged, steps=128, drive=drive)
     launch = {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks")}
     stepper = HipWaveSolver(merged, **launch)
     info = stepper.info()
@@ -519,6 +530,10 @@ def main(argv=None) -> int:
                          "float against double; the outdoor barrier's insertion loss)")
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
+    ap.add_argument("--face-blocks", type=int, default=-1, choices=(-1, 0, 1),
+                    help="face specials by blocks of their own (1) or in their tiles (0); with any "
+                         "of --tile-z, --graph-steps, --face-blocks the plain workloads skip the "
+                         "autotuner")
     ap.add_argument("--calibrate", action="store_true",
                     help="time the runtime's own device-to-device copy kernel instead (the bandwidth "
                          "reference and the rocprofv3 counter scale)")
