@@ -7,15 +7,20 @@ The reference's room-acoustic parameters of a band, restated for the wave solver
   (`parameters.py:306-333, 357-369, 442-452`);
 - the decay times' line fits (`parameters.py:266-296, 336-343`, with `products.dots`,
   `fixed_sums.hpp:44-47, 370-378`);
-- T30 (`parameters.py:464-475`) and the strength G (`parameters.py:385-440`).
+- T30 (`parameters.py:464-475`) and the strength G (`parameters.py:385-440`);
+- the synthesis's band split, zero-phase raised-cosine crossovers half a band wide whose
+  gains sum to one (`synthesis.py:31-76`, `materials/bands.py:18-29, 41-50, 78-84`);
+- band-limited resampling, polyphase Kaiser-windowed sinc (`render.py:46-54, 105-162`).
 The constants are the reference's.
 """
 
 from __future__ import annotations
 
 import math
+from fractions import Fraction
 
 import numpy as np
+from numpy.lib.stride_tricks import sliding_window_view
 
 OCTAVE_BANDS_HZ = (63, 125, 250, 500, 1000, 2000, 4000, 8000)
 OCTAVE_RATIO = 10.0 ** 0.3  # base-ten octave, IEC 61260-1
@@ -27,6 +32,13 @@ MIN_DECAY_DB = -20.0
 ENERGY_TO_IR = 4.0 * math.pi  # impulse-response energy per unit of simulated energy
 REFERENCE_ENERGY_10M = 0.01   # the direct sound's energy at 10 m: G's reference
 PAD_S = 0.2                   # zero padding round FFT filtering (s)
+THIRD_OCTAVE_BANDS_HZ = (50, 63, 80, 100, 125, 160, 200, 250, 315, 400, 500, 630, 800, 1000, 1250,
+                         1600, 2000, 2500, 3150, 4000, 5000, 6300, 8000, 10000)
+CROSSOVER_WIDTH = 0.5         # a crossover between octave bands, in octaves: half a band
+RESAMPLE_ZEROS = 32           # zero crossings of the resampling filter on each side
+RESAMPLE_BETA = 9.0           # its Kaiser window's beta
+RESAMPLE_MAX_PHASES = 4096    # most filter phases; finer ratios round the position to these
+RESAMPLE_CHUNK = 1 << 15      # outputs resampled at once when the rates have no short ratio
 
 
 def octave_midbands() -> np.ndarray:
@@ -224,3 +236,108 @@ def octave_results(signal: np.ndarray, rate: float, bands: int = 3, time_step: f
         out["g_db"].append(band_strength(row, time_step))
         out["t30_s"].append(band_t30(row[first:], time_step))
     return out
+
+
+def bands_per_octave(bands: int) -> int:
+    """1 for the 8 octave bands, 3 for the 24 third-octave bands (materials/bands.py:41-50)."""
+    if bands == len(OCTAVE_BANDS_HZ):
+        return 1
+    if bands == len(THIRD_OCTAVE_BANDS_HZ):
+        return 3
+    raise ValueError(f"expected {len(OCTAVE_BANDS_HZ)} octave or {len(THIRD_OCTAVE_BANDS_HZ)} "
+                     f"third-octave bands, got {bands}")
+
+
+def crossover_frequencies(bands: int = len(OCTAVE_BANDS_HZ)) -> np.ndarray:
+    """The edges between adjacent bands (Hz): the nominal octave centres times sqrt(2), or
+    between third octaves 1 kHz times 2^((k + 1/2) / 3) (materials/bands.py:78-84)."""
+    if bands_per_octave(bands) == 1:
+        return np.array(OCTAVE_BANDS_HZ[:-1], float) * math.sqrt(2.0)
+    k = np.arange(len(THIRD_OCTAVE_BANDS_HZ) - 1) - THIRD_OCTAVE_BANDS_HZ.index(1000)
+    return 1000.0 * 2.0 ** ((k + 0.5) / 3.0)
+
+
+def crossover_width(bands: int = len(OCTAVE_BANDS_HZ)) -> float:
+    """A crossover's width in octaves: half a band (synthesis.py:47-49)."""
+    return CROSSOVER_WIDTH / bands_per_octave(bands)
+
+
+def band_gains(n_fft: int, sample_rate: float, bands: int = len(OCTAVE_BANDS_HZ)) -> np.ndarray:
+    """The synthesis's zero-phase band gains (bands, n_fft // 2 + 1) on the rFFT grid: at
+    each edge a raised-cosine step sin^2(pi x / 2) over the crossover's width in log2 f,
+    each band the difference of the steps at its edges, so that they sum to one
+    (synthesis.py:50-71)."""
+    f = np.fft.rfftfreq(int(n_fft), 1.0 / float(sample_rate))
+    log_f = np.log2(np.maximum(f, 1e-6))
+    width = crossover_width(bands)
+    steps = np.array([np.sin(0.5 * np.pi * np.clip((log_f - (edge - width / 2)) / width, 0.0, 1.0)) ** 2
+                      for edge in np.log2(crossover_frequencies(bands))])
+    gains = np.empty((bands, len(f)))
+    gains[0] = 1.0 - steps[0]
+    gains[1:-1] = steps[:-1] - steps[1:]
+    gains[-1] = steps[-1]
+    return gains
+
+
+def kaiser_sinc(t: np.ndarray, cutoff: float) -> np.ndarray:
+    """Low-pass taps at offsets ``t`` (input samples), ``cutoff`` a share of the input's
+    Nyquist frequency: sinc times a Kaiser window reaching RESAMPLE_ZEROS zero crossings
+    (render.py:105-109)."""
+    half = RESAMPLE_ZEROS / cutoff
+    window = np.i0(RESAMPLE_BETA * np.sqrt(np.clip(1.0 - (t / half) ** 2, 0.0, 1.0)))
+    return cutoff * np.sinc(cutoff * t) * window / np.i0(RESAMPLE_BETA)
+
+
+def ordered_rows_dot(rows: np.ndarray, taps: np.ndarray) -> np.ndarray:
+    """rows (n, taps) times taps, each row's terms added in ascending tap order from the
+    first (the reference's products.product, products.py:39-56)."""
+    acc = rows[:, 0] * taps[0]
+    for k in range(1, len(taps)):
+        acc = acc + rows[:, k] * taps[k]
+    return acc
+
+
+def resample(signal, rate_in: float, rate_out: float) -> np.ndarray:
+    """``signal`` (samples,) or (samples, channels) at ``rate_out``, ceil(n rate_out /
+    rate_in) samples: polyphase windowed-sinc interpolation, the cutoff just below the
+    lower Nyquist frequency (render.py:112-162). Output k sits at input position
+    k rate_in / rate_out; where that ratio is p / q with q <= RESAMPLE_MAX_PHASES, each of
+    the q phases is one ordered product over the outputs that share it; otherwise the
+    position rounds to one of RESAMPLE_MAX_PHASES phases."""
+    x = np.asarray(signal, np.float64)
+    if rate_in <= 0 or rate_out <= 0:
+        raise ValueError("sample rates must be positive")
+    if math.isclose(rate_in, rate_out) or len(x) == 0:
+        return x.copy()
+    ratio = (Fraction(rate_in) / Fraction(rate_out)).limit_denominator(1 << 20)
+    exact = ratio.denominator <= RESAMPLE_MAX_PHASES
+    step, phases = (ratio.numerator, ratio.denominator) if exact else (0, RESAMPLE_MAX_PHASES)
+    cutoff = 0.97 * min(1.0, rate_out / rate_in)
+    reach = int(math.ceil(RESAMPLE_ZEROS / cutoff))
+    offsets = np.arange(-reach + 1, reach + 1)  # taps relative to the position's floor
+    table = kaiser_sinc(offsets[None, :] - (np.arange(phases) / phases)[:, None], cutoff)
+    columns = x.reshape(len(x), -1)
+    count = int(math.ceil(len(x) * rate_out / rate_in))
+    out = np.zeros((count, columns.shape[1]))
+    for c in range(columns.shape[1]):
+        padded = np.concatenate([np.zeros(reach), columns[:, c], np.zeros(reach + 1)])
+        window = sliding_window_view(padded, len(offsets))  # row r: the taps round input r - 1
+        if exact:
+            inverse = pow(step, -1, phases)
+            for p in range(phases):
+                k0 = p * inverse % phases  # the first output of phase p
+                if k0 >= count:
+                    continue
+                n = len(range(k0, count, phases))
+                first = (k0 * step - p) // phases + 1
+                out[k0::phases, c] = ordered_rows_dot(window[first: first + (n - 1) * step + 1: step], table[p])
+            continue
+        for start in range(0, count, RESAMPLE_CHUNK):
+            k = np.arange(start, min(start + RESAMPLE_CHUNK, count))
+            position = k * (rate_in / rate_out)
+            whole = np.floor(position).astype(np.int64)
+            phase = np.round((position - whole) * phases).astype(np.int64)
+            whole, phase = whole + phase // phases, phase % phases
+            out[k, c] = np.einsum("kt,kt->k", table[phase], window[whole + 1])
+    return out[:, 0] if x.ndim == 1 else out
+

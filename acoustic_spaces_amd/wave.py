@@ -7,7 +7,8 @@ stepping seam, for a scene of this package's (`scenes.Scene`):
   (1118-1128);
 - the grid is stepped on the GPU (`HipWaveSolver`);
 - each listener's pressure and gradient become W, X, Y, Z (`_first_order`, 757-802), in a
-  `WaveResponse` (330-454) with the reference's fields, `fed` and `energy`.
+  `WaveResponse` (330-454) with the reference's fields, `fed`, `field` (resampled to a
+  render rate), `high_pass` and `energy`.
 
 The reference's model types, band grids from materials, moving air and its CPU stepper are
 not here: a scene carries its own grid and admittances, and the stepping is the GPU's.
@@ -30,6 +31,19 @@ LOW_PASS_TAPS = 33
 LOW_PASS_CUTOFF = 0.2               # of the sample rate, on the 7-point grid
 PROBES = 13                         # points a listener is read at
 SOURCE_CELLS = {7: 0.0, 19: 1.5}    # a listener's points this near the source are left out
+LOW_EDGE_HZ = analysis.OCTAVE_BANDS_HZ[0] / (2.0 * math.sqrt(2.0))  # where the wave bands start
+
+
+def wave_gains(n_fft: int, sample_rate: float, bands: int, total: int = len(analysis.OCTAVE_BANDS_HZ)) -> np.ndarray:
+    """Zero-phase gains (bands, n_fft // 2 + 1) of the lowest ``bands`` of ``total`` bands as
+    the wave solver covers them: the synthesis's band gains, the lowest one rising from
+    LOW_EDGE_HZ over a crossover's width (engine/wave.py:195-206)."""
+    gains = analysis.band_gains(n_fft, sample_rate, total)[:bands]
+    f = np.fft.rfftfreq(n_fft, 1.0 / sample_rate)
+    width = analysis.CROSSOVER_WIDTH
+    x = (np.log2(np.maximum(f, 1e-6)) - math.log2(LOW_EDGE_HZ) + width / 2) / width
+    gains[0] *= np.sin(0.5 * np.pi * np.clip(x, 0.0, 1.0)) ** 2
+    return gains
 
 
 def cutoff(courant: float) -> float:
@@ -167,6 +181,42 @@ class WaveResponse:
 
     def _amplitudes(self) -> np.ndarray:
         return np.ones(self.bands) if self.band_gain is None else np.sqrt(self.band_gain)
+
+    def _resampled(self, sample_rate: float, n: int) -> tuple:
+        """The run at ``sample_rate`` (at least before + n samples, 4 channels), still
+        broadband, and ``before``: how many of its samples precede emission. Its start is
+        rounded to a sample once, and an impulse's height scales with the sample period
+        (engine/wave.py:366-387)."""
+        key = (float(sample_rate), int(n))
+        if key not in self._cache:
+            start = int(round((self.delay_s - self.lead_s) * sample_rate))
+            before = max(0, -start)
+            x = analysis.resample(self.signals, self.rate, sample_rate)
+            out = np.zeros((before + max(n, start + len(x)), 4))
+            out[before + start: before + start + len(x)] = x * (self.rate / sample_rate)
+            self._cache = {key: (out, before)}
+        return self._cache[key]
+
+    def field(self, sample_rate: float, n: int) -> np.ndarray:
+        """The first-order sound field (n, 4) of the wave bands together from emission, at
+        ``sample_rate``: the resampled run through the bands' summed zero-phase gains (each
+        band's times its feed's amplitude), whose spread before emission is left out
+        (engine/wave.py:389-399)."""
+        run, before = self._resampled(sample_rate, n)
+        n_fft = analysis.fft_size(len(run), sample_rate)
+        gains = wave_gains(n_fft, sample_rate, self.bands, self.total_bands)
+        gain = (gains * self._amplitudes()[:, None]).sum(axis=0)
+        spectrum = np.fft.rfft(run, n_fft, axis=0) * gain[:, None]
+        return np.fft.irfft(spectrum, n_fft, axis=0)[before: before + n]
+
+    def high_pass(self, signals, sample_rate: float, late: bool = False) -> np.ndarray:
+        """``signals`` (n, channels) without what the wave bands cover, which take them at a
+        gain g: at 1 - g (early paths, adding in pressure) or, with ``late``, at
+        sqrt(1 - g^2) (a tail, adding in energy) (engine/wave.py:401-409)."""
+        signals = np.asarray(signals, float)
+        n_fft = analysis.fft_size(len(signals), sample_rate)
+        g = wave_gains(n_fft, sample_rate, self.bands, self.total_bands).sum(axis=0)
+        return analysis.zero_phase(signals, np.sqrt(1.0 - g ** 2) if late else 1.0 - g, n_fft)
 
     def energy(self, time_step: float, slots: int, axis=None, exponent: int = 2) -> np.ndarray:
         """Energy per band and time slot (bands, slots), as the ray tracer counts it: each

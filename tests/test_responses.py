@@ -2,7 +2,8 @@
 listener's 13 points; the source signal's filters (the low-pass's unit gain, the double
 high-pass's zero, the 19-point taps' gain (5 + 4 cos wT) / 9); the first-order gradient's
 exactness on linear fields and its choice of differences; the trapezoidal integral of a
-band-limited pulse; WaveResponse.fed and energy. GPU: wave_responses' W, X, Y, Z equal the
+band-limited pulse; WaveResponse.fed, energy, field and high_pass (the wave bands' gains and
+the resampled run). GPU: wave_responses' W, X, Y, Z equal the
 oracle's run of the same layout and drive, to the bit."""
 
 import math
@@ -122,6 +123,54 @@ def test_response_fed_and_energy():
     assert np.allclose(a.sum(axis=1), b.sum(axis=1), rtol=1e-9)
     centroid = lambda x: (x * (np.arange(x.shape[1]) + 0.5)).sum(axis=1) / x.sum(axis=1) * 1e-3  # noqa: E731
     assert np.allclose(centroid(a) - centroid(b), 0.25, atol=2e-3)
+
+
+def test_wave_gains_rise_from_the_low_edge():
+    """The wave bands' gains are the synthesis's band gains, the lowest one rising from 0
+    below LOW_EDGE_HZ / 2^(1/4) to its band gain above LOW_EDGE_HZ 2^(1/4) (a raised cosine
+    half an octave wide, 1/2 at the edge). All eight together are that rise alone."""
+    rate, n_fft = 6000.0, 1 << 16
+    f = np.fft.rfftfreq(n_fft, 1.0 / rate)
+    g = wv.wave_gains(n_fft, rate, 8)
+    band = analysis.band_gains(n_fft, rate, 8)
+    assert np.array_equal(g[1:], band[1:])
+    rise = g.sum(axis=0)
+    assert np.max(rise[f < wv.LOW_EDGE_HZ * 2 ** -0.25]) == 0.0
+    assert np.allclose(rise[f > wv.LOW_EDGE_HZ * 2 ** 0.25], 1.0, atol=1e-12)
+    assert abs(np.interp(wv.LOW_EDGE_HZ, f, rise) - 0.5) < 1e-3
+    assert np.allclose(wv.wave_gains(n_fft, rate, 3), g[:3])
+
+
+def test_response_field_and_high_pass_split_the_run():
+    """field() and high_pass() (1 - g) of the same resampled run add up to it (the wave
+    bands' gains g and 1 - g); late=True filters at sqrt(1 - g^2); the run's start sits
+    (delay_s - lead_s) from emission, an impulse's height scaled by rate / sample_rate; a
+    band's feed scales its share."""
+    rng = np.random.default_rng(8)
+    rate, sr, n = 2000.0, 8000.0, 6000
+    sig = rng.standard_normal((1200, 4))
+    r = wv.WaveResponse(rate, sig, 3, 0.1, lead_s=0.01, delay_s=0.05)
+    run, before = r._resampled(sr, n)
+    start = int(round((0.05 - 0.01) * sr))
+    assert before == 0 and run.shape == (max(n, start + 4800), 4)
+    assert np.allclose(run[start:start + 4800], analysis.resample(sig, rate, sr) * (rate / sr))
+    field = r.field(sr, n)
+    assert field.shape == (n, 4)
+    assert np.allclose(field + r.high_pass(run, sr)[:n], run[:n], atol=1e-12)
+    n_fft = analysis.fft_size(len(run), sr)
+    g = wv.wave_gains(n_fft, sr, 3).sum(axis=0)
+    late = r.high_pass(run, sr, late=True)
+    assert np.allclose(late, analysis.zero_phase(run, np.sqrt(1.0 - g ** 2), n_fft))
+    # A lead longer than the delay: samples before emission, filtered with the rest and
+    # left out of the field, which is the same run's 720 samples later.
+    early = wv.WaveResponse(rate, sig, 3, 0.1, lead_s=0.05)
+    run2, before2 = early._resampled(sr, n)
+    assert before2 == int(round(0.05 * sr))
+    shift = start + before2
+    assert np.allclose(early.field(sr, n), r.field(sr, n + shift)[shift:shift + n], atol=1e-9)
+    # Feeding each band: the field of one band alone scales with its amplitude.
+    one = wv.WaveResponse(rate, sig, 1, 0.1)
+    assert np.allclose(one.fed([4.0], 0.0).field(sr, n), 2.0 * one.field(sr, n))
 
 
 @pytest.mark.gpu
