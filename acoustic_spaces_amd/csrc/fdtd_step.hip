@@ -111,6 +111,9 @@ struct StepArgs {
     const uint8_t *__restrict__ types;
     const float *__restrict__ scale;  // padded, or null
     const uint32_t *__restrict__ tiles;
+    // Per tile block and row (wavefront): the lanes whose column holds an interior cell in
+    // the tile's planes; the others skip the march (null: every lane marches).
+    const uint64_t *__restrict__ tile_lanes;
     uint32_t tile_blocks;
     uint32_t tiles_x, tiles_y;
     int nx, ny, nz, px;
@@ -399,7 +402,11 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
     const uint32_t tz = rest / s.tiles_y;
     const int x = static_cast<int>(tx) * kTileX + static_cast<int>(threadIdx.x % kTileX);
     const int y = 1 + static_cast<int>(ty) * kTileRows + static_cast<int>(threadIdx.x / kTileX);
-    const bool inside = x < s.nx && y < s.ny - 1;
+    bool inside = x < s.nx && y < s.ny - 1;
+    if (!all && s.tile_lanes != nullptr) {
+        const uint32_t row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
+        inside = inside && ((s.tile_lanes[block * kTileRows + row] >> (threadIdx.x % kTileX)) & 1u);
+    }
     const int z0 = 1 + static_cast<int>(tz) * s.tile_z;
     const int z1 = min(z0 + s.tile_z, s.nz - 1);
     // 32-bit indices: the padded grid holds fewer than 2^31 cells (checked at set-up).
@@ -1135,6 +1142,7 @@ struct as_fdtd {
     uint8_t *types = nullptr;
     float *scale = nullptr;
     uint32_t *tiles = nullptr;
+    uint64_t *tile_lanes = nullptr;  // per tile block and row: its lanes that march (see interior_tile)
     uint32_t *sp_cell = nullptr, *link_base = nullptr, *state_base = nullptr;
     uint8_t *sp_links = nullptr, *sp_states = nullptr;
     uint32_t *link_dirs = nullptr;
@@ -1480,6 +1488,25 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         tiles.swap(ordered);
     }
     h->tile_count = static_cast<uint32_t>(tiles.size());
+    // Each block's rows' lanes whose column holds an interior cell in the tile's planes.
+    std::vector<uint64_t> tile_lanes(static_cast<size_t>(h->tile_count) * kTileRows, 0);
+    in_slices(h->tile_count, 64, [&](unsigned, uint64_t a, uint64_t b) {
+        for (uint64_t r = a; r < b; ++r) {
+            if (tiles[r] == kNoTile) continue;
+            uint64_t lo[3], hi[3];
+            tile_box(tiles[r] & ~kAllInterior, lo, hi);
+            for (uint64_t y = lo[1]; y < hi[1]; ++y) {
+                uint64_t lanes = 0;
+                for (uint64_t x = lo[0]; x < hi[0]; ++x)
+                    for (uint64_t z = lo[2]; z < hi[2]; ++z)
+                        if (types[x + px * (y + ny * z)] == kClassInterior) {
+                            lanes |= uint64_t{1} << (x - lo[0]);
+                            break;
+                        }
+                tile_lanes[r * kTileRows + (y - lo[1])] = lanes;
+            }
+        }
+    });
 
     // General specials: sorted by their counts of links and states (a special's update reads
     // only its own cell, links and states, so their order is free), in blocks of kBlock;
@@ -1760,6 +1787,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->types = upload(h, types);
     h->scale = upload(h, scale);
     h->tiles = upload(h, tiles);
+    h->tile_lanes = upload(h, tile_lanes);
     h->sp_cell = upload(h, sp_cell);
     h->sp_links = upload(h, sp_links);
     h->sp_states = upload(h, sp_states);
@@ -1845,6 +1873,7 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.types = h->types;
     s.scale = h->scale;
     s.tiles = h->tiles;
+    s.tile_lanes = h->tile_lanes;
     s.tile_blocks = h->tile_count;
     s.tiles_x = h->tiles_x;
     s.tiles_y = h->tiles_y;

@@ -32,8 +32,11 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <limits>
@@ -55,6 +58,19 @@ inline V3 operator+(const V3 &a, const V3 &b) { return {a.x + b.x, a.y + b.y, a.
 inline V3 operator-(const V3 &a, const V3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline V3 operator*(const V3 &a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 inline double dot(const V3 &a, const V3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// The set-up's phases' times on stderr, where AS_FDTD_TIMING is set.
+struct SetupClock {
+    bool on = std::getenv("AS_FDTD_TIMING") != nullptr;
+    std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    void mark(const char *what) {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "as_setup_create: %-22s %9.1f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    }
+};
+
 inline V3 cross(const V3 &a, const V3 &b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
@@ -804,10 +820,13 @@ public:
                 scale[i] = static_cast<float>(r * r);
             }
         }
+        SetupClock clock;
         make_bins();
+        clock.mark("stencil, bins");
         require_off_faces(src, "the source");
         for (size_t p = 0; p < probes.size(); ++p) require_off_faces(probes[p], "a probe");
         cut_links();
+        clock.mark("cut links");
 
         // Air: the cells the source sees, and all they reach.
         type.assign(n, kSolid);
@@ -821,6 +840,7 @@ public:
         }
         if (queue.empty()) throw std::invalid_argument("the source sees no cell of the wave grid");
         flood(queue);
+        clock.mark("air");
 
         // The matched layer's air, slab by slab of planes on the threads (each cell's class
         // its own; the cells listed in order).
@@ -866,6 +886,7 @@ public:
             }
         }
 
+        clock.mark("layer");
         // Boundary cells: marks 1 and 2, then weights, materials and states. Each pass runs
         // on the threads by slabs of planes; the second writes its marks apart (a cell can
         // take mark 2 from several neighbours), and the specials of each slab are joined in
@@ -985,6 +1006,7 @@ public:
             for (int k = 0; k < 6; ++k) open_sides[static_cast<size_t>(k)] = open_sides[static_cast<size_t>(k)] || part.sides[static_cast<size_t>(k)];
         }
         boundary_cells = special.size();
+        clock.mark("marks, specials");
 
         // The source's cells.
         for (const auto &[i, w] : visible_cells(src, true)) {
@@ -1013,6 +1035,7 @@ public:
             const size_t l = k - k % per_listener;
             probe_in_sight.push_back(k == l || sees(probes[l], probes[k]) ? 1 : 0);
         }
+        clock.mark("source, probes");
     }
 };
 

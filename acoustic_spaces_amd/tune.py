@@ -17,8 +17,11 @@ from pathlib import Path
 
 import numpy as np
 
+# In order of preference: the first is the library's default on a large grid. A candidate
+# within TIE of the fastest counts as fast as it, so that the choice holds from run to run.
 CANDIDATES = tuple({"tile_z": tz, "graph_steps": gs, "face_blocks": fb}
-                   for tz in (4, 8) for gs in (32, 64, 128) for fb in (False, True))
+                   for tz in (8, 4) for gs in (64, 128, 32) for fb in (True, False))
+TIE = 0.01
 
 
 def cache_path() -> Path:
@@ -63,8 +66,9 @@ def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int 
 
 def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: bool = True,
              precision: int = 32, drive=None) -> dict:
-    """The fastest launch shape for the layout: the candidate's keys (tile_z, graph_steps,
-    face_blocks), "us_per_step" and "timings"; from the cache when it holds this layout's
+    """The fastest launch shape for the layout (the first candidate within TIE of the
+    fastest): the candidate's keys (tile_z, graph_steps, face_blocks), "us_per_step" and
+    "timings"; from the cache when it holds this layout's
     signature. With ``drive`` each candidate is timed on live fields (after 256 steps of it),
     which can rank shapes otherwise than zero fields do."""
     live = "|live" if drive is not None else ""
@@ -83,7 +87,8 @@ def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: b
         us = time_shape(layout, steps=max(steps, shape["graph_steps"]), warmup=64 if drive is None else 256,
                         precision=precision, drive=drive, **shape)
         timings.append({**shape, "us_per_step": round(us, 3)})
-    best = min(timings, key=lambda t: t["us_per_step"])
+    fastest = min(t["us_per_step"] for t in timings)
+    best = next(t for t in timings if t["us_per_step"] <= fastest * (1.0 + TIE))
     result = {**best, "timings": timings}
     if use_cache:
         try:

@@ -89,5 +89,9 @@ def test_autotuner_picks_a_measured_shape(tmp_path, monkeypatch):
     shapes = ({"tile_z": 4, "graph_steps": 32}, {"tile_z": 8, "graph_steps": 64})
     best = tune.autotune(layout, steps=64, candidates=shapes)
     assert (best["tile_z"], best["graph_steps"]) in {(s["tile_z"], s["graph_steps"]) for s in shapes}
-    assert best["us_per_step"] == min(t["us_per_step"] for t in best["timings"])
+    # The first shape, in the candidates' order, within TIE of the fastest.
+    fastest = min(t["us_per_step"] for t in best["timings"])
+    first = next(t for t in best["timings"] if t["us_per_step"] <= fastest * (1.0 + tune.TIE))
+    assert (best["tile_z"], best["graph_steps"], best["us_per_step"]) == (
+        first["tile_z"], first["graph_steps"], first["us_per_step"])
     assert tune.autotune(layout, steps=64, candidates=shapes) == best  # from the cache
