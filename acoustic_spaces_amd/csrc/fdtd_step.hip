@@ -153,7 +153,13 @@ struct StepArgs {
     uint32_t layer_cells, layer_blocks;
     const uint32_t *__restrict__ layer_cell;
     uint32_t px_magic, px_shift, ny_magic, ny_shift;  // division by px and by ny (see divide)
-    T *__restrict__ layer_state;            // 9 planes of layer_cells
+    T *__restrict__ layer_state;            // 3 planes of layer_cells: the pressure's parts by axis
+    // The face velocities: 3 planes (by axis) of the padded grid, a face at the index of the
+    // cell below it; read from one slot, written to the other (the slots swap each step).
+    const T *__restrict__ layer_face_in;
+    T *__restrict__ layer_face_out;
+    const uint8_t *__restrict__ layer_edge;  // bit axis: the cell below along axis is no layer cell
+    unsigned long long face_plane;           // the planes' pitch (the padded grid's cells)
     const float *__restrict__ layer_scale;  // or null
     const float *__restrict__ keep[6];
     const float *__restrict__ push[6];
@@ -621,40 +627,69 @@ __device__ __forceinline__ uint32_t divide(uint32_t n, uint32_t magic, uint32_t 
 
 // A matched-layer cell (fdtd.hpp:2242-2266): its six face velocities, then the three
 // parts of its pressure, from the current field into the next.
+//
+// The reference keeps six velocities a cell. Where two layer cells meet, the upper cell's
+// lower face and the lower cell's upper face start at zero and take the same update from
+// the same operands (keep and push at the face's index, times the same two pressures, in
+// the same order), so they hold the same value at every step. One copy a face is kept:
+// each cell writes its upper face, and its lower one only where the cell below is no layer
+// cell (layer_edge). The faces are read from one slot and written to the other, so that a
+// cell reads the lower face its neighbour rewrites in the same step before it changes.
 template <typename T>
 __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
     const uint32_t cell = s.layer_cell[c];
+    const uint32_t edge = s.layer_edge[c];
     const long long i = cell;
     // The cell's coordinates from its padded index (no table of them to read).
     const uint32_t yz = divide(cell, s.px_magic, s.px_shift);
     const uint32_t z = divide(yz, s.ny_magic, s.ny_shift);
     const uint32_t coord[3] = {cell - yz * static_cast<uint32_t>(s.px), yz - z * static_cast<uint32_t>(s.ny), z};
     const long long stride[3] = {1, s.px, s.sxy};
-    const uint32_t L = s.layer_cells;
-    T *__restrict__ st = s.layer_state;
+    const size_t L = s.layer_cells;
     const T *__restrict__ b = s.b;
+    const bool scaled = s.layer_scale != nullptr;
+    // Every load first, then the arithmetic, then the stores: the compiler cannot tell that
+    // the stores miss the later loads, and would otherwise wait for each axis's loads
+    // before issuing the next axis's.
+    T lo[3], hi[3], below[3], above[3], part[3];
+    float keep_lo[3], keep_hi[3], push_lo[3], push_hi[3], keep_c[3], push_c[3];
     const T bi = b[i];
-    T total = 0;
+    const float scale = scaled ? s.layer_scale[c] : 0.0f;
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
         const uint32_t x = coord[axis];
-        const float *fk = s.keep[2 * axis + 1];
-        const float *fp = s.push[2 * axis + 1];
-        T below = st[(3 + 2 * axis) * static_cast<size_t>(L) + c];
-        T above = st[(4 + 2 * axis) * static_cast<size_t>(L) + c];
-        below = static_cast<T>(fk[x]) * below - static_cast<T>(fp[x]) * (bi - b[i - stride[axis]]);
-        above = static_cast<T>(fk[x + 1]) * above - static_cast<T>(fp[x + 1]) * (b[i + stride[axis]] - bi);
-        st[(3 + 2 * axis) * static_cast<size_t>(L) + c] = below;
-        st[(4 + 2 * axis) * static_cast<size_t>(L) + c] = above;
-        T part = st[axis * static_cast<size_t>(L) + c];
-        if (s.layer_scale != nullptr) {
-            const T ps = static_cast<T>(s.push[2 * axis][x]) * static_cast<T>(s.layer_scale[c]);
-            part = static_cast<T>(s.keep[2 * axis][x]) * part - ps * (above - below);
+        const T *__restrict__ fin = s.layer_face_in + axis * s.face_plane;
+        lo[axis] = b[i - stride[axis]];
+        hi[axis] = b[i + stride[axis]];
+        below[axis] = fin[i - stride[axis]];
+        above[axis] = fin[i];
+        part[axis] = s.layer_state[axis * L + c];
+        keep_lo[axis] = s.keep[2 * axis + 1][x];
+        keep_hi[axis] = s.keep[2 * axis + 1][x + 1];
+        push_lo[axis] = s.push[2 * axis + 1][x];
+        push_hi[axis] = s.push[2 * axis + 1][x + 1];
+        keep_c[axis] = s.keep[2 * axis][x];
+        push_c[axis] = s.push[2 * axis][x];
+    }
+    T total = 0;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        below[axis] = static_cast<T>(keep_lo[axis]) * below[axis] - static_cast<T>(push_lo[axis]) * (bi - lo[axis]);
+        above[axis] = static_cast<T>(keep_hi[axis]) * above[axis] - static_cast<T>(push_hi[axis]) * (hi[axis] - bi);
+        if (scaled) {
+            const T ps = static_cast<T>(push_c[axis]) * static_cast<T>(scale);
+            part[axis] = static_cast<T>(keep_c[axis]) * part[axis] - ps * (above[axis] - below[axis]);
         } else {
-            part = static_cast<T>(s.keep[2 * axis][x]) * part - static_cast<T>(s.push[2 * axis][x]) * (above - below);
+            part[axis] = static_cast<T>(keep_c[axis]) * part[axis] - static_cast<T>(push_c[axis]) * (above[axis] - below[axis]);
         }
-        st[axis * static_cast<size_t>(L) + c] = part;
-        total += part;
+        total += part[axis];
+    }
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        T *__restrict__ fout = s.layer_face_out + axis * s.face_plane;
+        if ((edge >> axis) & 1u) fout[i - stride[axis]] = below[axis];
+        fout[i] = above[axis];
+        s.layer_state[axis * L + c] = part[axis];
     }
     s.o[i] = total;
 }
@@ -1176,6 +1211,8 @@ struct as_fdtd {
     std::vector<double> coef_inv_host;        // per layout special
     std::vector<uint64_t> special_cell_host;  // padded cell per layout special
     uint32_t *layer_cell = nullptr;
+    uint8_t *layer_edge = nullptr;             // see StepArgs::layer_edge
+    void *layer_face[2] = {nullptr, nullptr};  // the face velocities' slots; slot prev & 1 is current
     uint32_t px_magic = 0, px_shift = 0, ny_magic = 0, ny_shift = 0;
     float *layer_scale = nullptr;
     float *keep[6] = {}, *push[6] = {};
@@ -1752,7 +1789,6 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     clock.mark("boxes");
     // Matched layer.
     std::vector<uint32_t> layer_cell(L.layer_cells);
-
     in_slices(L.layer_cells, 1 << 16, [&](unsigned, uint64_t a, uint64_t b) {
         for (uint64_t c = a; c < b; ++c) {
             const int64_t i = L.layer_cell[c];
@@ -1766,6 +1802,24 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         }
     });
     h->layer_cells = static_cast<uint32_t>(L.layer_cells);
+    // Which neighbours below a layer cell are layer cells too (their shared face is theirs).
+    std::vector<uint8_t> layer_edge(L.layer_cells);
+    if (L.layer_cells > 0) {
+        std::vector<uint8_t> member(n_pad, 0);
+        in_slices(L.layer_cells, 1 << 16, [&](unsigned, uint64_t a, uint64_t b) {
+            for (uint64_t c = a; c < b; ++c) member[layer_cell[c]] = 1;
+        });
+        const uint64_t below[3] = {1, px, px * ny};
+        in_slices(L.layer_cells, 1 << 16, [&](unsigned, uint64_t a, uint64_t b) {
+            for (uint64_t c = a; c < b; ++c) {
+                uint8_t bits = 0;
+                for (int axis = 0; axis < 3; ++axis) {
+                    if (!member[layer_cell[c] - below[axis]]) bits |= static_cast<uint8_t>(1u << axis);
+                }
+                layer_edge[c] = bits;
+            }
+        });
+    }
 
     // Probes.
     std::vector<uint32_t> probe_start(L.probes + 1), probe_cell(L.taps);
@@ -1829,6 +1883,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (!h->face_lookup && !face_desc.empty()) face_branch.resize(256 * 4, 0.0f);
     h->face_branch = reinterpret_cast<float4 *>(upload(h, face_branch));
     h->layer_cell = upload(h, layer_cell);
+    h->layer_edge = upload(h, layer_edge);
     divisor_of(static_cast<uint32_t>(px), &h->px_magic, &h->px_shift);
     divisor_of(static_cast<uint32_t>(ny), &h->ny_magic, &h->ny_shift);
     if (L.layer_scale != nullptr && L.layer_cells > 0) {
@@ -1862,7 +1917,11 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         h->box_bits = upload(h, box_bits);
         h->cells = upload(h, cells);
     }
-    h->layer_state = zeros(h, 9 * L.layer_cells * e);
+    h->layer_state = zeros(h, 3 * L.layer_cells * e);
+    if (L.layer_cells > 0) {
+        h->layer_face[0] = zeros(h, 3 * n_pad * e);
+        h->layer_face[1] = zeros(h, 3 * n_pad * e);
+    }
     if (clock.on) check(hipDeviceSynchronize(), "set-up copies");  // so that the copies count
     clock.mark("device copies");
 }
@@ -1924,6 +1983,8 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.ny_magic = h->ny_magic;
     s.ny_shift = h->ny_shift;
     s.layer_state = static_cast<T *>(h->layer_state);
+    s.layer_edge = h->layer_edge;
+    s.face_plane = h->n_pad;
     s.layer_scale = h->layer_scale;
     for (int k = 0; k < 6; ++k) {
         s.keep[k] = h->keep[k];
@@ -1948,6 +2009,8 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned 
     s.wa = static_cast<const T *>(h->velocity[prev]);
     s.wo = static_cast<T *>(h->velocity[prev]);
     s.wb = static_cast<const T *>(h->velocity[prev ^ 1]);
+    s.layer_face_in = static_cast<const T *>(h->layer_face[prev & 1]);
+    s.layer_face_out = static_cast<T *>(h->layer_face[(prev & 1) ^ 1]);
     s.step = k;
     if (h->split) {
         if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
@@ -2037,7 +2100,9 @@ int enqueue_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t k
                 first.probe_blocks + first.special_blocks + first.face_blocks + x.box_blocks + first.layer_blocks;
             const uint32_t second_blocks = block_count(h, second);
             for (; k + 2 <= k1; k += 2) {
-                const int next = prev < 2 ? 2 : 0;  // p(k + 1) into slot next, p(k + 2) into next ^ 1
+                // p(k + 1) into slot next, p(k + 2) into next ^ 1. A pair keeps prev & 1, the
+                // slot of the face velocities, which it writes and then overwrites.
+                const int next = (prev < 2 ? 2 : 0) | (prev & 1);
                 float *f[4] = {static_cast<float *>(h->field[prev]), static_cast<float *>(h->field[prev ^ 1]),
                                static_cast<float *>(h->field[next]), static_cast<float *>(h->field[next ^ 1])};
                 float *v[4] = {static_cast<float *>(h->velocity[prev]), static_cast<float *>(h->velocity[prev ^ 1]),
@@ -2048,6 +2113,8 @@ int enqueue_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t k
                 first.wa = v[0];
                 first.wb = v[1];
                 first.wo = v[2];
+                first.layer_face_in = static_cast<const float *>(h->layer_face[prev & 1]);
+                first.layer_face_out = static_cast<float *>(h->layer_face[(prev & 1) ^ 1]);
                 first.step = k;
                 x.prev = f[0];
                 x.cur = f[1];
@@ -2066,6 +2133,8 @@ int enqueue_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t k
                 second.wa = v[1];
                 second.wb = v[2];
                 second.wo = v[3];
+                second.layer_face_in = static_cast<const float *>(h->layer_face[(prev & 1) ^ 1]);
+                second.layer_face_out = static_cast<float *>(h->layer_face[prev & 1]);
                 second.step = k + 1;
                 if (second_blocks) {
                     hipLaunchKernelGGL((step_kernel<float, false, false>), dim3(second_blocks), dim3(kBlock), 0, stream,
@@ -2247,8 +2316,11 @@ int as_fdtd_reset(as_fdtd *h) {
             check(hipMemsetAsync(h->velocity[k], 0, std::max<size_t>(h->state_slots * e, 256), h->stream),
                   "hipMemsetAsync");
         }
-        check(hipMemsetAsync(h->layer_state, 0, std::max<size_t>(9ull * h->layer_cells * e, 256), h->stream),
+        check(hipMemsetAsync(h->layer_state, 0, std::max<size_t>(3ull * h->layer_cells * e, 256), h->stream),
               "hipMemsetAsync");
+        for (void *f : h->layer_face) {
+            if (f != nullptr) check(hipMemsetAsync(f, 0, 3 * h->n_pad * e, h->stream), "hipMemsetAsync");
+        }
         check(hipStreamSynchronize(h->stream), "reset");
         h->done = 0;
         h->prev = 0;

@@ -32,13 +32,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 PER_AIR_CELL = 12  # bytes: read the current and previous pressure, write the next (float)
-PER_LAYER_CELL = 80  # bytes: the current pressure, nine state values read and written, the next
+PER_LAYER_CELL = 56  # bytes: the current pressure, three parts and three faces read and written, the next
 PER_STATE = 12  # bytes: a branch velocity read twice-over and written (float)
 
 
 def algorithmic_bytes(info: dict, air_cells: int, layer_cells: int, states: int) -> int:
-    """Bytes one step must move at least: 12 per air cell outside the matched layer, 80 per
-    layer cell, 12 per branch state (coefficient and index reads not counted)."""
+    """Bytes one step must move at least: 12 per air cell outside the matched layer, 56 per
+    layer cell (each face velocity shared by the two cells it lies between is one value),
+    12 per branch state (coefficient and index reads not counted)."""
     return PER_AIR_CELL * (air_cells - layer_cells) + PER_LAYER_CELL * layer_cells + PER_STATE * states
 
 

@@ -23,6 +23,7 @@ CASES = {
     "box7": lambda: (cases.small_box(), {}),
     "box19": lambda: (cases.small_box(stencil=19), {}),
     "open_layer": lambda: (cases.open_block(), {}),
+    "open_layer_speed": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.open_block()),
     "speed": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.small_box()),
     "speed19": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.small_box(stencil=19)),
     # Fitted walls (the reference's cut cells, fdtd.hpp:970-1264): general specials with
@@ -54,15 +55,17 @@ def test_steps_equal_the_oracle(reference_available, name, precision):
 
 
 def test_reset_starts_over(reference_available):
+    """After an odd number of steps (the matched layer's faces in their other slot), reset
+    and the same run again give the same probes."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
 
-    scene = cases.small_box()
-    layout = _layout(scene)
-    gpu = HipWaveSolver(layout)
-    first = gpu.run(scene.drive[:150])
-    gpu.reset()
-    again = gpu.run(scene.drive[:150])
-    assert np.array_equal(first, again)
+    for scene in (cases.small_box(), cases.open_block()):
+        layout = _layout(scene)
+        gpu = HipWaveSolver(layout)
+        first = gpu.run(scene.drive[:151])
+        gpu.reset()
+        again = gpu.run(scene.drive[:151])
+        assert np.array_equal(first, again)
 
 
 @pytest.mark.parametrize("tile_z,row_align,tile_order", [(4, 1, 0), (16, 16, 0), (64, 64, 0), (4, 64, 1), (8, 64, 2)])
@@ -103,22 +106,23 @@ def test_kernel_variants_equal_the_oracle(reference_available, pairs, face_block
 def test_two_steps_a_launch_in_any_pieces(reference_available, pieces, tile_z):
     """Runs of one step, two, odd and even counts, shorter and longer than a graph: the
     probes of every step and the fields match the oracle whatever the split into calls
-    (the odd steps one step a launch, by tiles of 1, 3 or 64 planes)."""
+    (the odd steps one step a launch, by tiles of 1, 3 or 64 planes), on a room with branch
+    states and on the matched-layer scene (whose face velocities change slot every step)."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
-    scene = cases.with_branches(cases.small_box())
-    layout = _layout(scene)
-    print(f"pairs pieces {pieces} tile_z {tile_z}", flush=True)
-    gpu = HipWaveSolver(layout, pairs=True, tile_z=tile_z)
-    assert gpu.info()["pair_planes"] == 8
-    cpu = OracleSolver(layout)
-    start = 0
-    for n in pieces:
-        d = scene.drive[start:start + n]
-        assert np.array_equal(gpu.run(d), cpu.run(d)), (start, n)
-        start += n
-    assert np.array_equal(gpu.field(), cpu.field())
+    for scene in (cases.with_branches(cases.small_box()), cases.open_block()):
+        layout = _layout(scene)
+        print(f"pairs pieces {pieces} tile_z {tile_z} shape {scene.grid.shape}", flush=True)
+        gpu = HipWaveSolver(layout, pairs=True, tile_z=tile_z)
+        assert gpu.info()["pair_planes"] == 8
+        cpu = OracleSolver(layout)
+        start = 0
+        for n in pieces:
+            d = scene.drive[start:start + n]
+            assert np.array_equal(gpu.run(d), cpu.run(d)), (start, n)
+            start += n
+        assert np.array_equal(gpu.field(), cpu.field())
 
 
 @pytest.mark.parametrize("planes", [8, 16])
