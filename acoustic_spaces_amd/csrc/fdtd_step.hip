@@ -153,13 +153,12 @@ struct StepArgs {
     uint32_t layer_cells, layer_blocks;
     const uint32_t *__restrict__ layer_cell;
     uint32_t px_magic, px_shift, ny_magic, ny_shift;  // division by px and by ny (see divide)
-    T *__restrict__ layer_state;            // 3 planes of layer_cells: the pressure's parts by axis
-    // The face velocities: 3 planes (by axis) of the padded grid, a face at the index of the
-    // cell below it; read from one slot, written to the other (the slots swap each step).
+    T *__restrict__ layer_state;            // the pressure's parts: x, y, z of each layer cell
+    // The face velocities: x, y, z of each padded cell, a face at the index of the cell
+    // below it; read from one slot, written to the other (the slots swap each step).
     const T *__restrict__ layer_face_in;
     T *__restrict__ layer_face_out;
     const uint8_t *__restrict__ layer_edge;  // bit axis: the cell below along axis is no layer cell
-    unsigned long long face_plane;           // the planes' pitch (the padded grid's cells)
     const float *__restrict__ layer_scale;  // or null
     const float *__restrict__ keep[6];
     const float *__restrict__ push[6];
@@ -645,7 +644,6 @@ __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
     const uint32_t z = divide(yz, s.ny_magic, s.ny_shift);
     const uint32_t coord[3] = {cell - yz * static_cast<uint32_t>(s.px), yz - z * static_cast<uint32_t>(s.ny), z};
     const long long stride[3] = {1, s.px, s.sxy};
-    const size_t L = s.layer_cells;
     const T *__restrict__ b = s.b;
     const bool scaled = s.layer_scale != nullptr;
     // Every load first, then the arithmetic, then the stores: the compiler cannot tell that
@@ -655,15 +653,15 @@ __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
     float keep_lo[3], keep_hi[3], push_lo[3], push_hi[3], keep_c[3], push_c[3];
     const T bi = b[i];
     const float scale = scaled ? s.layer_scale[c] : 0.0f;
+    // A cell's three parts, and the three faces at its index, lie together: one access each.
+    __builtin_memcpy(above, s.layer_face_in + 3 * i, sizeof(above));
+    __builtin_memcpy(part, s.layer_state + 3 * static_cast<size_t>(c), sizeof(part));
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
         const uint32_t x = coord[axis];
-        const T *__restrict__ fin = s.layer_face_in + axis * s.face_plane;
         lo[axis] = b[i - stride[axis]];
         hi[axis] = b[i + stride[axis]];
-        below[axis] = fin[i - stride[axis]];
-        above[axis] = fin[i];
-        part[axis] = s.layer_state[axis * L + c];
+        below[axis] = s.layer_face_in[3 * (i - stride[axis]) + axis];
         keep_lo[axis] = s.keep[2 * axis + 1][x];
         keep_hi[axis] = s.keep[2 * axis + 1][x + 1];
         push_lo[axis] = s.push[2 * axis + 1][x];
@@ -684,12 +682,11 @@ __device__ __forceinline__ void layer_cell(const StepArgs<T> &s, uint32_t c) {
         }
         total += part[axis];
     }
+    __builtin_memcpy(s.layer_face_out + 3 * i, above, sizeof(above));
+    __builtin_memcpy(s.layer_state + 3 * static_cast<size_t>(c), part, sizeof(part));
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis) {
-        T *__restrict__ fout = s.layer_face_out + axis * s.face_plane;
-        if ((edge >> axis) & 1u) fout[i - stride[axis]] = below[axis];
-        fout[i] = above[axis];
-        s.layer_state[axis * L + c] = part[axis];
+        if ((edge >> axis) & 1u) s.layer_face_out[3 * (i - stride[axis]) + axis] = below[axis];
     }
     s.o[i] = total;
 }
@@ -1984,7 +1981,6 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.ny_shift = h->ny_shift;
     s.layer_state = static_cast<T *>(h->layer_state);
     s.layer_edge = h->layer_edge;
-    s.face_plane = h->n_pad;
     s.layer_scale = h->layer_scale;
     for (int k = 0; k < 6; ++k) {
         s.keep[k] = h->keep[k];
