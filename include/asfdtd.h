@@ -105,7 +105,11 @@ typedef struct as_tuning {
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the
  * same scheme in double, coefficients widened). NULL on failure (see as_fdtd_create_error).
- * Replaces GpuWaveSolver.__init__ (engine/gpu.py:445-545). */
+ * Replaces GpuWaveSolver.__init__ (engine/gpu.py:445-545).
+ * Device memory, e = 4 or 8 bytes and n the padded grid's cells (rows padded to 64 cells):
+ * two fields of n e (four with pairs), a class byte per cell, two (four) slots of branch
+ * velocities, and with a matched layer 3 e per layer cell plus two slots of 3 n e for its
+ * face velocities (one per face, shared by the two cells beside it). */
 as_fdtd *as_fdtd_create(const as_layout *layout, int precision, const as_tuning *tuning);
 const char *as_fdtd_create_error(void);
 
