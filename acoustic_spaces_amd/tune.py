@@ -3,14 +3,16 @@ stepped, chosen by timing them.
 
 For a layout it times each candidate launch shape (tile_z planes per tile, graph_steps
 steps per captured graph, face specials by blocks of their own or in their tiles) over a
-short run on the GPU, with hipEvents on the stepper's stream, and keeps the fastest. Results are cached by the layout's signature (its grid,
-cell counts and stencil) in a JSON file, as the reference keeps its own device timings
+short run on the GPU, with hipEvents on the stepper's stream, and keeps the fastest. Results
+are cached by the layout's signature (its grid, cell counts and stencil) and the library's
+build (a hash of libasfdtd.so) in a JSON file, as the reference keeps its own device timings
 (engine/tuning.py:76-180); the reference tunes only the CPU/GPU choice, this tunes the
 kernels' shape.
 """
 
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 from pathlib import Path
@@ -34,6 +36,25 @@ def signature(layout: dict) -> str:
     air = int(np.count_nonzero(np.asarray(layout["types"])))
     return (f"{shape}|air{air}|sp{len(layout['special'])}|st{len(layout['state_branch'])}|"
             f"ly{len(layout['layer_cells'])}|s{int(layout['stencil'][0])}|sc{int(layout['scaled'][0])}")
+
+
+def library_tag() -> str:
+    """The first 16 hex digits of the SHA-256 of the built library: a rebuilt library, whose
+    kernels may rank the shapes otherwise, gets timings of its own."""
+    from acoustic_spaces_amd import native
+
+    digest = hashlib.sha256()
+    with open(native.LIBRARY, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            digest.update(chunk)
+    return digest.hexdigest()[:16]
+
+
+def cache_key(layout: dict, precision: int = 32, live: bool = False, candidates=CANDIDATES) -> str:
+    """The cache's key: the layout's signature, the precision, live or zero fields, the
+    library's build and the candidates' keys."""
+    keys = ",".join(sorted({k for c in candidates for k in c}))
+    return f"{signature(layout)}|p{precision}{'|live' if live else ''}|lib{library_tag()}|{keys}"
 
 
 def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int = 32, drive=None,
@@ -71,8 +92,7 @@ def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: b
     "timings"; from the cache when it holds this layout's
     signature. With ``drive`` each candidate is timed on live fields (after 256 steps of it),
     which can rank shapes otherwise than zero fields do."""
-    live = "|live" if drive is not None else ""
-    key = f"{signature(layout)}|p{precision}{live}|" + ",".join(sorted({k for c in candidates for k in c}))
+    key = cache_key(layout, precision, drive is not None, candidates)
     path = cache_path()
     cache = {}
     if use_cache and path.exists():

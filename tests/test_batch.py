@@ -95,3 +95,20 @@ def test_autotuner_picks_a_measured_shape(tmp_path, monkeypatch):
     assert (best["tile_z"], best["graph_steps"], best["us_per_step"]) == (
         first["tile_z"], first["graph_steps"], first["us_per_step"])
     assert tune.autotune(layout, steps=64, candidates=shapes) == best  # from the cache
+
+
+def test_tuning_cache_key_follows_the_library(tmp_path, monkeypatch):
+    """The autotuner's cache key changes with the library's bytes (a rebuilt library times
+    its shapes again), with the precision and with live fields, and not otherwise."""
+    from acoustic_spaces_amd import native, tune
+
+    layout = _layouts(_rooms(count=1))[0]
+    lib = tmp_path / "libasfdtd.so"
+    lib.write_bytes(b"one build")
+    monkeypatch.setattr(native, "LIBRARY", lib)
+    key = tune.cache_key(layout)
+    assert tune.cache_key(layout) == key
+    assert tune.cache_key(layout, precision=64) != key
+    assert tune.cache_key(layout, live=True) != key
+    lib.write_bytes(b"another build")
+    assert tune.cache_key(layout) != key
