@@ -146,6 +146,10 @@ class BatchedWaveSolver:
     one launch a step for each grid. ``run(signal)`` returns each room's probes (probes,
     steps), in the order given."""
 
+    #: The grouping measured fastest on the 64 rooms at 5 cm (groups of 128 to 384 MB and one
+    #: merged grid timed): grids of at most 192 MB, which the 256 MB Infinity Cache holds.
+    DEFAULT_GROUP_BYTES = 192 << 20
+
     def __init__(self, rooms, precision: int = 32, group_bytes=None, **tuning):
         layouts = [r if isinstance(r, dict) else r.layout() for r in rooms]
         self.groups = []

@@ -1240,6 +1240,26 @@ void check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw Failure{AS_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
 }
 
+// Captures what `enqueue` puts on `stream` into an executable graph at *exec. The capture is
+// ended on every path: where `enqueue` throws, the partial graph is dropped and the stream
+// leaves capture mode before the failure goes on, so the handle's stream stays usable.
+template <typename F>
+void capture_graph(hipStream_t stream, hipGraphExec_t *exec, F &&enqueue) {
+    hipGraph_t graph = nullptr;
+    check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        enqueue();
+    } catch (...) {
+        if (hipStreamEndCapture(stream, &graph) == hipSuccess && graph != nullptr) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+    const hipError_t e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) *exec = nullptr;
+    check(e, "hipGraphInstantiate");
+}
+
 template <typename U>
 U *upload(as_fdtd *h, const std::vector<U> &v) {
     if (v.empty()) return nullptr;
@@ -1322,6 +1342,20 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (L.nx < 3 || L.ny < 3 || L.nz < 3) refuse("the grid is too small");
     if (L.n_weights < (L.stencil == 7 ? 6u : 18u)) refuse("the layout's weights are too few for its stencil");
     if (L.types == nullptr) refuse("the layout has no cell types");
+    // The probes' ranges: from 0, never decreasing, ending at the taps (a range running
+    // backwards would make a probe's wavefront loop near 2^32 times; one past the taps would
+    // read past the cells).
+    if (L.probes > 0) {
+        if (L.probe_start == nullptr) refuse("the layout's probes have no ranges (probe_start)");
+        if (L.probe_start[0] != 0) refuse("the probes' ranges do not start at 0");
+        for (uint64_t p = 0; p < L.probes; ++p)
+            if (L.probe_start[p + 1] < L.probe_start[p]) refuse("the probes' ranges run backwards");
+        if (L.probe_start[L.probes] != L.taps) refuse("the probes' ranges do not end at the taps");
+    } else if (L.taps != 0) {
+        refuse("the layout has taps but no probes");
+    }
+    if (L.taps > 0 && (L.probe_cells == nullptr || L.probe_weights == nullptr)) refuse("the probes' taps are missing");
+    if (L.taps >= (1ull << 32)) refuse("more than 2^32 taps");
     h->precision = precision;
     h->wide = L.stencil == 19;
     h->scaled = L.scale != nullptr;
@@ -2166,12 +2200,8 @@ void ensure_graph(as_fdtd *h, int prev) {
     s.out = h->out_stage;
     s.out_stride = G;
     const uint32_t blocks = block_count(h, s);
-    hipGraph_t graph = nullptr;
-    check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    h->graph_end[prev] = enqueue_steps<T>(h, s, blocks, prev, 0, G, h->stream);
-    check(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
-    check(hipGraphInstantiate(&h->graph_exec[prev], graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    check(hipGraphDestroy(graph), "hipGraphDestroy");
+    capture_graph(h->stream, &h->graph_exec[prev],
+                  [&] { h->graph_end[prev] = enqueue_steps<T>(h, s, blocks, prev, 0, G, h->stream); });
 }
 
 template <typename T>
@@ -2606,13 +2636,7 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     } release_graphs{graphs};
     auto capture = [&](int which, auto &&enqueue) {
         if (graphs[which] != nullptr) return;
-        hipGraph_t graph = nullptr;
-        check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        enqueue();
-        check(hipStreamEndCapture(h->stream, &graph), "hipStreamEndCapture");
-        const hipError_t e = hipGraphInstantiate(&graphs[which], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        check(e, "hipGraphInstantiate");
+        capture_graph(h->stream, &graphs[which], enqueue);
     };
     unsigned long long m = 0;
     for (size_t seg = segments; seg-- > 0;) {
@@ -2739,13 +2763,17 @@ const char *as_fdtd_last_error(const as_fdtd *h) { return h ? h->error.c_str() :
 
 void as_fdtd_destroy(as_fdtd *h) {
     if (h == nullptr) return;
+    // Work in flight ends first; then the graphs go, before any buffer their kernel nodes
+    // point at is freed; the stream last.
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &g : h->graph_exec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
     free_record(h);
     for (void *p : h->allocations) (void)hipFree(p);
     if (h->signal_buf) (void)hipFree(h->signal_buf);
     if (h->out_buf) (void)hipFree(h->out_buf);
-    for (auto &g : h->graph_exec)
-        if (g) (void)hipGraphExecDestroy(g);
     if (h->sig_stage) (void)hipFree(h->sig_stage);
     if (h->out_stage) (void)hipFree(h->out_stage);
     if (h->stream) (void)hipStreamDestroy(h->stream);

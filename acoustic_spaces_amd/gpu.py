@@ -137,10 +137,25 @@ class HipWaveSolver:
         return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self) -> None:
+        """Frees the handle's device memory and graphs now (the garbage collector's __del__
+        is a fallback, not the way to release a handle)."""
         h = getattr(self, "_h", None)
         if h:
             self._lib.as_fdtd_destroy(h)
             self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    @classmethod
+    def run_once(cls, solver, signal, **kwargs) -> np.ndarray:
+        """One run from rest on a handle of its own, closed before returning."""
+        with cls(solver, **kwargs) as s:
+            return s.run(signal)
 
     def __del__(self):
         try:

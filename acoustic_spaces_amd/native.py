@@ -166,6 +166,18 @@ def pack_layout(layout: dict):
     s.taps = len(A["probe_cells"])
     if len(A["link_start"]) != s.specials + 1 or len(A["state_start"]) != s.specials + 1:
         raise ValueError("the layout's link and state ranges do not match its special cells")
+    # Every array against the count it is indexed by, so that the library never reads past
+    # the end of a host array.
+    counts = {"coef_a": s.specials, "coef_b": s.specials, "coef_inv": s.specials, "source": s.specials,
+              "link_weight": s.links, "state_weight": s.states, "branch_before": s.branches,
+              "branch_gain": s.branches, "probe_weights": s.taps}
+    for name, count in counts.items():
+        if len(A[name]) != count:
+            raise ValueError(f"the layout's {name} has {len(A[name])} values, not {count}")
+    if s.specials and (int(A["link_start"][-1]) != s.links or int(A["state_start"][-1]) != s.states):
+        raise ValueError("the layout's link and state ranges do not end at its links and states")
+    if len(A["weights"]) != (6 if s.stencil == 7 else 18):
+        raise ValueError(f"the layout's weights have {len(A['weights'])} values for a {s.stencil}-point stencil")
     for name in ("special", "link_start", "link_direction", "link_weight", "coef_a", "coef_b",
                  "coef_inv", "source", "state_start", "state_branch", "state_weight",
                  "branch_now", "branch_before", "branch_gain", "probe_start", "probe_cells",

@@ -82,8 +82,8 @@ def insertion_loss(scene, layout) -> dict:
 
     out = {}
     for precision in (32, 64):
-        with_barrier = levels(HipWaveSolver(layout, precision=precision).run(scene.drive))
-        without = levels(HipWaveSolver(bare, precision=precision).run(scene.drive))
+        with_barrier = levels(HipWaveSolver.run_once(layout, scene.drive, precision=precision))
+        without = levels(HipWaveSolver.run_once(bare, scene.drive, precision=precision))
         out[precision] = without - with_barrier
     return {"bands_hz": [63, 125], "il_db_f32": np.round(out[32], 2).tolist(),
             "max_abs_il_db_f32_vs_f64": float(np.abs(out[32] - out[64]).max()),
@@ -170,32 +170,56 @@ def calibrate() -> int:
     return 0
 
 
-def time_steps(stepper, drive: np.ndarray, warmup: int, steps: int):
-    """Warm-up, then exactly ``steps`` steps between synchronisations, timed with hipEvents
-    on the stepper's own stream; the drive and the probes' output in device memory.
-    Returns (milliseconds, the probes' output of the warm-up and timed steps, (probes,
-    warmup + steps))."""
+def time_steps(stepper, drive: np.ndarray, warmup: int, steps: int, lead: int = 0):
+    """A lead-in of ``lead`` steps from rest (so that the pulse has spread and the timed
+    steps meet live fields whatever ``warmup`` is), the warm-up, then exactly ``steps``
+    steps between synchronisations, timed with hipEvents on the stepper's own stream; the
+    drive and the probes' output in device memory. Returns (milliseconds, the probes'
+    output of all three, (probes, lead + warmup + steps))."""
     from acoustic_spaces_amd import hip
 
-    need = warmup + steps
+    need = lead + warmup + steps
     probes = max(stepper.probes, 1)
     d_drive = hip.DeviceArray(np.ascontiguousarray(drive[:need], np.float32))
     d_out = hip.DeviceArray(np.zeros((probes, need), np.float64))
     stream = stepper.stream
+    if lead:
+        stepper.run_device(d_drive.ptr, lead, d_out.ptr)
     if warmup:
-        stepper.run_device(d_drive.ptr, warmup, d_out.ptr)
+        stepper.run_device(d_drive.ptr + 4 * lead, warmup, d_out.ptr + 8 * probes * lead)
     hip.synchronize()
     start, stop = hip.Event(), hip.Event()
     start.record(stream)
-    stepper.run_device(d_drive.ptr + 4 * warmup, steps, d_out.ptr + 8 * probes * warmup)
+    stepper.run_device(d_drive.ptr + 4 * (lead + warmup), steps, d_out.ptr + 8 * probes * (lead + warmup))
     stop.record(stream)
     hip.synchronize()
     ms = start.elapsed_ms(stop)
-    # Each call wrote its own (probes, its steps) block: the warm-up's, then the timed one's.
+    # Each call wrote its own (probes, its steps) block, in the order of the calls.
     flat = d_out.to_host().reshape(-1)
-    warm = flat[: probes * warmup].reshape(probes, warmup)
-    timed = flat[probes * warmup: probes * need].reshape(probes, steps)
-    return ms, np.concatenate([warm, timed], axis=1)
+    blocks, at = [], 0
+    for n in (lead, warmup, steps):
+        blocks.append(flat[at: at + probes * n].reshape(probes, n))
+        at += probes * n
+    d_drive.free()
+    d_out.free()
+    return ms, np.concatenate(blocks, axis=1)
+
+
+def graph_length(tuned: int, steps: int) -> int:
+    """The steps a graph replays for a timed run of ``steps``: the tuned length where the
+    run holds at least one graph, else the largest divisor of ``steps`` not above it, so
+    that a short timed run still replays graphs, as a long one does."""
+    if tuned <= 0 or steps >= tuned:
+        return tuned
+    return max(d for d in range(1, steps + 1) if steps % d == 0 and d <= tuned)
+
+
+def launch_mode(graph_steps: int, steps: int) -> str:
+    if graph_steps <= 0:
+        return f"{steps} launches from the host, one a step"
+    whole, rest = divmod(steps, graph_steps)
+    tail = f" and {rest} launches from the host" if rest else ""
+    return f"{whole} replays of a captured graph of {graph_steps} steps{tail}"
 
 
 def measured_traffic(name: str):
@@ -250,8 +274,9 @@ def plain(scene, name, args, traffic_name=None):
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from acoustic_spaces_amd.solver import WaveSolver
 
-    K, W = args.steps, args.warmup
-    drive = scene.drive if W + K <= len(scene.drive) else sc.gaussian_drive(scene.rate, W + K)
+    K, W, lead = args.steps, args.warmup, args.lead_in
+    need = lead + W + K
+    drive = scene.drive if need <= len(scene.drive) else sc.gaussian_drive(scene.rate, need)
     t = time.perf_counter()
     solver = WaveSolver.from_scene(scene)
     layout = solver.layout()
@@ -265,9 +290,12 @@ def plain(scene, name, args, traffic_name=None):
         shape = tune.autotune(layout, steps=512, drive=drive)
         launch = {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks")}
         launch_from = "the autotuner, each candidate timed on the scene's drive (live fields)"
+    # A timed run shorter than the tuned graph replays graphs of a length that divides it
+    # (0 is the library's default, 64).
+    launch["graph_steps"] = graph_length(launch["graph_steps"] or 64, K)
     stepper = HipWaveSolver(layout, **launch)
     info = stepper.info()
-    ms, out = time_steps(stepper, drive, W, K)
+    ms, out = time_steps(stepper, drive, W, K, lead=lead)
     if not np.all(np.isfinite(out)):
         raise SystemExit("non-finite probe output")
     air = solver.air_cells
@@ -277,14 +305,18 @@ def plain(scene, name, args, traffic_name=None):
         "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
         "special_cells": info["specials"], "grid": list(scene.grid.shape), "setup_s": round(setup_s, 2),
         "launch": launch, "launch_from": launch_from,
+        "timed_launches": launch_mode(launch["graph_steps"], K),
+        "lead_in_steps": lead,
+        "lead_in": "steps of the drive from rest before the warm-up, so that the timed steps "
+                   "meet the spreading pulse (live fields), not fields at rest",
         "precision": "float (bit-identical to the reference's CPU scheme)",
     }
     traffic = measured_traffic(traffic_name) if traffic_name else None
     stepper.close()
     if args.outputs:
         # The receivers' outputs over the whole drive, in float and in double.
-        y32 = HipWaveSolver(layout, precision=32).run(scene.drive)
-        y64 = HipWaveSolver(layout, precision=64).run(scene.drive)
+        y32 = HipWaveSolver.run_once(layout, scene.drive, precision=32)
+        y64 = HipWaveSolver.run_once(layout, scene.drive, precision=64)
         o32, o64 = octave_outputs(y32, scene.rate), octave_outputs(y64, scene.rate)
         config["outputs"] = {"steps": len(scene.drive), "f32": o32,
                              "f32_vs_f64": compare_outputs(o32, o64, y32, y64)}
@@ -303,8 +335,8 @@ def source_outputs(scene, args) -> dict:
     from acoustic_spaces_amd.solver import WaveSolver
 
     layout = WaveSolver.from_scene(scene).layout()
-    y32 = HipWaveSolver(layout, precision=32).run(scene.drive)
-    y64 = HipWaveSolver(layout, precision=64).run(scene.drive)
+    y32 = HipWaveSolver.run_once(layout, scene.drive, precision=32)
+    y64 = HipWaveSolver.run_once(layout, scene.drive, precision=64)
     o32, o64 = octave_outputs(y32, scene.rate), octave_outputs(y64, scene.rate)
     args.others.append((scene, y32))
     return {"steps": len(scene.drive), "f32": o32, "f32_vs_f64": compare_outputs(o32, o64, y32, y64)}
@@ -507,6 +539,96 @@ def adjoint(args):
     }
 
 
+def batched_over_sequential() -> dict:
+    """The batching workload's ratio: 64 rooms at 5 cm (each 0.5 s, 5,941 steps, from rest)
+    as 64 plain solves one after another against the batched solver (the rooms merged into
+    grids of at most 192 MB, one launch a step each), both on the library's default launch
+    shape and timed with hipEvents over every step; plus every room's receivers batched
+    against its plain solve, bit for bit."""
+    from acoustic_spaces_amd import scenes
+    from acoustic_spaces_amd.batch import BatchedWaveSolver
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    batch = scenes.rooms_batch(count=64)
+    t = time.perf_counter()
+    layouts = [WaveSolver.from_scene(r).layout() for r in batch]
+    setup_s = time.perf_counter() - t
+    drive = batch[0].drive
+    steps = len(drive)
+    ms_seq, plain_out = 0.0, []
+    for L in layouts:
+        with HipWaveSolver(L) as s:
+            ms, y = time_steps(s, drive, 0, steps)
+        ms_seq += ms
+        plain_out.append(y)
+    b = BatchedWaveSolver(layouts, group_bytes=BatchedWaveSolver.DEFAULT_GROUP_BYTES)
+    ms_batch, same = 0.0, True
+    for idx, placements, s in b.groups:
+        ms, y = time_steps(s, drive, 0, steps)
+        ms_batch += ms
+        for i, p in zip(idx, placements):
+            same = same and np.array_equal(y[p.probes], plain_out[i])
+    groups = len(b.groups)
+    b.close()
+    air = int(sum(np.count_nonzero(L["types"]) for L in layouts))
+    return {
+        "value": round(ms_seq / ms_batch, 3),
+        "unit": "x (64 sequential plain solves' time over the batched solve's)",
+        "sequential_ms": round(ms_seq, 2), "batched_ms": round(ms_batch, 2), "groups": groups,
+        "steps_each": steps, "rooms": len(batch), "air_cells": air,
+        "batched_mcell_updates_per_s": round(air * steps / (ms_batch / 1e3) / 1e6, 1),
+        "batched_equals_plain_bitwise": bool(same), "setup_s": round(setup_s, 2),
+        "workload": "64 shoeboxes U(3,12)xU(3,10)xU(2.5,5) m at 5 cm, Monte Carlo wall absorption, "
+                    "1 source and 4 receivers each, 0.5 s",
+    }
+
+
+def gradient_over_forward() -> dict:
+    """The differentiation workload's cost: on the 6 x 4 x 3 m shoebox at 5 cm (5,941
+    steps, 6 receivers, 12 wall parameters), one forward solve against the gradient, as the
+    autograd function runs it: the forward run keeping checkpoints every 256 steps, then
+    the backward pass (each segment recomputed from its checkpoint, the adjoint stepped back
+    through it). Host clocks around the synchronous calls, after one warm gradient."""
+    import torch
+
+    from acoustic_spaces_amd import scenes
+    from acoustic_spaces_amd.fit import WallModel, decay_loss
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    rng = np.random.default_rng(44)
+    scene = scenes.shoebox(receivers=6, seed=44)
+    m32 = WallModel(scene, precision=32, every=256)
+    truth = np.concatenate([rng.uniform(0.02, 0.15, 6), rng.uniform(0.05, 0.3, 6)])
+    theta0 = np.concatenate([np.full(6, 0.08), np.full(6, 0.15)])
+    target = torch.from_numpy(m32.forward(truth))
+    with HipWaveSolver(m32.coefficients(theta0)) as stepper:
+        stepper.run(m32.drive[:256])
+        stepper.reset()
+        t = time.perf_counter()
+        y0 = stepper.run(m32.drive)
+        forward_s = time.perf_counter() - t
+    yt = torch.tensor(y0, requires_grad=True)
+    decay_loss(yt, target, scene.rate).backward()
+    dl = yt.grad.numpy()
+    m32.gradient(theta0, dl)
+    t = time.perf_counter()
+    _, stepper = m32.record(theta0)
+    record_s = time.perf_counter() - t
+    t = time.perf_counter()
+    m32.backward(stepper, dl)
+    backward_s = time.perf_counter() - t
+    return {
+        "value": round((record_s + backward_s) / forward_s, 3),
+        "unit": "x (the whole gradient's time over one forward solve's)",
+        "backward_only": round(backward_s / forward_s, 3),
+        "record_over_forward": round(record_s / forward_s, 3),
+        "forward_s": round(forward_s, 4), "record_s": round(record_s, 4), "backward_s": round(backward_s, 4),
+        "checkpoint_every": 256, "steps": int(m32.steps), "parameters": 12,
+        "workload": "shoebox 6x4x3 m at 5 cm, 6 receivers, 0.5 s, 12 wall parameters (6 walls x (beta, G))",
+    }
+
+
 def strict(value):
     """``value`` with every NaN or infinity (an unreadable T30, say) as None, so that the
     line is strict JSON."""
@@ -524,11 +646,17 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=256)
+    ap.add_argument("--lead-in", type=int, default=256,
+                    help="steps from rest before the warm-up (set-up, untimed), so that the timed "
+                         "steps meet live fields")
     ap.add_argument("--workload", choices=("hall", "shoebox", "outdoor", "rooms", "adjoint"), default="hall")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-outputs", action="store_true",
                     help="skip the receiver outputs (the hall's and shoebox's band levels and T30, "
                          "float against double; the outdoor barrier's insertion loss)")
+    ap.add_argument("--no-ratios", action="store_true",
+                    help="the hall line without the batched-over-sequential and gradient-over-forward "
+                         "ratios (the 64 rooms and the shoebox adjoint)")
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--face-blocks", type=int, default=-1, choices=(-1, 0, 1),
@@ -583,6 +711,10 @@ def main(argv=None) -> int:
         scene = None
     if not args.no_cpu and scene is not None:
         line["cpu_baseline"] = cpu_baseline(scene, compare=args.signals, others=args.others)
+    if args.workload == "hall" and not args.no_ratios:
+        # The metric's other two parts, on their own workloads, in the same run.
+        line["batched_over_sequential"] = batched_over_sequential()
+        line["gradient_over_forward"] = gradient_over_forward()
     print(json.dumps(strict(line), allow_nan=False), flush=True)
     return 0
 

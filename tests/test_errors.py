@@ -93,3 +93,28 @@ def test_adjoint_refusals():
         HipWaveSolver(L, no_faces=True).gradient(drive, c, every=0)
     with pytest.raises(ValueError, match="record"):
         HipWaveSolver(L, no_faces=True).adjoint(c)
+
+
+def _probes_with(layout, start):
+    L = dict(layout)
+    L["probe_start"] = np.asarray(start, np.uint64)
+    return L
+
+
+@pytest.mark.parametrize("what,start,match", [
+    ("ranges not from 0", lambda s: np.r_[1, s[1:]], "start at 0"),
+    ("a range running backwards", lambda s: np.r_[s[0], s[2], s[1], s[3:]], "backwards"),
+    ("ranges past the taps", lambda s: np.r_[s[:-1], s[-1] + 1], "end at the taps"),
+])
+def test_refused_probe_ranges(what, start, match):
+    """probe_start must run from 0 to the taps, never decreasing: a range running
+    backwards would make a probe's wavefront loop about 2^32 times, one past the taps read
+    past its cells. Refused on the host, before any device work."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    L = _layout()
+    s = np.asarray(L["probe_start"], np.int64)
+    assert len(s) >= 4 and s[1] < s[2]
+    print(f"refused: {what}", flush=True)
+    with pytest.raises(ValueError, match=match):
+        HipWaveSolver(_probes_with(L, start(s)))

@@ -79,3 +79,48 @@ def test_loss_gradient_equals_finite_differences():
         rel = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
         print(f"precision {precision}: |grad - fd| / |fd| = {rel:.2e}")
         assert rel < (1e-3 if precision == 64 else 1e-2)
+
+
+def cfg4_problem():
+    """The differentiation workload: the 6 x 4 x 3 m shoebox at 5 cm (0.5 s, 5,941 steps),
+    6 receivers, 12 unknowns (6 walls x (beta, G)), a seeded true model, targets from the
+    forward solve plus white noise at 40 dB SNR, and a flat start."""
+    rng = np.random.default_rng(44)
+    scene = scenes.shoebox(receivers=6, seed=44)
+    truth = np.concatenate([rng.uniform(0.02, 0.15, 6), rng.uniform(0.05, 0.3, 6)])
+    theta0 = np.concatenate([np.full(6, 0.08), np.full(6, 0.15)])
+    return scene, truth, theta0, rng
+
+
+@pytest.mark.gpu
+def test_shoebox_gradient_with_checkpoints_every_256_equals_finite_differences():
+    """On the workload itself: the double adjoint's gradient of the decay loss (checkpoints
+    every 256 steps, recomputation between them) against central differences, h = 1e-3, of
+    the double solve, on all 12 parameters: within 1e-3 of the differences' largest."""
+    import torch
+
+    from acoustic_spaces_amd.fit import WallModel, decay_loss, make_function
+
+    scene, truth, theta0, rng = cfg4_problem()
+    m64 = WallModel(scene, precision=64, every=256)
+    assert m64.steps == 5941 and m64.every == 256 and m64.probes == 6
+    clean = m64.forward(truth)
+    noise = rng.normal(size=clean.shape) * np.sqrt(np.mean(clean ** 2, axis=1, keepdims=True)) * 10 ** (-40 / 20)
+    target = torch.from_numpy(clean + noise)
+    WaveSolve = make_function()
+    t = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    decay_loss(WaveSolve.apply(t, m64), target, scene.rate).backward()
+    grad = t.grad.numpy()
+    h = 1e-3
+    fd = np.zeros(12)
+    for p in range(12):
+        e = np.zeros(12)
+        e[p] = h
+        up = decay_loss(torch.from_numpy(m64.forward(theta0 + e)), target, scene.rate).item()
+        dn = decay_loss(torch.from_numpy(m64.forward(theta0 - e)), target, scene.rate).item()
+        fd[p] = (up - dn) / (2 * h)
+    worst = np.max(np.abs(grad - fd)) / np.max(np.abs(fd))
+    rel = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
+    print(f"shoebox adjoint (double, every 256): largest |grad - fd| / max|fd| = {worst:.2e}, "
+          f"relative L2 {rel:.2e}", flush=True)
+    assert worst < 1e-3 and rel < 1e-3

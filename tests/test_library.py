@@ -27,3 +27,27 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(native.LIBRARY))
     for name in _declared():
         assert hasattr(lib, name), name
+
+
+def test_pack_layout_checks_each_array_against_its_count():
+    """The binding refuses, before the library sees them, arrays shorter or longer than the
+    count that indexes them (so the library never reads past a host array's end)."""
+    import numpy as np
+    import pytest
+
+    import cases
+    from acoustic_spaces_amd import native
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    L = WaveSolver.from_scene(cases.small_box()).layout()
+    native.pack_layout(L)
+    for key in ("coef_a", "coef_b", "coef_inv", "source", "link_weight", "state_weight",
+                "branch_before", "branch_gain", "probe_weights"):
+        bad = dict(L)
+        bad[key] = np.asarray(L[key])[:-1]
+        with pytest.raises(ValueError, match=key):
+            native.pack_layout(bad)
+    bad = dict(L)
+    bad["weights"] = np.asarray(L["weights"])[:4]
+    with pytest.raises(ValueError, match="weights"):
+        native.pack_layout(bad)
