@@ -39,6 +39,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -1167,6 +1168,7 @@ struct as_fdtd {
     // launch. p(k - 1) is in slot `prev` and p(k) in prev ^ 1 (the same for the velocities).
     void *field[4] = {}, *velocity[4] = {};
     int prev = 0;
+    int device = 0;     // the device the handle's memory and stream live on
     int pairs = 0;      // 1: two steps a launch (see pair_kernel)
     int pair_z = 8;     // planes a box marches
     uint32_t box_blocks = 0, boxes_x = 0, groups_y = 0, cell_count = 0;
@@ -1215,7 +1217,7 @@ struct as_fdtd {
     float *keep[6] = {}, *push[6] = {};
     uint32_t *probe_start = nullptr, *probe_cell = nullptr;
     double *probe_weight = nullptr;
-    // scratch for run() on host pointers
+    // staging for run() on host pointers
     float *signal_buf = nullptr;
     size_t signal_cap = 0;
     double *out_buf = nullptr;
@@ -1868,6 +1870,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     clock.mark("layer, probes");
     // Device copies.
     const size_t e = precision == 64 ? sizeof(double) : sizeof(float);
+    check(hipGetDevice(&h->device), "hipGetDevice");
     check(hipStreamCreate(&h->stream), "hipStreamCreate");
     h->types = upload(h, types);
     h->scale = upload(h, scale);
@@ -2242,6 +2245,65 @@ int fail(as_fdtd *h, const Failure &f) {
 
 }  // namespace
 
+// The no-allocation run (as_fdtd_run_static) stages each piece of a run in this array: the
+// probes' output (probes x piece doubles) from its start, the drive (piece floats) from the
+// next 256-byte boundary after it. Each device holds its own copy, at its own address, for
+// the life of the process. In host code `scratch` names a host-side stand-in, not device
+// memory: its device address comes from hipGetSymbolAddress (scratch_address).
+constexpr size_t kScratchBytes = size_t{1} << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+struct ScratchAddress {
+    std::once_flag once;
+    void *ptr = nullptr;
+    hipError_t error = hipSuccess;
+};
+ScratchAddress g_scratch[kMaxDevices];
+
+// The device address of `scratch` on `device`, looked up once per device with that device
+// current, and the caller's current device restored.
+void *scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) refuse("the handle's device lies beyond the library's table");
+    ScratchAddress &slot = g_scratch[device];
+    std::call_once(slot.once, [&] {
+        int current = -1;
+        slot.error = hipGetDevice(&current);
+        if (slot.error != hipSuccess) return;
+        if (current != device) {
+            slot.error = hipSetDevice(device);
+            if (slot.error != hipSuccess) return;
+        }
+        slot.error = hipGetSymbolAddress(&slot.ptr, HIP_SYMBOL(scratch));
+        if (current != device) (void)hipSetDevice(current);
+    });
+    check(slot.error, "hipGetSymbolAddress(scratch)");
+    if (slot.ptr == nullptr) throw Failure{AS_EDEVICE, "hipGetSymbolAddress(scratch): no address"};
+    return slot.ptr;
+}
+
+// `steps` steps launched one a step from the host (no graphs: capturing and instantiating
+// one allocates), on device pointers.
+template <typename T>
+void launch_direct(as_fdtd *h, const float *d_signal, size_t steps, double *d_out, size_t out_stride) {
+    StepArgs<T> s = step_args<T>(h);
+    const uint32_t blocks = block_count(h, s);
+    if (blocks == 0) return;
+    s.signal = d_signal;
+    s.out = d_out;
+    s.out_stride = out_stride;
+    h->prev = enqueue_steps<T>(h, s, blocks, h->prev, 0, steps, h->stream);
+    check(hipGetLastError(), "step launch");
+    h->done += steps;
+}
+
+size_t round_up_256(size_t n) { return (n + 255) / 256 * 256; }
+
+}  // namespace
+
 extern "C" {
 
 as_fdtd *as_fdtd_create(const as_layout *layout, int precision, const as_tuning *tuning) {
@@ -2325,6 +2387,45 @@ int as_fdtd_run(as_fdtd *h, const float *signal, size_t steps, double *out) {
                 std::memcpy(out + static_cast<size_t>(p) * steps + done, chunk.data() + static_cast<size_t>(p) * m,
                             m * sizeof(double));
             }
+        }
+        return AS_OK;
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+int as_fdtd_run_static(as_fdtd *h, const float *signal, size_t steps, double *out) {
+    if (h == nullptr) return AS_EINVAL;
+    try {
+        if (steps == 0) return AS_OK;
+        if (signal == nullptr || (out == nullptr && h->probes > 0)) refuse("run needs a signal and room for the probes");
+        const size_t probe_bytes = sizeof(double) * h->probes;
+        size_t piece = kScratchBytes / (probe_bytes + sizeof(float));
+        while (piece > 0 && round_up_256(piece * probe_bytes) + piece * sizeof(float) > kScratchBytes) --piece;
+        if (piece == 0) {
+            refuse("as_fdtd_run_static stages a step's probes in " + std::to_string(kScratchBytes) +
+                   " bytes: at most " + std::to_string((kScratchBytes - 256) / sizeof(double)) + " probes, not " +
+                   std::to_string(h->probes));
+        }
+        unsigned char *base = static_cast<unsigned char *>(scratch_address(h->device));
+        double *d_out = reinterpret_cast<double *>(base);
+        float *d_signal = reinterpret_cast<float *>(base + round_up_256(piece * probe_bytes));
+        for (size_t done = 0; done < steps; done += piece) {
+            const size_t m = std::min(piece, steps - done);
+            check(hipMemcpyAsync(d_signal, signal + done, m * sizeof(float), hipMemcpyHostToDevice, h->stream),
+                  "hipMemcpyAsync");
+            if (h->precision == 64) {
+                launch_direct<double>(h, d_signal, m, d_out, m);
+            } else {
+                launch_direct<float>(h, d_signal, m, d_out, m);
+            }
+            if (h->probes > 0) {
+                check(hipMemcpy2DAsync(out + done, steps * sizeof(double), d_out, m * sizeof(double),
+                                       m * sizeof(double), h->probes, hipMemcpyDeviceToHost, h->stream),
+                      "hipMemcpy2DAsync");
+            }
+            // The next piece reuses the staging: this one's copies end first.
+            check(hipStreamSynchronize(h->stream), "step");
         }
         return AS_OK;
     } catch (const Failure &f) {
@@ -2456,11 +2557,11 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     const size_t e = sizeof(T);
     const size_t G = h->specials, slots = h->state_slots, P = h->probes;
     const size_t segments = (steps + every - 1) / every;
-    std::vector<void *> scratch;
+    std::vector<void *> temporaries;
     auto alloc = [&](size_t bytes) {
         void *p = nullptr;
         check(hipMalloc(&p, std::max<size_t>(bytes, 256)), "hipMalloc");
-        scratch.push_back(p);
+        temporaries.push_back(p);
         check(hipMemsetAsync(p, 0, std::max<size_t>(bytes, 256), h->stream), "hipMemsetAsync");
         return p;
     };
@@ -2469,7 +2570,7 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
         ~Free() {
             for (void *p : v) (void)hipFree(p);
         }
-    } release{scratch};
+    } release{temporaries};
     const size_t field = h->n_pad * e, states = slots * e, per = 2 * field + 2 * states;
     T *history = static_cast<T *>(alloc(every * std::max<size_t>(G, 1) * e));
     T *q0 = static_cast<T *>(alloc(field)), *q1 = static_cast<T *>(alloc(field));

@@ -51,7 +51,7 @@ _lib = None
 
 #: Every symbol include/asfdtd.h declares.
 SYMBOLS = (
-    "as_fdtd_create", "as_fdtd_create_error", "as_fdtd_run", "as_fdtd_run_device",
+    "as_fdtd_create", "as_fdtd_create_error", "as_fdtd_run", "as_fdtd_run_device", "as_fdtd_run_static",
     "as_fdtd_reset", "as_fdtd_field", "as_fdtd_stream", "as_fdtd_info", "as_fdtd_last_error",
     "as_fdtd_destroy", "as_fdtd_gradient", "as_fdtd_record", "as_fdtd_adjoint",
 )
@@ -80,6 +80,8 @@ def load():
     lib.as_fdtd_run.argtypes = [P, P, SIZE, P]
     lib.as_fdtd_run_device.restype = ctypes.c_int
     lib.as_fdtd_run_device.argtypes = [P, P, SIZE, P]
+    lib.as_fdtd_run_static.restype = ctypes.c_int
+    lib.as_fdtd_run_static.argtypes = [P, P, SIZE, P]
     lib.as_fdtd_reset.restype = ctypes.c_int
     lib.as_fdtd_reset.argtypes = [P]
     lib.as_fdtd_field.restype = ctypes.c_int

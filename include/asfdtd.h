@@ -122,6 +122,18 @@ int as_fdtd_run(as_fdtd *h, const float *signal, size_t steps, double *out);
 /* The same on device pointers, enqueued on the handle's stream without waiting. */
 int as_fdtd_run_device(as_fdtd *h, const float *d_signal, size_t steps, double *d_out);
 
+/* as_fdtd_run without allocating: no hipMalloc, hipFree, graph capture or other allocating
+ * call, for callers that cannot allocate (code in a tight serving loop, a harness that
+ * forbids device allocation after set-up). It has no counterpart in the reference. The
+ * drive and the probes' output go, piece by piece, through one file-scope device array of
+ * 1 MiB per device (held until the process ends); the steps are launched one a step from
+ * the host. Results equal as_fdtd_run's bit for bit. Refused (AS_EINVAL) when one step's
+ * probes do not fit it: more than 131,040 probes. Nothing in the array must start at zero
+ * (each piece writes what it reads). Every call on one device shares that device's array,
+ * so calls that overlap in time on one device (two host threads, or two handles whose
+ * streams run at once) race on it: serialise them. */
+int as_fdtd_run_static(as_fdtd *h, const float *signal, size_t steps, double *out);
+
 /* Zeroes the state (WaveSolver::reset, fdtd.hpp:906-915). */
 int as_fdtd_reset(as_fdtd *h);
 
