@@ -1,33 +1,44 @@
 // The wave solver's set-up on walls on cell faces: from a triangle mesh, its materials and
-// a grid, the layout the time step reads (the reference's WaveSolver constructor,
-// cpp/include/acoustic_spaces/fdtd.hpp:360-690, and its layout(), 727-782).
+// a grid, the layout the time step reads. It stands in for the reference's WaveSolver
+// constructor on that path (cpp/include/acoustic_spaces/fdtd.hpp:360-690) and hands out
+// the arrays of its layout() (727-782), equal to the reference's bit for bit.
 //
-// What it computes, in the reference's order and arithmetic:
-//  - the stencil's weights, lowered by the stability margin and rounded down
-//    (set_up_stencil, fdtd.hpp:1649-1673; lowered/down, 2127-2135);
-//  - each branch's coefficients (fdtd.hpp:391-411);
-//  - the cut links: bit b of cut[i] when the segment between the centres of cell i and its
-//    neighbour along the b-th direction of the + half crosses a face (fdtd.hpp:446-477);
-//  - the air: every cell the source's visible cells reach through links that are not
-//    blocked (visible_cells, seen_cells 3291-3365; flood 3367-3382);
-//  - the matched layer's cells and coefficients (fdtd.hpp:521-542, 2219-2240);
-//  - the boundary cells: marks, folded and pair-averaged link weights, per-material
-//    boundary weights and branch states (fdtd.hpp:546-625, raw_weights 2162-2190,
-//    add_special 3499-3517);
-//  - the source's cells, by the cubic reading stencil (reading_cells, 2361-2488), and the
-//    probes' cells and weights.
+// How it is organised:
+//  - Stencil: the 18 link directions as a table of their steps, weights and boundary
+//    weights; a diagonal's two axial sides and each direction's opposite are derived from
+//    the steps.
+//  - The geometry: triangles binned in blocks of cells; a segment crosses a face when its
+//    ends lie strictly on either side of the face's plane and the point where it meets
+//    the plane lies inside the triangle.
+//  - The cut word of each cell (a bit per link of the + half: the direction whose cell
+//    offset is positive owns the bit) and, from it, each cell's LinkMask: which of its
+//    links are blocked (cut, or reaching a cell on the grid's edge).
+//  - The air: a span flood from the cells the source sees.
+//  - Wall cells from LinkMasks alone: the folded weights of a cell are a function of its
+//    mask; the marks, the pair-averaged weights, the wall contacts of each blocked link,
+//    their resolution to a material or the grid's edge, and the sum of wall term and
+//    branch states are separate stages, run on slabs of planes on all threads.
+//  - SpecialList, which builds the layout's arrays of special cells.
+//  - Readings: the source's and probes' stencils over a 4 x 4 x 4 block as 64-bit masks.
 //
-// Geometry. The reference decides crossings with a BVH and a set of tie rules for points
-// that lie on faces or segments that pass through their edges (kTie, crossing,
-// fdtd.hpp:2475-3257). This set-up decides them exactly where no tie can arise: a segment
-// crosses a face when its ends lie strictly on either side of the face's plane and the
-// point where it meets the plane lies inside the triangle. Every decision is made with a
-// margin (kGuard from a plane, kBarycentric from an edge) far wider than the reference's
-// tie and edge tolerances (1e-10 cells, 1e-9 barycentric); a decision inside a margin
-// that could change the layout raises "degenerate" instead of guessing, so every layout
-// this code hands out is the reference's. A meeting near an edge shared with another face
-// of the same material at the same point, or farther than a firm crossing, changes nothing
-// and is accepted. Sources and probes on faces (the reference's leans) are refused.
+// What parity fixes, and is therefore the reference's: the decisions' rules (which links
+// are blocked, which cells are marked, which wall a contact meets), every formula and its
+// operation order (the lowered and rounded-down weights, fdtd.hpp:2127-2135; the
+// pair-averaged weights; the wall term's sum in contact order; the branch coefficients,
+// 391-411; the specials' coefficients, 3499-3517; the Lagrange and trilinear weights,
+// 2361-2488 and 3305-3365; the matched layer's profile, 2219-2240) and the order of the
+// output arrays (specials by ascending cell, then the source's cells; taps in z, y, x
+// order).
+//
+// Geometry and ties. The reference decides crossings with a BVH and tie rules for points
+// on faces and segments through their edges (kTie, crossing, fdtd.hpp:2475-3257). Here
+// every decision is made with a margin (kGuard from a plane, kBarycentric from an edge)
+// far wider than the reference's tie and edge tolerances (1e-10 cells, 1e-9 barycentric);
+// a decision inside a margin that could change the layout raises "degenerate" instead of
+// guessing, so every layout this code hands out is the reference's. A meeting near an edge
+// shared with another face of the same material at the same point, or farther than a firm
+// crossing, changes nothing and is accepted. Sources and probes on faces (the reference's
+// leans) are refused.
 
 #include <algorithm>
 #include <array>
@@ -58,6 +69,11 @@ inline V3 operator+(const V3 &a, const V3 &b) { return {a.x + b.x, a.y + b.y, a.
 inline V3 operator-(const V3 &a, const V3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline V3 operator*(const V3 &a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 inline double dot(const V3 &a, const V3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(const V3 &a, const V3 &b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm(const V3 &a) { return std::sqrt(dot(a, a)); }
+
 // The set-up's phases' times on stderr, where AS_FDTD_TIMING is set.
 struct SetupClock {
     bool on = std::getenv("AS_FDTD_TIMING") != nullptr;
@@ -71,36 +87,25 @@ struct SetupClock {
     }
 };
 
-inline V3 cross(const V3 &a, const V3 &b) {
-    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-inline double norm(const V3 &a) { return std::sqrt(dot(a, a)); }
-
 struct Degenerate : std::invalid_argument {
     using std::invalid_argument::invalid_argument;
 };
 
 constexpr uint8_t kSolid = 0, kInterior = 1, kSpecial = 2, kLayer = 3;
-constexpr double kMargin = 1.0 / 262144.0;  // 2^-18 (fdtd.hpp:17-27)
+constexpr double kMargin = 1.0 / 262144.0;  // the links' stability margin, 2^-18 (fdtd.hpp:17-27)
 constexpr int kReach = 4;                   // cells a side of a cubic reading (fdtd.hpp:2321)
-constexpr double kNear = 3.0;               // probes nearer the source read trilinearly
+constexpr double kNear = 3.0;               // probes nearer the source (in cells) read trilinearly
 constexpr double kBarycentric = 1e-8;       // the edge margin (see above)
 constexpr size_t npos = static_cast<size_t>(-1);
 
+// The link directions, the layout's link_direction values: 6 axial, then 12 diagonal.
 constexpr int kSteps[18][3] = {
     {-1, 0, 0},  {1, 0, 0},  {0, -1, 0}, {0, 1, 0},  {0, 0, -1}, {0, 0, 1},
     {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 1, 0},  {-1, 0, -1}, {1, 0, -1},
     {-1, 0, 1},  {1, 0, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}, {0, 1, 1}};
 
-int opposite(int k) { return k < 6 ? k ^ 1 : 6 + 4 * ((k - 6) / 4) + 3 - (k - 6) % 4; }
-
-std::array<int, 2> steps_of(int k) {
-    static constexpr int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-    const int p = (k - 6) / 4, sa = (k - 6) % 2, sb = ((k - 6) / 2) % 2;
-    return {2 * pairs[p][0] + sa, 2 * pairs[p][1] + sb};
-}
-
-// The largest float not above w; a link's weight lowered by the margin (fdtd.hpp:2127-2135).
+// The largest float not above w, and a link's weight lowered by the margin then rounded
+// so (fdtd.hpp:2127-2135): no weight rounds up, so the links never sum above their bound.
 float down(double w) {
     float f = static_cast<float>(w);
     if (static_cast<double>(f) > w) f = std::nextafter(f, 0.0f);
@@ -108,18 +113,275 @@ float down(double w) {
 }
 float lowered(double w) { return down(w * (1.0 - kMargin)); }
 
+// --- the stencil ----------------------------------------------------------------------
+
+struct Direction {
+    int step[3] = {0, 0, 0};
+    ptrdiff_t offset = 0;  // in cells of the flat grid
+    double exact = 0;      // the link's weight (0 for a direction the stencil does not use)
+    float weight = 0;      // lowered to float
+    double wall = 0;       // its boundary weight: exact / (2 L), L the Courant number
+    int plus_bit = -1;     // its bit in a cell's cut word, where its offset is positive
+    int opposite = 0;
+    int side_a = -1, side_b = -1;  // a diagonal's axial components, lower axis first
+};
+
+// The 7-point stencil (axial 1/3, L = 1/sqrt 3) or the 19-point one (axial 1/4, diagonal
+// 1/8, L = sqrt(3/4)) (fdtd.hpp:7-15, set_up_stencil 1649-1673).
+struct Stencil {
+    int count = 6;  // directions used: 6 or 18
+    std::array<Direction, 18> d;
+    float centre = 0, total = 0;  // 2 less the weights' sum, and the sum, as floats
+};
+
+Stencil make_stencil(size_t points, ptrdiff_t sy, ptrdiff_t sz) {
+    Stencil st;
+    st.count = points == 7 ? 6 : 18;
+    const double axial = points == 7 ? 1.0 / 3.0 : 0.25;
+    const double diagonal = points == 7 ? 0.0 : 0.125;
+    const double courant = std::sqrt(axial + 4.0 * diagonal);
+    int bits = 0;
+    for (int k = 0; k < 18; ++k) {
+        Direction &d = st.d[static_cast<size_t>(k)];
+        int span = 0;
+        for (int a = 0; a < 3; ++a) {
+            d.step[a] = kSteps[k][a];
+            span += std::abs(d.step[a]);
+        }
+        d.offset = d.step[0] + d.step[1] * sy + d.step[2] * sz;
+        d.exact = k < st.count ? (span == 1 ? axial : diagonal) : 0.0;
+        d.weight = lowered(d.exact);
+        d.wall = d.exact / (2.0 * courant);
+        if (k < st.count && d.offset > 0) d.plus_bit = bits++;
+    }
+    for (int k = 0; k < 18; ++k) {
+        Direction &d = st.d[static_cast<size_t>(k)];
+        for (int j = 0; j < 18; ++j) {
+            const int *s = st.d[static_cast<size_t>(j)].step;
+            if (s[0] == -d.step[0] && s[1] == -d.step[1] && s[2] == -d.step[2]) d.opposite = j;
+        }
+        if (k < 6) continue;
+        for (int a = 0; a < 3; ++a) {
+            if (d.step[a] == 0) continue;
+            const int axial_dir = 2 * a + (d.step[a] > 0 ? 1 : 0);
+            (d.side_a < 0 ? d.side_a : d.side_b) = axial_dir;
+        }
+    }
+    double sum = 0.0;
+    for (int k = 0; k < st.count; ++k) sum += st.d[static_cast<size_t>(k)].weight;
+    st.centre = static_cast<float>(2.0 - sum);
+    st.total = static_cast<float>(sum);
+    return st;
+}
+
+// Which of a cell's links are blocked: bit k where its neighbour along k lies on the grid's
+// edge or a face cuts the link; edge_far, the first kind alone.
+struct LinkMask {
+    uint32_t blocked = 0, edge_far = 0;
+};
+
+// A cell's link weights before they are averaged with its neighbours' (raw_weights,
+// fdtd.hpp:2162-2190), from its mask alone: an open link keeps its weight; a blocked
+// diagonal whose far cell is not on the grid's edge hands its weight to the open axial
+// links it is made of, shared equally; every other blocked link weighs 0. (The shares are
+// sums of binary fractions, exact in any order.)
+void fold(const Stencil &st, const LinkMask &m, double *w) {
+    for (int k = 0; k < 18; ++k) {
+        const bool open = k < st.count && !((m.blocked >> k) & 1u);
+        w[k] = open ? st.d[static_cast<size_t>(k)].exact : 0.0;
+    }
+    for (int k = 6; k < st.count; ++k) {
+        if (!((m.blocked >> k) & 1u) || ((m.edge_far >> k) & 1u)) continue;
+        const Direction &d = st.d[static_cast<size_t>(k)];
+        const bool open_a = !((m.blocked >> d.side_a) & 1u), open_b = !((m.blocked >> d.side_b) & 1u);
+        const int sides = (open_a ? 1 : 0) + (open_b ? 1 : 0);
+        if (sides == 0) continue;
+        const double share = d.exact / sides;
+        if (open_a) w[d.side_a] += share;
+        if (open_b) w[d.side_b] += share;
+    }
+}
+
+// --- the matched layer's profile ----------------------------------------------------------
+
+// Its coefficients along one axis of `cells` cells (layer_coefficients, fdtd.hpp:2219-2240),
+// at x = 0 .. cells: u the distance in cells from the nearer end, of the centre
+// (min(x, last - x)) and of the lower face (min(x - 1/2, last - x + 1/2)); the damping
+// d(u) = damping ((width + 1/2 - u) / width)^2 where that is positive, else 0; keep =
+// (1 - d) / (1 + d), push = courant / (1 + d).
+struct LayerAxis {
+    std::vector<float> keep_centre, keep_face, push_centre, push_face;
+};
+
+LayerAxis layer_axis(size_t cells, size_t width, double damping, double courant) {
+    LayerAxis out;
+    const double w = static_cast<double>(width), last = static_cast<double>(cells - 1);
+    const auto profile = [&](double u) {
+        const double depth = std::max(0.0, w + 0.5 - u) / w;
+        return damping * depth * depth;
+    };
+    for (size_t x = 0; x <= cells; ++x) {
+        const double c = static_cast<double>(x);
+        const double at_centre = profile(std::min(c, last - c));
+        const double at_face = profile(std::min(c - 0.5, last - c + 0.5));
+        out.keep_centre.push_back(static_cast<float>((1.0 - at_centre) / (1.0 + at_centre)));
+        out.push_centre.push_back(static_cast<float>(courant / (1.0 + at_centre)));
+        out.keep_face.push_back(static_cast<float>((1.0 - at_face) / (1.0 + at_face)));
+        out.push_face.push_back(static_cast<float>(courant / (1.0 + at_face)));
+    }
+    return out;
+}
+
+// --- branches ---------------------------------------------------------------------------
+
+// Each admittance branch (m, r, q) centred in time (fdtd.hpp:391-411): a = m + r/2 + q/2,
+// now = 2m / a, before = (2m + q) / a, gain = 1 / (2a); and each material's branches.
+struct Branches {
+    std::vector<float> now, before, gain;
+    std::vector<double> a;
+    std::vector<std::vector<int32_t>> of_material;
+};
+
+Branches make_branches(const double *mrq, const int64_t *material, size_t count, size_t materials) {
+    Branches b;
+    b.of_material.resize(materials);
+    for (size_t j = 0; j < count; ++j) {
+        const double m = mrq[3 * j], r = mrq[3 * j + 1], q = mrq[3 * j + 2];
+        const double a = m + 0.5 * r + 0.5 * q;
+        if (!(m >= 0.0 && r >= 0.0 && q >= 0.0 && a > 0.0)) {
+            throw std::invalid_argument("branch " + std::to_string(j) +
+                                        ": mass, resistance and stiffness must be at least 0, and not all 0");
+        }
+        if (material[j] < 0 || static_cast<size_t>(material[j]) >= materials) {
+            throw std::invalid_argument("branch " + std::to_string(j) + " belongs to no material");
+        }
+        b.of_material[static_cast<size_t>(material[j])].push_back(static_cast<int32_t>(j));
+        b.now.push_back(static_cast<float>(2.0 * m / a));
+        b.before.push_back(static_cast<float>((2.0 * m + q) / a));
+        b.gain.push_back(static_cast<float>(0.5 / a));
+        b.a.push_back(a);
+    }
+    return b;
+}
+
+// --- the special cells' arrays -----------------------------------------------------------
+
+// The layout's arrays of special cells (fdtd.hpp:727-782), appended a cell at a time:
+// its links where its weight is positive, lowered; coef_a = 2 - s (sum of the weights)
+// (1 - 2^-18); coef_b = 1 - r g and coef_inv = 1 / (1 + r g), r = sqrt s, g its wall term;
+// its branch states (add_special, fdtd.hpp:3499-3517).
+class SpecialList {
+public:
+    std::vector<int64_t> cell;
+    std::vector<uint64_t> link_start{0};
+    std::vector<uint8_t> link_direction;
+    std::vector<float> link_weight;
+    std::vector<float> coef_a, coef_b, coef_inv, source;
+    std::vector<uint64_t> state_start{0};
+    std::vector<int32_t> state_branch;
+    std::vector<float> state_weight;
+    size_t sorted = 0;  // the first `sorted` cells ascend (the wall cells)
+
+    struct State {
+        int32_t branch;
+        float weight;
+    };
+
+    void append(size_t i, const double *w, int count, double s, double g, const State *states, size_t nstates) {
+        cell.push_back(static_cast<int64_t>(i));
+        double sum = 0.0;
+        for (int k = 0; k < count; ++k) {
+            if (!(w[k] > 0.0)) continue;
+            sum += w[k];
+            link_direction.push_back(static_cast<uint8_t>(k));
+            link_weight.push_back(lowered(s * w[k]));
+        }
+        link_start.push_back(link_direction.size());
+        const double r = std::sqrt(s);
+        coef_a.push_back(static_cast<float>(2.0 - s * sum * (1.0 - kMargin)));
+        coef_b.push_back(static_cast<float>(1.0 - r * g));
+        coef_inv.push_back(static_cast<float>(1.0 / (1.0 + r * g)));
+        source.push_back(0.0f);
+        for (size_t q = 0; q < nstates; ++q) {
+            state_branch.push_back(states[q].branch);
+            state_weight.push_back(states[q].weight);
+        }
+        state_start.push_back(state_branch.size());
+    }
+
+    // Appends another list's cells (their ranges shifted by this list's counts).
+    void extend(const SpecialList &o) {
+        const uint64_t links0 = link_direction.size(), states0 = state_branch.size();
+        cell.insert(cell.end(), o.cell.begin(), o.cell.end());
+        for (size_t q = 1; q < o.link_start.size(); ++q) link_start.push_back(links0 + o.link_start[q]);
+        for (size_t q = 1; q < o.state_start.size(); ++q) state_start.push_back(states0 + o.state_start[q]);
+        link_direction.insert(link_direction.end(), o.link_direction.begin(), o.link_direction.end());
+        link_weight.insert(link_weight.end(), o.link_weight.begin(), o.link_weight.end());
+        for (auto [to, from] : {std::pair{&coef_a, &o.coef_a}, std::pair{&coef_b, &o.coef_b},
+                                std::pair{&coef_inv, &o.coef_inv}, std::pair{&source, &o.source},
+                                std::pair{&state_weight, &o.state_weight}})
+            to->insert(to->end(), from->begin(), from->end());
+        state_branch.insert(state_branch.end(), o.state_branch.begin(), o.state_branch.end());
+    }
+
+    // The special index of cell i, or npos: a binary search of the ascending wall cells,
+    // then the few cells added after them.
+    size_t index_of(size_t i) const {
+        const auto key = static_cast<int64_t>(i);
+        const auto end = cell.begin() + static_cast<ptrdiff_t>(sorted);
+        const auto it = std::lower_bound(cell.begin(), end, key);
+        if (it != end && *it == key) return static_cast<size_t>(it - cell.begin());
+        for (size_t k = sorted; k < cell.size(); ++k)
+            if (cell[k] == key) return k;
+        return npos;
+    }
+
+    size_t size() const { return cell.size(); }
+};
+
+// --- readings ----------------------------------------------------------------------------
+
+// The weight of node j of four (0 .. 3) in the cubic Lagrange interpolation at x, the
+// product over the other nodes m, in ascending m, of (x - m) / (j - m) (fdtd.hpp:2361-2488).
+double lagrange4(double x, int j) {
+    double w = 1.0;
+    for (int m = 0; m < kReach; ++m)
+        if (m != j) w *= (x - m) / static_cast<double>(j - m);
+    return w;
+}
+
+// A 4 x 4 x 4 block of cells as a 64-bit set, bit x + 4 y + 16 z.
+constexpr uint64_t block_bit(int x, int y, int z) { return 1ull << (x + 4 * y + 16 * z); }
+
+// The cells of the block whose coordinate along axis a is in `planes` (a 4-bit set).
+uint64_t block_slab(int a, unsigned planes) {
+    uint64_t out = 0;
+    for (int z = 0; z < kReach; ++z)
+        for (int y = 0; y < kReach; ++y)
+            for (int x = 0; x < kReach; ++x) {
+                const int at[3] = {x, y, z};
+                if ((planes >> at[a]) & 1u) out |= block_bit(x, y, z);
+            }
+    return out;
+}
+
+// --- triangles ---------------------------------------------------------------------------
+
 struct Triangle {
     V3 v0, e1, e2, normal;
     V3 lo, hi;  // bounding box
     int64_t material = 0;
 };
 
-// A segment's meeting with the faces: the nearest firm crossing and whether any meeting
-// lies within the margins.
+// A segment's meeting with the faces: the nearest firm crossing.
 struct Hit {
     double t = -1.0;
     int64_t face = -1;
 };
+
+// What a wall contact meets: a material's face (>= 0), the grid's edge, or nothing (a link
+// that only grazes a face's edge counts as rigid, fdtd.hpp:592-593).
+constexpr int32_t kEdgeWall = -1, kNoWall = -2;
 
 class Setup {
 public:
@@ -134,32 +396,19 @@ public:
     size_t layer = 0;
     double layer_damping = 0;
     double edge_admittance = 1.0;
-    size_t stencil = 7;
-    int directions = 6;
+    size_t points = 7;
     int threads = 1;
     double guard = 0;  // kGuard: distance from a plane within which an end counts as on it
 
-    // stencil
-    std::array<double, 18> exact{}, boundary{};
-    std::array<float, 18> weight{};
-    std::array<ptrdiff_t, 18> offset{};
-    std::array<int, 18> bit{};
-    float centre = 0, total = 0;
+    Stencil st;
+    std::array<float, 18> weights{};  // the stencil's lowered weights, as the layout hands them out
+    Branches branches;
 
     // layout
     std::vector<float> scale, layer_scale;
     std::vector<uint8_t> type;
     std::vector<uint16_t> cut;
-    std::vector<int64_t> special;
-    std::vector<uint64_t> link_start{0};
-    std::vector<uint8_t> link_direction;
-    std::vector<float> link_weight;
-    std::vector<float> coef_a, coef_b, coef_inv, source;
-    std::vector<uint64_t> state_start{0};
-    std::vector<int32_t> state_branch;
-    std::vector<float> state_weight;
-    std::vector<float> branch_now, branch_before, branch_gain;
-    std::vector<double> branch_a;
+    SpecialList specials;
     std::vector<int64_t> layer_cells;
     std::array<std::vector<float>, 6> keep, push;
     std::vector<uint64_t> probe_start{0}, probe_cells;
@@ -187,31 +436,48 @@ public:
         return centre_at(static_cast<long long>(i % nx), static_cast<long long>((i / nx) % ny),
                          static_cast<long long>(i / sz));
     }
-    size_t neighbour(size_t i, int k) const { return static_cast<size_t>(static_cast<ptrdiff_t>(i) + offset[k]); }
+    size_t neighbour(size_t i, int k) const {
+        return static_cast<size_t>(static_cast<ptrdiff_t>(i) + st.d[static_cast<size_t>(k)].offset);
+    }
     bool edge(size_t i) const {
         const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
         return x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz;
     }
     double cell_scale(size_t i) const { return scale.empty() ? 1.0 : static_cast<double>(scale[i]); }
+    // Whether a face cuts the link from i along k (k < st.count): the bit of the link's end
+    // whose direction owns it.
     bool is_cut(size_t i, int k) const {
-        if (bit[k] >= 0) return (cut[i] >> bit[k]) & 1u;
-        return (cut[neighbour(i, k)] >> bit[opposite(k)]) & 1u;
+        const Direction &d = st.d[static_cast<size_t>(k)];
+        if (d.plus_bit >= 0) return (cut[i] >> d.plus_bit) & 1u;
+        return (cut[neighbour(i, k)] >> st.d[static_cast<size_t>(d.opposite)].plus_bit) & 1u;
     }
-    bool blocked(size_t i, int k) const { return edge(neighbour(i, k)) || is_cut(i, k); }
-    // edge(neighbour(i, k)) for cell i at (x, y, z), without dividing. A step past the
-    // grid lands, in neighbour()'s indexing, on a cell of the far edge: on the edge too.
+    // Whether the neighbour of the cell at (x, y, z) along k lies on the grid's edge (a step
+    // past the grid counts as on it).
     bool neighbour_on_edge(size_t x, size_t y, size_t z, int k) const {
-        const long long to[3] = {static_cast<long long>(x) + kSteps[k][0], static_cast<long long>(y) + kSteps[k][1],
-                                 static_cast<long long>(z) + kSteps[k][2]};
+        const int *s = st.d[static_cast<size_t>(k)].step;
+        const long long to[3] = {static_cast<long long>(x) + s[0], static_cast<long long>(y) + s[1],
+                                 static_cast<long long>(z) + s[2]};
         const long long size[3] = {static_cast<long long>(nx), static_cast<long long>(ny), static_cast<long long>(nz)};
         for (int a = 0; a < 3; ++a)
             if (to[a] <= 0 || to[a] >= size[a] - 1) return true;
         return false;
     }
-    // blocked(i, k) for cell i at (x, y, z), without dividing.
     bool blocked_at(size_t i, int k, size_t x, size_t y, size_t z) const {
         return neighbour_on_edge(x, y, z, k) || is_cut(i, k);
     }
+    LinkMask links_at(size_t i, size_t x, size_t y, size_t z) const {
+        LinkMask m;
+        for (int k = 0; k < st.count; ++k) {
+            if (neighbour_on_edge(x, y, z, k)) {
+                m.edge_far |= 1u << k;
+                m.blocked |= 1u << k;
+            } else if (is_cut(i, k)) {
+                m.blocked |= 1u << k;
+            }
+        }
+        return m;
+    }
+    LinkMask links_of(size_t i) const { return links_at(i, i % nx, (i / nx) % ny, i / sz); }
 
     // Runs work(t, z_lo, z_hi) for slabs of planes t = 0 .. T - 1 on T threads (T returned
     // by slabs()); the first slab's exception, by order of slabs, is thrown again.
@@ -364,27 +630,6 @@ public:
 
     // --- the grid ------------------------------------------------------------------
 
-    void set_up_stencil() {
-        directions = stencil == 7 ? 6 : 18;
-        const double axial = stencil == 7 ? 1.0 / 3.0 : 0.25;
-        const double diagonal = stencil == 7 ? 0.0 : 0.125;
-        const double courant = std::sqrt(axial + 4.0 * diagonal);
-        int bits = 0;
-        for (int k = 0; k < 18; ++k) {
-            exact[k] = k < directions ? (k < 6 ? axial : diagonal) : 0.0;
-            weight[k] = lowered(exact[k]);
-            boundary[k] = exact[k] / (2.0 * courant);
-            offset[k] = kSteps[k][0] + kSteps[k][1] * static_cast<ptrdiff_t>(sy) + kSteps[k][2] * static_cast<ptrdiff_t>(sz);
-            const int *d = kSteps[k];
-            const bool plus = d[2] > 0 || (d[2] == 0 && (d[1] > 0 || (d[1] == 0 && d[0] > 0)));
-            bit[k] = k < directions && plus ? bits++ : -1;
-        }
-        double sum = 0.0;
-        for (int k = 0; k < directions; ++k) sum += weight[k];
-        centre = static_cast<float>(2.0 - sum);
-        total = static_cast<float>(sum);
-    }
-
     void make_bins() {
         bx = (nx + bin - 1) / bin;
         by = (ny + bin - 1) / bin;
@@ -465,18 +710,18 @@ public:
                                 const size_t i = x + y * sy + z * sz;
                                 const V3 c = centre_at(static_cast<long long>(x), static_cast<long long>(y),
                                                        static_cast<long long>(z));
-                                for (int k = 0; k < directions; ++k) {
-                                    if (bit[k] < 0) continue;
-                                    const long long tx = static_cast<long long>(x) + kSteps[k][0];
-                                    const long long ty = static_cast<long long>(y) + kSteps[k][1];
-                                    const long long tz = static_cast<long long>(z) + kSteps[k][2];
+                                for (int k = 0; k < st.count; ++k) {
+                                    if (st.d[static_cast<size_t>(k)].plus_bit < 0) continue;
+                                    const long long tx = static_cast<long long>(x) + st.d[static_cast<size_t>(k)].step[0];
+                                    const long long ty = static_cast<long long>(y) + st.d[static_cast<size_t>(k)].step[1];
+                                    const long long tz = static_cast<long long>(z) + st.d[static_cast<size_t>(k)].step[2];
                                     if (tx < 0 || ty < 0 || tz < 0 || tx >= static_cast<long long>(nx) ||
                                         ty >= static_cast<long long>(ny) || tz >= static_cast<long long>(nz))
                                         continue;
                                     double frac = 0;
                                     const int r = meet(f, c, centre_at(tx, ty, tz), frac);
                                     if (r == 1) {
-                                        cut[i] = static_cast<uint16_t>(cut[i] | (1u << bit[k]));
+                                        cut[i] = static_cast<uint16_t>(cut[i] | (1u << st.d[static_cast<size_t>(k)].plus_bit));
                                     } else if (r == 2) {
                                         unsure[static_cast<size_t>(t)].push_back({i, k});
                                     }
@@ -497,145 +742,127 @@ public:
         // refused, unless another face cuts it firmly.
         for (const auto &list : unsure)
             for (const auto &[i, k] : list)
-                if (!((cut[i] >> bit[k]) & 1u)) throw Degenerate("degenerate geometry: a link passes through a face's edge");
+                if (!((cut[i] >> st.d[static_cast<size_t>(k)].plus_bit) & 1u)) throw Degenerate("degenerate geometry: a link passes through a face's edge");
     }
 
-    // The cells round p (up to eight) whose centres p sees, with trilinear weights that sum
-    // to one; only cells keep(i) holds (seen_cells, fdtd.hpp:3305-3365, for p on no face).
-    template <typename Keep>
-    std::vector<std::pair<size_t, double>> seen_cells(const V3 &p, Keep keep) const {
+    // --- readings ----------------------------------------------------------------------
+
+    // The cells round p (up to eight) whose centres p sees, weighed trilinearly and scaled
+    // to sum to one; only cells `admit` takes (seen_cells, fdtd.hpp:3305-3365, for p on no
+    // face). Empty when p lies within a cell of the grid's edge or what is seen weighs
+    // under 1e-9.
+    template <typename Admit>
+    std::vector<std::pair<size_t, double>> trilinear(const V3 &p, Admit admit) const {
         std::vector<std::pair<size_t, double>> out;
-        const double q[3] = {(p.x - origin.x) / cell - 0.5, (p.y - origin.y) / cell - 0.5, (p.z - origin.z) / cell - 0.5};
-        const double size[3] = {static_cast<double>(nx), static_cast<double>(ny), static_cast<double>(nz)};
-        long long base[3];
-        double frac[3];
+        const double at[3] = {(p.x - origin.x) / cell - 0.5, (p.y - origin.y) / cell - 0.5,
+                              (p.z - origin.z) / cell - 0.5};
+        const size_t size[3] = {nx, ny, nz};
+        size_t low[3];
+        double side[3][2];  // per axis: the weights of the lower and upper cell
         for (int a = 0; a < 3; ++a) {
-            if (!(q[a] >= 1.0 && q[a] <= size[a] - 3.0)) return out;
-            base[a] = static_cast<long long>(std::floor(q[a]));
-            frac[a] = q[a] - static_cast<double>(base[a]);
+            if (!(at[a] >= 1.0 && at[a] <= static_cast<double>(size[a]) - 3.0)) return out;
+            const double f = std::floor(at[a]);
+            low[a] = static_cast<size_t>(f);
+            side[a][1] = at[a] - f;
+            side[a][0] = 1.0 - side[a][1];
         }
-        std::pair<size_t, double> seen[8];
-        int count = 0;
-        for (int corner = 0; corner < 8; ++corner) {
-            double w = 1.0;
-            size_t i = 0, stride = 1;
-            for (int a = 0; a < 3; ++a) {
-                const int up = (corner >> a) & 1;
-                w *= up ? frac[a] : 1.0 - frac[a];
-                i += static_cast<size_t>(base[a] + up) * stride;
-                stride *= a == 0 ? nx : (a == 1 ? ny : 1);
-            }
+        double total = 0.0;
+        for (unsigned corner = 0; corner < 8; ++corner) {
+            const unsigned ux = corner & 1u, uy = (corner >> 1) & 1u, uz = (corner >> 2) & 1u;
+            const double w = side[0][ux] * side[1][uy] * side[2][uz];
             if (w <= 0.0) continue;
-            if (!keep(i)) continue;
-            if (!sees(p, centre_of(i))) continue;
-            seen[count++] = {i, w};
+            const size_t i = (low[0] + ux) + (low[1] + uy) * sy + (low[2] + uz) * sz;
+            if (!admit(i) || !sees(p, centre_of(i))) continue;
+            out.emplace_back(i, w);
+            total += w;
         }
-        double sum = 0.0;
-        for (int k = 0; k < count; ++k) {
-            out.push_back(seen[k]);
-            sum += seen[k].second;
-        }
-        if (sum < 1e-9) {
+        if (total < 1e-9) {
             out.clear();
             return out;
         }
-        for (auto &cw : out) cw.second /= sum;
+        for (auto &cw : out) cw.second /= total;
         return out;
     }
 
-    std::vector<std::pair<size_t, double>> visible_cells(const V3 &p, bool air_only) const {
-        return seen_cells(p, [&](size_t i) { return !air_only || type[i] != kSolid; });
+    std::vector<std::pair<size_t, double>> visible(const V3 &p, bool air_only) const {
+        return trilinear(p, [&](size_t i) { return !air_only || type[i] != kSolid; });
     }
 
-    // The cubic reading of p (reading_cells, fdtd.hpp:2361-2488), for p on no face.
-    std::vector<std::pair<size_t, double>> reading_cells(const V3 &p, bool cubic) const {
-        if (!cubic) return visible_cells(p, true);
-        const double q[3] = {(p.x - origin.x) / cell - 0.5, (p.y - origin.y) / cell - 0.5, (p.z - origin.z) / cell - 0.5};
+    // The reading of p (reading_cells, fdtd.hpp:2361-2488, for p on no face). Trilinear
+    // where `cubic` is false. Otherwise the 4 x 4 x 4 block round p: its cells that are air
+    // and in sight of p; the planes of each axis kept while each holds such a cell among
+    // the other axes' kept planes (the largest such set); per axis, cubic weights where all
+    // four planes are kept, linear where the middle two are, 1 where one of them is; every
+    // cell of the box so chosen must be in sight and linked to its next ones in the box.
+    // Trilinear wherever any of that fails.
+    std::vector<std::pair<size_t, double>> reading(const V3 &p, bool cubic) const {
+        if (!cubic) return visible(p, true);
+        const double at[3] = {(p.x - origin.x) / cell - 0.5, (p.y - origin.y) / cell - 0.5,
+                              (p.z - origin.z) / cell - 0.5};
         const size_t size[3] = {nx, ny, nz};
-        long long below[3], start[3];
-        double w3[3][kReach];
+        long long first[3];
+        double below_frac[3], w[3][kReach];
         for (int a = 0; a < 3; ++a) {
-            below[a] = static_cast<long long>(std::floor(q[a]));
-            if (!(q[a] >= 1.0 && q[a] < static_cast<double>(size[a]) - 3.0 && below[a] >= 1)) {
-                return visible_cells(p, true);
-            }
-            start[a] = below[a] - (kReach / 2 - 1);
-            const double x = q[a] - static_cast<double>(start[a]);
-            for (int j = 0; j < kReach; ++j) {
-                double w = 1.0;
-                for (int m = 0; m < kReach; ++m) {
-                    if (m != j) w *= (x - m) / static_cast<double>(j - m);
-                }
-                w3[a][j] = w;
-            }
+            const double f = std::floor(at[a]);
+            const auto below = static_cast<long long>(f);
+            if (!(at[a] >= 1.0 && at[a] < static_cast<double>(size[a]) - 3.0 && below >= 1)) return visible(p, true);
+            first[a] = below - 1;
+            below_frac[a] = at[a] - f;
+            const double x = at[a] - static_cast<double>(first[a]);
+            for (int j = 0; j < kReach; ++j) w[a][j] = lagrange4(x, j);
         }
-        auto index = [&](int x, int y, int z) {
-            return static_cast<size_t>(start[0] + x) + static_cast<size_t>(start[1] + y) * sy +
-                   static_cast<size_t>(start[2] + z) * sz;
+        const auto index = [&](int x, int y, int z) {
+            return static_cast<size_t>(first[0] + x) + static_cast<size_t>(first[1] + y) * sy +
+                   static_cast<size_t>(first[2] + z) * sz;
         };
-        bool own[kReach][kReach][kReach];
+        uint64_t seen = 0;
         for (int z = 0; z < kReach; ++z)
             for (int y = 0; y < kReach; ++y)
                 for (int x = 0; x < kReach; ++x) {
                     const size_t i = index(x, y, z);
-                    bool s = type[i] == kInterior || type[i] == kSpecial;
-                    if (s) s = sees(p, centre_of(i));
-                    own[z][y][x] = s;
+                    if ((type[i] == kInterior || type[i] == kSpecial) && sees(p, centre_of(i))) seen |= block_bit(x, y, z);
                 }
-        bool keep_layer[3][kReach];
-        for (auto &axis : keep_layer) std::fill(std::begin(axis), std::end(axis), true);
-        for (bool changed = true; changed;) {
-            changed = false;
+        unsigned kept[3] = {0xFu, 0xFu, 0xFu};
+        for (bool shrunk = true; shrunk;) {
+            shrunk = false;
             for (int a = 0; a < 3; ++a) {
-                const int b = (a + 1) % 3, c = (a + 2) % 3;
+                const uint64_t across = seen & block_slab((a + 1) % 3, kept[(a + 1) % 3]) & block_slab((a + 2) % 3, kept[(a + 2) % 3]);
                 for (int j = 0; j < kReach; ++j) {
-                    if (!keep_layer[a][j]) continue;
-                    bool any = false;
-                    int at[3];
-                    at[a] = j;
-                    for (at[b] = 0; at[b] < kReach && !any; ++at[b]) {
-                        for (at[c] = 0; at[c] < kReach && !any; ++at[c]) {
-                            any = keep_layer[b][at[b]] && keep_layer[c][at[c]] && own[at[2]][at[1]][at[0]];
-                        }
-                    }
-                    if (!any) {
-                        keep_layer[a][j] = false;
-                        changed = true;
+                    if (((kept[a] >> j) & 1u) && !(across & block_slab(a, 1u << j))) {
+                        kept[a] &= ~(1u << j);
+                        shrunk = true;
                     }
                 }
             }
         }
-        int from[3], count[3];
+        int lo[3], hi[3];  // the box, per axis [lo, hi)
         for (int a = 0; a < 3; ++a) {
-            const int inner = kReach / 2 - 1;
-            if (keep_layer[a][0] && keep_layer[a][1] && keep_layer[a][2] && keep_layer[a][3]) {
-                from[a] = 0;
-                count[a] = kReach;
-            } else if (keep_layer[a][inner] && keep_layer[a][inner + 1]) {
-                from[a] = inner;
-                count[a] = 2;
-                const double t = q[a] - static_cast<double>(below[a]);
-                w3[a][inner] = 1.0 - t;
-                w3[a][inner + 1] = t;
-            } else if (keep_layer[a][inner] || keep_layer[a][inner + 1]) {
-                from[a] = keep_layer[a][inner] ? inner : inner + 1;
-                count[a] = 1;
-                w3[a][from[a]] = 1.0;
+            if (kept[a] == 0xFu) {
+                lo[a] = 0;
+                hi[a] = kReach;
+            } else if ((kept[a] & 0x6u) == 0x6u) {
+                lo[a] = 1;
+                hi[a] = 3;
+                w[a][1] = 1.0 - below_frac[a];
+                w[a][2] = below_frac[a];
+            } else if (kept[a] & 0x6u) {
+                lo[a] = (kept[a] & 0x2u) ? 1 : 2;
+                hi[a] = lo[a] + 1;
+                w[a][lo[a]] = 1.0;
             } else {
-                return visible_cells(p, true);
+                return visible(p, true);
             }
         }
         std::vector<std::pair<size_t, double>> out;
-        for (int z = from[2]; z < from[2] + count[2]; ++z)
-            for (int y = from[1]; y < from[1] + count[1]; ++y)
-                for (int x = from[0]; x < from[0] + count[0]; ++x) {
+        for (int z = lo[2]; z < hi[2]; ++z)
+            for (int y = lo[1]; y < hi[1]; ++y)
+                for (int x = lo[0]; x < hi[0]; ++x) {
                     const size_t i = index(x, y, z);
-                    if (!own[z][y][x] || (x + 1 < from[0] + count[0] && is_cut(i, 1)) ||
-                        (y + 1 < from[1] + count[1] && is_cut(i, 3)) || (z + 1 < from[2] + count[2] && is_cut(i, 5))) {
-                        return visible_cells(p, true);
-                    }
-                    const double w = w3[0][x] * w3[1][y] * w3[2][z];
-                    if (w != 0.0) out.emplace_back(i, w);
+                    const bool linked = !(x + 1 < hi[0] && is_cut(i, 1)) && !(y + 1 < hi[1] && is_cut(i, 3)) &&
+                                        !(z + 1 < hi[2] && is_cut(i, 5));
+                    if (!(seen & block_bit(x, y, z)) || !linked) return visible(p, true);
+                    const double wi = w[0][x] * w[1][y] * w[2][z];
+                    if (wi != 0.0) out.emplace_back(i, wi);
                 }
         return out;
     }
@@ -670,7 +897,7 @@ public:
             const Span sp = spans.back();
             spans.pop_back();
             const size_t y = sp.row % ny, z = sp.row / ny;
-            for (int k = 2; k < directions; ++k) {
+            for (int k = 2; k < st.count; ++k) {
                 for (size_t x = sp.x0; x <= sp.x1; ++x) {
                     const size_t i = sp.row * nx + x;
                     if (blocked_at(i, k, x, y, z)) continue;
@@ -683,347 +910,245 @@ public:
         }
     }
 
-    void raw_weights(size_t i, double *w) const {
-        for (int k = 0; k < 18; ++k) w[k] = 0.0;
-        for (int k = 0; k < directions; ++k) {
-            if (!blocked(i, k)) {
-                w[k] += exact[k];
-                continue;
-            }
-            if (k < 6 || edge(neighbour(i, k))) continue;
-            const auto [ka, kb] = steps_of(k);
-            const bool behind_a = blocked(i, ka), behind_b = blocked(i, kb);
-            if (behind_a && behind_b) continue;
-            if (behind_a) {
-                w[kb] += exact[k];
-            } else if (behind_b) {
-                w[ka] += exact[k];
-            } else {
-                w[ka] += 0.5 * exact[k];
-                w[kb] += 0.5 * exact[k];
-            }
-        }
-    }
+    // --- wall cells ----------------------------------------------------------------------
 
-    void layer_coefficients(int a) {
-        const size_t size = a == 0 ? nx : (a == 1 ? ny : nz);
-        const double lay = static_cast<double>(layer);
-        const double courant = std::sqrt(static_cast<double>(weight[0]));
-        auto damping = [&](double from_edge) {
-            const double into = std::max(0.0, lay + 0.5 - from_edge) / lay;
-            return layer_damping * into * into;
-        };
-        auto &ck = keep[2 * a], &fk = keep[2 * a + 1];
-        auto &cp = push[2 * a], &fp = push[2 * a + 1];
-        for (size_t x = 0; x <= size; ++x) {
-            const double c = static_cast<double>(x), last = static_cast<double>(size - 1);
-            const double dc = damping(std::min(c, last - c));
-            const double df = damping(std::min(c - 0.5, last - c + 0.5));
-            ck.push_back(static_cast<float>((1.0 - dc) / (1.0 + dc)));
-            cp.push_back(static_cast<float>(courant / (1.0 + dc)));
-            fk.push_back(static_cast<float>((1.0 - df) / (1.0 + df)));
-            fp.push_back(static_cast<float>(courant / (1.0 + df)));
-        }
-    }
-
-    // The specials of a slab of planes, as the layout's arrays (starts from 0).
-    struct Specials {
-        std::vector<int64_t> special;
-        std::vector<uint64_t> link_start{0};
-        std::vector<uint8_t> link_direction;
-        std::vector<float> link_weight;
-        std::vector<float> coef_a, coef_b, coef_inv, source;
-        std::vector<uint64_t> state_start{0};
-        std::vector<int32_t> state_branch;
-        std::vector<float> state_weight;
-        size_t air = 0;
-        bool open = false;
-        std::array<bool, 6> sides{};
+    // A contact: a wall a blocked link's boundary weight goes to, across the cell's link
+    // along `dir`.
+    struct Contact {
+        int dir;
+        double weight;
     };
 
-    // Special i's links (the weights w it has, lowered), coefficients and the start of its
-    // states (already added) into `out`, whose layout arrays it extends.
-    template <typename Out>
-    void put_special(Out &out, size_t i, const double *w, double g) const {
-        out.special.push_back(static_cast<int64_t>(i));
-        const double s = cell_scale(i), r = std::sqrt(s);
-        double sum = 0.0;
-        for (int k = 0; k < directions; ++k) {
-            if (!(w[k] > 0.0)) continue;
-            sum += w[k];
-            out.link_direction.push_back(static_cast<uint8_t>(k));
-            out.link_weight.push_back(lowered(s * w[k]));
+    // A wall cell's contacts in direction order (fdtd.hpp:602-614): a blocked axial link
+    // meets the wall across itself; a blocked diagonal meets the walls across those of its
+    // two axial sides that are blocked (the lower axis first), or, where neither is, the
+    // wall across itself; each contact carries the blocked link's boundary weight.
+    int contacts(const LinkMask &m, Contact *out) const {
+        int count = 0;
+        for (int k = 0; k < st.count; ++k) {
+            if (!((m.blocked >> k) & 1u)) continue;
+            const Direction &d = st.d[static_cast<size_t>(k)];
+            if (k < 6) {
+                out[count++] = {k, d.wall};
+                continue;
+            }
+            const bool a = (m.blocked >> d.side_a) & 1u, b = (m.blocked >> d.side_b) & 1u;
+            if (a) out[count++] = {d.side_a, d.wall};
+            if (b) out[count++] = {d.side_b, d.wall};
+            if (!a && !b) out[count++] = {k, d.wall};
         }
-        out.link_start.push_back(out.link_direction.size());
-        out.coef_a.push_back(static_cast<float>(2.0 - s * sum * (1.0 - kMargin)));
-        out.coef_b.push_back(static_cast<float>(1.0 - r * g));
-        out.coef_inv.push_back(static_cast<float>(1.0 / (1.0 + r * g)));
-        out.source.push_back(0.0f);
-        out.state_start.push_back(out.state_branch.size());
+        return count;
     }
 
-    size_t add_special(size_t i, const double *w, double g) {
-        type[i] = kSpecial;
-        put_special(*this, i, w, g);
-        return special.size() - 1;
+    // What cell i's contact along `dir` meets (fdtd.hpp:581-593): the grid's edge where the
+    // neighbour lies on it; else the material of the face nearest i that the link between
+    // the two centres crosses, looked for from i, then from the neighbour; else nothing.
+    int32_t wall_of(size_t i, int dir, Stamps &stamps) const {
+        const size_t j = neighbour(i, dir);
+        if (edge(j)) return kEdgeWall;
+        int64_t face = crossing(centre_of(i), centre_of(j), stamps);
+        if (face < 0) face = crossing(centre_of(j), centre_of(i), stamps);
+        if (face < 0) return kNoWall;
+        return static_cast<int32_t>(face_material[static_cast<size_t>(face)]);
     }
 
-    size_t special_index(size_t i) const {
-        if (type[i] != kSpecial) return npos;
-        const auto it = std::find(special.begin(), special.end(), static_cast<int64_t>(i));
-        return it == special.end() ? npos : static_cast<size_t>(it - special.begin());
-    }
-
-    void build(const double *branches, const int64_t *branch_material, size_t branch_count, const V3 &src,
-               const std::vector<V3> &probes, size_t per_listener, const float *speed) {
-        if (stencil != 7 && stencil != 19) throw std::invalid_argument("the stencil has 7 or 19 points");
-        if (per_listener == 0 || probes.size() % per_listener != 0) {
-            throw std::invalid_argument("the probes come in whole groups, one per listener");
-        }
-        if (layer > 0 && stencil != 7) throw std::invalid_argument("the matched layer needs the 7-point stencil");
-        if (nx < 3 || ny < 3 || nz < 3) throw std::invalid_argument("the grid is too small");
-        if (!(cell > 0.0)) throw std::invalid_argument("cell must be positive");
-        if (2 * layer + 3 > std::min({nx, ny, nz})) throw std::invalid_argument("the matched layer leaves no grid inside it");
-        if (!(layer_damping >= 0.0)) throw std::invalid_argument("negative layer damping");
-        for (size_t f = 0; f < face_material.size(); ++f) {
-            if (face_material[f] < 0 || static_cast<size_t>(face_material[f]) >= materials) {
-                throw std::invalid_argument("a face has no material");
-            }
-        }
-        std::vector<std::vector<int32_t>> material_branches(materials);
-        for (size_t m = 0; m < materials; ++m) {
-            if (!(beta[m] >= 0.0)) throw std::invalid_argument("negative admittance");
-        }
-        for (size_t j = 0; j < branch_count; ++j) {
-            const double m = branches[3 * j], r = branches[3 * j + 1], q = branches[3 * j + 2];
-            const double a = m + 0.5 * r + 0.5 * q;
-            if (!(m >= 0.0 && r >= 0.0 && q >= 0.0 && a > 0.0)) {
-                throw std::invalid_argument("a branch needs mass, resistance and stiffness >= 0");
-            }
-            if (branch_material[j] < 0 || static_cast<size_t>(branch_material[j]) >= materials) {
-                throw std::invalid_argument("a branch has no material");
-            }
-            material_branches[static_cast<size_t>(branch_material[j])].push_back(static_cast<int32_t>(j));
-            branch_now.push_back(static_cast<float>(2.0 * m / a));
-            branch_before.push_back(static_cast<float>((2.0 * m + q) / a));
-            branch_gain.push_back(static_cast<float>(0.5 / a));
-            branch_a.push_back(a);
-        }
-        set_up_stencil();
-        if (speed != nullptr) {
-            scale.resize(n);
-            for (size_t i = 0; i < n; ++i) {
-                const double r = static_cast<double>(speed[i]);
-                if (!(r > 0.0 && r <= 1.0)) {
-                    throw std::invalid_argument("each cell's speed of sound over the grid's lies in (0, 1]");
-                }
-                scale[i] = static_cast<float>(r * r);
-            }
-        }
-        SetupClock clock;
-        make_bins();
-        clock.mark("stencil, bins");
-        require_off_faces(src, "the source");
-        for (size_t p = 0; p < probes.size(); ++p) require_off_faces(probes[p], "a probe");
-        cut_links();
-        clock.mark("cut links");
-
-        // Air: the cells the source sees, and all they reach.
-        type.assign(n, kSolid);
-        std::vector<size_t> queue;
-        for (const auto &[i, w] : visible_cells(src, false)) {
-            (void)w;
-            if (type[i] == kSolid) {
-                type[i] = kInterior;
-                queue.push_back(i);
-            }
-        }
-        if (queue.empty()) throw std::invalid_argument("the source sees no cell of the wave grid");
-        flood(queue);
-        clock.mark("air");
-
-        // The matched layer's air, slab by slab of planes on the threads (each cell's class
-        // its own; the cells listed in order).
-        if (layer > 0) {
-            const size_t T = slabs();
-            std::vector<std::vector<int64_t>> found(T);
-            std::vector<std::array<bool, 6>> sides(T);
-            in_slabs(T, [&](size_t t, size_t z0, size_t z1) {
-                std::array<bool, 6> reach{};
-                const size_t size[3] = {nx, ny, nz};
-                for (size_t z = z0; z < z1; ++z)
-                    for (size_t y = 0; y < ny; ++y)
-                        for (size_t x = 0; x < nx; ++x) {
-                            const size_t i = x + y * sy + z * sz;
-                            if (type[i] == kSolid) continue;
-                            if (std::min({x, y, z, nx - 1 - x, ny - 1 - y, nz - 1 - z}) > layer) continue;
-                            const size_t at[3] = {x, y, z};
-                            for (int k = 0; k < 6; ++k) {
-                                if (!neighbour_on_edge(x, y, z, k) && is_cut(i, k)) {
-                                    throw std::invalid_argument("the model reaches into the matched layer");
-                                }
-                            }
-                            type[i] = kLayer;
-                            found[t].push_back(static_cast<int64_t>(i));
-                            for (size_t a = 0; a < 3; ++a) {
-                                if (at[a] <= layer) reach[2 * a] = true;
-                                if (at[a] + 1 + layer >= size[a]) reach[2 * a + 1] = true;
-                            }
-                        }
-                sides[t] = reach;
-            });
-            for (size_t t = 0; t < T; ++t) {
-                layer_cells.insert(layer_cells.end(), found[t].begin(), found[t].end());
-                air += found[t].size();
-                if (!found[t].empty()) open = true;
-                for (int k = 0; k < 6; ++k) open_sides[static_cast<size_t>(k)] = open_sides[static_cast<size_t>(k)] || sides[t][static_cast<size_t>(k)];
-            }
-        }
-        if (!layer_cells.empty()) {
-            for (int a = 0; a < 3; ++a) layer_coefficients(a);
-            for (size_t c = 0; c < layer_cells.size() && !scale.empty(); ++c) {
-                layer_scale.push_back(scale[static_cast<size_t>(layer_cells[c])]);
-            }
-        }
-
-        clock.mark("layer");
-        // Boundary cells: marks 1 and 2, then weights, materials and states. Each pass runs
-        // on the threads by slabs of planes; the second writes its marks apart (a cell can
-        // take mark 2 from several neighbours), and the specials of each slab are joined in
-        // the order of the slabs, so of the cells, as one pass in order would list them.
-        const size_t T = slabs();
+    // Boundary cells (fdtd.hpp:546-625), in stages over slabs of planes on the threads:
+    //  1. an interior cell with a blocked link is a wall cell (mark 1);
+    //  2. an interior cell that is not, next to a wall cell across a link whose folded
+    //     weight differs from the stencil's, is marked too (mark 2): averaging changes it;
+    //  3. each marked cell, in grid order: its link weights, each the mean of its folded
+    //     weight and its neighbour's towards it (the stencil's where the neighbour is
+    //     unmarked air or in the matched layer; 0 towards solid and edge cells); its wall
+    //     contacts, resolved; the wall term g, the sum of boundary weight times admittance
+    //     over the contacts in order, plus, per material met (in the order first met), its
+    //     contacts' weight times each of its branches' gain; one branch state per branch of
+    //     each material met. The slabs' lists are joined in slab order: specials ascend.
+    void wall_cells(size_t T) {
         std::vector<uint8_t> mark(n, 0);
         in_slabs(T, [&](size_t, size_t z0, size_t z1) {
             for (size_t z = z0; z < z1; ++z)
                 for (size_t y = 0; y < ny; ++y)
                     for (size_t x = 0; x < nx; ++x) {
                         const size_t i = x + y * sy + z * sz;
-                        if (type[i] != kInterior) continue;
-                        for (int k = 0; k < directions && !mark[i]; ++k) mark[i] = blocked_at(i, k, x, y, z);
+                        if (type[i] == kInterior && links_at(i, x, y, z).blocked) mark[i] = 1;
                     }
         });
-        std::vector<uint8_t> second(n, 0);
+        std::vector<uint8_t> reached(n, 0);
         in_slabs(T, [&](size_t, size_t z0, size_t z1) {
-            std::array<double, 18> wi{};
+            std::array<double, 18> w{};
             for (size_t i = z0 * sz; i < z1 * sz; ++i) {
                 if (mark[i] != 1) continue;
-                raw_weights(i, wi.data());
-                for (int k = 0; k < directions; ++k) {
+                fold(st, links_of(i), w.data());
+                for (int k = 0; k < st.count; ++k) {
                     const size_t j = neighbour(i, k);
-                    if (wi[k] != exact[k] && type[j] == kInterior && !mark[j]) {
-                        __atomic_store_n(&second[j], static_cast<uint8_t>(1), __ATOMIC_RELAXED);
+                    if (w[static_cast<size_t>(k)] != st.d[static_cast<size_t>(k)].exact && type[j] == kInterior &&
+                        !mark[j]) {
+                        __atomic_store_n(&reached[j], static_cast<uint8_t>(1), __ATOMIC_RELAXED);
                     }
                 }
             }
         });
         for (size_t i = 0; i < n; ++i)
-            if (second[i]) mark[i] = 2;
-        std::vector<Specials> parts(T);
+            if (reached[i]) mark[i] = 2;
+
+        struct Part {
+            SpecialList list;
+            size_t air = 0;
+            bool open = false;
+            std::array<bool, 6> sides{};
+        };
+        std::vector<Part> parts(T);
         in_slabs(T, [&](size_t t, size_t z0, size_t z1) {
-          Specials &out = parts[t];
-          Stamps stamps;
-          std::array<double, 18> wi{}, wj{};
-          for (size_t i = z0 * sz; i < z1 * sz; ++i) {
-            if (type[i] != kInterior) continue;
-            ++out.air;
-            if (!mark[i]) continue;
-            raw_weights(i, wi.data());
-            std::array<double, 18> w{};
-            double beta_sum = 0.0;
-            std::array<int32_t, 18> crossed{};
-            std::array<double, 18> crossed_weight{};
-            size_t kinds = 0;
-            auto meet_face = [&](int k, double bw) {
-                const size_t j = neighbour(i, k);
-                if (edge(j)) {
-                    beta_sum += bw * edge_admittance;
-                    out.open = true;
-                    const size_t at[3] = {j % nx, (j / nx) % ny, j / sz};
-                    const size_t size[3] = {nx, ny, nz};
-                    for (size_t a = 0; a < 3; ++a) {
-                        if (at[a] == 0) out.sides[2 * a] = true;
-                        if (at[a] + 1 >= size[a]) out.sides[2 * a + 1] = true;
+            Part &part = parts[t];
+            Stamps stamps;
+            std::array<double, 18> mine{}, theirs{}, weight{};
+            std::array<Contact, 36> touch{};
+            std::array<int32_t, 36> met{};        // materials met, in the order first met
+            std::array<double, 36> met_weight{};  // and their contacts' weights
+            std::vector<SpecialList::State> states;
+            for (size_t i = z0 * sz; i < z1 * sz; ++i) {
+                if (type[i] != kInterior) continue;
+                ++part.air;
+                if (!mark[i]) continue;
+                const LinkMask m = links_of(i);
+                fold(st, m, mine.data());
+                weight.fill(0.0);
+                for (int k = 0; k < st.count; ++k) {
+                    const auto ku = static_cast<size_t>(k);
+                    const size_t j = neighbour(i, k);
+                    if (((m.edge_far >> k) & 1u) || type[j] == kSolid) continue;
+                    double toward = st.d[ku].exact;
+                    if (type[j] == kInterior && mark[j]) {
+                        fold(st, links_of(j), theirs.data());
+                        toward = theirs[static_cast<size_t>(st.d[ku].opposite)];
                     }
-                    return;
+                    weight[ku] = 0.5 * (mine[ku] + toward);
                 }
-                int64_t face = crossing(centre_of(i), centre_of(j), stamps);
-                if (face < 0) face = crossing(centre_of(j), centre_of(i), stamps);
-                if (face < 0) return;
-                const auto m = static_cast<int32_t>(face_material[static_cast<size_t>(face)]);
-                beta_sum += bw * beta[static_cast<size_t>(m)];
-                size_t kind = 0;
-                while (kind < kinds && crossed[kind] != m) ++kind;
-                if (kind == kinds) crossed[kinds++] = m;
-                crossed_weight[kind] += bw;
-            };
-            for (int k = 0; k < directions; ++k) {
-                const size_t j = neighbour(i, k);
-                if (!edge(j) && type[j] != kSolid && type[j] != kLayer && mark[j]) {
-                    raw_weights(j, wj.data());
-                    w[k] = 0.5 * (wi[k] + wj[opposite(k)]);
-                } else if (!edge(j) && type[j] != kSolid) {
-                    w[k] = 0.5 * (wi[k] + exact[k]);
+                const int count = contacts(m, touch.data());
+                double g = 0.0;
+                size_t kinds = 0;
+                for (int c = 0; c < count; ++c) {
+                    const Contact &ct = touch[static_cast<size_t>(c)];
+                    const int32_t wall = wall_of(i, ct.dir, stamps);
+                    if (wall == kNoWall) continue;
+                    if (wall == kEdgeWall) {
+                        g += ct.weight * edge_admittance;
+                        part.open = true;
+                        const size_t j = neighbour(i, ct.dir);
+                        const size_t at[3] = {j % nx, (j / nx) % ny, j / sz}, size[3] = {nx, ny, nz};
+                        for (size_t a = 0; a < 3; ++a) {
+                            if (at[a] == 0) part.sides[2 * a] = true;
+                            if (at[a] + 1 >= size[a]) part.sides[2 * a + 1] = true;
+                        }
+                        continue;
+                    }
+                    g += ct.weight * beta[static_cast<size_t>(wall)];
+                    size_t kind = 0;
+                    while (kind < kinds && met[kind] != wall) ++kind;
+                    if (kind == kinds) {
+                        met[kinds] = wall;
+                        met_weight[kinds++] = 0.0;
+                    }
+                    met_weight[kind] += ct.weight;
                 }
-                if (!blocked(i, k)) continue;
-                if (k < 6) {
-                    meet_face(k, boundary[k]);
-                    continue;
+                states.clear();
+                const double r = std::sqrt(cell_scale(i));
+                for (size_t kind = 0; kind < kinds; ++kind) {
+                    for (const int32_t jb : branches.of_material[static_cast<size_t>(met[kind])]) {
+                        g += met_weight[kind] * (0.5 / branches.a[static_cast<size_t>(jb)]);
+                        states.push_back({jb, static_cast<float>(r * met_weight[kind])});
+                    }
                 }
-                const auto [ka, kb] = steps_of(k);
-                const bool behind_a = blocked(i, ka), behind_b = blocked(i, kb);
-                if (behind_a) meet_face(ka, boundary[k]);
-                if (behind_b) meet_face(kb, boundary[k]);
-                if (!behind_a && !behind_b) meet_face(k, boundary[k]);
+                part.list.append(i, weight.data(), st.count, cell_scale(i), g, states.data(), states.size());
             }
-            double g = beta_sum;
-            const double r = std::sqrt(cell_scale(i));
-            for (size_t kind = 0; kind < kinds; ++kind) {
-                for (const int32_t jb : material_branches[static_cast<size_t>(crossed[kind])]) {
-                    g += crossed_weight[kind] * (0.5 / branch_a[static_cast<size_t>(jb)]);
-                    out.state_branch.push_back(jb);
-                    out.state_weight.push_back(static_cast<float>(r * crossed_weight[kind]));
-                }
-            }
-            put_special(out, i, w.data(), g);
-          }
         });
-        for (const Specials &part : parts) {
-            const uint64_t links0 = link_direction.size(), states0 = state_branch.size();
-            for (const int64_t i : part.special) type[static_cast<size_t>(i)] = kSpecial;
-            special.insert(special.end(), part.special.begin(), part.special.end());
-            for (size_t q = 1; q < part.link_start.size(); ++q) link_start.push_back(links0 + part.link_start[q]);
-            for (size_t q = 1; q < part.state_start.size(); ++q) state_start.push_back(states0 + part.state_start[q]);
-            link_direction.insert(link_direction.end(), part.link_direction.begin(), part.link_direction.end());
-            link_weight.insert(link_weight.end(), part.link_weight.begin(), part.link_weight.end());
-            coef_a.insert(coef_a.end(), part.coef_a.begin(), part.coef_a.end());
-            coef_b.insert(coef_b.end(), part.coef_b.begin(), part.coef_b.end());
-            coef_inv.insert(coef_inv.end(), part.coef_inv.begin(), part.coef_inv.end());
-            source.insert(source.end(), part.source.begin(), part.source.end());
-            state_branch.insert(state_branch.end(), part.state_branch.begin(), part.state_branch.end());
-            state_weight.insert(state_weight.end(), part.state_weight.begin(), part.state_weight.end());
+        for (const Part &part : parts) {
+            for (const int64_t i : part.list.cell) type[static_cast<size_t>(i)] = kSpecial;
+            specials.extend(part.list);
             air += part.air;
             open = open || part.open;
-            for (int k = 0; k < 6; ++k) open_sides[static_cast<size_t>(k)] = open_sides[static_cast<size_t>(k)] || part.sides[static_cast<size_t>(k)];
+            for (size_t k = 0; k < 6; ++k) open_sides[k] = open_sides[k] || part.sides[k];
         }
-        boundary_cells = special.size();
-        clock.mark("marks, specials");
+        specials.sorted = specials.size();
+        boundary_cells = specials.size();
+    }
 
-        // The source's cells.
-        for (const auto &[i, w] : visible_cells(src, true)) {
-            (void)w;
-            if (type[i] == kLayer) throw std::invalid_argument("the source is in the matched layer");
+    // The matched layer: every air cell within `layer` cells of the grid's edge, which no
+    // cut link may reach (fdtd.hpp:521-542), slab by slab on the threads (each cell's class
+    // its own; the cells listed in grid order), and its coefficients.
+    void matched_layer(size_t T) {
+        std::vector<std::vector<int64_t>> found(T);
+        std::vector<std::array<bool, 6>> sides(T);
+        in_slabs(T, [&](size_t t, size_t z0, size_t z1) {
+            std::array<bool, 6> reach{};
+            const size_t size[3] = {nx, ny, nz};
+            for (size_t z = z0; z < z1; ++z)
+                for (size_t y = 0; y < ny; ++y)
+                    for (size_t x = 0; x < nx; ++x) {
+                        const size_t i = x + y * sy + z * sz;
+                        if (type[i] == kSolid) continue;
+                        if (std::min({x, y, z, nx - 1 - x, ny - 1 - y, nz - 1 - z}) > layer) continue;
+                        for (int k = 0; k < 6; ++k) {
+                            if (!neighbour_on_edge(x, y, z, k) && is_cut(i, k)) {
+                                throw std::invalid_argument("the model reaches into the matched layer: a face cuts a "
+                                                            "link between two of its cells");
+                            }
+                        }
+                        type[i] = kLayer;
+                        found[t].push_back(static_cast<int64_t>(i));
+                        const size_t at[3] = {x, y, z};
+                        for (size_t a = 0; a < 3; ++a) {
+                            if (at[a] <= layer) reach[2 * a] = true;
+                            if (at[a] + 1 + layer >= size[a]) reach[2 * a + 1] = true;
+                        }
+                    }
+            sides[t] = reach;
+        });
+        for (size_t t = 0; t < T; ++t) {
+            layer_cells.insert(layer_cells.end(), found[t].begin(), found[t].end());
+            air += found[t].size();
+            if (!found[t].empty()) open = true;
+            for (size_t k = 0; k < 6; ++k) open_sides[k] = open_sides[k] || sides[t][k];
         }
-        for (const auto &[i, w] : reading_cells(src, true)) {
-            size_t k = special_index(i);
-            if (k == npos) k = add_special(i, exact.data(), 0.0);
-            source[k] += static_cast<float>(cell_scale(i) * w);
+        if (layer_cells.empty()) return;
+        const double courant = std::sqrt(static_cast<double>(st.d[0].weight));
+        const size_t size[3] = {nx, ny, nz};
+        for (size_t a = 0; a < 3; ++a) {
+            LayerAxis ax = layer_axis(size[a], layer, layer_damping, courant);
+            keep[2 * a] = std::move(ax.keep_centre);
+            keep[2 * a + 1] = std::move(ax.keep_face);
+            push[2 * a] = std::move(ax.push_centre);
+            push[2 * a + 1] = std::move(ax.push_face);
         }
-        // The probes' cells.
-        for (size_t k = 0; k < probes.size(); ++k) {
-            const V3 &p = probes[k];
-            const V3 d = p - src;
-            const bool near = norm(d) < (kNear - 1e-9) * cell;
-            const auto cells = reading_cells(p, !near);
+        if (!scale.empty())
+            for (const int64_t c : layer_cells) layer_scale.push_back(scale[static_cast<size_t>(c)]);
+    }
+
+    // The source's cells (fdtd.hpp:627-638): none of the cells round it in the matched
+    // layer; each cell of its reading made special (with the stencil's weights where it was
+    // interior) and fed s w, s its speed factor.
+    void place_source(const V3 &src) {
+        for (const auto &cw : visible(src, true))
+            if (type[cw.first] == kLayer) throw std::invalid_argument("the source lies in the matched layer");
+        std::array<double, 18> exact{};
+        for (int k = 0; k < 18; ++k) exact[static_cast<size_t>(k)] = st.d[static_cast<size_t>(k)].exact;
+        for (const auto &[i, w] : reading(src, true)) {
+            size_t k = specials.index_of(i);
+            if (k == npos) {
+                type[i] = kSpecial;
+                specials.append(i, exact.data(), st.count, cell_scale(i), 0.0, nullptr, 0);
+                k = specials.size() - 1;
+            }
+            specials.source[k] += static_cast<float>(cell_scale(i) * w);
+        }
+    }
+
+    // The probes' readings (fdtd.hpp:639-690): trilinear within three cells of the source,
+    // cubic elsewhere; whether each probe's listener (the first of its group) sees it.
+    void place_probes(const std::vector<V3> &probes, const V3 &src, size_t per_listener) {
+        for (const V3 &p : probes) {
+            const bool near = norm(p - src) < (kNear - 1e-9) * cell;
+            const auto cells = reading(p, !near);
             for (const auto &[i, w] : cells) {
                 probe_cells.push_back(i);
                 probe_weights.push_back(w);
@@ -1032,9 +1157,81 @@ public:
             probe_found.push_back(cells.empty() ? 0 : 1);
         }
         for (size_t k = 0; k < probes.size(); ++k) {
-            const size_t l = k - k % per_listener;
-            probe_in_sight.push_back(k == l || sees(probes[l], probes[k]) ? 1 : 0);
+            const size_t listener = k - k % per_listener;
+            probe_in_sight.push_back(k == listener || sees(probes[listener], probes[k]) ? 1 : 0);
         }
+    }
+
+    void check_arguments(size_t probe_count, size_t per_listener) const {
+        if (points != 7 && points != 19) {
+            throw std::invalid_argument("a stencil of 7 or 19 points is needed, not " + std::to_string(points));
+        }
+        if (per_listener == 0 || probe_count % per_listener != 0) {
+            throw std::invalid_argument("the probes must come in whole groups of " + std::to_string(per_listener) +
+                                        " (one group a listener)");
+        }
+        if (layer > 0 && points != 7) throw std::invalid_argument("a matched layer works with the 7-point stencil only");
+        if (nx < 3 || ny < 3 || nz < 3) throw std::invalid_argument("the grid needs at least 3 cells along each axis");
+        if (!(cell > 0.0)) throw std::invalid_argument("the cell size must be positive");
+        if (2 * layer + 3 > std::min({nx, ny, nz})) {
+            throw std::invalid_argument("the matched layer is too thick: it leaves no cells of the grid inside it");
+        }
+        if (!(layer_damping >= 0.0)) throw std::invalid_argument("the matched layer's damping must not be negative");
+        for (size_t f = 0; f < face_material.size(); ++f) {
+            if (face_material[f] < 0 || static_cast<size_t>(face_material[f]) >= materials) {
+                throw std::invalid_argument("face " + std::to_string(f) + " has no material");
+            }
+        }
+        for (size_t m = 0; m < materials; ++m) {
+            if (!(beta[m] >= 0.0)) {
+                throw std::invalid_argument("material " + std::to_string(m) + " has a negative admittance");
+            }
+        }
+    }
+
+    void build(const double *mrq, const int64_t *branch_material, size_t branch_count, const V3 &src,
+               const std::vector<V3> &probes, size_t per_listener, const float *speed) {
+        check_arguments(probes.size(), per_listener);
+        branches = make_branches(mrq, branch_material, branch_count, materials);
+        st = make_stencil(points, static_cast<ptrdiff_t>(sy), static_cast<ptrdiff_t>(sz));
+        for (size_t k = 0; k < 18; ++k) weights[k] = st.d[k].weight;
+        if (speed != nullptr) {
+            scale.resize(n);
+            for (size_t i = 0; i < n; ++i) {
+                const double r = static_cast<double>(speed[i]);
+                if (!(r > 0.0 && r <= 1.0)) {
+                    throw std::invalid_argument("cell " + std::to_string(i) +
+                                                ": its speed of sound over the grid's must lie in (0, 1]");
+                }
+                scale[i] = static_cast<float>(r * r);
+            }
+        }
+        SetupClock clock;
+        make_bins();
+        clock.mark("stencil, bins");
+        require_off_faces(src, "the source");
+        for (const V3 &p : probes) require_off_faces(p, "a probe");
+        cut_links();
+        clock.mark("cut links");
+
+        // Air: the cells the source sees, and all they reach (fdtd.hpp:498-517).
+        type.assign(n, kSolid);
+        std::vector<size_t> queue;
+        for (const auto &cw : visible(src, false)) {
+            if (type[cw.first] != kSolid) continue;
+            type[cw.first] = kInterior;
+            queue.push_back(cw.first);
+        }
+        if (queue.empty()) throw std::invalid_argument("no cell centre of the wave grid is in sight of the source");
+        flood(queue);
+        clock.mark("air");
+        const size_t T = slabs();
+        if (layer > 0) matched_layer(T);
+        clock.mark("layer");
+        wall_cells(T);
+        clock.mark("marks, specials");
+        place_source(src);
+        place_probes(probes, src, per_listener);
         clock.mark("source, probes");
     }
 };
@@ -1069,9 +1266,6 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
         s.materials = materials;
         s.beta.assign(beta, beta + materials);
         s.face_material.assign(face_material, face_material + nf);
-        V3 lo{std::numeric_limits<double>::infinity(), std::numeric_limits<double>::infinity(),
-              std::numeric_limits<double>::infinity()};
-        V3 hi{-lo.x, -lo.y, -lo.z};
         for (size_t f = 0; f < nf; ++f) {
             V3 p[3];
             for (int k = 0; k < 3; ++k) {
@@ -1085,15 +1279,13 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
             t.e2 = p[2] - p[0];
             const V3 c = cross(t.e1, t.e2);
             const double len = norm(c);
-            t.normal = len > 0.0 ? c * (1.0 / len) : V3{};
+            if (!(len > 0.0)) throw std::invalid_argument("a face of the mesh has no area");
+            t.normal = c * (1.0 / len);
             t.lo = {std::min({p[0].x, p[1].x, p[2].x}), std::min({p[0].y, p[1].y, p[2].y}),
                     std::min({p[0].z, p[1].z, p[2].z})};
             t.hi = {std::max({p[0].x, p[1].x, p[2].x}), std::max({p[0].y, p[1].y, p[2].y}),
                     std::max({p[0].z, p[1].z, p[2].z})};
             t.material = face_material[f];
-            lo = {std::min(lo.x, t.lo.x), std::min(lo.y, t.lo.y), std::min(lo.z, t.lo.z)};
-            hi = {std::max(hi.x, t.hi.x), std::max(hi.y, t.hi.y), std::max(hi.z, t.hi.z)};
-            if (!(len > 0.0)) throw std::invalid_argument("a face of the mesh has no area");
             s.tri.push_back(t);
         }
         s.origin = V3{origin[0], origin[1], origin[2]};
@@ -1107,12 +1299,10 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
         s.layer = layer;
         s.layer_damping = layer_damping;
         s.edge_admittance = edge_admittance;
-        s.stencil = stencil;
+        s.points = stencil;
         s.threads = threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
-        const double scale = std::max(norm(hi - lo), 1e-9);
         // A hundred times the reference's tie (kTie, 1e-10 cells); beyond it, and off the
         // faces' edges, its casts and tie rules decide as the plain geometry does.
-        (void)scale;
         s.guard = 1e-8 * cell;
         std::vector<V3> points;
         for (size_t p = 0; p < nprobes; ++p) points.push_back(V3{probes[3 * p], probes[3 * p + 1], probes[3 * p + 2]});
@@ -1131,41 +1321,42 @@ const char *as_setup_create_error(void) { return g_setup_error.c_str(); }
 int as_setup_layout(const as_setup *h, as_layout *out) {
     if (h == nullptr || out == nullptr) return AS_EINVAL;
     const Setup &s = h->s;
+    const SpecialList &sp = s.specials;
     std::memset(out, 0, sizeof(*out));
     out->nx = s.nx;
     out->ny = s.ny;
     out->nz = s.nz;
-    out->stencil = s.stencil;
-    out->weights = s.weight.data();
-    out->n_weights = static_cast<uint64_t>(s.directions);
-    out->centre = s.centre;
-    out->total = s.total;
+    out->stencil = s.points;
+    out->weights = s.weights.data();
+    out->n_weights = static_cast<uint64_t>(s.st.count);
+    out->centre = s.st.centre;
+    out->total = s.st.total;
     out->scale = s.scale.empty() ? nullptr : s.scale.data();
     out->layer_scale = s.layer_scale.empty() ? nullptr : s.layer_scale.data();
     out->types = s.type.data();
-    out->specials = s.special.size();
-    out->special = s.special.data();
-    out->link_start = s.link_start.data();
-    out->links = s.link_direction.size();
-    out->link_direction = s.link_direction.data();
-    out->link_weight = s.link_weight.data();
-    out->coef_a = s.coef_a.data();
-    out->coef_b = s.coef_b.data();
-    out->coef_inv = s.coef_inv.data();
-    out->source = s.source.data();
-    out->state_start = s.state_start.data();
-    out->states = s.state_branch.size();
-    out->state_branch = s.state_branch.data();
-    out->state_weight = s.state_weight.data();
-    out->branches = s.branch_now.size();
-    out->branch_now = s.branch_now.data();
-    out->branch_before = s.branch_before.data();
-    out->branch_gain = s.branch_gain.data();
+    out->specials = sp.size();
+    out->special = sp.cell.data();
+    out->link_start = sp.link_start.data();
+    out->links = sp.link_direction.size();
+    out->link_direction = sp.link_direction.data();
+    out->link_weight = sp.link_weight.data();
+    out->coef_a = sp.coef_a.data();
+    out->coef_b = sp.coef_b.data();
+    out->coef_inv = sp.coef_inv.data();
+    out->source = sp.source.data();
+    out->state_start = sp.state_start.data();
+    out->states = sp.state_branch.size();
+    out->state_branch = sp.state_branch.data();
+    out->state_weight = sp.state_weight.data();
+    out->branches = s.branches.now.size();
+    out->branch_now = s.branches.now.data();
+    out->branch_before = s.branches.before.data();
+    out->branch_gain = s.branches.gain.data();
     out->layer_cells = s.layer_cells.size();
     out->layer_cell = s.layer_cells.data();
     for (int k = 0; k < 6; ++k) {
-        out->keep[k] = s.keep[k].empty() ? nullptr : s.keep[k].data();
-        out->push[k] = s.push[k].empty() ? nullptr : s.push[k].data();
+        out->keep[k] = s.keep[static_cast<size_t>(k)].empty() ? nullptr : s.keep[static_cast<size_t>(k)].data();
+        out->push[k] = s.push[static_cast<size_t>(k)].empty() ? nullptr : s.push[static_cast<size_t>(k)].data();
     }
     out->probes = s.probe_start.size() - 1;
     out->probe_start = s.probe_start.data();
@@ -1180,12 +1371,12 @@ int as_setup_info(const as_setup *h, uint64_t *out) {
     const Setup &s = h->s;
     out[0] = s.air;
     out[1] = s.boundary_cells;
-    out[2] = s.state_branch.size();
+    out[2] = s.specials.state_branch.size();
     out[3] = s.layer_cells.size();
     out[4] = s.probe_start.size() - 1;
     out[5] = s.open ? 1 : 0;
     uint64_t sides = 0;
-    for (int k = 0; k < 6; ++k) sides |= (s.open_sides[static_cast<size_t>(k)] ? 1ull : 0ull) << k;
+    for (size_t k = 0; k < 6; ++k) sides |= (s.open_sides[k] ? 1ull : 0ull) << k;
     out[6] = sides;
     out[7] = 0;  // the source on a face: refused here
     return AS_OK;
