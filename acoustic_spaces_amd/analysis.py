@@ -20,7 +20,6 @@ import math
 from fractions import Fraction
 
 import numpy as np
-from numpy.lib.stride_tricks import sliding_window_view
 
 OCTAVE_BANDS_HZ = (63, 125, 250, 500, 1000, 2000, 4000, 8000)
 OCTAVE_RATIO = 10.0 ** 0.3  # base-ten octave, IEC 61260-1
@@ -51,13 +50,14 @@ def octave_midbands() -> np.ndarray:
 
 def band_pass(f, order: int = FILTER_ORDER) -> np.ndarray:
     """Magnitudes (8, len(f)) at frequencies ``f`` (Hz, > 0) of the octave bands'
-    Butterworth band-pass filters of ``order``: 1 at mid-band, -3 dB at fm G^(+-1/2),
-    |H| = 1 / sqrt(1 + ((f / fm - fm / f) / (G^(1/2) - G^(-1/2)))^(2 order))."""
+    Butterworth band-pass filters of ``order`` (parameters.py:1062-1085): with the
+    detuning v = (f / fm - fm / f) / (G^(1/2) - G^(-1/2)), which is +-1 at the band's edges
+    fm G^(+-1/2), |H| = 1 / sqrt(1 + v^(2 order)): 1 at mid-band, -3 dB at the edges."""
     f = np.asarray(f, float)
-    fm = octave_midbands()[:, None]
-    half = OCTAVE_RATIO ** 0.5
-    width = half - 1.0 / half
-    return 1.0 / np.sqrt(1.0 + ((f / fm - fm / f) / width) ** (2 * order))
+    mid = octave_midbands()[:, None]
+    edge = OCTAVE_RATIO ** 0.5
+    detuning = (f / mid - mid / f) / (edge - 1.0 / edge)
+    return 1.0 / np.sqrt(1.0 + detuning ** (2 * order))
 
 
 def band_filter_gains(n_fft: int, sample_rate: float, order: int = FILTER_ORDER) -> np.ndarray:
@@ -95,100 +95,131 @@ def fixed_dots(x, y) -> tuple[float, float, float]:
     return fixed_dot(x, x), fixed_dot(y, y), fixed_dot(x, y)
 
 
-def fit_line(level: np.ndarray, t: np.ndarray, top: float, bottom: float):
-    """Least-squares line through ``level`` from where it first falls to ``top`` to where it
-    first falls to ``bottom``: (slope dB/s, intercept dB, r^2), or None when the curve does
-    not fall that far, or reaches ``bottom`` only where no energy is left (-inf), or does
-    not fall."""
-    below = level <= bottom
-    if not below.any():
+def _line_window(level: np.ndarray, top: float, bottom: float):
+    """The samples a decay line is fitted over: from the first at or below ``top`` to the
+    first at or below ``bottom`` (parameters.py:266-296). None where the curve never
+    reaches ``bottom``, reaches it only where no energy is left (-inf: the end of the data,
+    not a decay), or leaves fewer than three samples between the two."""
+    reached = np.flatnonzero(level <= bottom)
+    if reached.size == 0 or not np.isfinite(level[reached[0]]):
         return None
-    stop = int(np.argmax(below))
-    if not np.isfinite(level[stop]):
-        return None
-    start = int(np.argmax(level <= top))
-    if stop - start < 2:
-        return None
-    x, y = t[start:stop + 1], level[start:stop + 1]
+    stop = int(reached[0])
+    above = np.flatnonzero(level <= top)
+    start = int(above[0]) if above.size else 0
+    return (start, stop + 1) if stop - start >= 2 else None
+
+
+def _least_squares(x: np.ndarray, y: np.ndarray):
+    """The closed-form straight line through (x, y): (slope, intercept, r^2), the sums of
+    squares and products from fixed_dots (a machine-independent order)."""
     n = len(x)
-    sx, sy = float(x.sum()), float(y.sum())
+    sum_x, sum_y = float(x.sum()), float(y.sum())
     xx, yy, xy = fixed_dots(x, y)
-    sxx, syy, sxy = n * xx - sx * sx, n * yy - sy * sy, n * xy - sx * sy
-    slope = sxy / sxx
-    if slope >= 0:
+    var_x = n * xx - sum_x * sum_x
+    var_y = n * yy - sum_y * sum_y
+    cov = n * xy - sum_x * sum_y
+    slope = cov / var_x
+    r2 = min(cov * cov / (var_x * var_y), 1.0) if var_y > 0 else 1.0
+    return slope, (sum_y - slope * sum_x) / n, r2
+
+
+def fit_line(level: np.ndarray, t: np.ndarray, top: float, bottom: float):
+    """The least-squares line of a decay curve between ``top`` and ``bottom`` dB:
+    (slope dB/s, intercept dB, r^2), or None without a window (see _line_window) or where
+    the line does not fall."""
+    window = _line_window(level, top, bottom)
+    if window is None:
         return None
-    intercept = (sy - slope * sx) / n
-    r2 = min(sxy * sxy / (sxx * syy), 1.0) if syy > 0 else 1.0
-    return slope, intercept, r2
+    a, b = window
+    line = _least_squares(t[a:b], level[a:b])
+    return line if line[0] < 0 else None
+
+
+def _still_to_come(e: np.ndarray, extra: float = 0.0, closing_edge: bool = False) -> np.ndarray:
+    """The energy still to come at the start of each slot of ``e`` (the backward, or
+    Schroeder, integral), ``extra`` added after the last slot; with ``closing_edge``, one more
+    value at the end of the last slot."""
+    backward = np.cumsum(e[::-1])[::-1]
+    if closing_edge:
+        backward = np.append(backward, 0.0)
+    return backward + extra
+
+
+def _level_db(remaining: np.ndarray, reference: float) -> np.ndarray:
+    """10 log10(remaining / reference), -inf where nothing remains."""
+    with np.errstate(divide="ignore"):
+        return 10.0 * np.log10(np.maximum(remaining / reference, 0.0))
 
 
 def tail(e: np.ndarray, dt: float) -> tuple[float, float]:
-    """The energy after the end of ``e`` (per-slot energies) and its decay rate (1/s): a
-    line through the data's own backward-integrated curve, fitted low on the decay but
-    TAIL_HEADROOM_DB above where it would reach by the end, extended to the end."""
-    remaining = np.cumsum(e[::-1])[::-1]
-    if len(remaining) == 0 or remaining[0] <= 0:
+    """The energy after the end of ``e`` (per-slot energies) and its decay rate (1/s)
+    (parameters.py:306-333): a line through the data's own backward-integrated curve,
+    first between -5 and -15 dB, then twice moved down to end TAIL_HEADROOM_DB above where
+    the previous line reaches at the end of the data (at most -15 dB, at least -60 dB) and
+    span TAIL_SPAN_DB above that (at most up to -5 dB); the last line, extended to the end,
+    gives the energy still to come. (0, inf) where any fit fails."""
+    remaining = _still_to_come(e)
+    if remaining.size == 0 or remaining[0] <= 0:
         return 0.0, math.inf
     total = float(remaining[0])
     t = np.arange(len(e)) * dt
     end = len(e) * dt
-    with np.errstate(divide="ignore"):
-        level = 10.0 * np.log10(np.maximum(remaining / total, 0.0))
-    line = fit_line(level, t, -5.0, -15.0)
-    for _ in range(2):
+    level = _level_db(remaining, total)
+    window = (-5.0, -15.0)
+    for refinement in range(3):
+        line = fit_line(level, t, *window)
         if line is None:
             return 0.0, math.inf
+        if refinement == 2:
+            break
         bottom = min(max(line[0] * end + line[1] + TAIL_HEADROOM_DB, -60.0), -15.0)
-        line = fit_line(level, t, min(bottom + TAIL_SPAN_DB, -5.0), bottom)
-    if line is None:
-        return 0.0, math.inf
-    slope, intercept = line[:2]
+        window = (min(bottom + TAIL_SPAN_DB, -5.0), bottom)
+    slope, intercept = line[0], line[1]
     return float(total * 10.0 ** ((slope * end + intercept) / 10.0)), float(-slope * math.log(10.0) / 10.0)
 
 
 def decay_curve(energy: np.ndarray, time_step: float) -> np.ndarray:
-    """The backward-integrated (Schroeder) decay in dB at the start of each slot, the
-    estimated tail included; 0 dB at the start."""
+    """The Schroeder decay in dB at the start of each slot, the estimated tail included,
+    0 dB at the start (parameters.py:357-369); -inf throughout without energy."""
     e = np.asarray(energy, float)
-    extra, _ = tail(e, time_step)
-    remaining = np.cumsum(e[::-1])[::-1] + extra
-    if len(remaining) == 0 or remaining[0] <= 0:
+    remaining = _still_to_come(e, tail(e, time_step)[0])
+    if remaining.size == 0 or remaining[0] <= 0:
         return np.full(len(e), -np.inf)
-    with np.errstate(divide="ignore"):
-        return 10.0 * np.log10(np.maximum(remaining, 0.0) / remaining[0])
+    return _level_db(remaining, remaining[0])
 
 
 def decay_level(e: np.ndarray, extra: float) -> tuple[np.ndarray, np.ndarray]:
     """The energy still to come at each slot edge (``extra`` after the last) and its level
-    in dB against the curve's own start; -inf where none is left."""
-    remaining = np.concatenate([np.cumsum(e[::-1])[::-1], [0.0]]) + extra
-    with np.errstate(divide="ignore"):
-        level = 10.0 * np.log10(np.maximum(remaining / remaining[0], 0.0))
-    return remaining, level
+    in dB against the curve's own start (parameters.py:442-452)."""
+    remaining = _still_to_come(e, extra, closing_edge=True)
+    return remaining, _level_db(remaining, remaining[0])
 
 
 def decay_time(level: np.ndarray, t: np.ndarray, top: float, bottom: float) -> float:
     """-60 dB over the slope of the line fitted between ``top`` and ``bottom``, or NaN."""
-    fit = fit_line(level, t, top, bottom)
-    return math.nan if fit is None else -60.0 / fit[0]
+    line = fit_line(level, t, top, bottom)
+    return math.nan if line is None else -60.0 / line[0]
 
 
 def too_short(level: np.ndarray, edges: np.ndarray, edt: float) -> bool:
-    """Whether the response stops before its decay has fallen MIN_DECAY_DB, at its end or
-    along the early decay's slope."""
-    return bool(level[-1] > MIN_DECAY_DB and not 60.0 * edges[-1] / edt >= -MIN_DECAY_DB)
+    """Whether the decay stops short of MIN_DECAY_DB (parameters.py:455-461): its curve's
+    end (the tail included) still above it, and the early decay's slope not reaching it by
+    the last edge either (a NaN EDT never does)."""
+    end_above = level[-1] > MIN_DECAY_DB
+    slope_reaches = 60.0 * edges[-1] / edt >= -MIN_DECAY_DB
+    return bool(end_above and not slope_reaches)
 
 
 def band_t30(e: np.ndarray, dt: float, offset: float = 0.0) -> float:
-    """T30 (s) of one band's slot energies ``e``, starting at the slot of the onset, which
-    lies ``offset`` seconds into it; NaN where it cannot be read."""
-    edges = np.arange(len(e) + 1) * dt - offset
-    extra, _ = tail(e, dt)
+    """T30 (s) of one band's slot energies ``e`` from the slot of the onset, which lies
+    ``offset`` seconds into it (parameters.py:464-475); NaN where the decay has no energy,
+    is too short, or gives no line between -5 and -35 dB."""
+    extra = tail(e, dt)[0]
     if float(e.sum()) + extra <= 0:
         return math.nan
-    _, level = decay_level(e, extra)
-    edt = decay_time(level, edges, *DECAY_RANGES["edt"])
-    if too_short(level, edges, edt):
+    edges = np.arange(len(e) + 1) * dt - offset
+    level = decay_level(e, extra)[1]
+    if too_short(level, edges, decay_time(level, edges, *DECAY_RANGES["edt"])):
         return math.nan
     return decay_time(level, edges, *DECAY_RANGES["t30"])
 
@@ -280,64 +311,77 @@ def band_gains(n_fft: int, sample_rate: float, bands: int = len(OCTAVE_BANDS_HZ)
 
 
 def kaiser_sinc(t: np.ndarray, cutoff: float) -> np.ndarray:
-    """Low-pass taps at offsets ``t`` (input samples), ``cutoff`` a share of the input's
-    Nyquist frequency: sinc times a Kaiser window reaching RESAMPLE_ZEROS zero crossings
-    (render.py:105-109)."""
-    half = RESAMPLE_ZEROS / cutoff
-    window = np.i0(RESAMPLE_BETA * np.sqrt(np.clip(1.0 - (t / half) ** 2, 0.0, 1.0)))
-    return cutoff * np.sinc(cutoff * t) * window / np.i0(RESAMPLE_BETA)
+    """The resampling low-pass at offsets ``t`` (input samples), ``cutoff`` a share of the
+    input's Nyquist frequency: the ideal low-pass's impulse response cutoff sinc(cutoff t)
+    under a Kaiser window of shape RESAMPLE_BETA that closes at RESAMPLE_ZEROS of its zero
+    crossings (the reference's filter, render.py:105-109)."""
+    support = RESAMPLE_ZEROS / cutoff
+    inside = np.sqrt(np.clip(1.0 - (t / support) ** 2, 0.0, 1.0))
+    return cutoff * np.sinc(cutoff * t) * (np.i0(RESAMPLE_BETA * inside) / np.i0(RESAMPLE_BETA))
 
 
 def ordered_rows_dot(rows: np.ndarray, taps: np.ndarray) -> np.ndarray:
     """rows (n, taps) times taps, each row's terms added in ascending tap order from the
     first (the reference's products.product, products.py:39-56)."""
-    acc = rows[:, 0] * taps[0]
+    return _tap_major(lambda k: rows[:, k], taps)
+
+
+def _tap_major(column, taps):
+    """sum over k of column(k) * taps[k], for every row at once, added in ascending k from
+    the first term: the order of a plain loop over the taps."""
+    acc = column(0) * taps[0]
     for k in range(1, len(taps)):
-        acc = acc + rows[:, k] * taps[k]
+        acc = acc + column(k) * taps[k]
     return acc
+
+
+def _output_positions(count: int, rate_in: float, rate_out: float):
+    """Where each of ``count`` outputs lies among the inputs: (the input sample at or
+    below it, its phase, the number of phases). Output k sits at k rate_in / rate_out;
+    where that ratio is p / q with q <= RESAMPLE_MAX_PHASES the phase is exact integer
+    arithmetic (k p mod q of q); otherwise the position's fraction rounds to one of
+    RESAMPLE_MAX_PHASES phases, a round up to the next sample carried into it."""
+    ratio = (Fraction(rate_in) / Fraction(rate_out)).limit_denominator(1 << 20)
+    k = np.arange(count, dtype=np.int64)
+    if ratio.denominator <= RESAMPLE_MAX_PHASES:
+        p, q = ratio.numerator, ratio.denominator
+        return (k * p) // q, (k * p) % q, q
+    position = k * (rate_in / rate_out)
+    whole = np.floor(position).astype(np.int64)
+    phase = np.round((position - whole) * RESAMPLE_MAX_PHASES).astype(np.int64)
+    return whole + phase // RESAMPLE_MAX_PHASES, phase % RESAMPLE_MAX_PHASES, RESAMPLE_MAX_PHASES
 
 
 def resample(signal, rate_in: float, rate_out: float) -> np.ndarray:
     """``signal`` (samples,) or (samples, channels) at ``rate_out``, ceil(n rate_out /
-    rate_in) samples: polyphase windowed-sinc interpolation, the cutoff just below the
-    lower Nyquist frequency (render.py:112-162). Output k sits at input position
-    k rate_in / rate_out; where that ratio is p / q with q <= RESAMPLE_MAX_PHASES, each of
-    the q phases is one ordered product over the outputs that share it; otherwise the
-    position rounds to one of RESAMPLE_MAX_PHASES phases."""
+    rate_in) samples, by band-limited interpolation (the reference's render.resample,
+    render.py:112-162): the cutoff just below the lower Nyquist frequency, 0.97 of it;
+    a bank of the filter's phases (one row of taps per phase, taps at whole-sample offsets
+    round the output's position); every output the dot product of its phase's row with
+    the inputs round it (zeros past either end), in ascending tap order. The outputs are
+    formed all at once, tap after tap, in CHUNK-sized blocks."""
     x = np.asarray(signal, np.float64)
     if rate_in <= 0 or rate_out <= 0:
         raise ValueError("sample rates must be positive")
     if math.isclose(rate_in, rate_out) or len(x) == 0:
         return x.copy()
-    ratio = (Fraction(rate_in) / Fraction(rate_out)).limit_denominator(1 << 20)
-    exact = ratio.denominator <= RESAMPLE_MAX_PHASES
-    step, phases = (ratio.numerator, ratio.denominator) if exact else (0, RESAMPLE_MAX_PHASES)
     cutoff = 0.97 * min(1.0, rate_out / rate_in)
     reach = int(math.ceil(RESAMPLE_ZEROS / cutoff))
-    offsets = np.arange(-reach + 1, reach + 1)  # taps relative to the position's floor
-    table = kaiser_sinc(offsets[None, :] - (np.arange(phases) / phases)[:, None], cutoff)
-    columns = x.reshape(len(x), -1)
     count = int(math.ceil(len(x) * rate_out / rate_in))
-    out = np.zeros((count, columns.shape[1]))
-    for c in range(columns.shape[1]):
-        padded = np.concatenate([np.zeros(reach), columns[:, c], np.zeros(reach + 1)])
-        window = sliding_window_view(padded, len(offsets))  # row r: the taps round input r - 1
-        if exact:
-            inverse = pow(step, -1, phases)
-            for p in range(phases):
-                k0 = p * inverse % phases  # the first output of phase p
-                if k0 >= count:
-                    continue
-                n = len(range(k0, count, phases))
-                first = (k0 * step - p) // phases + 1
-                out[k0::phases, c] = ordered_rows_dot(window[first: first + (n - 1) * step + 1: step], table[p])
-            continue
-        for start in range(0, count, RESAMPLE_CHUNK):
-            k = np.arange(start, min(start + RESAMPLE_CHUNK, count))
-            position = k * (rate_in / rate_out)
-            whole = np.floor(position).astype(np.int64)
-            phase = np.round((position - whole) * phases).astype(np.int64)
-            whole, phase = whole + phase // phases, phase % phases
-            out[k, c] = np.einsum("kt,kt->k", table[phase], window[whole + 1])
+    whole, phase, phases = _output_positions(count, rate_in, rate_out)
+    offsets = np.arange(1 - reach, reach + 1)  # taps' inputs, relative to the one at or below
+    bank = kaiser_sinc(offsets[None, :] - (np.arange(phases) / phases)[:, None], cutoff)
+    columns = x.reshape(len(x), -1)
+    # Input i + reach of `padded` is input i: every tap of every output lies inside it.
+    padded = np.zeros((len(x) + 2 * reach + 1, columns.shape[1]))
+    padded[reach: reach + len(x)] = columns
+    out = np.empty((count, columns.shape[1]))
+    for a in range(0, count, RESAMPLE_CHUNK):
+        b = min(a + RESAMPLE_CHUNK, count)
+        rows = bank[phase[a:b]]
+        first = whole[a:b] + 1  # padded index of the first tap (offset 1 - reach)
+        acc = padded[first] * rows[:, 0, None]
+        for k in range(1, len(offsets)):
+            acc += padded[first + k] * rows[:, k, None]
+        out[a:b] = acc
     return out[:, 0] if x.ndim == 1 else out
-

@@ -48,52 +48,65 @@ def wave_gains(n_fft: int, sample_rate: float, bands: int, total: int = len(anal
 
 def cutoff(courant: float) -> float:
     """The source's low-pass cutoff (of the grid's sample rate) on the grid of Courant
-    number ``courant``."""
+    number ``courant``: LOW_PASS_CUTOFF on the 7-point grid, in proportion on others."""
     return LOW_PASS_CUTOFF * courant / COURANT
+
+
+def _low_pass_taps(courant: float) -> np.ndarray:
+    """LOW_PASS_TAPS taps of a Blackman-windowed sinc at ``cutoff(courant)``, unit gain at
+    DC (engine/wave.py:281-327)."""
+    c = cutoff(courant)
+    k = np.arange(LOW_PASS_TAPS) - LOW_PASS_TAPS // 2
+    taps = np.blackman(LOW_PASS_TAPS) * (2.0 * c) * np.sinc(2.0 * c * k)
+    return taps / taps.sum()
+
+
+def _hann_high_pass_taps(rate: float) -> np.ndarray:
+    """An impulse less a unit-area Hann window of HIGH_PASS_S (at least 3 samples): it
+    passes no DC (engine/wave.py:281-327)."""
+    length = max(3, int(round(HIGH_PASS_S * rate)))
+    hann = np.hanning(length + 2)[1:-1]
+    taps = hann * (-1.0 / hann.sum())
+    taps[0] += 1.0
+    return taps
+
+
+def _isotropic_level_taps(speed: float) -> np.ndarray:
+    """Three taps (2/9, 5/9, 2/9) with the side taps over speed^2: their gain
+    (5 + 4 cos wT) / 9 undoes the 19-point scheme's at the source."""
+    side = 2.0 / (9.0 * speed ** 2)
+    return np.array([side, 1.0 - 2.0 * side, side])
 
 
 def source_signal(rate: float, cell: float, courant: float = COURANT, speed: float = 1.0, *,
                   level: bool = True) -> tuple[np.ndarray, int]:
-    """What the source adds to the grid each step, and its delay in steps: an impulse,
-    low-passed (a Blackman-windowed sinc of LOW_PASS_TAPS taps at ``cutoff(courant)``) and
-    high-passed twice (one minus a Hann window of HIGH_PASS_S, so it leaves no air in the
-    room), scaled to 4 pi courant^2 / cell so that the free-field pressure at r metres is
-    that impulse over r. On the 19-point grid, with ``level``, three taps 2/9, 5/9, 2/9
-    (the side taps over speed^2) undo the scheme's gain 9 / (5 + 4 cos wT) at the source,
-    with a step more of delay."""
-    half = LOW_PASS_TAPS // 2
-    k = np.arange(-half, half + 1)
-    c = cutoff(courant)
-    low = 2 * c * np.sinc(2 * c * k) * np.blackman(LOW_PASS_TAPS)
-    low /= low.sum()
-    length = max(3, int(round(HIGH_PASS_S * rate)))
-    window = np.hanning(length + 2)[1:-1]
-    high = -window / window.sum()
-    high[0] += 1.0
-    signal = np.convolve(np.convolve(low, high), high)
-    delay = half
+    """What the source adds to the grid each step, and its delay in steps (the
+    reference's drive, engine/wave.py:281-327): an impulse low-passed by _low_pass_taps and
+    high-passed twice by _hann_high_pass_taps (twice, so that it leaves no air in a closed
+    room), scaled by 4 pi courant^2 / cell, so that the free-field pressure r metres away is
+    that impulse over r. On the 19-point grid, with ``level``, also through
+    _isotropic_level_taps, a step more of delay."""
+    high = _hann_high_pass_taps(rate)
+    chain = [_low_pass_taps(courant), high, high]
+    delay = LOW_PASS_TAPS // 2
     if level and math.isclose(courant, ISOTROPIC_COURANT, rel_tol=1e-9):
-        side = 2.0 / (9.0 * speed ** 2)
-        signal = np.convolve(signal, [side, 1.0 - 2.0 * side, side])
+        chain.append(_isotropic_level_taps(speed))
         delay += 1
-    return 4.0 * math.pi * courant ** 2 / cell * signal, delay
+    signal = chain[0]
+    for taps in chain[1:]:
+        signal = np.convolve(signal, taps)
+    return (4.0 * math.pi * courant ** 2 / cell) * signal, delay
 
 
 def high_pass(x, rate: float) -> np.ndarray:
-    """``x`` (samples, ...) through the source signal's high-pass twice (one minus a Hann
-    window of HIGH_PASS_S): what a drive that carries air into a closed room (a Gaussian
-    pulse) would give had it been high-passed as source_signal's is, the scheme being
-    linear; same length."""
+    """``x`` (samples, ...) through the source signal's high-pass twice, same length: by
+    the scheme's linearity, what a drive that carries air into a closed room (a Gaussian
+    pulse) would give had it been high-passed as source_signal's is."""
     x = np.asarray(x, float)
-    length = max(3, int(round(HIGH_PASS_S * rate)))
-    window = np.hanning(length + 2)[1:-1]
-    high = -window / window.sum()
-    high[0] += 1.0
-    h2 = np.convolve(high, high)
-    n_fft = 1 << int(math.ceil(math.log2(len(x) + len(h2))))
-    shape = (-1,) + (1,) * (x.ndim - 1)
-    y = np.fft.irfft(np.fft.rfft(x, n_fft, axis=0) * np.fft.rfft(h2, n_fft).reshape(shape), n_fft, axis=0)
-    return y[: len(x)]
+    twice = np.convolve(*([_hann_high_pass_taps(rate)] * 2))
+    n_fft = 1 << int(math.ceil(math.log2(len(x) + len(twice))))
+    response = np.fft.rfft(twice, n_fft).reshape((-1,) + (1,) * (x.ndim - 1))
+    return np.fft.irfft(np.fft.rfft(x, n_fft, axis=0) * response, n_fft, axis=0)[: len(x)]
 
 
 def probe_points(listener, cell: float) -> np.ndarray:
@@ -120,41 +133,62 @@ def off_the_source(found, probes, source, cell: float, stencil: int) -> np.ndarr
     return after.reshape(-1)
 
 
+def _cell_difference(rows: np.ndarray, found, axis: int):
+    """The change of pressure over one cell along ``axis`` at the listener, from the 13
+    rows (probe_points' order), or None: the fourth-order central difference over two
+    cells either side; the second-order one where a point two cells away is missing; the
+    one-sided one where a point a cell away is (engine/wave.py:757-802)."""
+    up, down = 1 + 2 * axis, 2 + 2 * axis
+    if found[up] and found[down]:
+        near = 0.5 * (rows[up] - rows[down])
+        if not (found[up + 6] and found[down + 6]):
+            return near
+        far = rows[up + 6] - rows[down + 6]
+        return near * (4 / 3) - far / 12
+    if found[up]:
+        return rows[up] - rows[0]
+    if found[down]:
+        return rows[0] - rows[down]
+    return None
+
+
+def _trapezoid_integral(d: np.ndarray) -> np.ndarray:
+    """The running trapezoidal integral of d (one sample = 1), from a zero before the
+    first sample: I[n] = I[n - 1] + (d[n - 1] + d[n]) / 2."""
+    halves = 0.5 * d
+    return np.cumsum(halves + np.concatenate(([0.0], halves[:-1])))
+
+
+def _undo_trapezoid_gain(x: np.ndarray, cut: float) -> np.ndarray:
+    """``x`` (samples, channels) with the trapezoidal rule's gain (wT/2) / tan(wT/2)
+    divided out at frequencies up to ``cut`` of the sample rate (zero-padded FFT)."""
+    n_fft = 1 << int(math.ceil(math.log2(2 * len(x) + 1)))
+    half_turn = np.pi * np.fft.rfftfreq(n_fft)
+    correction = np.ones(len(half_turn))
+    inside = (half_turn > 0) & (half_turn <= np.pi * cut)
+    correction[inside] = np.tan(half_turn[inside]) / half_turn[inside]
+    return np.fft.irfft(np.fft.rfft(x, n_fft, axis=0) * correction[:, None], n_fft, axis=0)[: len(x)]
+
+
 def first_order(probes: np.ndarray, found, name: str = "", courant: float = COURANT,
                 cut: float = LOW_PASS_CUTOFF) -> np.ndarray:
-    """W, X, Y, Z (samples, 4) from the pressure at the listener and round it (the 13
-    rows of ``probes``, at probe_points). X, Y, Z are c times the time integral of the
-    pressure gradient, X = -rho c u_x (AmbiX: towards the sound):
-    - the gradient is a fourth-order central difference over two cells either side,
-      second-order where a point two cells away is not ``found``, one-sided where a
-      point a cell away is not;
-    - the integral is the trapezoidal rule, its gain (wT/2) / tan(wT/2) divided out up
-      to ``cut`` of the sample rate.
-    ``courant`` is the Courant number of the speed of sound at the listener."""
+    """W, X, Y, Z (samples, 4) from the pressure at a listener's 13 points (the rows of
+    ``probes``, in probe_points' order; the reference's _first_order, engine/wave.py:
+    757-802). W is the pressure at the listener. X, Y, Z are c times the time integral of
+    the pressure gradient, X = -rho c u_x (AmbiX: towards the sound): per axis the cell
+    difference (_cell_difference) integrated by the trapezoidal rule, times the Courant
+    number ``courant`` of the speed of sound at the listener (c dt / h), then the rule's
+    gain undone up to ``cut`` of the sample rate. An axis with no point a cell away either
+    side reads 0."""
     if not found[0]:
         raise ValueError(f"listener {name!r} is not in the air of the wave grid")
     out = np.zeros((probes.shape[1], 4))
     out[:, 0] = probes[0]
     for axis in range(3):
-        plus, minus = 1 + 2 * axis, 2 + 2 * axis
-        if found[plus] and found[minus]:
-            difference = 0.5 * (probes[plus] - probes[minus])
-            if found[plus + 6] and found[minus + 6]:
-                difference = difference * (4 / 3) - (probes[plus + 6] - probes[minus + 6]) / 12
-        elif found[plus]:
-            difference = probes[plus] - probes[0]
-        elif found[minus]:
-            difference = probes[0] - probes[minus]
-        else:
-            continue
-        out[:, 1 + axis] = courant * (np.cumsum(difference) - 0.5 * difference)
-    n_fft = 1 << int(math.ceil(math.log2(2 * len(out) + 1)))
-    turn = np.pi * np.fft.rfftfreq(n_fft)  # wT / 2
-    gain = np.ones(len(turn))
-    below = (turn > 0) & (turn <= np.pi * cut)
-    gain[below] = np.tan(turn[below]) / turn[below]
-    spectrum = np.fft.rfft(out[:, 1:], n_fft, axis=0) * gain[:, None]
-    out[:, 1:] = np.fft.irfft(spectrum, n_fft, axis=0)[: len(out)]
+        d = _cell_difference(probes, found, axis)
+        if d is not None:
+            out[:, 1 + axis] = courant * _trapezoid_integral(d)
+    out[:, 1:] = _undo_trapezoid_gain(out[:, 1:], cut)
     return out
 
 
@@ -182,70 +216,79 @@ class WaveResponse:
     def _amplitudes(self) -> np.ndarray:
         return np.ones(self.bands) if self.band_gain is None else np.sqrt(self.band_gain)
 
-    def _resampled(self, sample_rate: float, n: int) -> tuple:
-        """The run at ``sample_rate`` (at least before + n samples, 4 channels), still
-        broadband, and ``before``: how many of its samples precede emission. Its start is
-        rounded to a sample once, and an impulse's height scales with the sample period
-        (engine/wave.py:366-387)."""
+    def _timeline(self, sample_rate: float, n: int) -> tuple:
+        """(run, before): the run resampled to ``sample_rate`` (4 channels, still broadband)
+        and laid on a timeline that starts ``before`` samples ahead of emission and holds
+        at least before + n samples; the run's start rounds to a sample once, and an
+        impulse's height scales with the sample period (engine/wave.py:366-387). The last
+        timeline asked for is kept."""
         key = (float(sample_rate), int(n))
         if key not in self._cache:
-            start = int(round((self.delay_s - self.lead_s) * sample_rate))
-            before = max(0, -start)
-            x = analysis.resample(self.signals, self.rate, sample_rate)
-            out = np.zeros((before + max(n, start + len(x)), 4))
-            out[before + start: before + start + len(x)] = x * (self.rate / sample_rate)
-            self._cache = {key: (out, before)}
+            at = int(round((self.delay_s - self.lead_s) * sample_rate))  # the run's start, from emission
+            before = max(0, -at)
+            x = analysis.resample(self.signals, self.rate, sample_rate) * (self.rate / sample_rate)
+            run = np.zeros((before + max(n, at + len(x)), 4))
+            run[before + at: before + at + len(x)] = x
+            self._cache = {key: (run, before)}
         return self._cache[key]
 
+    def _covered(self, n_fft: int, sample_rate: float) -> np.ndarray:
+        """The wave bands' summed zero-phase gain on the rFFT grid, each band's at its
+        feed's amplitude."""
+        return self._amplitudes() @ wave_gains(n_fft, sample_rate, self.bands, self.total_bands)
+
     def field(self, sample_rate: float, n: int) -> np.ndarray:
-        """The first-order sound field (n, 4) of the wave bands together from emission, at
-        ``sample_rate``: the resampled run through the bands' summed zero-phase gains (each
-        band's times its feed's amplitude), whose spread before emission is left out
-        (engine/wave.py:389-399)."""
-        run, before = self._resampled(sample_rate, n)
+        """The first-order sound field (n, 4) of the wave bands together, from emission, at
+        ``sample_rate`` (engine/wave.py:389-399): the timeline through _covered, what the
+        filter spreads ahead of emission dropped."""
+        run, before = self._timeline(sample_rate, n)
         n_fft = analysis.fft_size(len(run), sample_rate)
-        gains = wave_gains(n_fft, sample_rate, self.bands, self.total_bands)
-        gain = (gains * self._amplitudes()[:, None]).sum(axis=0)
-        spectrum = np.fft.rfft(run, n_fft, axis=0) * gain[:, None]
-        return np.fft.irfft(spectrum, n_fft, axis=0)[before: before + n]
+        return analysis.zero_phase(run, self._covered(n_fft, sample_rate), n_fft)[before: before + n]
 
     def high_pass(self, signals, sample_rate: float, late: bool = False) -> np.ndarray:
-        """``signals`` (n, channels) without what the wave bands cover, which take them at a
-        gain g: at 1 - g (early paths, adding in pressure) or, with ``late``, at
-        sqrt(1 - g^2) (a tail, adding in energy) (engine/wave.py:401-409)."""
+        """``signals`` (n, channels) without what the wave bands cover at their gain g:
+        through 1 - g (early paths, which add in pressure) or, with ``late``, through
+        sqrt(1 - g^2) (a tail, which adds in energy) (engine/wave.py:401-409)."""
         signals = np.asarray(signals, float)
         n_fft = analysis.fft_size(len(signals), sample_rate)
         g = wave_gains(n_fft, sample_rate, self.bands, self.total_bands).sum(axis=0)
-        return analysis.zero_phase(signals, np.sqrt(1.0 - g ** 2) if late else 1.0 - g, n_fft)
+        rest = np.sqrt(1.0 - g * g) if late else 1.0 - g
+        return analysis.zero_phase(signals, rest, n_fft)
 
     def energy(self, time_step: float, slots: int, axis=None, exponent: int = 2) -> np.ndarray:
-        """Energy per band and time slot (bands, slots), as the ray tracer counts it: each
-        band's filtered pressure's energy over its filter's share of the spectrum. With
-        ``axis`` (a unit vector), a figure of eight facing it, from X, Y, Z; with
-        ``exponent`` 1, the magnitude of its product with the pressure."""
-        out = np.zeros((self.bands, slots))
+        """Energy per band and time slot (bands, slots), as the ray tracer counts it
+        (engine/wave.py:411-454): every band's zero-phase filtered pressure squared, over
+        ENERGY_TO_IR times the filter's share of the spectrum, times the band's feed, into
+        the slot of its time from emission (what the filters spread before emission into
+        the first). With ``axis`` (a unit vector), a figure of eight facing it (X, Y, Z
+        along it) squared, or with ``exponent`` 1 the magnitude of its product with the
+        pressure."""
         pad = int(round(analysis.PAD_S * self.rate))
-        n_fft = analysis.fft_size(len(self.signals) + pad, self.rate)
+        n = len(self.signals) + pad
+        n_fft = analysis.fft_size(n, self.rate)
         gains = analysis.band_filter_gains(n_fft, self.rate)[: self.bands]
-        weights = np.full(gains.shape[1], 2.0)
-        weights[0] = weights[-1] = 1.0
+        one_sided = np.full(gains.shape[1], 2.0)  # bins counted twice on the one-sided grid
+        one_sided[0] = one_sided[-1] = 1.0
+        per_energy = self._amplitudes() ** 2 / (analysis.ENERGY_TO_IR * (gains ** 2 @ one_sided) / n_fft)
+
+        def banded(series):  # (n, bands): the series after `pad` zeros, through each band
+            spectrum = np.fft.rfft(np.concatenate([np.zeros(pad), series]), n_fft)
+            return np.fft.irfft(spectrum[:, None] * gains.T, n_fft, axis=0)[:n]
+
         lead = int(round(self.lead_s * self.rate))
-        time = (np.arange(len(self.signals) + pad) - pad - lead) / self.rate + self.delay_s
-        slot = np.maximum(time / time_step, 0).astype(np.int64)
-        amplitude = self._amplitudes()
+        t = (np.arange(n) - pad - lead) / self.rate + self.delay_s
+        slot = np.maximum(t / time_step, 0).astype(np.int64)
         keep = slot < slots
-        padded = np.concatenate([np.zeros(pad), self.signals[:, 0]])
-        if axis is not None:
-            figure8 = np.concatenate([np.zeros(pad), self.signals[:, 1:] @ np.asarray(axis, float)])
-        for b, gain in enumerate(gains):
-            pressure = analysis.zero_phase(padded, gain, n_fft)
-            share = float((gain ** 2 * weights).sum() / n_fft)
-            power = pressure[keep] ** 2
-            if axis is not None:
-                side = analysis.zero_phase(figure8, gain, n_fft)[keep]
-                power = side ** 2 if exponent == 2 else np.abs(side * pressure[keep])
-            energy = power * amplitude[b] ** 2 / (analysis.ENERGY_TO_IR * share)
-            out[b] = np.bincount(slot[keep], energy, minlength=slots)[:slots]
+        pressure = banded(self.signals[:, 0])
+        if axis is None:
+            power = pressure * pressure
+        else:
+            side = banded(self.signals[:, 1:] @ np.asarray(axis, float))
+            power = side * side if exponent == 2 else np.abs(side * pressure)
+        power = power[keep] * per_energy
+        out = np.zeros((self.bands, slots))
+        for b in range(self.bands):
+            out[b] = np.bincount(slot[keep], power[:, b], minlength=slots)[:slots]
         return out
 
 

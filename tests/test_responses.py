@@ -150,7 +150,7 @@ def test_response_field_and_high_pass_split_the_run():
     rate, sr, n = 2000.0, 8000.0, 6000
     sig = rng.standard_normal((1200, 4))
     r = wv.WaveResponse(rate, sig, 3, 0.1, lead_s=0.01, delay_s=0.05)
-    run, before = r._resampled(sr, n)
+    run, before = r._timeline(sr, n)
     start = int(round((0.05 - 0.01) * sr))
     assert before == 0 and run.shape == (max(n, start + 4800), 4)
     assert np.allclose(run[start:start + 4800], analysis.resample(sig, rate, sr) * (rate / sr))
@@ -164,7 +164,7 @@ def test_response_field_and_high_pass_split_the_run():
     # A lead longer than the delay: samples before emission, filtered with the rest and
     # left out of the field, which is the same run's 720 samples later.
     early = wv.WaveResponse(rate, sig, 3, 0.1, lead_s=0.05)
-    run2, before2 = early._resampled(sr, n)
+    run2, before2 = early._timeline(sr, n)
     assert before2 == int(round(0.05 * sr))
     shift = start + before2
     assert np.allclose(early.field(sr, n), r.field(sr, n + shift)[shift:shift + n], atol=1e-9)
