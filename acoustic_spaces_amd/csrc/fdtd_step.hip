@@ -1202,6 +1202,16 @@ struct as_fdtd {
     size_t rec_steps = 0, rec_every = 0;
     char *rec_ckpt = nullptr;
     float *rec_signal = nullptr;
+    // With the history kept (keep_history), every step's p(n + 1) - p(n - 1) of each general
+    // special (steps x specials, step-major), so that the adjoint needs no recompute; and
+    // the record's graphs of rec_graph_every steps (by starting slot) on their staging.
+    int keep_history = 0;  // 1: always, -1: never, 0: where it fits kHistoryBudget
+    void *rec_hist = nullptr;
+    hipGraphExec_t rec_graph[2] = {nullptr, nullptr};
+    size_t rec_graph_every = 0;
+    float *rec_sig_stage = nullptr;
+    double *rec_out_stage = nullptr;
+    void *rec_hist_stage = nullptr;
     std::vector<uint64_t> sorted_to_special;  // general special (sorted) -> layout index
     std::vector<uint64_t> state_slot;         // layout state -> its slot
     std::vector<uint32_t> probe_start_host;
@@ -1379,6 +1389,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (tuning && tuning->split > 0) h->split = 1;
     if (tuning && tuning->tile_order > 0) h->tile_order = tuning->tile_order == 2 ? 2 : 1;
     if (tuning && tuning->no_faces > 0) h->no_faces = 1;
+    if (tuning) h->keep_history = tuning->keep_history > 0 ? 1 : (tuning->keep_history < 0 ? -1 : 0);
     const bool pairs_on = tuning && tuning->pairs > 0;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
     const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);
@@ -2491,9 +2502,41 @@ namespace {
 void free_record(as_fdtd *h) {
     if (h->rec_ckpt) (void)hipFree(h->rec_ckpt);
     if (h->rec_signal) (void)hipFree(h->rec_signal);
+    if (h->rec_hist) (void)hipFree(h->rec_hist);
     h->rec_ckpt = nullptr;
     h->rec_signal = nullptr;
+    h->rec_hist = nullptr;
     h->rec_steps = h->rec_every = 0;
+}
+
+// The record's graphs and their staging (see as_fdtd::rec_graph): graphs first, then the
+// buffers their nodes point at.
+void free_record_graphs(as_fdtd *h) {
+    for (auto &g : h->rec_graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    for (void *p : {static_cast<void *>(h->rec_sig_stage), static_cast<void *>(h->rec_out_stage), h->rec_hist_stage})
+        if (p) (void)hipFree(p);
+    h->rec_sig_stage = nullptr;
+    h->rec_out_stage = nullptr;
+    h->rec_hist_stage = nullptr;
+    h->rec_graph_every = 0;
+}
+
+// The most device memory a record keeps its history in by default (keep_history 0): at
+// most this, and at most a quarter of what is free.
+constexpr size_t kHistoryBudget = size_t{16} << 30;
+
+// Steps 0 .. count - 1 of a recorded segment, from slot `prev`, one launch a step, s's
+// drive, probes' output and history (row k: step k) as given; returns the slot after.
+template <typename T>
+int record_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t count) {
+    for (size_t k = 0; k < count; ++k) {
+        launch_one<T>(h, s, blocks, prev, k, h->stream);
+        prev ^= 1;
+    }
+    return prev;
 }
 
 void check_adjoint(const as_fdtd *h) {
@@ -2524,6 +2567,28 @@ void record(as_fdtd *h, const float *signal, size_t steps, size_t every, double 
     check(hipMemcpyAsync(h->rec_signal, signal, steps * sizeof(float), hipMemcpyHostToDevice, h->stream), "hipMemcpyAsync");
     const int code = as_fdtd_reset(h);
     if (code != AS_OK) throw Failure{code, h->error};
+    // Keep the history where asked, or by default where it fits the budget.
+    const size_t G = std::max<size_t>(h->specials, 1);
+    const size_t hist_bytes = steps * G * e;
+    bool keep = h->keep_history > 0;
+    if (h->keep_history == 0) {
+        size_t free_bytes = 0, total_bytes = 0;
+        check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
+        keep = hist_bytes <= std::min(kHistoryBudget, free_bytes / 4);
+    }
+    StepArgs<T> hs = step_args<T>(h);
+    const uint32_t hblocks = block_count(h, hs);
+    if (keep) {
+        check(hipMalloc(&h->rec_hist, hist_bytes), "hipMalloc");
+        if (h->rec_graph_every != every) {
+            free_record_graphs(h);
+            check(hipMalloc(reinterpret_cast<void **>(&h->rec_sig_stage), every * sizeof(float)), "hipMalloc");
+            check(hipMalloc(reinterpret_cast<void **>(&h->rec_out_stage), std::max<size_t>(P, 1) * every * sizeof(double)),
+                  "hipMalloc");
+            check(hipMalloc(&h->rec_hist_stage, every * G * e), "hipMalloc");
+            h->rec_graph_every = every;
+        }
+    }
     for (size_t seg = 0; seg < segments; ++seg) {
         char *base = h->rec_ckpt + seg * per;
         // By role: p(k - 1), p(k), w(k - 1), w(k).
@@ -2536,7 +2601,43 @@ void record(as_fdtd *h, const float *signal, size_t steps, size_t every, double 
                   "ckpt");
         }
         const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
-        launch_steps<T>(h, h->rec_signal + k0, k1 - k0, d_out + k0, steps);
+        if (!keep) {
+            launch_steps<T>(h, h->rec_signal + k0, k1 - k0, d_out + k0, steps);
+            continue;
+        }
+        T *hist = static_cast<T *>(h->rec_hist) + k0 * G;
+        if (k1 - k0 < every || hblocks == 0) {
+            StepArgs<T> d = hs;
+            d.signal = h->rec_signal + k0;
+            d.out = d_out + k0;
+            d.out_stride = steps;
+            d.history = hist;
+            if (hblocks) h->prev = record_steps<T>(h, d, hblocks, h->prev, k1 - k0);
+            h->done += k1 - k0;
+            continue;
+        }
+        // A whole segment: its graph (by starting slot) on the staging buffers.
+        const int slot = h->prev;
+        if (h->rec_graph[slot] == nullptr) {
+            StepArgs<T> g = hs;
+            g.signal = h->rec_sig_stage;
+            g.out = h->rec_out_stage;
+            g.out_stride = every;
+            g.history = static_cast<T *>(h->rec_hist_stage);
+            capture_graph(h->stream, &h->rec_graph[slot], [&] { (void)record_steps<T>(h, g, hblocks, slot, every); });
+        }
+        check(hipMemcpyAsync(h->rec_sig_stage, h->rec_signal + k0, every * sizeof(float), hipMemcpyDeviceToDevice,
+                             h->stream),
+              "hipMemcpyAsync");
+        check(hipGraphLaunch(h->rec_graph[slot], h->stream), "hipGraphLaunch");
+        if (P > 0) {
+            check(hipMemcpy2DAsync(d_out + k0, steps * sizeof(double), h->rec_out_stage, every * sizeof(double),
+                                   every * sizeof(double), P, hipMemcpyDeviceToDevice, h->stream),
+                  "hipMemcpy2DAsync");
+        }
+        check(hipMemcpyAsync(hist, h->rec_hist_stage, every * G * e, hipMemcpyDeviceToDevice, h->stream), "history");
+        h->prev = static_cast<int>(slot ^ (every & 1u));
+        h->done += every;
     }
     if (out != nullptr && P > 0) {
         check(hipMemcpyAsync(out, d_out, P * steps * sizeof(double), hipMemcpyDeviceToHost, h->stream), "hipMemcpyAsync");
@@ -2742,29 +2843,41 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     unsigned long long m = 0;
     for (size_t seg = segments; seg-- > 0;) {
         const size_t k0 = seg * every, k1 = std::min(steps, k0 + every);
-        // The checkpoint into slots 0 (p(k0 - 1)) and 1 (p(k0)).
-        const char *base = h->rec_ckpt + seg * per;
-        check(hipMemcpyAsync(h->field[0], base, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        check(hipMemcpyAsync(h->field[1], base + field, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-        if (states) {
-            check(hipMemcpyAsync(h->velocity[0], base + 2 * field, states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
-            check(hipMemcpyAsync(h->velocity[1], base + 2 * field + states, states, hipMemcpyDeviceToDevice, h->stream),
-                  "ckpt");
+        // The segment's history: the record's, copied in where it kept it; else recomputed
+        // from the checkpoint, restored into slots 0 (p(k0 - 1)) and 1 (p(k0)).
+        const bool kept = h->rec_hist != nullptr;
+        if (kept) {
+            const size_t rows = std::max<size_t>(G, 1) * e;
+            check(hipMemcpyAsync(history, static_cast<const char *>(h->rec_hist) + k0 * rows, (k1 - k0) * rows,
+                                 hipMemcpyDeviceToDevice, h->stream),
+                  "history");
+        } else {
+            const char *base = h->rec_ckpt + seg * per;
+            check(hipMemcpyAsync(h->field[0], base, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+            check(hipMemcpyAsync(h->field[1], base + field, field, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+            if (states) {
+                check(hipMemcpyAsync(h->velocity[0], base + 2 * field, states, hipMemcpyDeviceToDevice, h->stream), "ckpt");
+                check(hipMemcpyAsync(h->velocity[1], base + 2 * field + states, states, hipMemcpyDeviceToDevice,
+                                     h->stream),
+                      "ckpt");
+            }
         }
         // Adjoint steps n from min(k1, steps - 1) down to k0 + 1 (Q(steps) is 0: p(steps) is
         // read by no probe; Q(0) enters no gradient).
         const size_t top = std::min(k1, steps - 1);
         const bool whole = k1 - k0 == every && top == k1 && h->graph_steps > 0;
         if (!whole) {
-            recompute(k1 - k0, h->rec_signal + k0);
+            if (!kept) recompute(k1 - k0, h->rec_signal + k0);
             if (top > k0) backward(k0, top - k0, d_g, steps, 0, m);
             m += top > k0 ? top - k0 : 0;
             continue;
         }
         check(hipMemcpyAsync(sig_stage, h->rec_signal + k0, every * sizeof(float), hipMemcpyDeviceToDevice, h->stream),
               "hipMemcpyAsync");
-        capture(0, [&] { recompute(every, sig_stage); });
-        check(hipGraphLaunch(graphs[0], h->stream), "hipGraphLaunch");
+        if (!kept) {
+            capture(0, [&] { recompute(every, sig_stage); });
+            check(hipGraphLaunch(graphs[0], h->stream), "hipGraphLaunch");
+        }
         if (P > 0) {
             check(hipMemcpy2DAsync(g_stage, (every + 1) * sizeof(double), d_g + k0, steps * sizeof(double),
                                    (every + 1) * sizeof(double), P, hipMemcpyDeviceToDevice, h->stream),
@@ -2871,6 +2984,7 @@ void as_fdtd_destroy(as_fdtd *h) {
         if (g) (void)hipGraphExecDestroy(g);
         g = nullptr;
     }
+    free_record_graphs(h);
     free_record(h);
     for (void *p : h->allocations) (void)hipFree(p);
     if (h->signal_buf) (void)hipFree(h->signal_buf);

@@ -30,13 +30,14 @@ import numpy as np
 
 from acoustic_spaces_amd.gpu import HipWaveSolver
 from acoustic_spaces_amd.solver import WaveSolver
+from acoustic_spaces_amd.wave import high_pass
 
 
 class WallModel:
     """The admittance model above on a scene's room (its mesh's materials are the walls)."""
 
     def __init__(self, scene, corner_hz: float = 177.0, every: int = 256, precision: int = 32,
-                 steps: int | None = None):
+                 steps: int | None = None, keep_history=None):
         self.scene = scene
         self.walls = scene.mesh.materials
         T = 1.0 / scene.rate
@@ -45,9 +46,15 @@ class WallModel:
         self.before = 2.0 * self.wcT / (1.0 + self.wcT)
         self.corner_hz = corner_hz
         self.every = every
+        self.keep_history = keep_history  # see HipWaveSolver: None, the library's choice
         self.precision = precision
         self.steps = steps or scene.steps
-        self.drive = np.ascontiguousarray(scene.drive[: self.steps], np.float32)
+        # The scene's pulse high-passed as the reference's source signal is (engine/wave.py:
+        # 281-327, "so it leaves no air in the room"): a closed room would otherwise hold the
+        # pulse's net volume as a constant pressure as large as the sound, on which the
+        # float scheme's rounding (relative to |p|) sits some 55 dB under the early sound,
+        # at the decay curves' floor.
+        self.drive = np.ascontiguousarray(high_pass(scene.drive[: self.steps], scene.rate), np.float32)
         # The layout's structure (which cells are special, their states) holds for every theta;
         # lay it out once with placeholder values, one branch per wall.
         placeholder = 0.1
@@ -101,7 +108,8 @@ class WallModel:
     def record(self, theta):
         """The forward half of :meth:`gradient`: (y, the stepper holding the run's
         checkpoints, for :meth:`backward`)."""
-        stepper = HipWaveSolver(self.coefficients(theta), precision=self.precision, no_faces=True)
+        stepper = HipWaveSolver(self.coefficients(theta), precision=self.precision, no_faces=True,
+                                keep_history=self.keep_history)
         try:
             y = stepper.record(self.drive, every=self.every)
         except Exception:
@@ -204,28 +212,74 @@ def band_gains(n: int, rate: float, centre: float, order: int = 3):
     return hp * lp
 
 
+def source_high_pass(n_fft: int, rate: float) -> np.ndarray:
+    """The frequency response, on the rFFT grid of n_fft, of the reference's source
+    high-pass taken twice (an impulse less a unit-area Hann window of 0.12 s, engine/wave.py:
+    281-327; acoustic_spaces_amd.wave.high_pass): zero at DC, so a signal's constant part,
+    the air a pulse leaves in a closed room, does not pass."""
+    from acoustic_spaces_amd.wave import _hann_high_pass_taps
+
+    taps = _hann_high_pass_taps(rate)
+    return np.fft.rfft(np.convolve(taps, taps), n_fft)
+
+
 def decay_loss(y, target, rate: float, bands=(125.0, 250.0), early_s: float = 0.05, floor_db: float = -40.0,
-               early_weight: float = 100.0):
+               early_weight: float = 1e4):
     """L2 on the band-filtered Schroeder decay curves (dB, down to floor_db of the target's
     start, averaged over the samples kept) plus early_weight times the L2 on the first
     early_s seconds of pressure (relative to the target's energy there): y and target
     (probes, steps) torch tensors. The decay curves fix the walls' absorption together; the
-    early reflections tell the walls apart."""
+    early reflections tell the walls apart. early_weight 1e4 is 10^(SNR / 10) for targets at
+    40 dB SNR: a pressure misfit at the noise level then weighs about as much as 1 dB^2 of
+    decay curve (the early pressure is the maximum-likelihood term under white noise; on
+    the 6 x 4 x 3 m shoebox a weight of 100 left the fit's minimum 0.02 from the true
+    absorption, 1e4 within 0.007).
+
+    The model's curve carries the target's noise: the target's band power per sample,
+    read late in the record, integrated backwards and added, so that the curves compare the
+    decay, not the noise floor the target has and the model lacks; they are compared while
+    the target's remaining sound is at least its remaining noise.
+
+    Both terms read the signals high-passed as the reference's source signal is
+    (source_high_pass), through filters zero-padded to twice the record (no wrap-around):
+    a Gaussian pulse leaves air in a closed room, a constant pressure as large as the
+    sound, which in float walks with the rounding (about 0.7 % of the signal over 0.5 s on
+    the 6 x 4 x 3 m shoebox); an unpadded band filter wraps its step at the record's end
+    into the bands near the decay's floor, where the curves' logarithm magnifies it."""
     torch = _torch()
     n = y.shape[-1]
+    n_fft = 1 << int(math.ceil(math.log2(2 * n)))
+    high = source_high_pass(n_fft, rate)
+
+    def filtered(x, gain):
+        spectrum = torch.fft.rfft(x, n=n_fft, dim=-1) * torch.from_numpy(gain)
+        return torch.fft.irfft(spectrum, n=n_fft, dim=-1)[..., :n]
+
+    def backward_integral(power):
+        return torch.flip(torch.cumsum(torch.flip(power, dims=[-1]), dim=-1), dims=[-1])
+
+    def level(energy):
+        return 10.0 * torch.log10(energy / energy[..., :1] + 1e-30)
+
+    remaining = torch.arange(n, 0, -1, dtype=y.dtype)  # samples from each to the end
     total = y.new_zeros(())
     for centre in bands:
-        gains = torch.from_numpy(band_gains(n, rate, centre)).to(y.dtype)
-
-        def curve(x):
-            xb = torch.fft.irfft(torch.fft.rfft(x, dim=-1) * gains, n=n, dim=-1)
-            energy = torch.flip(torch.cumsum(torch.flip(xb * xb, dims=[-1]), dim=-1), dims=[-1])
-            return 10.0 * torch.log10(energy / energy[..., :1] + 1e-30)
-
-        cy, ct = curve(y), curve(target)
-        mask = (ct > floor_db).to(y.dtype)
+        gain = high * band_gains(n_fft, rate, centre)
+        tb = filtered(target, gain)
+        # The target's noise in the band, a power per sample read from 75 to 95 % of the
+        # record (where the sound has decayed under it, clear of the filter's taper at the
+        # end): added, as its backward integral, to the model's curve, so that both level
+        # off alike; compared where the target's remaining sound is at least its remaining
+        # noise (its energy twice the noise's) and above floor_db.
+        noise = torch.mean(tb[..., (3 * n) // 4: (19 * n) // 20] ** 2, dim=-1, keepdim=True).detach()
+        left_t = backward_integral(tb * tb)
+        ct = level(left_t)
+        yb = filtered(y, gain)
+        cy = level(backward_integral(yb * yb) + noise * remaining)
+        mask = ((left_t > 2.0 * noise * remaining) & (ct > floor_db)).to(y.dtype).detach()
         total = total + torch.sum(mask * (cy - ct) ** 2) / torch.sum(mask)
     k = int(early_s * rate)
-    scale = torch.sum(target[..., :k] ** 2) + 1e-30
-    total = total + early_weight * torch.sum((y[..., :k] - target[..., :k]) ** 2) / scale
+    yh, th = filtered(y, high), filtered(target, high)
+    scale = torch.sum(th[..., :k] ** 2) + 1e-30
+    total = total + early_weight * torch.sum((yh[..., :k] - th[..., :k]) ** 2) / scale
     return total

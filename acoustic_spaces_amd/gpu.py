@@ -26,18 +26,22 @@ class HipWaveSolver:
     each kind of cell on its own instead of one launch per step; ``face_blocks`` steps the
     face specials by blocks of their own (True) or in their tiles (False; None: the
     library's choice); ``pairs`` takes two steps by two launches where that applies (True,
-    or 8 or 16 for the planes a box marches)."""
+    or 8 or 16 for the planes a box marches). ``keep_history`` (record and adjoint): True keeps
+    every step's history of the special cells in the record, so that the adjoint recomputes
+    nothing; False keeps the checkpoints only, each segment recomputed from its own; None,
+    the library's choice (the history where it fits 16 GiB and a quarter of free memory)."""
 
     def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
                  graph_steps: int = 0, split: bool = False, tile_order: int = 0,
-                 no_faces: bool = False, face_blocks=None, pairs: bool = False) -> None:
+                 no_faces: bool = False, face_blocks=None, pairs: bool = False, keep_history=None) -> None:
         lib = native.load()
         layout = solver if isinstance(solver, dict) else solver.layout()
         struct, keep = native.pack_layout(layout)
         tuning = native.Tuning(0, int(tile_z), int(row_align), int(graph_steps), 1 if split else 0,
                                int(tile_order), 1 if no_faces else 0,
                                0 if face_blocks is None else (1 if face_blocks else -1),
-                               int(pairs) if not isinstance(pairs, bool) and int(pairs) > 1 else (1 if pairs else 0))
+                               int(pairs) if not isinstance(pairs, bool) and int(pairs) > 1 else (1 if pairs else 0),
+                               0 if keep_history is None else (1 if keep_history else -1))
         h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
         if not h:
             message = lib.as_fdtd_create_error().decode()

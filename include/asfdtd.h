@@ -101,6 +101,12 @@ typedef struct as_tuning {
                           one speed of sound): the first steps every interior cell twice where
                           its neighbours are interior, by boxes of 60 x 4 cells marching 8
                           planes (16 here asks for 16); 0: one step a launch */
+    int keep_history;  /* the adjoint's record (as_fdtd_record): 1 keeps every step's history of
+                          the special cells (steps x specials values), so that the backward
+                          pass needs no recompute; -1 keeps only the checkpoints, and the
+                          backward pass recomputes each segment from its own; 0, the library's
+                          choice: the history where it takes at most 16 GiB and a quarter of
+                          the free device memory */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the

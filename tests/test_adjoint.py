@@ -134,3 +134,29 @@ def test_record_then_adjoint_equals_gradient():
     assert np.array_equal(y, y2) and np.array_equal(g, g2) and np.array_equal(n, n2)
     assert not np.any(gpu.field())
     assert np.array_equal(gpu.run(drive[:50]), y[:, :50])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,stencil", [(32, 7), (64, 7), (32, 19)])
+def test_kept_history_equals_recompute(precision, stencil):
+    """The record that keeps every step's history of the special cells (so that the
+    backward pass recomputes nothing) gives the gradients of the record that keeps only
+    checkpoints and recomputes each segment, bit for bit: both are the same kernels'
+    arithmetic. Whole segments and a partial last one (250 steps, every 64 and 48), and
+    the probes' signals too."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    sc, layout = _layout(stencil)
+    drive = sc.drive[:250]
+    rng = np.random.default_rng(17)
+    probes = len(layout["probe_start"]) - 1
+    c = rng.normal(size=(probes, len(drive)))
+    for every in (64, 48):
+        out = {}
+        for keep in (True, False):
+            with HipWaveSolver(layout, precision=precision, no_faces=True, keep_history=keep) as s:
+                out[keep] = s.gradient(drive, c, every=every)
+        print(f"precision {precision}, stencil {stencil}, every {every}", flush=True)
+        for a, b in zip(out[True], out[False]):
+            assert np.array_equal(a, b)
+        assert np.any(out[True][1] != 0)
