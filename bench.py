@@ -688,7 +688,7 @@ def main(argv=None) -> int:
         args.steps = args.steps or 5941
         scene = scenes.shoebox()
         args.outputs = not args.no_outputs
-        line = plain(scene, "shoebox", args)
+        line = plain(scene, "shoebox", args, traffic_name="step_kernel_shoebox_traffic")
     elif args.workload == "outdoor":
         args.steps = args.steps or 2377
         scene = scenes.outdoor()
