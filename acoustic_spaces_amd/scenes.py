@@ -232,10 +232,12 @@ class Scene:
     layer: int = 0
     layer_damping: float = 0.0
     edge_admittance: float = 1.0
+    speed: np.ndarray | None = None  # (nx ny nz,) each cell's speed of sound over the grid's, or None
 
     def solver_args(self) -> dict:
         """The keyword arguments of the solver's constructor."""
-        return dict(
+        extra = {} if self.speed is None else {"speed": self.speed}
+        return extra | dict(
             vertices=self.mesh.vertices,
             faces=self.mesh.faces,
             face_material=self.mesh.face_material,
@@ -442,3 +444,24 @@ def rooms_batch(count=64, cell=0.05, seed=3, length_s=0.5, receivers=4, size_low
         rooms.append(shoebox(size=size, cell=cell, seed=int(rng.integers(1 << 30)), length_s=length_s,
                              receivers=receivers, absorption=alpha))
     return rooms
+
+
+def with_weather(scene: Scene, atmosphere, ground_c: float = 20.0, ground_z: float | None = None) -> Scene:
+    """``scene`` outdoors under ``atmosphere`` (acoustic_spaces_amd.weather.Atmosphere):
+    each cell's effective speed of sound from the source (the wind's component along the
+    horizontal from it added to the still air's at the cell's height above the ground at
+    ``ground_z``, whose temperature is ``ground_c``), over the grid's speed, the fastest
+    over the grid; the time step set by that speed (the reference's prepare,
+    engine/wave.py:1000-1060), the length and the drive kept."""
+    from acoustic_spaces_amd.weather import Weather
+
+    g = scene.grid
+    ground = float(scene.mesh.vertices[:, 2].min()) if ground_z is None else float(ground_z)
+    weather = Weather(atmosphere, ground_c, ground, scene.source)
+    fastest = weather.extremes(g.origin, g.cell, g.shape)[1]
+    courant = COURANT if scene.stencil == 7 else ISOTROPIC_COURANT
+    rate = fastest / (courant * g.cell)
+    length_s = scene.steps / scene.rate
+    steps = int(round(length_s * rate))
+    return dataclasses.replace(scene, name=scene.name + ", weather", rate=rate, steps=steps,
+                               drive=gaussian_drive(rate, steps), speed=weather.ratios(g.origin, g.cell, g.shape, fastest))
