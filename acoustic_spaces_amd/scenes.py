@@ -465,3 +465,24 @@ def with_weather(scene: Scene, atmosphere, ground_c: float = 20.0, ground_z: flo
     steps = int(round(length_s * rate))
     return dataclasses.replace(scene, name=scene.name + ", weather", rate=rate, steps=steps,
                                drive=gaussian_drive(rate, steps), speed=weather.ratios(g.origin, g.cell, g.shape, fastest))
+
+
+def trace_model(mesh: Mesh, absorption, *, transparent=None, outdoor=None, ground: float = 0.0, atmosphere=None,
+                temperature_c: float = 20.0, bands: str = "octave", volume_m3: float | None = None):
+    """A model as the reference's wave stage reads its TraceModel (engine/model.py:146-227):
+    ``room.mesh`` (vertices, faces), ``room.face_materials``, ``room.material_names``,
+    ``room.outdoor``, ``room.volume_m3``, ``room.scene.atmosphere`` and
+    ``room.scene.air.temperature_c``; ``absorption`` per face (faces x bands, from the
+    materials' rows of ``absorption``, materials x bands), ``transparent``, ``transmission``
+    (None), ``face_impedance`` (None), ``bands``, ``ground``."""
+    from types import SimpleNamespace
+
+    rows = np.asarray(absorption, float)
+    fm = np.asarray(mesh.face_material, np.int64)
+    face_count = len(mesh.faces)
+    scene = SimpleNamespace(atmosphere=atmosphere, air=SimpleNamespace(temperature_c=temperature_c))
+    room = SimpleNamespace(mesh=SimpleNamespace(vertices=mesh.vertices, faces=mesh.faces), face_materials=fm,
+                           material_names=list(mesh.names), outdoor=outdoor, volume_m3=volume_m3, scene=scene)
+    return SimpleNamespace(room=room, absorption=rows[fm],
+                           transparent=np.zeros(face_count, bool) if transparent is None else np.asarray(transparent),
+                           transmission=None, face_impedance=None, bands=bands, ground=ground)

@@ -292,8 +292,8 @@ class WaveResponse:
         return out
 
 
-def wave_responses(scene, listeners, names, *, length_s: float, bands: int = 3, precision: int = 32,
-                   say=None, label: str = "") -> tuple[list, dict]:
+def scene_responses(scene, listeners, names, *, length_s: float, bands: int = 3, precision: int = 32,
+                    say=None, label: str = "") -> tuple[list, dict]:
     """The responses from ``scene``'s source to each of ``listeners`` (L, 3) on the scene's
     own grid (its cell and rate), ``length_s`` seconds from emission, and run details:
     the scene's set-up with 13 points a listener, the reference's drive with its lead, the
@@ -337,4 +337,294 @@ def wave_responses(scene, listeners, names, *, length_s: float, bands: int = 3, 
         part = slice(PROBES * i, PROBES * (i + 1))
         sig = first_order(recorded[part], found[part], name, courant, cutoff(courant))
         responses.append(WaveResponse(scene.rate, sig, bands, h, lead_s=lead / scene.rate))
+    return responses, info
+
+
+# --- the reference's entry point -----------------------------------------------------------
+
+#: The stencils' points and Courant numbers, by scheme (engine/wave.py:116-130).
+STENCILS = {"standard": (7, COURANT), "isotropic": (19, ISOTROPIC_COURANT)}
+#: Link weights of an axial and an edge link, by the stencil's points.
+LINK_WEIGHTS = {7: (1.0 / 3.0, 0.0), 19: (0.25, 0.125)}
+MAX_CELLS = 40_000_000                      # grids of more cells are refused
+WAVE_BANDS_HZ = analysis.OCTAVE_BANDS_HZ[:4]  # the highest band a run may reach
+LAYER_CELLS = 8                              # the matched layer round an open model
+LAYER_DAMPING = 0.5
+LAYER_ROOM = 0.15   # room past the points, a share of their farthest spread across each axis
+LONG_RUN = 2e9      # cell updates beyond which a run reports its tenths
+
+
+def band_top(band: int, total: int = len(analysis.OCTAVE_BANDS_HZ)) -> float:
+    """The highest frequency band ``band`` covers (Hz): its upper crossover's top."""
+    edge = analysis.crossover_frequencies(total)[band]
+    return float(edge * 2.0 ** (analysis.crossover_width(total) / 2))
+
+
+@dataclass(frozen=True)
+class BandGrid:
+    """A band's grid: its cell (m) and its sample rate (Hz)."""
+
+    band: int
+    cell: float
+    rate: float
+
+
+def band_grid(band: int, sample_rate: float, speed_of_sound: float, points_per_wavelength: float = 8.0,
+              total: int = len(analysis.OCTAVE_BANDS_HZ), courant: float = COURANT) -> BandGrid:
+    """The coarsest grid with ``points_per_wavelength`` cells a wavelength at the top of
+    ``band`` whose step is a whole number of output samples, or one over a whole number
+    (engine/wave.py:253-272): the rate speed / (courant cell) moved to the nearest such
+    rate at or above it, the cell then taken from the rate."""
+    cell = speed_of_sound / (points_per_wavelength * band_top(band, total))
+    rate = speed_of_sound / (courant * cell)
+    ratio = sample_rate / rate
+    rate = sample_rate / math.floor(ratio) if ratio >= 1.0 else sample_rate * math.ceil(1.0 / ratio)
+    return BandGrid(band, speed_of_sound / (courant * rate), rate)
+
+
+def slow_by(stencil: int, per_wavelength: float, ratio: float = 1.0) -> float:
+    """How much slower than sound (a fraction) a wave of ``per_wavelength`` cells a
+    wavelength runs along the grid's axes: from the stencil's dispersion relation
+    2 - 2 cos(w T) = ratio^2 sum_k g_k (1 - cos(k . d_k)) (engine/wave.py:216-245)."""
+    axial, edge = LINK_WEIGHTS[stencil]
+    courant = STENCILS["standard" if stencil == 7 else "isotropic"][1]
+    theta = 2.0 * math.pi / per_wavelength
+    c = math.cos(theta)
+    total = 2.0 * axial * (1.0 - c) + 4.0 * edge * (2.0 * (1.0 - c))  # two edge pairs hold the axis
+    turned = math.acos(1.0 - ratio ** 2 * total / 2.0)
+    return 1.0 - turned / (ratio * courant * theta)
+
+
+def _spread_across(source, listeners) -> np.ndarray:
+    """(3,): the farthest a listener lies from the source in the plane across each axis."""
+    d = np.asarray(listeners, float) - np.asarray(source, float)
+    if len(d) == 0:
+        return np.zeros(3)
+    square = (d * d).sum(axis=1)[:, None] - d * d
+    return np.sqrt(square.max(axis=0))
+
+
+def _band_centres(bands_kind: str) -> list:
+    return list(analysis.OCTAVE_BANDS_HZ) if bands_kind == "octave" else list(analysis.THIRD_OCTAVE_BANDS_HZ)
+
+
+def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_s: float, sample_rate: float,
+                   speed_of_sound: float, points_per_wavelength: float = 8.0, threads: int = 0,
+                   device: str = "gpu", scheme: str = "auto", say=None, label: str = "",
+                   wind: str = "effective", walls: str = "cell_faces") -> tuple[list, dict]:
+    """The wave bands of the responses from ``source`` to each of ``listeners`` in the room
+    of ``model``, and run details: the reference's entry point (engine/wave.py:805-1271),
+    its arguments, checks, results (``WaveResponse`` each) and ``info`` keys, stepped on the
+    GPU.
+
+    ``model`` is read as the reference's TraceModel is: ``room.mesh`` (``vertices``,
+    ``faces``), ``room.face_materials``, ``room.material_names``, ``room.outdoor`` (None
+    indoors), ``room.volume_m3``, ``absorption`` (faces x bands), ``transparent`` (faces),
+    ``transmission`` (faces x bands, or None), ``face_impedance`` (None), ``bands``
+    ("octave" or "third_octave"), ``ground`` (outdoors) and, outdoors,
+    ``room.scene.atmosphere`` (an acoustic_spaces_amd.weather.Atmosphere, or None) and
+    ``room.scene.air.temperature_c``.
+
+    As the reference: each material's admittance is fitted to its absorption in the wave
+    bands at the grid's rate; the grid is ``band_grid``'s for the highest band; the stencil
+    is the isotropic one (``scheme`` "auto") unless the model is open, where the standard
+    one with its 8-cell matched layer steps it (outdoors at once); outdoors under a weather,
+    each cell's effective speed of sound from the source (the wind's component along the
+    horizontal added). Differences, each refused or documented: ``device`` "cpu" (this
+    package steps on the GPU), ``wind`` "moving" (the reference steps moving air on the CPU
+    only), ``walls`` "fitted" (not laid out here), materials with an impedance (a porous
+    ground's fit), and the room round an open model's points taken on every side (the
+    reference leaves it off sides its surfaces close, ``_open_sides``)."""
+    from acoustic_spaces_amd.admittance import MAX_ABSORPTION, fit_admittance
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import Bvh, WaveSolver
+    from acoustic_spaces_amd.weather import Weather
+
+    say = say or (lambda _: None)
+    if device not in ("gpu", "auto", "hip", "cpu"):
+        raise ValueError(f"unknown device {device!r}; use cpu, gpu or auto")
+    if device == "cpu":
+        raise ValueError("this package steps the wave grid on the GPU: set simulation.wave.device to gpu or auto")
+    if max_band_hz not in WAVE_BANDS_HZ:
+        raise ValueError(f"max_band_hz must be one of {WAVE_BANDS_HZ}, got {max_band_hz}")
+    if scheme not in ("auto", "standard", "isotropic"):
+        raise ValueError(f"unknown scheme {scheme!r}; use auto, standard or isotropic")
+    if wind not in ("effective", "moving"):
+        raise ValueError(f"unknown wind {wind!r}; use effective or moving")
+    if walls not in ("cell_faces", "fitted"):
+        raise ValueError(f"unknown walls {walls!r}; use cell_faces or fitted")
+    if walls == "fitted":
+        raise ValueError("fitted walls: not laid out by this package; set simulation.wave.walls to cell_faces")
+    for what, value in (("points_per_wavelength", points_per_wavelength), ("length_s", length_s),
+                        ("sample_rate", sample_rate), ("speed_of_sound", speed_of_sound)):
+        if not (math.isfinite(value) and value > 0.0):
+            raise ValueError(f"{what} must be a positive number, not {value!r}")
+    listeners = np.asarray(listeners, float).reshape(-1, 3)
+    if len(names) != len(listeners):
+        raise ValueError(f"{len(names)} names for {len(listeners)} listeners")
+    room = model.room
+    mesh = room.mesh
+    solid = ~np.asarray(model.transparent, bool)
+    if not solid.any():
+        raise ValueError("the wave solver needs a surface that reflects: every face of the model lets all "
+                         "sound through (an opening)")
+    face_materials = np.asarray(room.face_materials)
+    if getattr(model, "transmission", None) is not None:
+        partial = np.unique(face_materials[solid & np.any(np.asarray(model.transmission) > 0, 1)])
+        if len(partial):
+            kinds = ", ".join(repr(room.material_names[m]) for m in partial)
+            raise ValueError(f"the wave solver cannot pass sound through a surface; {kinds} let some through "
+                             "(openings, with transmission 1, are fine): set simulation.wave.enabled to false or "
+                             "give them no transmission")
+    if getattr(model, "face_impedance", None) is not None:
+        raise ValueError("materials with an impedance (a porous ground's) are not fitted by this package")
+    faces = np.ascontiguousarray(np.asarray(mesh.faces)[solid], np.int64)
+    vertices = np.asarray(mesh.vertices, float)
+    bvh = Bvh(vertices, faces)
+    source = np.asarray(source, float).reshape(3)
+    points = np.vstack([source[None, :], listeners])
+    centres = _band_centres(model.bands)
+    per_octave = 1 if model.bands == "octave" else 3
+    bands = (analysis.OCTAVE_BANDS_HZ.index(max_band_hz) + 1) * per_octave
+    total = len(centres)
+    rows, face_material = np.unique(np.round(np.asarray(model.absorption, float)[solid, :bands], 6), axis=0,
+                                    return_inverse=True)
+    face_material = np.asarray(face_material, np.int64).reshape(-1)
+    outdoor = getattr(room, "outdoor", None) is not None
+    weather = None
+    if outdoor:
+        scene = getattr(room, "scene", None)
+        atmosphere = getattr(scene, "atmosphere", None)
+        if atmosphere is not None and not atmosphere.still:
+            if wind == "moving" and atmosphere.wind(np.array([10.0]))[0] != 0.0:
+                raise ValueError("moving air: the reference steps it on the CPU only; set simulation.wave.wind "
+                                 "to effective")
+            weather = Weather(atmosphere, float(scene.air.temperature_c), float(model.ground), source)
+
+    def layout(courant, layer, speed, per_wavelength):
+        grid = band_grid(bands - 1, sample_rate, speed, per_wavelength, total, courant)
+        h = grid.cell
+        low = np.minimum(vertices.min(axis=0), points.min(axis=0) - 2.0 * h)
+        high = np.maximum(vertices.max(axis=0), points.max(axis=0) + 2.0 * h)
+        offset = h * (2.0 + layer + np.array([0.0137, 0.0219, 0.0291]))
+        if layer:
+            room_past = LAYER_ROOM * _spread_across(points[0], points[1:])
+            below = np.ceil(np.maximum(low - (points.min(axis=0) - room_past), 0.0) / h)
+            if outdoor:
+                below[2] = 0.0
+            low = low - h * below
+            high = np.maximum(high, points.max(axis=0) + room_past)
+        origin = low - offset
+        shape = np.ceil((high - origin) / h).astype(np.int64) + 2 + layer
+        if int(np.prod(shape)) > MAX_CELLS:
+            what = " with the matched layer round this open model" if layer else ""
+            raise ValueError(f"the wave grid up to the {max_band_hz} Hz band would have {int(np.prod(shape)):,} "
+                             f"cells{what} (at most {MAX_CELLS:,}); lower max_band_hz or points_per_wavelength")
+        return grid, origin, shape
+
+    def prepare(stencil, layer):
+        points_n, courant = STENCILS[stencil]
+        speeds, extremes, still = None, None, speed_of_sound
+        if weather is None:
+            grid, origin, shape = layout(courant, layer, speed_of_sound, points_per_wavelength)
+        else:
+            # Fine enough for the slowest sound, in steps for the fastest, over the grid
+            # those give (the reference's eight rounds).
+            slowest = fastest = float(weather.at(source)[0])
+            for _ in range(8):
+                grid, origin, shape = layout(courant, layer, fastest, points_per_wavelength * fastest / slowest)
+                low_speed, high_speed = weather.extremes(origin, grid.cell, shape)
+                if low_speed >= slowest and high_speed <= fastest:
+                    break
+                slowest, fastest = min(slowest, low_speed), max(fastest, high_speed)
+            speeds = weather.ratios(origin, grid.cell, shape, courant * grid.cell * grid.rate)
+            extremes, still = (low_speed, high_speed), low_speed
+        fits = [fit_admittance(row, total, grid.rate) for row in rows]
+        branches = [f.branches(grid.rate) for f in fits]
+        branch_material = np.concatenate([np.full(len(b), m, np.int64) for m, b in enumerate(branches)]) \
+            if branches else np.zeros(0, np.int64)
+        probes = np.vstack([probe_points(p, grid.cell) for p in listeners])
+        solver = WaveSolver(bvh, face_material, np.array([f.beta for f in fits]),
+                            np.vstack(branches) if branches else np.zeros((0, 3)), branch_material, origin,
+                            grid.cell, int(shape[0]), int(shape[1]), int(shape[2]), source, probes, 1.0, layer,
+                            LAYER_DAMPING, threads, points_n, speeds, probes_per_listener=PROBES)
+        return solver, grid, fits, probes, courant, extremes, still
+
+    if scheme == "isotropic" and outdoor:
+        raise ValueError("outdoors the isotropic stencil is not available (it has no matched layer for the sky): "
+                         "set simulation.wave.scheme to auto or standard")
+    built = None
+    if outdoor:
+        built = prepare("standard", LAYER_CELLS)
+        if not built[0].open:
+            built = None
+    if built is None:
+        built = prepare("standard" if scheme == "standard" else "isotropic", 0)
+        if built[0].open:
+            if scheme == "isotropic":
+                raise ValueError("the isotropic stencil has no matched layer for an open model: set "
+                                 "simulation.wave.scheme to auto or standard")
+            built = prepare("standard", LAYER_CELLS)
+    solver, grid, fits, probes, courant, extremes, still = built
+    h = grid.cell
+    at_source = 1.0
+    if weather is not None:
+        at_source = min(float(weather.at(source)[0]) / (courant * h * grid.rate), 1.0)
+    signal, delay = source_signal(grid.rate, h, courant, at_source)
+    samples = int(math.ceil(length_s * grid.rate))
+    whole = max(1, round(grid.rate / sample_rate))  # steps per output sample
+    lead = -(-delay // whole) * whole
+    steps = lead + samples
+    drive = np.zeros(steps, np.float32)
+    drive[lead - delay: lead - delay + len(signal)] = signal[: steps - lead + delay]
+    found = np.array([solver.probe_found(p) and solver.probe_in_sight(p) for p in range(len(probes))])
+    found = off_the_source(found, probes, source, h, solver.stencil).tolist()
+    missing = [n for i, n in enumerate(names) if not found[PROBES * i]]
+    if missing:
+        air, volume = solver.air_cells * h ** 3, getattr(room, "volume_m3", None)
+        if volume and air < 0.1 * volume:
+            who = f"source {label!r}" if label else "the source"
+            raise ValueError(f"{who} is closed in: the wave grid has {air:.2g} m³ of air round it, in a room of "
+                             f"{volume:.3g} m³")
+        raise ValueError(f"listener {missing[0]!r} is not in the air of the wave grid")
+    band_name = f"{centres[0]} Hz" if bands == 1 else f"{centres[0]}-{centres[bands - 1]} Hz"
+    where = ""
+    if extremes is not None:
+        where = f", sound at {extremes[0]:.0f} to {extremes[1]:.0f} m/s"
+    say(f"{label}: wave solver, {band_name} ({solver.air_cells:,} cells of {100 * h:.0f} cm, {steps:,} steps "
+        f"on the GPU{where})")
+    who = f"source {label!r}" if label else "the source"
+    tenths = 10 if solver.air_cells * steps > LONG_RUN else 1
+    with HipWaveSolver(solver) as stepper:
+        if tenths == 1:
+            recorded = stepper.run(drive)
+        else:
+            recorded = np.empty((len(probes), steps))
+            bounds = np.linspace(0, steps, tenths + 1).astype(np.int64)
+            for c in range(tenths):
+                if c:
+                    say(f"{label}: wave solver, {band_name}, {100 * c // tenths} %")
+                recorded[:, bounds[c]: bounds[c + 1]] = stepper.run(drive[bounds[c]: bounds[c + 1]])
+    if not np.all(np.isfinite(recorded)):
+        raise ValueError(f"{who}: the wave solver's run did not stay finite")
+    info = {"wave_cells": float(solver.air_cells), "wave_cell_m": h, "wave_steps": float(steps),
+            "wave_stencil": float(solver.stencil), "wave_gpu": 1.0}
+    if solver.open:
+        info["wave_open"] = 1.0
+    if extremes is not None:
+        info["wave_speed_min_m_s"], info["wave_speed_max_m_s"] = extremes
+    slowest = speed_of_sound if extremes is None else extremes[0]
+    per_wavelength = slowest / (band_top(bands - 1, total) * h)
+    info["wave_slow_percent"] = 100.0 * slow_by(solver.stencil, per_wavelength, still / (courant * h * grid.rate))
+    used = np.unique(face_material)
+    wanted = np.minimum(rows[used], MAX_ABSORPTION)
+    info["wave_absorption_miss"] = float(np.max(np.abs(np.array([fits[m].achieved for m in used]) - wanted)))
+    responses = []
+    for i, name in enumerate(names):
+        part = slice(PROBES * i, PROBES * (i + 1))
+        local = courant
+        if weather is not None:
+            local *= float(weather.at(listeners[i])[0]) / (courant * h * grid.rate)
+        signals = first_order(recorded[part], found[part], name, local, cutoff(courant))
+        responses.append(WaveResponse(grid.rate, signals, bands, h, total, lead / grid.rate))
     return responses, info

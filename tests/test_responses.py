@@ -3,7 +3,7 @@ listener's 13 points; the source signal's filters (the low-pass's unit gain, the
 high-pass's zero, the 19-point taps' gain (5 + 4 cos wT) / 9); the first-order gradient's
 exactness on linear fields and its choice of differences; the trapezoidal integral of a
 band-limited pulse; WaveResponse.fed, energy, field and high_pass (the wave bands' gains and
-the resampled run). GPU: wave_responses' W, X, Y, Z equal the
+the resampled run). GPU: scene_responses' W, X, Y, Z equal the
 oracle's run of the same layout and drive, to the bit."""
 
 import math
@@ -182,7 +182,7 @@ def test_wave_responses_equal_the_oracle():
     scene = cases.small_box(seed=31)
     listeners = cases.listeners13().probes[[0, 13]]
     print("wave_responses small box", flush=True)
-    responses, info = wv.wave_responses(scene, listeners, ["a", "b"], length_s=0.05)
+    responses, info = wv.scene_responses(scene, listeners, ["a", "b"], length_s=0.05)
     points = np.vstack([wv.probe_points(p, scene.grid.cell) for p in listeners])
     solver = WaveSolver.from_scene(dataclasses.replace(scene, probes=points), probes_per_listener=13)
     signal, delay = wv.source_signal(scene.rate, scene.grid.cell)
