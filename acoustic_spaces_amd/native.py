@@ -56,6 +56,9 @@ SYMBOLS = (
     "as_fdtd_destroy", "as_fdtd_gradient", "as_fdtd_record", "as_fdtd_adjoint",
 )
 
+#: The device post-processing's symbols (acoustic_spaces_amd.device_responses binds them).
+RESPONSE_SYMBOLS = ("as_first_order_device",)
+
 #: The set-up's symbols (acoustic_spaces_amd.solver binds them).
 SETUP_SYMBOLS = ("as_setup_create", "as_setup_create_error", "as_setup_layout", "as_setup_info",
                  "as_setup_probe", "as_setup_destroy")
@@ -100,6 +103,8 @@ def load():
     lib.as_fdtd_record.argtypes = [P, P, SIZE, SIZE, P]
     lib.as_fdtd_adjoint.restype = ctypes.c_int
     lib.as_fdtd_adjoint.argtypes = [P, P, P, P]
+    lib.as_first_order_device.restype = ctypes.c_int
+    lib.as_first_order_device.argtypes = [P, SIZE, SIZE, P, P, P, P]
     _lib = lib
     return lib
 

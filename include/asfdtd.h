@@ -184,6 +184,22 @@ int as_fdtd_record(as_fdtd *h, const float *signal, size_t steps, size_t every, 
 int as_fdtd_adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gain);
 
 /*
+ * The receivers' first-order reading on the device (the reference's _first_order,
+ * src/acoustic_spaces/engine/wave.py:757-802), on device pointers: d_probes holds the
+ * stepper's output of `listeners` groups of 13 points (13 x listeners rows of `steps`
+ * doubles, each group the listener, then one cell either side along x, y, z, then two),
+ * d_found one byte per point (read and in sight), d_courant each listener's Courant number.
+ * Writes d_out (listeners x 4 x steps doubles): W, then X, Y, Z as the Courant number times
+ * the running trapezoidal integral of the cell difference along each axis (fourth-order,
+ * second-order or one-sided by the points found; 0 where no point a cell away is), before
+ * the trapezoid's gain correction (a spectral step left to the caller). Enqueued on
+ * `stream` (a hipStream_t; NULL the default stream). The same values as the host reading,
+ * bit for bit.
+ */
+int as_first_order_device(const double *d_probes, size_t steps, size_t listeners, const uint8_t *d_found,
+                          const double *d_courant, double *d_out, void *stream);
+
+/*
  * The set-up on walls on cell faces (the reference's WaveSolver constructor,
  * fdtd.hpp:360-690, bound at cpp/bindings/native_module.cpp:1238-1285): the arguments
  * of the binding, the mesh as vertices (nv x 3) and faces (nf x 3); threads 0 takes the

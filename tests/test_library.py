@@ -9,7 +9,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def _declared():
     text = (ROOT / "include" / "asfdtd.h").read_text()
-    return sorted(set(re.findall(r"\b(as_(?:fdtd|setup)_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(as_(?:fdtd|setup|first)_[a-z_]+)\s*\(", text)))
 
 
 def test_header_declares_the_bound_symbols():
