@@ -12,15 +12,14 @@ Here:
   surface of specific admittance y, in closed form (London's integral of
   1 - |R(theta)|^2 over sin 2 theta); ``admittance(alpha)`` its inverse for a real
   admittance, on the side of hard surfaces (engine/impedance.py:98-112);
-- ``Admittance``: a constant beta plus first-order branches, rising (mass-free,
-  Y_j(s) = G_j (s / w_j) / (1 + s / w_j), the first-order boundary filters of the configs)
-  or falling (stiffness-free, G_j / (1 + s / w_j)), with ``branches(rate)`` in the solver's
-  (m, r, q) form and ``response(f, rate)`` the admittance the scheme realises at ``rate``
-  steps a second;
-- ``fit_admittance(absorption, total, rate)``: beta >= 0, a rising branch from each band's
-  lower edge and a falling one from its upper edge, fitted by bounded least squares so that
-  the band-weighted random-incidence absorption of the scheme's admittance meets each
-  band's.
+- ``Admittance``: a constant beta plus resonant branches G_j / (1 + i Q_j (f/f_j - f_j/f)),
+  with ``branches(rate)`` in the solver's (m, r, q) form and ``response(f, rate)`` the
+  admittance the scheme realises at ``rate`` steps a second;
+- ``fit_admittance(absorption, total, rate)``: beta >= 0 and a branch at each band's centre
+  and a step beyond either end, at the bands' own quality, their conductances fitted by
+  bounded least squares so that the band-weighted random-incidence absorption of the
+  scheme's admittance meets each band's;
+- ``fit_admittance_batch``: the same for many rows at once on the GPU (Monte Carlo draws).
 """
 
 from __future__ import annotations
@@ -94,49 +93,48 @@ def admittance(alpha) -> np.ndarray:
 
 @dataclasses.dataclass(frozen=True)
 class Admittance:
-    """beta plus first-order branches of conductance ``gain`` (1 / R) and corner
-    ``corner_hz``, each rising (``rise`` +1: mass-free, 1 / (R + S / s), the admittance
-    climbing to G above its corner) or falling (-1: stiffness-free, 1 / (M s + R), G below
-    its corner, falling above it); ``achieved``, the band absorption the fit reached."""
+    """beta plus resonant branches Y_j = G_j / (1 + i Q_j (f / f_j - f_j / f)) of
+    conductance ``gain`` (1 / R), centre ``centre_hz`` and quality ``quality`` (the
+    reference's branch 1 / (M s + R + S / s), impedance.py:168-211); ``achieved``, the band
+    absorption the fit reached. Branches of zero conductance add nothing and are not handed
+    to the solver."""
 
     beta: float
     gain: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
-    corner_hz: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
-    rise: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    centre_hz: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    quality: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
     achieved: np.ndarray | None = None
 
     def branches(self, rate: float) -> np.ndarray:
-        """(branches, 3): each branch as the solver takes it (fdtd.hpp:391-411): mass over
-        the step m = M / T, resistance r = R = 1 / G, stiffness times the step q = S T;
-        rising m = 0, q = R w_c T; falling q = 0, m = R / (w_c T)."""
+        """(branches, 3): each branch of positive conductance as the solver takes it
+        (fdtd.hpp:391-411): m = M / T = Q R / (w0 T), r = R = 1 / G, q = S T = Q R w0 T."""
         g = np.asarray(self.gain, float)
-        if len(g) == 0:
-            return np.zeros((0, 3))
-        r = 1.0 / g
-        wct = 2.0 * np.pi * np.asarray(self.corner_hz, float) / rate
-        up = np.asarray(self.rise) > 0
-        return np.column_stack([np.where(up, 0.0, r / wct), r, np.where(up, r * wct, 0.0)])
+        live = g > 0
+        r = 1.0 / g[live]
+        w0t = 2.0 * np.pi * np.asarray(self.centre_hz, float)[live] / rate
+        q = np.asarray(self.quality, float)[live]
+        return np.column_stack([q * r / w0t, r, q * r * w0t]).reshape(-1, 3)
 
     def response(self, f, rate: float | None = None) -> np.ndarray:
         """The admittance at ``f`` (Hz): exact, or with ``rate`` as the scheme realises its
         branches centred in time: from m (z - 2 + 1/z) W + r (z - 1/z) W / 2 + q (z + 1/z) W / 2
         = (z - 1/z) P / 2 on z = exp(i w T), W / P = i sin wT / (-4 m sin^2(wT/2) + i r sin wT
-        + q cos wT)."""
+        + q cos wT), taken per unit conductance and scaled."""
         f = np.asarray(f, float)
-        w = 2.0 * np.pi * f[..., None]
-        if len(np.asarray(self.gain)) == 0:
-            return self.beta + np.zeros(f.shape, complex)
-        if rate is None:
-            s = 1j * w
-            wc = 2.0 * np.pi * np.asarray(self.corner_hz, float)
-            g = np.asarray(self.gain, float)
-            up = np.asarray(self.rise) > 0
-            branch = np.where(up, g * (s / wc) / (1.0 + s / wc), g / (1.0 + s / wc))
-        else:
-            m, r, q = self.branches(rate).T
-            wt = w / rate
-            branch = 1j * np.sin(wt) / (-4.0 * m * np.sin(0.5 * wt) ** 2 + 1j * r * np.sin(wt) + q * np.cos(wt))
-        return self.beta + branch.sum(axis=-1)
+        unit = unit_branches(f.reshape(-1), self.centre_hz, self.quality, rate)
+        return (self.beta + unit @ np.asarray(self.gain, float).astype(complex)).reshape(f.shape)
+
+
+def unit_branches(f, centre_hz, quality, rate: float | None) -> np.ndarray:
+    """(frequencies, branches): each resonant branch's admittance at conductance 1."""
+    w = 2.0 * np.pi * np.asarray(f, float)[:, None]
+    w0 = 2.0 * np.pi * np.asarray(centre_hz, float)[None, :]
+    q = np.asarray(quality, float)[None, :]
+    if rate is None:
+        return 1.0 / (1.0 + 1j * q * (w / w0 - w0 / w))
+    t = 1.0 / rate
+    m, qs = q / (w0 * t), q * w0 * t
+    return 1j * np.sin(w * t) / (-4.0 * m * np.sin(0.5 * w * t) ** 2 + 1j * np.sin(w * t) + qs * np.cos(w * t))
 
 
 def _band_grid(bands: int, total: int):
@@ -154,7 +152,17 @@ def _band_grid(bands: int, total: int):
         half = 2.0 ** (1.0 / (2 * per))
         mid = centres[:bands, None]
         w = 1.0 / (1.0 + ((f / mid - mid / f) / (half - 1.0 / half)) ** (2 * analysis.FILTER_ORDER))
-    return f, w / w.sum(axis=1, keepdims=True), centres[:bands] / math.sqrt(step)
+    return f, w / w.sum(axis=1, keepdims=True), centres[:bands]
+
+
+def _resonances(centres: np.ndarray, total: int):
+    """The branches' centres (a band's step below the lowest band, each band's centre, a
+    step above the highest) and their quality, each band's own: 1 / (s^(1/2) - s^(-1/2)), s
+    the band's width ratio (as the reference's fit starts from, impedance.py:320-330)."""
+    per = 1 if total == len(analysis.OCTAVE_BANDS_HZ) else 3
+    step = 2.0 ** (1.0 / per)
+    at = np.concatenate([[centres[0] / step], centres, [centres[-1] * step]])
+    return at, np.full(len(at), 1.0 / (math.sqrt(step) - 1.0 / math.sqrt(step)))
 
 
 def band_absorption(adm: Admittance, bands: int, total: int, rate: float | None) -> np.ndarray:
@@ -174,22 +182,17 @@ def _fit(target: tuple, total: int, rate: float | None) -> Admittance:
     if np.ptp(goal) < 1e-9:
         plain = Admittance(float(admittance(goal[:1])[0]))
         return dataclasses.replace(plain, achieved=band_absorption(plain, bands, total, rate))
-    f, weights, lows = _band_grid(bands, total)
-    # A rising branch from each band's lower edge and a falling one from its upper edge,
-    # so that the absorption may climb or fall from band to band.
-    step = lows[1] / lows[0] if bands > 1 else 2.0
-    corners = np.concatenate([lows, lows * step])
-    rise = np.concatenate([np.ones(bands), -np.ones(bands)])
+    f, weights, centres = _band_grid(bands, total)
+    at, quality = _resonances(centres, total)
+    unit = unit_branches(f, at, quality, rate)
     base = float(admittance(np.array([goal.min()]))[0])
-    start = np.concatenate([[base], np.full(2 * bands, 1e-3)])
+    start = np.concatenate([[base], np.full(len(at), 1e-3)])
 
     def misses(x):
-        y = Admittance(x[0], x[1:], corners, rise).response(f, rate)
-        return weights @ random_incidence(y) - goal
+        return weights @ random_incidence(x[0] + unit @ x[1:].astype(complex)) - goal
 
     fit = least_squares(misses, start, bounds=(0.0, np.inf), xtol=1e-12, ftol=1e-12, gtol=1e-12)
-    keep = fit.x[1:] > 1e-9
-    out = Admittance(float(fit.x[0]), fit.x[1:][keep], corners[keep], rise[keep])
+    out = Admittance(float(fit.x[0]), fit.x[1:], at, quality)
     return dataclasses.replace(out, achieved=band_absorption(out, bands, total, rate))
 
 
@@ -200,3 +203,78 @@ def fit_admittance(absorption, total: int, rate: float | None = None) -> Admitta
     [0, MAX_ABSORPTION]."""
     target = tuple(float(a) for a in np.clip(np.asarray(absorption, float), 0.0, MAX_ABSORPTION))
     return _fit(target, int(total), None if rate is None else float(rate))
+
+
+def _random_incidence_torch(torch, y):
+    """random_incidence on a complex torch tensor (every entry with Re y > 0)."""
+    zeta = 1.0 / y
+    r, x = zeta.real, zeta.imag
+    m2 = r * r + x * x
+    small = x.abs() < 1e-12 * torch.clamp(r.abs(), min=1.0)
+    safe_x = torch.where(small, torch.ones_like(x), x)
+    bend = torch.where(small, r * r / (m2 * (1.0 + r)), (r * r - x * x) / (safe_x * m2) * torch.atan(safe_x / (1.0 + r)))
+    return 8.0 * r / m2 * (1.0 - r / m2 * torch.log(1.0 + 2.0 * r + m2) + bend)
+
+
+def fit_admittance_batch(absorption, total: int, rate: float | None = None, device: str = "cuda",
+                         iterations: int = 400, tolerance: float = 1e-9):
+    """``fit_admittance`` for many rows at once (absorption (N, bands), Monte Carlo draws of
+    materials), on ``device``: the same model (beta, a rising and a falling first-order
+    branch per band, the scheme's admittance at ``rate``) and band weights, fitted by a
+    bounded Levenberg-Marquardt run on all rows together (each row's Jacobian by automatic
+    differentiation, its own damping, steps projected onto x >= 0), until every row is
+    within ``tolerance`` or after ``iterations`` rounds. Returns (beta (N,),
+    gains (N, branches), centres (branches,), quality (branches,), achieved (N, bands)); a
+    row's Admittance is ``Admittance(beta[i], gains[i], centres, quality)``."""
+    import torch
+
+    goal_np = np.clip(np.asarray(absorption, float), 0.0, MAX_ABSORPTION)
+    n, bands = goal_np.shape
+    f, weights, centres = _band_grid(bands, total)
+    at, quality = _resonances(centres, total)
+    unit = unit_branches(f, at, quality, rate)
+    dev = torch.device(device)
+    U = torch.as_tensor(unit, dtype=torch.complex128, device=dev)          # (F, 2B)
+    Wt = torch.as_tensor(weights, dtype=torch.float64, device=dev)         # (B, F)
+    goal = torch.as_tensor(goal_np, dtype=torch.float64, device=dev)       # (N, B)
+
+    def misses(x, g):  # one row: x (1 + 2B,) -> (B,)
+        y = x[0] + U @ x[1:].to(torch.complex128)
+        return Wt @ _random_incidence_torch(torch, y) - g
+
+    jac = torch.func.vmap(torch.func.jacrev(misses))
+    res = torch.func.vmap(misses)
+    base = admittance(goal_np.min(axis=1))
+    x = torch.zeros((n, 1 + len(at)), dtype=torch.float64, device=dev)
+    x[:, 0] = torch.as_tensor(np.maximum(base, 1e-6), device=dev)
+    x[:, 1:] = 1e-3
+    lam = torch.full((n,), 1e-3, dtype=torch.float64, device=dev)
+    r = res(x, goal)
+    cost = (r * r).sum(dim=1)
+    eye = torch.eye(1 + len(at), dtype=torch.float64, device=dev)
+    for it in range(iterations):
+        if it % 10 == 0 and float(r.abs().max()) < tolerance:
+            break  # every row met (high absorption with a dip takes a few hundred rounds)
+        J = jac(x, goal)                                   # (N, B, P)
+        JT = J.transpose(1, 2)
+        A = JT @ J
+        diag = torch.diagonal(A, dim1=1, dim2=2).clamp(min=1e-12)
+        g = (JT @ r[..., None])[..., 0]
+        delta = torch.linalg.solve(A + lam[:, None, None] * eye * diag[:, None, :], -g[..., None])[..., 0]
+        trial = torch.clamp(x + delta, min=0.0)
+        r_new = res(trial, goal)
+        cost_new = (r_new * r_new).sum(dim=1)
+        better = cost_new < cost
+        x = torch.where(better[:, None], trial, x)
+        r = torch.where(better[:, None], r_new, r)
+        cost = torch.where(better, cost_new, cost)
+        lam = torch.where(better, lam * 0.3, lam * 10.0).clamp(1e-12, 1e12)
+    achieved = (r + goal).cpu().numpy()
+    xs = x.cpu().numpy()
+    # Rows that ask for nothing, or the same in every band, as the single fit takes them.
+    flat = np.ptp(goal_np, axis=1) < 1e-9
+    if flat.any():
+        xs[flat, 0] = admittance(goal_np[flat, 0])
+        xs[flat, 1:] = 0.0
+        achieved[flat] = np.array([band_absorption(Admittance(float(b)), bands, total, rate) for b in xs[flat, 0]])
+    return xs[:, 0], xs[:, 1:], at, quality, achieved
