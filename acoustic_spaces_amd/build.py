@@ -18,7 +18,7 @@ ROOT = PACKAGE.parent
 CSRC = PACKAGE / "csrc"
 LIBRARY = PACKAGE / "libasfdtd.so"
 
-HIP_SOURCES = ["fdtd_step.hip", "responses.hip"]
+HIP_SOURCES = ["fdtd_step.hip", "responses.hip", "setup_device.hip"]
 HOST_SOURCES = ["setup.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

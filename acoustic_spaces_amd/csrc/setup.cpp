@@ -58,6 +58,7 @@
 #include <vector>
 
 #include "asfdtd.h"
+#include "setup_device.h"
 
 namespace {
 
@@ -399,6 +400,8 @@ public:
     size_t points = 7;
     int threads = 1;
     double guard = 0;  // kGuard: distance from a plane within which an end counts as on it
+    bool on_device = false;  // the cut links and the flood on the current GPU
+    size_t flood_rounds = 0;  // the device flood's rounds
 
     Stencil st;
     std::array<float, 18> weights{};  // the stencil's lowered weights, as the layout hands them out
@@ -1189,6 +1192,72 @@ public:
         }
     }
 
+    static constexpr const char *kNoSight = "no cell centre of the wave grid is in sight of the source";
+
+    // The cut links and the air on the current GPU (setup_device.hip), with the host
+    // passes' results and refusals: the same cut words, the same air, the same errors in
+    // the same order (a guard fault, then a link through a face's edge, then no air).
+    void device_passes(const V3 &src) {
+        asdev::Grid g{};
+        g.origin[0] = origin.x;
+        g.origin[1] = origin.y;
+        g.origin[2] = origin.z;
+        g.cell = cell;
+        g.guard = guard;
+        g.barycentric = kBarycentric;
+        g.nx = static_cast<uint32_t>(nx);
+        g.ny = static_cast<uint32_t>(ny);
+        g.nz = static_cast<uint32_t>(nz);
+        g.bin = static_cast<uint32_t>(bin);
+        g.bx = static_cast<uint32_t>(bx);
+        g.by = static_cast<uint32_t>(by);
+        g.bz = static_cast<uint32_t>(bz);
+        g.directions = static_cast<uint32_t>(st.count);
+        for (size_t k = 0; k < 18; ++k) {
+            for (int a = 0; a < 3; ++a) g.step[k][a] = static_cast<int8_t>(st.d[k].step[a]);
+            g.plus_bit[k] = static_cast<int8_t>(st.d[k].plus_bit);
+            g.opposite[k] = static_cast<int8_t>(st.d[k].opposite);
+        }
+        if (nx >= (1u << 31) || ny >= (1u << 31) || nz >= (1u << 31))
+            throw std::invalid_argument("the grid is too large for the device set-up");
+        std::vector<asdev::Tri> faces(tri.size());
+        const auto put = [](const V3 &v, double *o) {
+            o[0] = v.x;
+            o[1] = v.y;
+            o[2] = v.z;
+        };
+        for (size_t f = 0; f < tri.size(); ++f) {
+            put(tri[f].v0, faces[f].v0);
+            put(tri[f].e1, faces[f].e1);
+            put(tri[f].e2, faces[f].e2);
+            put(tri[f].normal, faces[f].n);
+            put(tri[f].lo, faces[f].lo);
+            put(tri[f].hi, faces[f].hi);
+        }
+        std::vector<uint8_t> air(n, 0);
+        size_t seeds = 0;
+        for (const auto &cw : visible(src, false)) {
+            if (air[cw.first]) continue;
+            air[cw.first] = 1;
+            ++seeds;
+        }
+        double seconds[2] = {0.0, 0.0};
+        const uint32_t flags = asdev::cut_and_flood(faces, bin_start, bin_faces, g, cut, air, flood_rounds, seconds);
+        if (std::getenv("AS_FDTD_TIMING") != nullptr) {
+            std::fprintf(stderr, "as_setup_create: device cut links %.1f ms, flood %.1f ms in %zu rounds\n",
+                         1e3 * seconds[0], 1e3 * seconds[1], flood_rounds);
+        }
+        if (flags & 1u) {
+            throw Degenerate("degenerate geometry: a cell centre, the source or a probe lies within " +
+                             std::to_string(guard) + " m of a face's plane next to the face");
+        }
+        if (flags & 2u) throw Degenerate("degenerate geometry: a link passes through a face's edge");
+        if (seeds == 0) throw std::invalid_argument(kNoSight);
+        type.assign(n, kSolid);
+        for (size_t i = 0; i < n; ++i)
+            if (air[i]) type[i] = kInterior;
+    }
+
     void build(const double *mrq, const int64_t *branch_material, size_t branch_count, const V3 &src,
                const std::vector<V3> &probes, size_t per_listener, const float *speed) {
         check_arguments(probes.size(), per_listener);
@@ -1211,20 +1280,25 @@ public:
         clock.mark("stencil, bins");
         require_off_faces(src, "the source");
         for (const V3 &p : probes) require_off_faces(p, "a probe");
-        cut_links();
-        clock.mark("cut links");
+        if (on_device) {
+            device_passes(src);
+            clock.mark("cut links, air (device)");
+        } else {
+            cut_links();
+            clock.mark("cut links");
 
-        // Air: the cells the source sees, and all they reach (fdtd.hpp:498-517).
-        type.assign(n, kSolid);
-        std::vector<size_t> queue;
-        for (const auto &cw : visible(src, false)) {
-            if (type[cw.first] != kSolid) continue;
-            type[cw.first] = kInterior;
-            queue.push_back(cw.first);
+            // Air: the cells the source sees, and all they reach (fdtd.hpp:498-517).
+            type.assign(n, kSolid);
+            std::vector<size_t> queue;
+            for (const auto &cw : visible(src, false)) {
+                if (type[cw.first] != kSolid) continue;
+                type[cw.first] = kInterior;
+                queue.push_back(cw.first);
+            }
+            if (queue.empty()) throw std::invalid_argument(kNoSight);
+            flood(queue);
+            clock.mark("air");
         }
-        if (queue.empty()) throw std::invalid_argument("no cell centre of the wave grid is in sight of the source");
-        flood(queue);
-        clock.mark("air");
         const size_t T = slabs();
         if (layer > 0) matched_layer(T);
         clock.mark("layer");
@@ -1238,6 +1312,16 @@ public:
 
 thread_local std::string g_setup_error;
 
+// Threads for `threads` 0: OMP_NUM_THREADS where it is set to a positive count (the share
+// of the machine the process is given), else every hardware thread.
+int default_threads() {
+    if (const char *env = std::getenv("OMP_NUM_THREADS")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v > 0 && v <= 4096) return static_cast<int>(v);
+    }
+    return static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+}
+
 }  // namespace
 
 struct as_setup {
@@ -1246,13 +1330,13 @@ struct as_setup {
 
 extern "C" {
 
-as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
-                          const int64_t *face_material, const double *beta, size_t materials,
-                          const double *branches, const int64_t *branch_material, size_t nbranch,
-                          const double *origin, double cell, size_t nx, size_t ny, size_t nz,
-                          const double *source, const double *probes, size_t nprobes, double edge_admittance,
-                          size_t layer, double layer_damping, size_t stencil, const float *speed,
-                          size_t probes_per_listener, int threads) {
+as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
+                             const int64_t *face_material, const double *beta, size_t materials,
+                             const double *branches, const int64_t *branch_material, size_t nbranch,
+                             const double *origin, double cell, size_t nx, size_t ny, size_t nz,
+                             const double *source, const double *probes, size_t nprobes, double edge_admittance,
+                             size_t layer, double layer_damping, size_t stencil, const float *speed,
+                             size_t probes_per_listener, int threads, unsigned flags) {
     g_setup_error.clear();
     as_setup *h = new (std::nothrow) as_setup();
     if (h == nullptr) {
@@ -1262,6 +1346,8 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
     try {
         Setup &s = h->s;
         if (nf == 0) throw std::invalid_argument("the mesh has no faces");
+        if (flags & ~static_cast<unsigned>(AS_SETUP_ON_DEVICE)) throw std::invalid_argument("unknown set-up flags");
+        s.on_device = (flags & AS_SETUP_ON_DEVICE) != 0;
         if (nx * ny * nz >= (1ull << 40)) throw std::invalid_argument("the grid is too large");
         s.materials = materials;
         s.beta.assign(beta, beta + materials);
@@ -1300,7 +1386,7 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
         s.layer_damping = layer_damping;
         s.edge_admittance = edge_admittance;
         s.points = stencil;
-        s.threads = threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+        s.threads = threads > 0 ? threads : default_threads();
         // A hundred times the reference's tie (kTie, 1e-10 cells); beyond it, and off the
         // faces' edges, its casts and tie rules decide as the plain geometry does.
         s.guard = 1e-8 * cell;
@@ -1314,6 +1400,18 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
     }
     delete h;
     return nullptr;
+}
+
+as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
+                          const int64_t *face_material, const double *beta, size_t materials,
+                          const double *branches, const int64_t *branch_material, size_t nbranch,
+                          const double *origin, double cell, size_t nx, size_t ny, size_t nz,
+                          const double *source, const double *probes, size_t nprobes, double edge_admittance,
+                          size_t layer, double layer_damping, size_t stencil, const float *speed,
+                          size_t probes_per_listener, int threads) {
+    return as_setup_create_ex(vertices, nv, faces, nf, face_material, beta, materials, branches, branch_material,
+                              nbranch, origin, cell, nx, ny, nz, source, probes, nprobes, edge_admittance, layer,
+                              layer_damping, stencil, speed, probes_per_listener, threads, 0u);
 }
 
 const char *as_setup_create_error(void) { return g_setup_error.c_str(); }

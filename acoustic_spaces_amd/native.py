@@ -60,7 +60,7 @@ SYMBOLS = (
 RESPONSE_SYMBOLS = ("as_first_order_device",)
 
 #: The set-up's symbols (acoustic_spaces_amd.solver binds them).
-SETUP_SYMBOLS = ("as_setup_create", "as_setup_create_error", "as_setup_layout", "as_setup_info",
+SETUP_SYMBOLS = ("as_setup_create", "as_setup_create_ex", "as_setup_create_error", "as_setup_layout", "as_setup_info",
                  "as_setup_probe", "as_setup_destroy")
 
 

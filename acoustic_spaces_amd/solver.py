@@ -9,6 +9,10 @@ the reference's (WaveSolver::layout, fdtd.hpp:727-782), to be stepped by
 ``acoustic_spaces_amd.gpu.HipWaveSolver``. ``run`` steps it on the GPU, so that the object is
 also the reference solver's drop-in where it is stepped directly.
 
+``on_device=True`` runs the set-up's two passes over the whole grid, the cut links and the
+air's flood, on the GPU (csrc/setup_device.hip); the layout and every refusal are the host
+set-up's bit for bit.
+
 Not here: moving air (the reference steps it on the CPU only), fitted walls (refused with
 a ValueError), and sources or probes that lie on a face (the reference's leans): refused.
 """
@@ -26,6 +30,7 @@ S = ctypes.c_size_t
 D = ctypes.c_double
 
 _setup_lib = None
+SETUP_ON_DEVICE = 1  # AS_SETUP_ON_DEVICE: the cut links and the air's flood on the GPU
 
 
 def _lib():
@@ -35,6 +40,8 @@ def _lib():
         lib.as_setup_create.restype = P
         lib.as_setup_create.argtypes = [P, S, P, S, P, P, S, P, P, S, P, D, S, S, S, P, P, S, D, S, D,
                                         S, P, S, ctypes.c_int]
+        lib.as_setup_create_ex.restype = P
+        lib.as_setup_create_ex.argtypes = [*lib.as_setup_create.argtypes, ctypes.c_uint]
         lib.as_setup_create_error.restype = ctypes.c_char_p
         lib.as_setup_create_error.argtypes = []
         lib.as_setup_layout.restype = ctypes.c_int
@@ -81,7 +88,7 @@ class WaveSolver:
     def __init__(self, bvh, face_material, beta, branches, branch_material, origin, cell, nx, ny,
                  nz, source, probes, edge_admittance=1.0, layer=0, layer_damping=0.0, threads=0,
                  stencil=7, speed=None, flow_x=None, flow_y=None, probes_per_listener=1,
-                 fitted_walls=False, vertices=None, faces=None):
+                 fitted_walls=False, vertices=None, faces=None, on_device=False):
         if fitted_walls:
             raise ValueError("fitted walls: not laid out by this set-up; take walls on cell faces")
         if flow_x is not None or flow_y is not None:
@@ -107,12 +114,12 @@ class WaveSolver:
                 raise ValueError("WaveSolver: one speed per cell of the grid")
         self._keep = (bvh, fm, b, br, bm, o, s, p, sp)
         ptr = lambda a: None if a is None or len(a) == 0 else a.ctypes.data  # noqa: E731
-        h = lib.as_setup_create(
+        h = lib.as_setup_create_ex(
             bvh.vertices.ctypes.data, len(bvh.vertices), bvh.faces.ctypes.data, len(bvh.faces),
             fm.ctypes.data, b.ctypes.data, len(b), ptr(br), ptr(bm), len(bm), o.ctypes.data,
             float(cell), int(nx), int(ny), int(nz), s.ctypes.data, ptr(p), len(p),
             float(edge_admittance), int(layer), float(layer_damping), int(stencil), ptr(sp),
-            int(probes_per_listener), int(threads))
+            int(probes_per_listener), int(threads), SETUP_ON_DEVICE if on_device else 0)
         if not h:
             raise ValueError(lib.as_setup_create_error().decode())
         self._h = h

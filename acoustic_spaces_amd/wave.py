@@ -307,7 +307,7 @@ def scene_responses(scene, listeners, names, *, length_s: float, bands: int = 3,
     courant = COURANT if scene.stencil == 7 else ISOTROPIC_COURANT
     points = np.vstack([probe_points(p, h) for p in listeners])
     run = dataclasses.replace(scene, probes=points)
-    solver = WaveSolver.from_scene(run, probes_per_listener=PROBES)
+    solver = WaveSolver.from_scene(run, probes_per_listener=PROBES, on_device=True)
     signal, delay = source_signal(scene.rate, h, courant)
     samples = int(math.ceil(length_s * scene.rate))
     lead = delay
@@ -547,7 +547,7 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
         solver = WaveSolver(bvh, face_material, np.array([f.beta for f in fits]),
                             np.vstack(branches) if branches else np.zeros((0, 3)), branch_material, origin,
                             grid.cell, int(shape[0]), int(shape[1]), int(shape[2]), source, probes, 1.0, layer,
-                            LAYER_DAMPING, threads, points_n, speeds, probes_per_listener=PROBES)
+                            LAYER_DAMPING, threads, points_n, speeds, probes_per_listener=PROBES, on_device=True)
         return solver, grid, fits, probes, courant, extremes, still
 
     if scheme == "isotropic" and outdoor:

@@ -277,10 +277,23 @@ def plain(scene, name, args, traffic_name=None):
     K, W, lead = args.steps, args.warmup, args.lead_in
     need = lead + W + K
     drive = scene.drive if need <= len(scene.drive) else sc.gaussian_drive(scene.rate, need)
+    # The set-up on the GPU (the cut links and the air's flood on the device), then the host
+    # set-up alone for comparison; the two layouts must be the same. The runtime is brought
+    # up first, as it is in a process that steps.
+    import torch
+
+    torch.zeros(1, device="cuda")
+    torch.cuda.synchronize()
     t = time.perf_counter()
-    solver = WaveSolver.from_scene(scene)
+    solver = WaveSolver.from_scene(scene, on_device=True)
     layout = solver.layout()
     setup_s = time.perf_counter() - t
+    t = time.perf_counter()
+    host_layout = WaveSolver.from_scene(scene).layout()
+    host_setup_s = time.perf_counter() - t
+    if sorted(host_layout) != sorted(layout) or not all(np.array_equal(layout[k], host_layout[k]) for k in layout):
+        raise SystemExit("the device set-up's layout differs from the host's")
+    del host_layout
     if args.tile_z or args.graph_steps or args.face_blocks >= 0:
         launch = {"tile_z": args.tile_z, "graph_steps": args.graph_steps,
                   "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks)}
@@ -303,7 +316,10 @@ def plain(scene, name, args, traffic_name=None):
     step_bytes = algorithmic_bytes(info, air, solver.layer_cells, solver.branch_states)
     config = {
         "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
-        "special_cells": info["specials"], "grid": list(scene.grid.shape), "setup_s": round(setup_s, 2),
+        "special_cells": info["specials"], "grid": list(scene.grid.shape),
+        "setup_s": {"device": round(setup_s, 3), "host": round(host_setup_s, 3),
+                    "device_is": "the cut links and the air's flood on the GPU, the rest on the host's threads",
+                    "host_threads": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())},
         "launch": launch, "launch_from": launch_from,
         "timed_launches": launch_mode(launch["graph_steps"], K),
         "lead_in_steps": lead,
@@ -621,6 +637,40 @@ def gradient_over_forward() -> dict:
     }
 
 
+def setup_times() -> int:
+    """The set-up's wall times, host against device passes, on the hall at 10 and 5 cm (both
+    stencils) and outdoors at 25 cm: one JSON line; the stages' times on stderr."""
+    import torch
+
+    from acoustic_spaces_amd import scenes
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    torch.zeros(1, device="cuda")
+    torch.cuda.synchronize()
+    os.environ["AS_FDTD_TIMING"] = "1"
+    rows = []
+    for name, scene, stencils in (("hall 10 cm", scenes.hall(cell=0.1)[0], (7, 19)),
+                                  ("hall 5 cm", scenes.hall(cell=0.05)[0], (7, 19)),
+                                  ("outdoor 25 cm", scenes.outdoor(), (7,))):
+        for stencil in stencils:
+            scene.stencil = stencil
+            row = {"scene": name, "stencil": stencil, "grid": list(scene.grid.shape)}
+            layouts = []
+            for on_device in (False, True):
+                print(f"== {name}, {stencil} points, on_device={on_device}", file=sys.stderr, flush=True)
+                t = time.perf_counter()
+                solver = WaveSolver.from_scene(scene, on_device=on_device)
+                row["device_s" if on_device else "host_s"] = round(time.perf_counter() - t, 3)
+                layouts.append(solver.layout())
+                row["air_cells"] = solver.air_cells
+            row["same_layout"] = all(np.array_equal(layouts[0][k], layouts[1][k]) for k in layouts[0])
+            rows.append(row)
+            del layouts
+    print(json.dumps({"setup_times": rows, "host_threads": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())}),
+          flush=True)
+    return 0
+
+
 def strict(value):
     """``value`` with every NaN or infinity (an unreadable T30, say) as None, so that the
     line is strict JSON."""
@@ -658,10 +708,14 @@ def main(argv=None) -> int:
     ap.add_argument("--calibrate", action="store_true",
                     help="time the runtime's own device-to-device copy kernel instead (the bandwidth "
                          "reference and the rocprofv3 counter scale)")
+    ap.add_argument("--setup-times", action="store_true",
+                    help="time the set-up instead, host against device passes (hall at 10 and 5 cm, outdoors)")
     args = ap.parse_args(argv)
     args.outputs, args.signals, args.insertion_loss, args.others = False, None, False, []
     if args.calibrate:
         return calibrate()
+    if args.setup_times:
+        return setup_times()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the solver is built and measured at N=1 only "
               f"(--gpus {args.gpus} asked for); a multi-GPU run is not part of this build.",

@@ -202,8 +202,8 @@ int as_first_order_device(const double *d_probes, size_t steps, size_t listeners
 /*
  * The set-up on walls on cell faces (the reference's WaveSolver constructor,
  * fdtd.hpp:360-690, bound at cpp/bindings/native_module.cpp:1238-1285): the arguments
- * of the binding, the mesh as vertices (nv x 3) and faces (nf x 3); threads 0 takes the
- * machine's. NULL on failure (ValueError in the reference; see as_setup_create_error).
+ * of the binding, the mesh as vertices (nv x 3) and faces (nf x 3); threads 0 takes
+ * OMP_NUM_THREADS where set, else the machine's. NULL on failure (ValueError in the reference; see as_setup_create_error).
  */
 typedef struct as_setup as_setup;
 as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
@@ -213,6 +213,20 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
                           const double *source, const double *probes, size_t nprobes,
                           double edge_admittance, size_t layer, double layer_damping, size_t stencil,
                           const float *speed, size_t probes_per_listener, int threads);
+/*
+ * as_setup_create with flags. AS_SETUP_ON_DEVICE: the two passes over the whole grid, the
+ * cut links (fdtd.hpp:446-477) and the air's flood (fdtd.hpp:498-517), run on the current
+ * GPU (hipMalloc'd temporaries, freed before the call returns); the layout, and every
+ * refusal, is the host set-up's bit for bit. Grids of 2^32 cells or more are refused in this mode.
+ */
+#define AS_SETUP_ON_DEVICE 1u
+as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
+                             const int64_t *face_material, const double *beta, size_t materials,
+                             const double *branches, const int64_t *branch_material, size_t nbranch,
+                             const double *origin, double cell, size_t nx, size_t ny, size_t nz,
+                             const double *source, const double *probes, size_t nprobes,
+                             double edge_admittance, size_t layer, double layer_damping, size_t stencil,
+                             const float *speed, size_t probes_per_listener, int threads, unsigned flags);
 const char *as_setup_create_error(void);
 /* Points `out` at the set-up's layout arrays (WaveSolver::layout, fdtd.hpp:727-782); they
  * live as long as the handle. */

@@ -78,3 +78,27 @@ def listeners13(stencil=7, seed=31):
     listeners = [np.array([1.21, 0.73, 0.52]), np.array([0.5 * c + 0.013, 0.61, 0.44])]
     sc.probes = np.vstack([probe_points(p, c) for p in listeners])
     return sc
+
+
+def baffled_room(stencil=7, seed=41, cell=0.1):
+    """A 3 x 2 x 1.6 m room split by two baffles into an S-shaped channel (each baffle a
+    solid slab through the floor, the ceiling and one side wall): the air reaches the far
+    end only round two bends."""
+    rng = np.random.default_rng(seed)
+    room = scenes.box_mesh((3.0, 2.0, 1.6))
+    a = scenes.box_mesh((0.14, 1.65, 2.0), low=(0.93, -0.2, -0.2), inward=False)
+    b = scenes.box_mesh((0.14, 1.65, 2.0), low=(1.93, 0.55, -0.2), inward=False)
+    mesh = scenes.merge(room, a, b)
+    grid = scenes.grid_round((0.0, 0.0, 0.0), (3.0, 2.0, 1.6), cell)
+    courant = scenes.COURANT if stencil == 7 else scenes.ISOTROPIC_COURANT
+    rate = scenes.C / (courant * cell)
+    steps = 200
+    source = np.array([0.41, 0.52, 0.77]) + rng.uniform(-0.02, 0.02, 3)
+    probes = np.array([[0.45, 1.71, 0.63], [1.52, 0.33, 0.91], [2.61, 1.42, 1.13]])
+    sc = scenes.Scene(
+        name="baffled room", mesh=mesh, beta=np.full(mesh.materials, 0.05),
+        branches=np.zeros((0, 3)), branch_material=np.zeros(0, np.int64), grid=grid,
+        source=source, probes=probes, rate=rate, steps=steps,
+        drive=scenes.gaussian_drive(rate, steps), stencil=stencil,
+    )
+    return with_branches(sc, per_material=((0.0, 4.0, 0.2),))

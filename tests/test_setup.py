@@ -47,6 +47,9 @@ SCENES = {
     # (engine/wave.py:677-682), two listeners one a cell from a wall.
     "listeners13": lambda: (cases.listeners13(), {"probes_per_listener": 13}),
     "listeners13_19": lambda: (cases.listeners13(stencil=19), {"probes_per_listener": 13}),
+    # The air round two bends: an S-shaped channel between two baffles.
+    "baffled7": lambda: (cases.baffled_room(), {}),
+    "baffled19": lambda: (cases.baffled_room(stencil=19), {}),
 }
 
 

@@ -1,0 +1,109 @@
+"""The set-up's passes over the whole grid on the GPU (csrc/setup_device.hip, the
+``on_device`` set-up): the cut links and the air's flood give the host set-up's layout bit
+for bit, array for array, and the same refusals. The host set-up is pinned to the reference's
+own layouts by test_setup.py; this pins the device passes to the host's."""
+
+import time
+
+import numpy as np
+import pytest
+
+import cases
+from acoustic_spaces_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(scene, **kw):
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    return WaveSolver.from_scene(scene, threads=8, **kw), WaveSolver.from_scene(scene, threads=8, on_device=True, **kw)
+
+
+def _same(a, b):
+    A, B = a.layout(), b.layout()
+    assert sorted(A) == sorted(B)
+    for k in A:
+        assert A[k].dtype == B[k].dtype, k
+        assert np.array_equal(A[k], B[k]), k
+    for name in ("air_cells", "boundary_cells", "branch_states", "layer_cells", "open", "open_sides"):
+        assert getattr(a, name) == getattr(b, name), name
+    for p in range(a.probes):
+        assert a.probe_found(p) == b.probe_found(p)
+        assert a.probe_in_sight(p) == b.probe_in_sight(p)
+
+
+SCENES = {
+    "box7": lambda: (cases.small_box(), {}),
+    "box19": lambda: (cases.small_box(stencil=19), {}),
+    "open_layer": lambda: (cases.open_block(), {}),
+    "speed19": lambda: (lambda sc: (sc, {"speed": cases.speeds_for(sc)}))(cases.small_box(stencil=19)),
+    "shoebox": lambda: (scenes.shoebox(), {}),
+    "shoebox19": lambda: (scenes.shoebox(stencil=19, seed=4), {}),
+    "near_source": lambda: (cases.probes_near_source(), {}),
+    "listeners13_19": lambda: (cases.listeners13(stencil=19), {"probes_per_listener": 13}),
+    "baffled7": lambda: (cases.baffled_room(), {}),
+    "baffled19": lambda: (cases.baffled_room(stencil=19), {}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_device_layout_equals_the_host(name):
+    print(f"device set-up: {name}", flush=True)
+    scene, kw = SCENES[name]()
+    _same(*_pair(scene, **kw))
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13, 14])
+def test_random_boxes(seed):
+    rng = np.random.default_rng(seed)
+    size = tuple(rng.uniform(1.0, 2.2, 3))
+    print(f"device set-up: box seed {seed} size {size}", flush=True)
+    scene = cases.with_branches(scenes.shoebox(size=size, cell=0.1, seed=seed, length_s=0.05,
+                                               stencil=7 if seed % 2 else 19))
+    _same(*_pair(scene))
+
+
+@pytest.mark.parametrize("stencil", [7, 19])
+def test_hall(stencil):
+    """The 30 x 20 x 12 m hall at 10 cm (tilted ceiling planes, splayed side walls): the
+    same layout, and the two set-ups' times."""
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    print(f"device set-up: hall, {stencil} points", flush=True)
+    scene = scenes.hall()[0]
+    scene.stencil = stencil
+    t0 = time.perf_counter()
+    host = WaveSolver.from_scene(scene, threads=16)
+    t1 = time.perf_counter()
+    dev = WaveSolver.from_scene(scene, threads=16, on_device=True)
+    t2 = time.perf_counter()
+    print(f"hall {stencil}: host set-up {t1 - t0:.2f} s, device passes {t2 - t1:.2f} s", flush=True)
+    _same(host, dev)
+
+
+def test_outdoor():
+    """The outdoor scene at 25 cm (a barrier and buildings on an absorbing ground, a matched
+    layer round the grid): 102M cells."""
+    print("device set-up: outdoor", flush=True)
+    _same(*_pair(scenes.outdoor()))
+
+
+def test_refusals_match_the_host():
+    """A wall on a plane of cell centres (an end within the guard of a face) and a scene
+    with no air in sight of the source are refused as the host refuses them."""
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    sc = cases.small_box(branches=False)
+    sc.grid.origin = np.array([-2.5 * sc.grid.cell, sc.grid.origin[1], sc.grid.origin[2]])
+    for on_device in (False, True):
+        with pytest.raises(ValueError, match="degenerate geometry: a cell centre"):
+            WaveSolver.from_scene(sc, on_device=on_device)
+    sc = cases.small_box(branches=False)
+    sc.source = np.array([-0.35, 0.7, 0.5])  # outside the room, next to its wall
+    messages = []
+    for on_device in (False, True):
+        with pytest.raises(ValueError) as e:
+            WaveSolver.from_scene(sc, on_device=on_device)
+        messages.append(str(e.value))
+    assert messages[0] == messages[1]
