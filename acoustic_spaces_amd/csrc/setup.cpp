@@ -424,6 +424,7 @@ public:
     // triangle bins for crossing queries
     size_t bin = 8, bx = 0, by = 0, bz = 0;
     std::vector<uint32_t> bin_start, bin_faces;
+    std::vector<std::array<uint32_t, 6>> face_bins;  // each face's bins: from x, y, z, to x, y, z
     // Which faces a query has met (each thread's own: the bins list a face more than once).
     struct Stamps {
         std::vector<uint32_t> mark;
@@ -650,6 +651,7 @@ public:
                 to[k] = static_cast<size_t>(std::clamp(l, 0.0, static_cast<double>(dims[k] - 1)));
             }
         };
+        face_bins.assign(tri.size(), {});
         for (int pass = 0; pass < 2; ++pass) {
             std::vector<uint32_t> fill;
             if (pass == 1) {
@@ -661,6 +663,10 @@ public:
             for (size_t f = 0; f < tri.size(); ++f) {
                 size_t from[3], to[3];
                 range(tri[f], from, to);
+                for (int a = 0; a < 3; ++a) {
+                    face_bins[f][static_cast<size_t>(a)] = static_cast<uint32_t>(from[a]);
+                    face_bins[f][static_cast<size_t>(a) + 3] = static_cast<uint32_t>(to[a]);
+                }
                 for (size_t z = from[2]; z <= to[2]; ++z)
                     for (size_t y = from[1]; y <= to[1]; ++y)
                         for (size_t x = from[0]; x <= to[0]; ++x) {
@@ -1113,6 +1119,11 @@ public:
             if (!found[t].empty()) open = true;
             for (size_t k = 0; k < 6; ++k) open_sides[k] = open_sides[k] || sides[t][k];
         }
+        layer_profile();
+    }
+
+    // The matched layer's coefficients along each axis, and its cells' speeds.
+    void layer_profile() {
         if (layer_cells.empty()) return;
         const double courant = std::sqrt(static_cast<double>(st.d[0].weight));
         const size_t size[3] = {nx, ny, nz};
@@ -1194,11 +1205,13 @@ public:
 
     static constexpr const char *kNoSight = "no cell centre of the wave grid is in sight of the source";
 
-    // The cut links and the air on the current GPU (setup_device.hip), with the host
-    // passes' results and refusals: the same cut words, the same air, the same errors in
-    // the same order (a guard fault, then a link through a face's edge, then no air).
+    // The passes over the whole grid on the current GPU (setup_device.hip): the cut links,
+    // the air, the matched layer and the wall cells, with the host passes' results and
+    // refusals, in the host's order (a guard fault or a link through a face's edge, then
+    // no air, then the layer, then the wall cells' refusal at the lowest cell).
     void device_passes(const V3 &src) {
-        asdev::Grid g{};
+        asdev::Input in;
+        asdev::Grid &g = in.g;
         g.origin[0] = origin.x;
         g.origin[1] = origin.y;
         g.origin[2] = origin.z;
@@ -1213,49 +1226,111 @@ public:
         g.by = static_cast<uint32_t>(by);
         g.bz = static_cast<uint32_t>(bz);
         g.directions = static_cast<uint32_t>(st.count);
+        g.layer = static_cast<uint32_t>(layer);
+        g.edge_admittance = edge_admittance;
         for (size_t k = 0; k < 18; ++k) {
-            for (int a = 0; a < 3; ++a) g.step[k][a] = static_cast<int8_t>(st.d[k].step[a]);
-            g.plus_bit[k] = static_cast<int8_t>(st.d[k].plus_bit);
-            g.opposite[k] = static_cast<int8_t>(st.d[k].opposite);
+            const Direction &d = st.d[k];
+            for (int a = 0; a < 3; ++a) g.step[k][a] = static_cast<int8_t>(d.step[a]);
+            g.plus_bit[k] = static_cast<int8_t>(d.plus_bit);
+            g.opposite[k] = static_cast<int8_t>(d.opposite);
+            g.side_a[k] = static_cast<int8_t>(d.side_a);
+            g.side_b[k] = static_cast<int8_t>(d.side_b);
+            g.exact[k] = d.exact;
+            g.wall[k] = d.wall;
         }
-        if (nx >= (1u << 31) || ny >= (1u << 31) || nz >= (1u << 31))
+        if (nx >= (1u << 31) || ny >= (1u << 31) || nz >= (1u << 31) || layer >= (1u << 31))
             throw std::invalid_argument("the grid is too large for the device set-up");
-        std::vector<asdev::Tri> faces(tri.size());
+        in.tri.resize(tri.size());
         const auto put = [](const V3 &v, double *o) {
             o[0] = v.x;
             o[1] = v.y;
             o[2] = v.z;
         };
         for (size_t f = 0; f < tri.size(); ++f) {
-            put(tri[f].v0, faces[f].v0);
-            put(tri[f].e1, faces[f].e1);
-            put(tri[f].e2, faces[f].e2);
-            put(tri[f].normal, faces[f].n);
-            put(tri[f].lo, faces[f].lo);
-            put(tri[f].hi, faces[f].hi);
+            asdev::Tri &t = in.tri[f];
+            put(tri[f].v0, t.v0);
+            put(tri[f].e1, t.e1);
+            put(tri[f].e2, t.e2);
+            put(tri[f].normal, t.n);
+            put(tri[f].lo, t.lo);
+            put(tri[f].hi, t.hi);
+            t.material = face_material[f];
+            for (int a = 0; a < 3; ++a) {
+                t.bin_from[a] = face_bins[f][static_cast<size_t>(a)];
+                t.bin_to[a] = face_bins[f][static_cast<size_t>(a) + 3];
+            }
         }
-        std::vector<uint8_t> air(n, 0);
+        in.bin_start = bin_start;
+        in.bin_faces = bin_faces;
+        in.seeds.assign(n, 0);
         size_t seeds = 0;
         for (const auto &cw : visible(src, false)) {
-            if (air[cw.first]) continue;
-            air[cw.first] = 1;
+            if (in.seeds[cw.first]) continue;
+            in.seeds[cw.first] = 1;
             ++seeds;
         }
-        double seconds[2] = {0.0, 0.0};
-        const uint32_t flags = asdev::cut_and_flood(faces, bin_start, bin_faces, g, cut, air, flood_rounds, seconds);
+        in.scale = scale.empty() ? nullptr : scale.data();
+        in.beta = beta;
+        in.branch_start.assign(1, 0);
+        for (const auto &list : branches.of_material) {
+            in.branch_of.insert(in.branch_of.end(), list.begin(), list.end());
+            in.branch_start.push_back(static_cast<uint32_t>(in.branch_of.size()));
+        }
+        in.branch_a = branches.a;
+
+        asdev::Output out;
+        asdev::set_up(in, out);
         if (std::getenv("AS_FDTD_TIMING") != nullptr) {
-            std::fprintf(stderr, "as_setup_create: device cut links %.1f ms, flood %.1f ms in %zu rounds\n",
-                         1e3 * seconds[0], 1e3 * seconds[1], flood_rounds);
+            std::fprintf(stderr,
+                         "as_setup_create: device cut links %.1f ms, flood %.1f ms in %zu rounds, layer and marks "
+                         "%.1f ms, wall cells %.1f ms\n",
+                         1e3 * out.seconds[0], 1e3 * out.seconds[1], out.rounds, 1e3 * out.seconds[2],
+                         1e3 * out.seconds[3]);
         }
-        if (flags & 1u) {
-            throw Degenerate("degenerate geometry: a cell centre, the source or a probe lies within " +
-                             std::to_string(guard) + " m of a face's plane next to the face");
+        flood_rounds = out.rounds;
+        const std::string guard_fault = "degenerate geometry: a cell centre, the source or a probe lies within " +
+                                        std::to_string(guard) + " m of a face's plane next to the face";
+        switch (out.fault) {
+            case asdev::kGuard: throw Degenerate(guard_fault);
+            case asdev::kEdgeLink: throw Degenerate("degenerate geometry: a link passes through a face's edge");
+            case asdev::kEdgeOnly: throw Degenerate("degenerate geometry: a segment passes through a face's edge");
+            default: break;
         }
-        if (flags & 2u) throw Degenerate("degenerate geometry: a link passes through a face's edge");
         if (seeds == 0) throw std::invalid_argument(kNoSight);
-        type.assign(n, kSolid);
-        for (size_t i = 0; i < n; ++i)
-            if (air[i]) type[i] = kInterior;
+        switch (out.fault) {
+            case asdev::kLayerReach:
+                throw std::invalid_argument("the model reaches into the matched layer: a face cuts a link between two "
+                                            "of its cells");
+            case asdev::kManyEdges: throw Degenerate("degenerate geometry: a segment passes through many faces' edges");
+            case asdev::kTwice:
+                throw Degenerate("degenerate geometry: a segment crosses two faces of different materials at one point");
+            case asdev::kEdgeNear:
+                throw Degenerate("degenerate geometry: a segment passes through a face's edge next to its nearest "
+                                 "crossing");
+            default: break;
+        }
+        cut = std::move(out.cut);
+        type = std::move(out.type);
+        layer_cells = std::move(out.layer_cells);
+        air += layer_cells.size() + out.interior;
+        open = open || out.open;
+        for (size_t k = 0; k < 6; ++k) open_sides[k] = open_sides[k] || ((out.sides >> k) & 1u);
+        layer_profile();
+        SpecialList &sp = specials;
+        sp.cell = std::move(out.cell);
+        sp.link_start = std::move(out.link_start);
+        sp.state_start = std::move(out.state_start);
+        sp.link_direction = std::move(out.link_direction);
+        sp.link_weight = std::move(out.link_weight);
+        sp.coef_a = std::move(out.coef_a);
+        sp.coef_b = std::move(out.coef_b);
+        sp.coef_inv = std::move(out.coef_inv);
+        sp.source.reserve(sp.cell.size() + 64);
+        sp.source.assign(sp.cell.size(), 0.0f);
+        sp.state_branch = std::move(out.state_branch);
+        sp.state_weight = std::move(out.state_weight);
+        sp.sorted = sp.size();
+        boundary_cells = sp.size();
     }
 
     void build(const double *mrq, const int64_t *branch_material, size_t branch_count, const V3 &src,
@@ -1282,7 +1357,11 @@ public:
         for (const V3 &p : probes) require_off_faces(p, "a probe");
         if (on_device) {
             device_passes(src);
-            clock.mark("cut links, air (device)");
+            clock.mark("device passes");
+            place_source(src);
+            place_probes(probes, src, per_listener);
+            clock.mark("source, probes");
+            return;
         } else {
             cut_links();
             clock.mark("cut links");

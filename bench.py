@@ -277,7 +277,7 @@ def plain(scene, name, args, traffic_name=None):
     K, W, lead = args.steps, args.warmup, args.lead_in
     need = lead + W + K
     drive = scene.drive if need <= len(scene.drive) else sc.gaussian_drive(scene.rate, need)
-    # The set-up on the GPU (the cut links and the air's flood on the device), then the host
+    # The set-up on the GPU (its passes over the whole grid on the device), then the host
     # set-up alone for comparison; the two layouts must be the same. The runtime is brought
     # up first, as it is in a process that steps.
     import torch
@@ -318,7 +318,7 @@ def plain(scene, name, args, traffic_name=None):
         "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
         "special_cells": info["specials"], "grid": list(scene.grid.shape),
         "setup_s": {"device": round(setup_s, 3), "host": round(host_setup_s, 3),
-                    "device_is": "the cut links and the air's flood on the GPU, the rest on the host's threads",
+                    "device_is": "every pass over the whole grid on the GPU (cut links, flood, layer, wall cells); the source's and probes' readings on the host",
                     "host_threads": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())},
         "launch": launch, "launch_from": launch_from,
         "timed_launches": launch_mode(launch["graph_steps"], K),

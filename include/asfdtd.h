@@ -214,10 +214,10 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
                           double edge_admittance, size_t layer, double layer_damping, size_t stencil,
                           const float *speed, size_t probes_per_listener, int threads);
 /*
- * as_setup_create with flags. AS_SETUP_ON_DEVICE: the two passes over the whole grid, the
- * cut links (fdtd.hpp:446-477) and the air's flood (fdtd.hpp:498-517), run on the current
- * GPU (hipMalloc'd temporaries, freed before the call returns); the layout, and every
- * refusal, is the host set-up's bit for bit. Grids of 2^32 cells or more are refused in this mode.
+ * as_setup_create with flags. AS_SETUP_ON_DEVICE: the passes over the whole grid, the cut
+ * links (fdtd.hpp:446-477), the air's flood (498-517), the matched layer (521-542) and the
+ * boundary cells (544-625), run on the current GPU (hipMalloc'd temporaries, freed before
+ * the call returns); the layout, and every refusal, is the host set-up's bit for bit. Grids of 2^32 cells or more are refused in this mode.
  */
 #define AS_SETUP_ON_DEVICE 1u
 as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *faces, size_t nf,

@@ -107,3 +107,15 @@ def test_refusals_match_the_host():
             WaveSolver.from_scene(sc, on_device=on_device)
         messages.append(str(e.value))
     assert messages[0] == messages[1]
+
+
+def test_a_model_in_the_layer_is_refused_as_on_the_host():
+    """A block whose faces cut links between cells of the matched layer."""
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    sc = cases.open_block()
+    sc.mesh = scenes.box_mesh((0.6, 0.5, 0.4), low=(-0.37, 0.4, 0.3), inward=False)
+    sc.beta = np.full(sc.mesh.materials, 0.05)
+    for on_device in (False, True):
+        with pytest.raises(ValueError, match="reaches into the matched layer"):
+            WaveSolver.from_scene(sc, on_device=on_device)
