@@ -823,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const StepArgs<T> s) {
 
 // ---- The adjoint (time-reversed) scheme ----------------------------------------------
 //
-// For a closed grid (no matched layer, one speed of sound) the step is linear in the fields,
+// For a closed grid (no matched layer) the step is linear in the fields,
 // p(n+1) = L p(n) - d p(n-1) - coef_inv sum_s weight (now w(n) - before w(n-1)) + source, with
 // L the links' row operator times coef_inv, d = coef_inv coef_b; and the branch velocities
 // w(n+1) = w(n-1) + gain (p(n+1) - p(n-1)) + now w(n) - before w(n-1). Its adjoint runs the
@@ -836,8 +836,16 @@ __global__ __launch_bounds__(kBlock) void probe_kernel(const StepArgs<T> s) {
 // respect to a special's wall term g (coef_b = 1 - g, coef_inv = 1 / (1 + g)) and a state's
 // gain is, with D(n) = p(n) - p(n-2) at the special's cell,
 //   dJ/dg = -sum_n D(n) Q(n),   dJ/dgain = sum_n D(n) W(n).
+//
+// With a speed of sound per cell (scale s, a cell's speed squared over the grid's) a row's
+// links carry its own cell's s (the interior's s axial, a special's lowered s w), so the
+// transpose carries each neighbour's. The fields then hold R = s Q: in R the adjoint's
+// interior rows are again the forward's scaled interior update, and a special's row is
+//   R(n) = coef_inv (coef_a R(n+1) + sum_l w_l R(n+1)[i + l] - coef_b R(n+2)
+//                    + s (sum_s gain (W(n) - W(n+2)) + (C^T dJ/dy(n))_i)),
+// its states and sums reading Q = R / s.
 
-template <typename T>
+template <typename T, bool SCALED>
 __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t block) {
     const uint32_t t = threadIdx.x;
     const uint32_t k = block * kBlock + t;
@@ -873,9 +881,12 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
             if (c + u < nl) sum += static_cast<T>(w[u]) * v[u];
         }
     }
+    // R = s Q where the grid has a speed per cell: the states and the sums read Q.
+    const T sc = SCALED ? static_cast<T>(s.scale[i]) : static_cast<T>(1);
+    const T qbq = SCALED ? qb / sc : qb, qaq = SCALED ? qa / sc : qa;
     // The previous adjoint step's sums: D(n + 1) Q(n + 1), D(n + 1) W(n + 1).
     const double d = s.delta != nullptr ? static_cast<double>(s.delta[k]) : 0.0;
-    if (s.delta != nullptr) s.acc_g[k] += d * static_cast<double>(qb);
+    if (s.delta != nullptr) s.acc_g[k] += d * static_cast<double>(qbq);
     T gsum = 0;
     for (int j = 0; j < ns; ++j) {
         const uint32_t q = sb + j * kBlock;
@@ -884,10 +895,11 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
         const T old = s.wa[q], wb = s.wb[q];
         if (s.delta != nullptr) s.acc_gain[q] += d * static_cast<double>(wb);
         const T fresh = now * wb + (static_cast<T>(1) - before) * old +
-                        static_cast<T>(s.state_weight[q]) * (before * qa - now * qb);
+                        static_cast<T>(s.state_weight[q]) * (before * qaq - now * qbq);
         gsum += static_cast<T>(s.branch_gain[kd]) * (fresh - old);
         s.wo[q] = fresh;
     }
+    if (SCALED) gsum = sc * gsum;
     T q = ci * (ca * qb + sum - cb * qa + gsum);
     if (s.sp_tap != nullptr) {
         const uint32_t u = s.sp_tap[k];
@@ -896,18 +908,19 @@ __device__ __forceinline__ void adjoint_special(const StepArgs<T> &s, uint32_t b
     s.o[i] = q;
 }
 
-template <typename T, bool WIDE>
+template <typename T, bool WIDE, bool SCALED>
 __global__ __launch_bounds__(kBlock) void adjoint_kernel(const StepArgs<T> s) {
     const uint32_t block = blockIdx.x;
     if (block < s.special_blocks) {
-        adjoint_special<T>(s, block);
+        adjoint_special<T, SCALED>(s, block);
         return;
     }
-    interior_tile<T, WIDE, false, true>(s, block - s.special_blocks);
+    interior_tile<T, WIDE, SCALED, true>(s, block - s.special_blocks);
 }
 
 // The probes' sensitivities into the adjoint field: Q[cell] += coef_inv (sum over the taps
-// on the cell of weight dJ/dy(probe, n)), one thread per distinct tap cell.
+// on the cell of weight dJ/dy(probe, n)), one thread per distinct tap cell; ci holds each
+// cell's coef_inv, times its scale where the grid has a speed per cell (R = s Q).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void tap_values(T *__restrict__ inj, const double *__restrict__ ci,
                                                     const uint32_t *__restrict__ start,
@@ -923,8 +936,9 @@ __global__ __launch_bounds__(kBlock) void tap_values(T *__restrict__ inj, const 
     inj[id] = static_cast<T>(ci[u] * sum);
 }
 
-// The gradient's sums: each general special's D(n) Q(n) and each of its states' D(n) W(n).
-template <typename T>
+// The gradient's sums: each general special's D(n) Q(n) and each of its states' D(n) W(n)
+// (Q = R / s where the grid has a speed per cell).
+template <typename T, bool SCALED>
 __global__ __launch_bounds__(kBlock) void accumulate_kernel(const StepArgs<T> s, const T *__restrict__ q,
                                                            const T *__restrict__ w, const T *__restrict__ delta,
                                                            double *__restrict__ acc_g, double *__restrict__ acc_gain) {
@@ -932,7 +946,9 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(const StepArgs<T> s,
     const uint32_t k = block * kBlock + t;
     if (k >= s.specials) return;
     const double d = static_cast<double>(delta[k]);
-    acc_g[k] += d * static_cast<double>(q[s.sp_cell[k]]);
+    const uint32_t cell = s.sp_cell[k];
+    const T qk = SCALED ? q[cell] / static_cast<T>(s.scale[cell]) : q[cell];
+    acc_g[k] += d * static_cast<double>(qk);
     const uint32_t sb = s.state_base[block] + t;
     for (int j = 0; j < s.sp_states[k]; ++j) {
         const uint32_t slot = sb + j * kBlock;
@@ -2541,7 +2557,6 @@ int record_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t co
 
 void check_adjoint(const as_fdtd *h) {
     if (h->layer_cells > 0) refuse("the adjoint needs a closed grid: no matched layer");
-    if (h->scaled) refuse("the adjoint needs one speed of sound for the whole grid");
     if (h->face_specials > 0) refuse("the adjoint needs a handle made with no_faces set");
 }
 
@@ -2707,6 +2722,12 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     }
     tstart.push_back(static_cast<uint32_t>(tprobe.size()));
     const uint32_t unique = static_cast<uint32_t>(tcell.size());
+    if (h->scaled && unique > 0) {
+        // R = s Q: a tap's dJ/dy enters times its cell's scale.
+        std::vector<float> scale_host(h->n_pad);
+        check(hipMemcpy(scale_host.data(), h->scale, h->n_pad * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+        for (uint32_t u = 0; u < unique; ++u) tci[u] *= static_cast<double>(scale_host[tcell[u]]);
+    }
     auto put = [&](const auto &v) {
         using U = typename std::decay_t<decltype(v)>::value_type;
         U *p = static_cast<U *>(alloc(std::max<size_t>(v.size(), 1) * sizeof(U)));
@@ -2811,16 +2832,25 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
             a.delta = n < k0 + top ? history + (n - k0) * G : nullptr;
             a.inj = inj + (n - k0 - 1) * unique;
             if (ablocks) {
-                if (h->wide) {
-                    hipLaunchKernelGGL((adjoint_kernel<T, true>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                if (h->wide && h->scaled) {
+                    hipLaunchKernelGGL((adjoint_kernel<T, true, true>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                } else if (h->wide) {
+                    hipLaunchKernelGGL((adjoint_kernel<T, true, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                } else if (h->scaled) {
+                    hipLaunchKernelGGL((adjoint_kernel<T, false, true>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
                 } else {
-                    hipLaunchKernelGGL((adjoint_kernel<T, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
+                    hipLaunchKernelGGL((adjoint_kernel<T, false, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
                 }
             }
         }
         if (sblocks) {
-            hipLaunchKernelGGL((accumulate_kernel<T>), dim3(sblocks), dim3(kBlock), 0, h->stream, a, a.a, a.wa,
-                               history, acc_g, acc_gain);
+            if (h->scaled) {
+                hipLaunchKernelGGL((accumulate_kernel<T, true>), dim3(sblocks), dim3(kBlock), 0, h->stream, a, a.a, a.wa,
+                                   history, acc_g, acc_gain);
+            } else {
+                hipLaunchKernelGGL((accumulate_kernel<T, false>), dim3(sblocks), dim3(kBlock), 0, h->stream, a, a.a,
+                                   a.wa, history, acc_g, acc_gain);
+            }
         }
     };
     // Whole segments (every steps, the adjoint's top at the segment's end) replay captured

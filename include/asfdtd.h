@@ -169,7 +169,7 @@ void as_fdtd_destroy(as_fdtd *h);
  * checkpoint and the adjoint stepped back through it. dl_dout holds dJ/dy[probe][step]
  * (probes x steps). Out: the forward probes (probes x steps, or NULL), dJ/dg per layout
  * special (specials) and dJ/dgain per layout state (states). The handle must be made with
- * no_faces set, on a closed grid (no matched layer) with one speed of sound. It is left at
+ * no_faces set, on a closed grid (no matched layer), with a speed of sound per cell or not. It is left at
  * rest.
  */
 int as_fdtd_gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout,

@@ -1,8 +1,9 @@
 """TEST INFRASTRUCTURE ONLY — the adjoint's oracle.
 
 A double-precision NumPy restatement of the wave solver's step (the reference's run(),
-cpp/include/acoustic_spaces/fdtd.hpp:786-904, for a closed grid: interior, special cells
-with their branch states, probes read before each step), driven by a layout whose special
+cpp/include/acoustic_spaces/fdtd.hpp:786-904, for a closed grid: interior, with a speed of
+sound per cell or not, special cells with their branch states, probes read before each
+step), driven by a layout whose special
 cells' wall terms and branch gains are given as doubles: coef_b = 1 - g, coef_inv =
 1 / (1 + g) for each special's g (add_special, fdtd.hpp:3499-3517) and each state's gain
 (fdtd.hpp:391-411). Central finite differences of this forward are the ground truth the
@@ -27,8 +28,12 @@ class Forward:
     state's gain given (see the module's notes)."""
 
     def __init__(self, layout: dict):
-        if len(layout["layer_cells"]) or int(layout["scaled"][0]):
-            raise ValueError("a closed grid with one speed of sound")
+        if len(layout["layer_cells"]):
+            raise ValueError("a closed grid: no matched layer")
+        # A speed of sound per cell: the interior's scaled update (fdtd.hpp:240-288),
+        # (2 - total s) p + (s axial) sum [+ (s diagonal) edges] - p_prev.
+        self.scale = np.asarray(layout["scale"], np.float64) if int(layout["scaled"][0]) else None
+        self.total = float(layout["total"][0])
         nx, ny, nz = (int(v) for v in layout["shape"])
         self.n = nx * ny * nz
         offsets = STEPS[:, 0] + STEPS[:, 1] * nx + STEPS[:, 2] * nx * ny
@@ -83,9 +88,15 @@ class Forward:
                                        minlength=self.probes)
             nxt = a.copy()
             i = self.interior
-            total = self.centre * b[i] + self.axial * b[self.nb].sum(axis=1)
-            if self.wide:
-                total = total + self.diagonal * b[self.edges].sum(axis=1)
+            if self.scale is None:
+                total = self.centre * b[i] + self.axial * b[self.nb].sum(axis=1)
+                if self.wide:
+                    total = total + self.diagonal * b[self.edges].sum(axis=1)
+            else:
+                sc = self.scale[i]
+                total = (2.0 - self.total * sc) * b[i] + (sc * self.axial) * b[self.nb].sum(axis=1)
+                if self.wide:
+                    total = total + (sc * self.diagonal) * b[self.edges].sum(axis=1)
             nxt[i] = total - a[i]
             sp = self.special
             links = np.bincount(self.link_owner, self.link_weight * b[self.link_cell], minlength=K)

@@ -20,11 +20,20 @@ def _layout(stencil=7, scene=None):
     return sc, WaveSolver.from_scene(sc).layout()
 
 
-def test_numpy_forward_equals_the_double_oracle():
+def _speed_layout(stencil=7):
+    """The small box with a speed of sound per cell in (0.9, 1]."""
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    sc = cases.small_box(stencil=stencil)
+    return sc, WaveSolver.from_scene(sc, speed=cases.speeds_for(sc)).layout()
+
+
+@pytest.mark.parametrize("speed", [False, True])
+def test_numpy_forward_equals_the_double_oracle(speed):
     from oracle.adjoint import Forward
     from oracle.stepper import OracleSolver
 
-    sc, layout = _layout()
+    sc, layout = _speed_layout() if speed else _layout()
     f = Forward(layout)
     g, gain = f.layout_coefficients(layout)
     drive = sc.drive[:200]
@@ -88,6 +97,33 @@ def test_gradient_equals_finite_differences(stencil, listeners):
     for j in rng.choice(len(gain), 5, replace=False):
         fd = _fd(f, layout, drive, gain, c, "gain", j, 1e-6)
         assert abs(fd - sens_gain[j]) <= 1e-6 * scale_gain + 1e-5 * abs(fd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stencil", [7, 19])
+def test_gradient_with_a_speed_per_cell_equals_finite_differences(stencil):
+    """A speed of sound per cell (scales in (0.9, 1]): the adjoint steps R = s Q, the
+    scaled interior's own update, and each special reads Q = R / s."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.adjoint import Forward
+
+    sc, layout = _speed_layout(stencil)
+    drive = sc.drive[:300]
+    rng = np.random.default_rng(7)
+    gpu = HipWaveSolver(layout, precision=64, no_faces=True)
+    c = rng.normal(size=(gpu.probes, len(drive)))
+    y, sens_g, sens_gain = gpu.gradient(drive, c, every=64)
+    f = Forward(layout)
+    g, gain = f.layout_coefficients(layout)
+    assert np.allclose(y, f.run(drive, g, gain, coef_b=np.asarray(layout["coef_b"], np.float64),
+                                coef_inv=np.asarray(layout["coef_inv"], np.float64)), rtol=0, atol=1e-9 * np.abs(y).max())
+    scale_g, scale_gain = np.abs(sens_g).max(), np.abs(sens_gain).max()
+    for k in rng.choice(len(g), 6, replace=False):
+        fd = _fd(f, layout, drive, gain, c, "g", k, 1e-6)
+        assert abs(fd - sens_g[k]) <= 1e-6 * scale_g + 1e-5 * abs(fd), (k, fd, sens_g[k])
+    for j in rng.choice(len(gain), 6, replace=False):
+        fd = _fd(f, layout, drive, gain, c, "gain", j, 1e-6)
+        assert abs(fd - sens_gain[j]) <= 1e-6 * scale_gain + 1e-5 * abs(fd), (j, fd, sens_gain[j])
 
 
 @pytest.mark.gpu
