@@ -1,9 +1,10 @@
-"""The stepper's autotuner: tile depth, steps per graph and where the face specials are
-stepped, chosen by timing them.
+"""The stepper's autotuner: tile depth, steps per graph, where the face specials are
+stepped and the steps a launch takes, chosen by timing them.
 
 For a layout it times each candidate launch shape (tile_z planes per tile, graph_steps
-steps per captured graph, face specials by blocks of their own or in their tiles) over a
-short run on the GPU, with hipEvents on the stepper's stream, and keeps the fastest. Results
+steps per captured graph, face specials by blocks of their own or in their tiles, one step
+a launch or two steps by two launches with boxes of 8 or 16 planes, `pairs`) over a short
+run on the GPU, with hipEvents on the stepper's stream, and keeps the fastest. Results
 are cached by the layout's signature (its grid, cell counts and stencil) and the library's
 build (a hash of libasfdtd.so) in a JSON file, as the reference keeps its own device timings
 (engine/tuning.py:76-180); the reference tunes only the CPU/GPU choice, this tunes the
@@ -23,9 +24,21 @@ import numpy as np
 # within TIE of the fastest counts as fast as it, so that the choice holds from run to run.
 # Tiles of 16 planes re-read less of the planes above and below a tile; they pay off where
 # tiles are mostly interior (outdoors, 1.5 %), and lose in a room with walls all round.
-CANDIDATES = tuple({"tile_z": tz, "graph_steps": gs, "face_blocks": fb}
-                   for tz in (8, 4, 16) for gs in (64, 128, 32) for fb in (True, False))
+ONE_STEP = tuple({"tile_z": tz, "graph_steps": gs, "face_blocks": fb, "pairs": 0}
+                 for tz in (8, 4, 16) for gs in (64, 128, 32) for fb in (True, False))
+# Two steps by two launches: the first steps the cells whose neighbours are all interior
+# twice, in boxes marching 8 or 16 planes in registers (float, 7 points, one speed of sound).
+TWO_STEPS = tuple({"tile_z": 8, "graph_steps": 64, "face_blocks": fb, "pairs": planes}
+                  for planes in (8, 16) for fb in (True, False))
+CANDIDATES = ONE_STEP + TWO_STEPS
 TIE = 0.01
+
+
+def applicable(layout: dict, candidates, precision: int = 32) -> tuple:
+    """The candidates the layout can take: two steps by two launches only for float grids
+    of 7 points with one speed of sound."""
+    pairs_ok = precision == 32 and int(layout["stencil"][0]) == 7 and not int(layout["scaled"][0])
+    return tuple(c for c in candidates if pairs_ok or not c.get("pairs"))
 
 
 def cache_path() -> Path:
@@ -90,10 +103,11 @@ def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int 
 def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: bool = True,
              precision: int = 32, drive=None) -> dict:
     """The fastest launch shape for the layout (the first candidate within TIE of the
-    fastest): the candidate's keys (tile_z, graph_steps, face_blocks), "us_per_step" and
+    fastest): the candidate's keys (tile_z, graph_steps, face_blocks, pairs), "us_per_step" and
     "timings"; from the cache when it holds this layout's
     signature. With ``drive`` each candidate is timed on live fields (after 256 steps of it),
     which can rank shapes otherwise than zero fields do."""
+    candidates = applicable(layout, candidates, precision)
     key = cache_key(layout, precision, drive is not None, candidates)
     path = cache_path()
     cache = {}

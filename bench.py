@@ -296,12 +296,12 @@ def plain(scene, name, args, traffic_name=None):
     del host_layout
     if args.tile_z or args.graph_steps or args.face_blocks >= 0:
         launch = {"tile_z": args.tile_z, "graph_steps": args.graph_steps,
-                  "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks)}
+                  "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks), "pairs": 0}
         launch_from = "flags (0 or null: the library's default)"
     else:
         # Set-up, outside the timed region: the launch shape timed on the scene's drive.
         shape = tune.autotune(layout, steps=512, drive=drive)
-        launch = {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks")}
+        launch = {k: shape.get(k, 0) for k in ("tile_z", "graph_steps", "face_blocks", "pairs")}
         launch_from = "the autotuner, each candidate timed on the scene's drive (live fields)"
     # A timed run shorter than the tuned graph replays graphs of a length that divides it
     # (0 is the library's default, 64).
@@ -321,7 +321,9 @@ def plain(scene, name, args, traffic_name=None):
                     "device_is": "every pass over the whole grid on the GPU (cut links, flood, layer, wall cells); the source's and probes' readings on the host",
                     "host_threads": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())},
         "launch": launch, "launch_from": launch_from,
-        "timed_launches": launch_mode(launch["graph_steps"], K),
+        "timed_launches": launch_mode(launch["graph_steps"], K) + (
+            f", two steps by two launches (boxes of {launch['pairs'] if launch['pairs'] > 1 else 8} planes)"
+            if launch.get("pairs") else ", one step a launch"),
         "lead_in_steps": lead,
         "lead_in": "steps of the drive from rest before the warm-up, so that the timed steps "
                    "meet the spreading pulse (live fields), not fields at rest",
@@ -341,6 +343,8 @@ def plain(scene, name, args, traffic_name=None):
         config["insertion_loss"] = insertion_loss(scene, layout)
     return line_for(f"FDTD Mcell-updates/s ({name}, plain solver)", air / per_step_s / 1e6, K, W, ms, scene.name,
                     config, roofline_of(step_bytes, per_step_s, traffic,
+                                        "pair_kernel + step_kernel<float,false,false> (two steps by two launches)"
+                                        if launch.get("pairs") else
                                         "step_kernel<float,false,false> (one fused launch per step)"))
 
 
@@ -376,7 +380,7 @@ def rooms(args):
     drive = scenes.gaussian_drive(batch[0].rate, W + K)
     merged, placements = merge_layouts(layouts)
     shape = tune.autotune(merged, steps=128, drive=drive)
-    launch = {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks")}
+    launch = {k: shape.get(k, 0) for k in ("tile_z", "graph_steps", "face_blocks", "pairs")}
     stepper = HipWaveSolver(merged, **launch)
     info = stepper.info()
     ms_one, out = time_steps(stepper, drive, W, K)
@@ -424,7 +428,7 @@ def rooms(args):
     step_bytes = PER_AIR_CELL * air + PER_STATE * states
     config = {
         "rooms": len(batch), "air_cells": air, "merged_grid": merged_shape,
-        "setup_s": round(setup_s, 2), "tuned": {k: shape[k] for k in ("tile_z", "graph_steps", "face_blocks", "us_per_step")},
+        "setup_s": round(setup_s, 2), "tuned": {k: shape.get(k, 0) for k in ("tile_z", "graph_steps", "face_blocks", "pairs", "us_per_step")},
         "sequential_ms": round(ms_seq, 2), "one_grid_ms": round(ms_one, 2),
         "grouped_ms": {f"{mb} MB": {"ms": round(v[0], 2), "groups": v[1]} for mb, v in grouped.items()},
         "batched": "one grid" if ms_one <= grouped[best_mb][0] else f"groups of at most {best_mb} MB",
