@@ -956,6 +956,105 @@ __global__ __launch_bounds__(kBlock) void accumulate_kernel(const StepArgs<T> s,
     }
 }
 
+// ---- The adjoint of the matched layer --------------------------------------------------
+//
+// The layer (layer_cell) steps each cell's three parts and the velocities of its six faces
+// from the pressures before the step:
+//   lo_a(n+1) = kf[x] lo_a(n) - pf[x] (p(n) - p_below(n)),  hi_a likewise at x + 1 with
+//   (p_above(n) - p(n));  part_a(n+1) = kc[x] part_a(n) - pc[x] (hi_a(n+1) - lo_a(n+1));
+//   p(n+1) = part_x + part_y + part_z
+// (pc times the cell's scale where the grid has a speed per cell; a face shared by two layer
+// cells is one variable of each, of equal values). Its adjoint keeps, per layer cell, the
+// adjoints of its parts (LP) and of its six face velocities (LF), apart from the fields:
+// the fields' layer entries stay 0, so the rows off the layer read nothing there. Each
+// adjoint step, after adjoint_kernel:
+//   layer_pressure: the adjoint of the cell's p(n), P = taps + sum over the faces that read
+//     p(n) (its own six and its layer neighbours' facing it) of their coefficient times LF(n+1)
+//     + sum over the cells off the layer whose rows read it (alpha, the row's coefficient on
+//     it, in the fields' variables) of alpha times their field at n+1; and into each such
+//     cell's field at n, its face's coefficient on that cell's p times LF(n+1) (mult: in the
+//     field's variables); then LP(n) = kc LP(n+1) + P;
+//   layer_faces: LF_lo(n) = kf[x] LF_lo(n+1) + pc LP(n), LF_hi(n) = kf[x+1] LF_hi(n+1) - pc LP(n).
+template <typename T>
+struct LayerAdjoint {
+    uint32_t cells;
+    const uint32_t *__restrict__ cell;    // padded index of each layer cell
+    const uint32_t *__restrict__ index;   // per padded cell: its layer index, or kNoTap
+    const float *__restrict__ keep[6];
+    const float *__restrict__ push[6];
+    const float *__restrict__ scale;      // per layer cell, or null
+    int px, ny;
+    long long sxy;
+    T *__restrict__ part;                 // 3 per layer cell
+    T *__restrict__ face;                 // 6 per layer cell: lo x, hi x, lo y, hi y, lo z, hi z
+    const uint32_t *__restrict__ couple_start;  // per layer cell, into the couplings
+    const uint8_t *__restrict__ couple_dir;     // 0..5: the direction from the layer cell
+    const uint32_t *__restrict__ couple_cell;
+    const T *__restrict__ couple_alpha;
+    const T *__restrict__ couple_mult;
+    const uint32_t *__restrict__ tap;     // per layer cell: its tap, or kNoTap
+    const T *__restrict__ inj;
+    const T *__restrict__ b;              // the fields at n + 1
+    T *o;                                 // the fields at n (written before, added to here)
+};
+
+template <typename T>
+__device__ __forceinline__ void layer_coords(const LayerAdjoint<T> &s, uint32_t i, uint32_t *at) {
+    const uint32_t yz = i / static_cast<uint32_t>(s.px);
+    at[0] = i - yz * static_cast<uint32_t>(s.px);
+    at[2] = yz / static_cast<uint32_t>(s.ny);
+    at[1] = yz - at[2] * static_cast<uint32_t>(s.ny);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void layer_pressure_adjoint(const LayerAdjoint<T> s) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= s.cells) return;
+    const uint32_t i = s.cell[c];
+    uint32_t at[3];
+    layer_coords(s, i, at);
+    const long long stride[3] = {1, s.px, s.sxy};
+    const T *lf = s.face + 6 * static_cast<size_t>(c);
+    T p = 0;
+    if (s.tap != nullptr && s.tap[c] != kNoTap) p = s.inj[s.tap[c]];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T pf_lo = static_cast<T>(s.push[2 * a + 1][at[a]]), pf_hi = static_cast<T>(s.push[2 * a + 1][at[a] + 1]);
+        p += pf_hi * lf[2 * a + 1] - pf_lo * lf[2 * a];
+        const uint32_t up = s.index[i + stride[a]], down = s.index[i - stride[a]];
+        if (up != kNoTap) p += pf_hi * s.face[6 * static_cast<size_t>(up) + 2 * a];
+        if (down != kNoTap) p -= pf_lo * s.face[6 * static_cast<size_t>(down) + 2 * a + 1];
+    }
+    for (uint32_t q = s.couple_start[c]; q < s.couple_start[c + 1]; ++q) {
+        const uint32_t j = s.couple_cell[q], dir = s.couple_dir[q], a = dir / 2;
+        p += s.couple_alpha[q] * s.b[j];
+        // The face between the two cells reads the other's p with this coefficient.
+        const T term = (dir & 1u) ? -static_cast<T>(s.push[2 * a + 1][at[a] + 1]) * lf[2 * a + 1]
+                                  : static_cast<T>(s.push[2 * a + 1][at[a]]) * lf[2 * a];
+        atomicAdd(&s.o[j], s.couple_mult[q] * term);
+    }
+    T *lp = s.part + 3 * static_cast<size_t>(c);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) lp[a] = static_cast<T>(s.keep[2 * a][at[a]]) * lp[a] + p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void layer_faces_adjoint(const LayerAdjoint<T> s) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= s.cells) return;
+    uint32_t at[3];
+    layer_coords(s, s.cell[c], at);
+    T *lf = s.face + 6 * static_cast<size_t>(c);
+    const T *lp = s.part + 3 * static_cast<size_t>(c);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        T pc = static_cast<T>(s.push[2 * a][at[a]]);
+        if (s.scale != nullptr) pc = pc * static_cast<T>(s.scale[c]);
+        lf[2 * a] = static_cast<T>(s.keep[2 * a + 1][at[a]]) * lf[2 * a] + pc * lp[a];
+        lf[2 * a + 1] = static_cast<T>(s.keep[2 * a + 1][at[a] + 1]) * lf[2 * a + 1] - pc * lp[a];
+    }
+}
+
 // ---- Two steps a launch: `pair_kernel` --------------------------------------------------
 //
 // Steps k and k + 1 as two launches instead of two steps of the one-step kernel (float, 7
@@ -1235,6 +1334,10 @@ struct as_fdtd {
     std::vector<double> probe_weight_host;
     std::vector<double> coef_inv_host;        // per layout special
     std::vector<uint64_t> special_cell_host;  // padded cell per layout special
+    // Where the grid has a matched layer, for its adjoint: its cells (padded) and each
+    // layout special's axial link weights (0 where it has no link that way).
+    std::vector<uint32_t> layer_cell_host;
+    std::vector<float> special_axial_host;
     uint32_t *layer_cell = nullptr;
     uint8_t *layer_edge = nullptr;             // see StepArgs::layer_edge
     void *layer_face[2] = {nullptr, nullptr};  // the face velocities' slots; slot prev & 1 is current
@@ -1675,6 +1778,12 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     for (uint64_t k = 0; k < K; ++k) h->coef_inv_host[k] = exact ? L.coef_inv64[k] : L.coef_inv[k];
     h->special_cell_host.resize(K);
     for (uint64_t k = 0; k < K; ++k) h->special_cell_host[k] = pad(static_cast<uint64_t>(L.special[k]));
+    if (L.layer_cells > 0) {
+        h->special_axial_host.assign(6 * K, 0.0f);
+        for (uint64_t k = 0; k < K; ++k)
+            for (uint64_t l = L.link_start[k]; l < L.link_start[k + 1]; ++l)
+                if (L.link_direction[l] < 6) h->special_axial_host[6 * k + L.link_direction[l]] = L.link_weight[l];
+    }
     for (uint64_t k = 0; k < G; ++k) {
         const uint64_t src = order[k], B = k / kBlock, t = k % kBlock;
         for (uint64_t u = 0; u < states_of(src); ++u) h->state_slot[L.state_start[src] + u] = state_base[B] + u * kBlock + t;
@@ -1944,6 +2053,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (!h->face_lookup && !face_desc.empty()) face_branch.resize(256 * 4, 0.0f);
     h->face_branch = reinterpret_cast<float4 *>(upload(h, face_branch));
     h->layer_cell = upload(h, layer_cell);
+    h->layer_cell_host = layer_cell;
     h->layer_edge = upload(h, layer_edge);
     divisor_of(static_cast<uint32_t>(px), &h->px_magic, &h->px_shift);
     divisor_of(static_cast<uint32_t>(ny), &h->ny_magic, &h->ny_shift);
@@ -2556,8 +2666,9 @@ int record_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t co
 }
 
 void check_adjoint(const as_fdtd *h) {
-    if (h->layer_cells > 0) refuse("the adjoint needs a closed grid: no matched layer");
     if (h->face_specials > 0) refuse("the adjoint needs a handle made with no_faces set");
+    if (h->layer_cells > 0 && (h->px > (1 << 30) || h->ny > (1 << 30)))
+        refuse("the adjoint's matched layer needs a grid under 2^30 cells along each axis");
 }
 
 // The forward half of the gradient: from rest, `steps` steps, the state saved before each
@@ -2590,6 +2701,11 @@ void record(as_fdtd *h, const float *signal, size_t steps, size_t every, double 
         size_t free_bytes = 0, total_bytes = 0;
         check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
         keep = hist_bytes <= std::min(kHistoryBudget, free_bytes / 4);
+    }
+    if (h->layer_cells > 0 && !keep) {
+        // Its checkpoints hold no layer state to recompute a segment from.
+        refuse("the adjoint of a grid with a matched layer keeps the record's history: keep_history off, or a "
+               "history beyond 16 GiB or a quarter of free device memory, is refused");
     }
     StepArgs<T> hs = step_args<T>(h);
     const uint32_t hblocks = block_count(h, hs);
@@ -2669,6 +2785,7 @@ template <typename T>
 void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gain) {
     check_adjoint(h);
     if (h->rec_ckpt == nullptr) refuse("no forward run recorded (as_fdtd_record)");
+    if (h->layer_cells > 0 && h->rec_hist == nullptr) refuse("the adjoint of a grid with a matched layer needs the record's history");
     const size_t steps = h->rec_steps, every = h->rec_every;
     const size_t e = sizeof(T);
     const size_t G = h->specials, slots = h->state_slots, P = h->probes;
@@ -2722,11 +2839,24 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     }
     tstart.push_back(static_cast<uint32_t>(tprobe.size()));
     const uint32_t unique = static_cast<uint32_t>(tcell.size());
-    if (h->scaled && unique > 0) {
-        // R = s Q: a tap's dJ/dy enters times its cell's scale.
-        std::vector<float> scale_host(h->n_pad);
+    std::vector<float> scale_host;
+    if (h->scaled) {
+        scale_host.resize(h->n_pad);
         check(hipMemcpy(scale_host.data(), h->scale, h->n_pad * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
-        for (uint32_t u = 0; u < unique; ++u) tci[u] *= static_cast<double>(scale_host[tcell[u]]);
+    }
+    // The layer's cells by padded cell (the layer's adjoint keeps its own variables).
+    const size_t layer_count = h->layer_cells;
+    std::vector<uint32_t> layer_index;
+    if (layer_count > 0) {
+        layer_index.assign(h->n_pad, kNoTap);
+        for (uint32_t c = 0; c < layer_count; ++c) layer_index[h->layer_cell_host[c]] = c;
+    }
+    for (uint32_t u = 0; u < unique; ++u) {
+        if (!layer_index.empty() && layer_index[tcell[u]] != kNoTap) {
+            tci[u] = 1.0;  // a layer cell's own adjoint: dJ/dy as it is
+        } else if (h->scaled) {
+            tci[u] *= static_cast<double>(scale_host[tcell[u]]);  // R = s Q: times the cell's scale
+        }
     }
     auto put = [&](const auto &v) {
         using U = typename std::decay_t<decltype(v)>::value_type;
@@ -2750,13 +2880,16 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
     std::vector<std::pair<uint64_t, uint32_t>> special_at;  // (cell, sorted special)
     for (size_t k = 0; k < G; ++k) special_at.push_back({h->special_cell_host[h->sorted_to_special[k]], static_cast<uint32_t>(k)});
     std::sort(special_at.begin(), special_at.end());
-    std::vector<uint32_t> sp_tap(G, kNoTap), tap_tiles;
+    std::vector<uint32_t> sp_tap(G, kNoTap), tap_tiles, layer_tap;
+    if (layer_count > 0) layer_tap.assign(layer_count, kNoTap);
     std::vector<std::pair<uint32_t, uint32_t>> interior_taps;  // (cell, tap)
     for (uint32_t u = 0; u < unique; ++u) {
         const uint64_t c = tcell[u];
         const auto it = std::lower_bound(special_at.begin(), special_at.end(), std::make_pair(c, uint32_t{0}));
         if (it != special_at.end() && it->first == c) {
             sp_tap[it->second] = u;
+        } else if (!layer_index.empty() && layer_index[c] != kNoTap) {
+            layer_tap[layer_index[c]] = u;
         } else if (types_host[c] == kClassInterior) {
             types_host[c] = kClassTap;
             interior_taps.push_back({static_cast<uint32_t>(c), u});
@@ -2782,6 +2915,67 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
         check(hipStreamSynchronize(h->stream), "adjoint");  // before tap_of goes
     }
     uint32_t *d_sp_tap = G ? put(sp_tap) : nullptr;
+    // The matched layer's adjoint (see LayerAdjoint): its couplings with the cells off the
+    // layer next to it, each with the coefficient that cell's row puts on the layer cell's
+    // p (in the fields' variables: Q, or R = s Q) and the factor that takes the layer's term
+    // into that cell's field.
+    LayerAdjoint<T> la{};
+    if (layer_count > 0) {
+        std::vector<std::pair<uint64_t, uint32_t>> special_of;  // (cell, layout special)
+        for (size_t k = 0; k < h->special_cell_host.size(); ++k) special_of.push_back({h->special_cell_host[k], static_cast<uint32_t>(k)});
+        std::sort(special_of.begin(), special_of.end());
+        std::vector<uint32_t> cstart{0}, ccell;
+        std::vector<uint8_t> cdir;
+        std::vector<T> calpha, cmult;
+        const long long stride[3] = {1, h->px, static_cast<long long>(h->sxy)};
+        for (uint32_t c = 0; c < layer_count; ++c) {
+            const uint64_t i = h->layer_cell_host[c];
+            for (uint32_t dir = 0; dir < 6; ++dir) {
+                const long long step = (dir & 1u) ? stride[dir / 2] : -stride[dir / 2];
+                const uint64_t j = static_cast<uint64_t>(static_cast<long long>(i) + step);
+                if (layer_index[j] != kNoTap || types_host[j] == 0) continue;  // a layer cell, or solid
+                const double s_j = h->scaled ? static_cast<double>(scale_host[j]) : 1.0;
+                double alpha = 0.0, mult = 0.0;
+                const auto it = std::lower_bound(special_of.begin(), special_of.end(), std::make_pair(j, uint32_t{0}));
+                if (it != special_of.end() && it->first == j) {
+                    // Its row: coef_inv (... + w p_layer ...): w on Q, w / s on R; its field
+                    // takes coef_inv (times s) of the layer's term.
+                    const uint32_t k = it->second;
+                    alpha = static_cast<double>(h->special_axial_host[6 * static_cast<size_t>(k) + (dir ^ 1u)]) / s_j;
+                    mult = h->coef_inv_host[k] * s_j;
+                } else {
+                    // An interior row: s axial on p_layer, axial on R (axial on Q unscaled).
+                    alpha = h->axial;
+                    mult = s_j;
+                }
+                cdir.push_back(static_cast<uint8_t>(dir));
+                ccell.push_back(static_cast<uint32_t>(j));
+                calpha.push_back(static_cast<T>(alpha));
+                cmult.push_back(static_cast<T>(mult));
+            }
+            cstart.push_back(static_cast<uint32_t>(ccell.size()));
+        }
+        la.cells = static_cast<uint32_t>(layer_count);
+        la.cell = h->layer_cell;
+        la.index = put(layer_index);
+        for (int k = 0; k < 6; ++k) {
+            la.keep[k] = h->keep[k];
+            la.push[k] = h->push[k];
+        }
+        la.scale = h->layer_scale;
+        la.px = static_cast<int>(h->px);
+        la.ny = static_cast<int>(h->ny);
+        la.sxy = static_cast<long long>(h->sxy);
+        la.part = static_cast<T *>(alloc(3 * layer_count * e));
+        la.face = static_cast<T *>(alloc(6 * layer_count * e));
+        la.couple_start = put(cstart);
+        la.couple_dir = put(cdir);
+        la.couple_cell = put(ccell);
+        la.couple_alpha = put(calpha);
+        la.couple_mult = put(cmult);
+        la.tap = put(layer_tap);
+    }
+    const uint32_t layer_blocks = static_cast<uint32_t>((layer_count + kBlock - 1) / kBlock);
     T *inj = static_cast<T *>(alloc((every + 1) * std::max<uint32_t>(unique, 1) * e));
 
     StepArgs<T> r = step_args<T>(h);  // recompute: no probes, history on
@@ -2841,6 +3035,13 @@ void adjoint(as_fdtd *h, const double *dl_dout, double *sens_g, double *sens_gai
                 } else {
                     hipLaunchKernelGGL((adjoint_kernel<T, false, false>), dim3(ablocks), dim3(kBlock), 0, h->stream, a);
                 }
+            }
+            if (layer_blocks) {
+                la.b = a.b;
+                la.o = a.o;
+                la.inj = a.inj;
+                hipLaunchKernelGGL((layer_pressure_adjoint<T>), dim3(layer_blocks), dim3(kBlock), 0, h->stream, la);
+                hipLaunchKernelGGL((layer_faces_adjoint<T>), dim3(layer_blocks), dim3(kBlock), 0, h->stream, la);
             }
         }
         if (sblocks) {

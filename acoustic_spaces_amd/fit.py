@@ -63,8 +63,11 @@ class WallModel:
             branches=np.array([(0.0, 1.0 / placeholder, self.wcT / placeholder)] * self.walls),
             branch_material=np.arange(self.walls, dtype=np.int64))
         solver = WaveSolver.from_scene(laid)
-        if solver.open or solver.layer_cells:
-            raise ValueError("the fit needs a closed room")
+        # A wall cell next to the grid's edge adds the edge's admittance to its wall term,
+        # which the model leaves out: a room closed by its walls, or a matched layer round
+        # the grid (whose cells are all the air next to the edge).
+        if solver.open and not solver.layer_cells:
+            raise ValueError("the fit needs a room closed by its walls or a matched layer round the grid")
         self.base = solver.layout()
         self.probes = solver.probes
         L = self.base

@@ -164,12 +164,14 @@ void as_fdtd_destroy(as_fdtd *h);
  * term g (coef_b = 1 - g, coef_inv = 1 / (1 + g), fdtd.hpp:3499-3517) and each branch
  * state's gain (fdtd.hpp:391-411), by the adjoint (time-reversed) scheme. The reference
  * has no adjoint (no counterpart); it serves fitting wall admittance to measured
- * responses. From rest, `steps` steps of `signal` are run, saving the state every `every`
- * steps; then, segment by segment from the last, the segment is recomputed from its
- * checkpoint and the adjoint stepped back through it. dl_dout holds dJ/dy[probe][step]
+ * responses. From rest, `steps` steps of `signal` are run, keeping every general special's
+ * p(n + 1) - p(n - 1) where that history fits (keep_history) and the state every `every`
+ * steps; then, segment by segment from the last, the segment's history is copied in (or
+ * recomputed from its checkpoint) and the adjoint stepped back through it. dl_dout holds dJ/dy[probe][step]
  * (probes x steps). Out: the forward probes (probes x steps, or NULL), dJ/dg per layout
  * special (specials) and dJ/dgain per layout state (states). The handle must be made with
- * no_faces set, on a closed grid (no matched layer), with a speed of sound per cell or not. It is left at
+ * no_faces set; a speed of sound per cell and a matched layer are allowed (with a matched
+ * layer the record keeps its history: keep_history not -1, and the history within its budget). It is left at
  * rest.
  */
 int as_fdtd_gradient(as_fdtd *h, const float *signal, size_t steps, size_t every, const double *dl_dout,

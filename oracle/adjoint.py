@@ -1,9 +1,9 @@
 """TEST INFRASTRUCTURE ONLY — the adjoint's oracle.
 
 A double-precision NumPy restatement of the wave solver's step (the reference's run(),
-cpp/include/acoustic_spaces/fdtd.hpp:786-904, for a closed grid: interior, with a speed of
-sound per cell or not, special cells with their branch states, probes read before each
-step), driven by a layout whose special
+cpp/include/acoustic_spaces/fdtd.hpp:786-904: interior, with a speed of sound per cell or
+not, special cells with their branch states, the matched layer's split fields, probes read
+before each step), driven by a layout whose special
 cells' wall terms and branch gains are given as doubles: coef_b = 1 - g, coef_inv =
 1 / (1 + g) for each special's g (add_special, fdtd.hpp:3499-3517) and each state's gain
 (fdtd.hpp:391-411). Central finite differences of this forward are the ground truth the
@@ -24,12 +24,28 @@ STEPS = np.array([
 
 
 class Forward:
-    """The closed grid of ``layout`` stepped in double with each special's g and each
-    state's gain given (see the module's notes)."""
+    """The grid of ``layout`` stepped in double with each special's g and each state's gain
+    given (see the module's notes)."""
 
     def __init__(self, layout: dict):
-        if len(layout["layer_cells"]):
-            raise ValueError("a closed grid: no matched layer")
+        # The matched layer (the reference's update_layer, fdtd.hpp:2242-2266): each layer
+        # cell's three parts and six face velocities (its lower and upper face along each
+        # axis), stepped from the pressures before the step.
+        nx_, ny_, _ = (int(v) for v in layout["shape"])
+        self.layer = np.asarray(layout["layer_cells"], np.int64)
+        if len(self.layer):
+            c = self.layer
+            self.layer_at = np.stack([c % nx_, (c // nx_) % ny_, c // (nx_ * ny_)])  # (3, L)
+            self.layer_stride = np.array([1, nx_, nx_ * ny_])
+            axes = "xyz"
+            self.keep_c = np.stack([np.asarray(layout[f"keep_{a}"], np.float64)[self.layer_at[k]] for k, a in enumerate(axes)])
+            self.push_c = np.stack([np.asarray(layout[f"push_{a}"], np.float64)[self.layer_at[k]] for k, a in enumerate(axes)])
+            self.keep_lo = np.stack([np.asarray(layout[f"keep_face_{a}"], np.float64)[self.layer_at[k]] for k, a in enumerate(axes)])
+            self.push_lo = np.stack([np.asarray(layout[f"push_face_{a}"], np.float64)[self.layer_at[k]] for k, a in enumerate(axes)])
+            self.keep_hi = np.stack([np.asarray(layout[f"keep_face_{a}"], np.float64)[self.layer_at[k] + 1] for k, a in enumerate(axes)])
+            self.push_hi = np.stack([np.asarray(layout[f"push_face_{a}"], np.float64)[self.layer_at[k] + 1] for k, a in enumerate(axes)])
+            if len(layout["layer_scale"]):
+                self.push_c = self.push_c * np.asarray(layout["layer_scale"], np.float64)[None, :]
         # A speed of sound per cell: the interior's scaled update (fdtd.hpp:240-288),
         # (2 - total s) p + (s axial) sum [+ (s diagonal) edges] - p_prev.
         self.scale = np.asarray(layout["scale"], np.float64) if int(layout["scaled"][0]) else None
@@ -82,6 +98,8 @@ class Forward:
         b = np.zeros(self.n)
         wa = np.zeros(len(self.state_weight))
         wb = np.zeros(len(self.state_weight))
+        L = len(self.layer)
+        part, lo, hi = np.zeros((3, L)), np.zeros((3, L)), np.zeros((3, L))
         out = np.zeros((self.probes, len(signal)))
         for step, s in enumerate(np.asarray(signal, np.float64)):
             out[:, step] = np.bincount(self.probe_owner, self.probe_weights * b[self.probe_cells],
@@ -106,6 +124,14 @@ class Forward:
             new = ci * (self.coef_a * b[sp] + links - cb * prev - flow + self.source * s)
             nxt[sp] = new
             wa = wa + (new - prev)[self.state_owner] * gain + own
+            if L:
+                c = self.layer
+                for ax in range(3):
+                    st = self.layer_stride[ax]
+                    lo[ax] = self.keep_lo[ax] * lo[ax] - self.push_lo[ax] * (b[c] - b[c - st])
+                    hi[ax] = self.keep_hi[ax] * hi[ax] - self.push_hi[ax] * (b[c + st] - b[c])
+                    part[ax] = self.keep_c[ax] * part[ax] - self.push_c[ax] * (hi[ax] - lo[ax])
+                nxt[c] = part[0] + part[1] + part[2]
             a, b = b, nxt
             wa, wb = wb, wa
         return out

@@ -28,12 +28,24 @@ def _speed_layout(stencil=7):
     return sc, WaveSolver.from_scene(sc, speed=cases.speeds_for(sc)).layout()
 
 
-@pytest.mark.parametrize("speed", [False, True])
-def test_numpy_forward_equals_the_double_oracle(speed):
+def _layer_layout(speed=False):
+    """The rigid block in open air with a matched layer of 4 cells round the grid, and a
+    fourth probe inside the layer (read from eight layer cells)."""
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    sc = cases.open_block()
+    sc.probes = np.vstack([sc.probes, [[-0.33, 0.61, 0.52]]])
+    kw = {"speed": cases.speeds_for(sc)} if speed else {}
+    return sc, WaveSolver.from_scene(sc, **kw).layout()
+
+
+@pytest.mark.parametrize("case", ["box", "speed", "layer", "layer_speed"])
+def test_numpy_forward_equals_the_double_oracle(case):
     from oracle.adjoint import Forward
     from oracle.stepper import OracleSolver
 
-    sc, layout = _speed_layout() if speed else _layout()
+    sc, layout = {"box": _layout, "speed": _speed_layout, "layer": _layer_layout,
+                  "layer_speed": lambda: _layer_layout(True)}[case]()
     f = Forward(layout)
     g, gain = f.layout_coefficients(layout)
     drive = sc.drive[:200]
@@ -124,6 +136,47 @@ def test_gradient_with_a_speed_per_cell_equals_finite_differences(stencil):
     for j in rng.choice(len(gain), 6, replace=False):
         fd = _fd(f, layout, drive, gain, c, "gain", j, 1e-6)
         assert abs(fd - sens_gain[j]) <= 1e-6 * scale_gain + 1e-5 * abs(fd), (j, fd, sens_gain[j])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("speed", [False, True])
+def test_gradient_through_the_matched_layer_equals_finite_differences(speed):
+    """Open air with a matched layer (its parts and face velocities stepped back by their
+    own adjoint kernels, coupled to the cells off the layer through the faces between),
+    with and without a speed of sound per cell; a probe reads layer cells."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.adjoint import Forward
+
+    sc, layout = _layer_layout(speed)
+    drive = sc.drive[:300]
+    rng = np.random.default_rng(11)
+    gpu = HipWaveSolver(layout, precision=64, no_faces=True)
+    c = rng.normal(size=(gpu.probes, len(drive)))
+    y, sens_g, sens_gain = gpu.gradient(drive, c, every=64)
+    f = Forward(layout)
+    g, gain = f.layout_coefficients(layout)
+    assert np.allclose(y, f.run(drive, g, gain, coef_b=np.asarray(layout["coef_b"], np.float64),
+                                coef_inv=np.asarray(layout["coef_inv"], np.float64)), rtol=0, atol=1e-9 * np.abs(y).max())
+    scale_g, scale_gain = np.abs(sens_g).max(), np.abs(sens_gain).max()
+    # Steps of 3e-5: open air leaves J small against the signals, and differences over 1e-6
+    # steps wander by rounding (1e-5 of a gain's sensitivity); over 3e-5 they settle to
+    # within 1e-7 of those over 1e-4.
+    for k in rng.choice(len(g), 8, replace=False):
+        fd = _fd(f, layout, drive, gain, c, "g", k, 3e-5)
+        assert abs(fd - sens_g[k]) <= 1e-6 * scale_g + 1e-5 * abs(fd), (k, fd, sens_g[k])
+    for j in rng.choice(len(gain), 6, replace=False):
+        fd = _fd(f, layout, drive, gain, c, "gain", j, 3e-5)
+        assert abs(fd - sens_gain[j]) <= 1e-6 * scale_gain + 1e-5 * abs(fd), (j, fd, sens_gain[j])
+
+
+@pytest.mark.gpu
+def test_a_layer_without_the_history_is_refused():
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+
+    sc, layout = _layer_layout()
+    gpu = HipWaveSolver(layout, precision=64, no_faces=True, keep_history=False)
+    with pytest.raises(ValueError, match="matched layer keeps the record's history"):
+        gpu.record(sc.drive[:100], every=32)
 
 
 @pytest.mark.gpu

@@ -9,6 +9,7 @@ import math
 import numpy as np
 import pytest
 
+import cases
 from acoustic_spaces_amd import scenes
 
 
@@ -81,6 +82,43 @@ def test_loss_gradient_equals_finite_differences():
         # A 700-step box with noiseless targets: the float solve's rounding matters more
         # against its short decay than on the 0.5 s shoebox, whose test holds float to 1e-3.
         assert rel < (1e-3 if precision == 64 else 1e-2)
+
+
+@pytest.mark.gpu
+def test_open_air_with_a_matched_layer_gradient_equals_finite_differences():
+    """The rigid block in open air with a matched layer round the grid: the gradient of a
+    misfit of the probes' signals through the layer's adjoint, against central differences
+    of the double solve."""
+    import torch
+
+    from acoustic_spaces_amd.fit import WallModel, make_function
+
+    sc = cases.open_block()
+    truth = WallModel(sc, precision=64, every=64, steps=300)
+    walls = truth.walls
+    theta_true = np.concatenate([np.linspace(0.03, 0.15, walls), np.linspace(0.25, 0.05, walls)])
+    target = torch.from_numpy(truth.forward(theta_true))
+    theta0 = theta_true * np.array([1.3, 0.8, 1.1, 0.9, 1.2, 0.7] * 2)[: 2 * walls]
+
+    def loss_of(y):
+        return torch.sum((y - target) ** 2)
+
+    WaveSolve = make_function()
+    t = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+    loss = loss_of(WaveSolve.apply(t, truth))
+    loss.backward()
+    grad = t.grad.numpy()
+    fd = np.zeros_like(theta0)
+    h = 1e-4
+    for p in range(len(theta0)):
+        up, dn = theta0.copy(), theta0.copy()
+        up[p] += h
+        dn[p] -= h
+        fd[p] = (loss_of(torch.from_numpy(truth.forward(up))).item()
+                 - loss_of(torch.from_numpy(truth.forward(dn))).item()) / (2 * h)
+    rel = np.linalg.norm(grad - fd) / np.linalg.norm(fd)
+    print(f"open air with a layer: |grad - fd| / |fd| = {rel:.2e}")
+    assert rel < 1e-4
 
 
 def cfg4_problem():
