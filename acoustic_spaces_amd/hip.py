@@ -4,7 +4,6 @@ the stepper on its own stream without going through torch."""
 from __future__ import annotations
 
 import ctypes
-import os
 
 import numpy as np
 
@@ -12,10 +11,15 @@ _hip = None
 
 
 def runtime():
+    """The HIP runtime the wave-solver library is bound to: loaded by its soname once the
+    library is, so that the process holds one runtime (its streams, events and buffers are
+    then the library's own)."""
     global _hip
     if _hip is None:
-        path = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so")
-        lib = ctypes.CDLL(path)
+        from acoustic_spaces_amd import native
+
+        native.load()
+        lib = ctypes.CDLL("libamdhip64.so.7")
         P = ctypes.c_void_p
         lib.hipMalloc.argtypes = [ctypes.POINTER(P), ctypes.c_size_t]
         lib.hipFree.argtypes = [P]

@@ -74,6 +74,13 @@ def load():
             f"the HIP wave-solver library {LIBRARY} is not built: run "
             "`python -c 'import __graft_entry__ as g; g.build()'` (or python -m acoustic_spaces_amd.build)"
         )
+    # One HIP runtime a process: torch carries its own libamdhip64. Imported first, it is the
+    # one the library binds to; loaded after the library's (/opt/rocm), a second runtime would
+    # claim the device beside the first, and torch's then finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(str(LIBRARY))
     lib.as_fdtd_create.restype = P
     lib.as_fdtd_create.argtypes = [ctypes.POINTER(Layout), ctypes.c_int, ctypes.POINTER(Tuning)]

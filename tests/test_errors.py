@@ -85,10 +85,11 @@ def test_adjoint_refusals():
     c = np.zeros((len(L["probe_start"]) - 1, len(drive)))
     with pytest.raises(ValueError, match="no_faces"):
         HipWaveSolver(L).gradient(drive, c)
+    # A grid with a matched layer keeps the record's history: refused without it.
     open_layout = _layout(cases.open_block())
-    with pytest.raises(ValueError, match="closed grid"):
-        HipWaveSolver(open_layout, no_faces=True).gradient(cases.open_block().drive[:40],
-                                                           np.zeros((3, 40)))
+    with pytest.raises(ValueError, match="matched layer keeps the record's history"):
+        HipWaveSolver(open_layout, no_faces=True, keep_history=False).gradient(cases.open_block().drive[:40],
+                                                                               np.zeros((3, 40)))
     with pytest.raises(ValueError, match="every 0"):
         HipWaveSolver(L, no_faces=True).gradient(drive, c, every=0)
     with pytest.raises(ValueError, match="record"):
