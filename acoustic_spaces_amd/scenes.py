@@ -446,18 +446,24 @@ def rooms_batch(count=64, cell=0.05, seed=3, length_s=0.5, receivers=4, size_low
     return rooms
 
 
-def with_weather(scene: Scene, atmosphere, ground_c: float = 20.0, ground_z: float | None = None) -> Scene:
+def with_weather(scene: Scene, atmosphere, ground_c: float = 20.0, ground_z: float | None = None,
+                 extent=None) -> Scene:
     """``scene`` outdoors under ``atmosphere`` (acoustic_spaces_amd.weather.Atmosphere):
     each cell's effective speed of sound from the source (the wind's component along the
     horizontal from it added to the still air's at the cell's height above the ground at
-    ``ground_z``, whose temperature is ``ground_c``), over the grid's speed, the fastest
-    over the grid; the time step set by that speed (the reference's prepare,
-    engine/wave.py:1000-1060), the length and the drive kept."""
+    ``ground_z``, whose temperature is ``ground_c``; where the atmosphere has zones, the
+    weather of the zone over the cell, their polygons clipped to ``extent``, the grid's plan
+    by default), over the grid's speed, the fastest over the grid; the time step set by that
+    speed (the reference's prepare, engine/wave.py:1000-1060), the length and the drive
+    kept."""
     from acoustic_spaces_amd.weather import Weather
 
     g = scene.grid
     ground = float(scene.mesh.vertices[:, 2].min()) if ground_z is None else float(ground_z)
-    weather = Weather(atmosphere, ground_c, ground, scene.source)
+    if extent is None:
+        far = g.origin + np.asarray(g.shape, float) * g.cell
+        extent = (g.origin[0], g.origin[1], far[0], far[1])
+    weather = Weather(atmosphere, ground_c, ground, scene.source, extent)
     fastest = weather.extremes(g.origin, g.cell, g.shape)[1]
     courant = COURANT if scene.stencil == 7 else ISOTROPIC_COURANT
     rate = fastest / (courant * g.cell)

@@ -499,7 +499,8 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
             if wind == "moving" and atmosphere.wind(np.array([10.0]))[0] != 0.0:
                 raise ValueError("moving air: the reference steps it on the CPU only; set simulation.wave.wind "
                                  "to effective")
-            weather = Weather(atmosphere, float(scene.air.temperature_c), float(model.ground), source)
+            extent = getattr(room.outdoor, "extent", None)  # the ground's, for the zones' polygons
+            weather = Weather(atmosphere, float(scene.air.temperature_c), float(model.ground), source, extent)
 
     def layout(courant, layer, speed, per_wavelength):
         grid = band_grid(bands - 1, sample_rate, speed, per_wavelength, total, courant)

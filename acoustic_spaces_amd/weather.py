@@ -17,9 +17,13 @@ Here the same, for one atmosphere over the whole scene:
 - the still air's speed c(z) = 331.3 sqrt(1 + T(z) / 273.15) (air.py:23);
 - heights below the ground count as at the ground.
 
-The reference's weather zones (polygons with weather of their own, engine/weather.py) are
-not here. The speeds are formed plane by plane in double and rounded once to float, as the
-reference's are.
+Weather zones (engine/weather.py, scene/model.py:2955-3067): parts of the ground (polygons
+in plan, clipped to the ground's extent) with weather of their own, each given over the
+atmosphere's (what a zone gives replaces the atmosphere's, what it leaves out stays). A cell
+takes the weather of the first zone whose polygon holds its centre's (x, y) (even-odd rule,
+edges half-open along y), else the atmosphere's own; the cosine is to that weather's wind.
+The zones' named presets are not here. The speeds are formed plane by plane in double and
+rounded once to float, as the reference's are.
 """
 
 from __future__ import annotations
@@ -31,6 +35,35 @@ import numpy as np
 
 SOUND_SPEED_0C = 331.3  # m/s in still air at 0 degrees C (air.py:23)
 KELVIN = 273.15
+
+
+@dataclasses.dataclass
+class WeatherZone:
+    """A part of the ground over which the weather differs (scene/model.py:3051-3067): the
+    ``polygon``'s (x, y) corners in order (m); each field given (not None) replaces the
+    atmosphere's: the wind's speed, direction, height and profile, the temperature's
+    gradient and logarithmic term, the roughness, a measured table, and the temperature at
+    the ground (C). Giving the gradient or the log term drops the atmosphere's table; giving
+    a table drops them (scene/model.py:3377-3413)."""
+
+    polygon: tuple
+    wind_speed_m_s: float | None = None
+    wind_direction_deg: float | None = None
+    wind_height_m: float | None = None
+    wind_profile: str | None = None
+    temperature_gradient_k_m: float | None = None
+    temperature_log_k: float | None = None
+    roughness_m: float | None = None
+    table: tuple | None = None
+    temperature_c: float | None = None
+    name: str | None = None
+
+    def __post_init__(self):
+        corners = np.asarray(self.polygon, float)
+        if corners.ndim != 2 or corners.shape[1] != 2 or len(corners) < 3:
+            raise ValueError("a weather zone's polygon is three (x, y) corners or more")
+        if self.table is not None and (self.temperature_gradient_k_m is not None or self.temperature_log_k is not None):
+            raise ValueError("give a weather zone a table or the temperature's gradient and log terms, not both")
 
 
 @dataclasses.dataclass
@@ -49,6 +82,7 @@ class Atmosphere:
     wind_height_m: float = 10.0
     wind_profile: str = "log"
     table: tuple | None = None
+    zones: tuple = ()
 
     def __post_init__(self):
         if self.wind_profile not in ("log", "linear", "constant"):
@@ -62,12 +96,44 @@ class Atmosphere:
 
     @property
     def still(self) -> bool:
-        """No wind and one temperature: sound goes straight, the grid needs no field."""
+        """No wind and one temperature: sound goes straight, the grid needs no field (with
+        zones: in every weather, scene/model.py:3471-3480)."""
+        if self.zones:
+            return all(w.still for w in self.weathers())
         if self.table is not None:
             rows = np.asarray(self.table, float)
             return bool(np.all(rows[:, 1] == rows[0, 1]) and np.all(rows[:, 2] == 0.0))
         return (self.wind_speed_m_s == 0.0 and self.temperature_gradient_k_m == 0.0
                 and self.temperature_log_k == 0.0)
+
+    def weathers(self) -> list:
+        """The atmosphere's own weather (first, without zones) and each zone's, resolved
+        (scene/model.py:3354-3413)."""
+        own = dataclasses.replace(self, zones=())
+        return [own, *(self.resolved(z) for z in self.zones)]
+
+    def resolved(self, zone: WeatherZone) -> "Atmosphere":
+        """The atmosphere with ``zone``'s weather given over it."""
+        values = {f.name: getattr(self, f.name) for f in dataclasses.fields(self)}
+        values["zones"] = ()
+        for mine, theirs in (("wind_speed_m_s", "wind_speed_m_s"), ("wind_direction_deg", "wind_direction_deg"),
+                             ("wind_height_m", "wind_height_m"), ("wind_profile", "wind_profile"),
+                             ("temperature_gradient_k_m", "temperature_gradient_k_m"),
+                             ("temperature_log_k", "temperature_log_k"), ("roughness_m", "roughness_m")):
+            if getattr(zone, theirs) is not None:
+                values[mine] = getattr(zone, theirs)
+        if zone.table is not None:
+            values["table"] = zone.table
+        elif zone.temperature_gradient_k_m is not None or zone.temperature_log_k is not None:
+            values["table"] = None
+        if values["table"] is not None:
+            values["temperature_gradient_k_m"] = values["temperature_log_k"] = 0.0
+        return Atmosphere(**values)
+
+    def ground_temperatures(self, ground_c: float) -> list:
+        """The temperature at the ground (C) under each weather, in :meth:`weathers` order."""
+        return [float(ground_c), *(float(ground_c) if z.temperature_c is None else float(z.temperature_c)
+                                   for z in self.zones)]
 
     def direction(self) -> np.ndarray:
         """The unit vector (2,) the wind blows along."""
@@ -102,26 +168,98 @@ class Atmosphere:
         return SOUND_SPEED_0C * np.sqrt(1.0 + self.temperature(height, ground_c) / KELVIN)
 
 
+def inside(points, polygon) -> np.ndarray:
+    """(n,) whether each of ``points`` (n, 2) lies inside ``polygon`` (m, 2), by the
+    even-odd rule: the ray from the point towards +x crosses an odd number of its edges. An
+    edge counts where exactly one of its ends lies strictly above the point's y and it
+    passes to the right of the point at that y."""
+    p = np.asarray(points, float).reshape(-1, 2)
+    a = np.asarray(polygon, float)
+    b = np.roll(a, -1, axis=0)
+    y = p[:, 1][:, None]
+    spans = (a[None, :, 1] > y) != (b[None, :, 1] > y)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x_at = a[None, :, 0] + (y - a[None, :, 1]) * (b[None, :, 0] - a[None, :, 0]) / (b[None, :, 1] - a[None, :, 1])
+    return np.count_nonzero(spans & (p[:, 0][:, None] < x_at), axis=1) % 2 == 1
+
+
+def clip(polygon, extent):
+    """The part of ``polygon`` inside the rectangle ``extent`` (x_min, y_min, x_max,
+    y_max), clipped against its four sides in turn (x_min, x_max, y_min, y_max); None where
+    nothing with an area is left."""
+    x0, y0, x1, y1 = (float(v) for v in extent)
+    pts = [tuple(float(c) for c in q) for q in np.asarray(polygon, float)]
+    for axis, bound, keep_above in ((0, x0, True), (0, x1, False), (1, y0, True), (1, y1, False)):
+        keep = [(q[axis] >= bound) if keep_above else (q[axis] <= bound) for q in pts]
+        out = []
+        for k, q in enumerate(pts):
+            prev, prev_keep = pts[k - 1], keep[k - 1]
+            if keep[k] != prev_keep:
+                t = (bound - prev[axis]) / (q[axis] - prev[axis])
+                cut = [prev[0] + t * (q[0] - prev[0]), prev[1] + t * (q[1] - prev[1])]
+                cut[axis] = bound
+                out.append(tuple(cut))
+            if keep[k]:
+                out.append(q)
+        pts = out
+        if not pts:
+            return None
+    c = np.array(pts)
+    tolerance = 1e-9 * max(x1 - x0, y1 - y0)
+    c = c[np.linalg.norm(c - np.roll(c, 1, axis=0), axis=1) > tolerance]
+    if len(c) < 3:
+        return None
+    area = 0.5 * float(np.sum(c[:, 0] * np.roll(c[:, 1], -1) - np.roll(c[:, 0], -1) * c[:, 1]))
+    if abs(area) <= tolerance * max(x1 - x0, y1 - y0):
+        return None
+    return c
+
+
 class Weather:
     """The effective speed of sound over a grid from a source (``_Weather``,
     engine/wave.py:457-627): the atmosphere over a ground at height ``ground`` (m) whose
-    temperature is ``ground_c``, seen from ``source`` (3,)."""
+    temperature is ``ground_c``, seen from ``source`` (3,); with zones, each over the part
+    of the ground its polygon covers, clipped to ``extent`` (x_min, y_min, x_max, y_max:
+    the ground's; None keeps the polygons whole)."""
 
-    def __init__(self, atmosphere: Atmosphere, ground_c: float, ground: float, source):
+    def __init__(self, atmosphere: Atmosphere, ground_c: float, ground: float, source, extent=None):
         self.atmosphere = atmosphere
         self.ground_c = float(ground_c)
         self.ground = float(ground)
         self.source = np.asarray(source, float).reshape(3)
+        self.weathers = atmosphere.weathers()
+        self.grounds = atmosphere.ground_temperatures(ground_c)
+        self.polygons = []
+        for zone in atmosphere.zones:
+            poly = np.asarray(zone.polygon, float)
+            self.polygons.append(poly if extent is None else clip(poly, extent))
 
-    def profiles(self, z) -> tuple[np.ndarray, np.ndarray]:
-        """The still air's speed and the wind's at heights z (m, the scene's coordinates)."""
+    @property
+    def zoned(self) -> bool:
+        return bool(self.polygons)
+
+    def zone_at(self, points) -> np.ndarray:
+        """(n,) the weather over each of ``points`` (n, 2 or 3): 0 the atmosphere's own,
+        k the k-th zone's (the first where zones overlap)."""
+        p = np.asarray(points, float)
+        p = p.reshape(-1, p.shape[-1])[:, :2]
+        out = np.zeros(len(p), np.int64)
+        for k, poly in enumerate(self.polygons, start=1):
+            if poly is not None:
+                out[(out == 0) & inside(p, poly)] = k
+        return out
+
+    def profiles(self, z, k: int = 0) -> tuple[np.ndarray, np.ndarray]:
+        """The still air's speed and the wind's at heights z (m, the scene's coordinates),
+        under weather k (0: the atmosphere's own)."""
         height = np.asarray(z, float) - self.ground
-        return self.atmosphere.sound_speed(height, self.ground_c), self.atmosphere.wind(height)
+        w = self.weathers[k]
+        return w.sound_speed(height, self.grounds[k]), w.wind(height)
 
-    def cosine(self, x, y) -> np.ndarray:
-        """The cosine between the wind and the horizontal from the source to (x, y); 0
-        straight above or below the source."""
-        wx, wy = self.atmosphere.direction()
+    def cosine(self, x, y, k: int = 0) -> np.ndarray:
+        """The cosine between weather k's wind and the horizontal from the source to
+        (x, y); 0 straight above or below the source."""
+        wx, wy = self.weathers[k].direction()
         dx, dy = np.asarray(x, float) - self.source[0], np.asarray(y, float) - self.source[1]
         reach = np.hypot(dx, dy)
         along = wx * dx + wy * dy
@@ -130,30 +268,61 @@ class Weather:
     def at(self, points) -> np.ndarray:
         """The effective speed of sound (m/s) at ``points`` (n, 3)."""
         p = np.asarray(points, float).reshape(-1, 3)
-        still, wind = self.profiles(p[:, 2])
-        return still + wind * self.cosine(p[:, 0], p[:, 1])
+        zone = self.zone_at(p)
+        out = np.zeros(len(p))
+        for k in np.unique(zone).tolist():
+            rows = zone == k
+            still, wind = self.profiles(p[rows, 2], k)
+            out[rows] = still + wind * self.cosine(p[rows, 0], p[rows, 1], k)
+        return out
 
-    def extremes(self, origin, cell: float, shape) -> tuple[float, float]:
-        """The slowest and fastest effective speed over the cells of a grid."""
-        origin = np.asarray(origin, float)
-        centres = [origin[a] + (np.arange(int(shape[a])) + 0.5) * cell for a in range(3)]
-        still, wind = self.profiles(centres[2])
-        cos = self.cosine(centres[0][None, :], centres[1][:, None])
-        ends = wind[:, None] * np.array([cos.min(), cos.max()])[None, :]
-        return float((still + ends.min(axis=1)).min()), float((still + ends.max(axis=1)).max())
-
-    def ratios(self, origin, cell: float, shape, speed: float) -> np.ndarray:
-        """(nx ny nz,) float32, x fastest: each cell's effective speed of sound over
-        ``speed`` (the grid's: at least the fastest), at most 1 (``_Weather.ratios``). The
-        cosines of a plane are formed once; each plane's speeds in double, then rounded."""
+    def _grid(self, origin, cell: float, shape):
+        """Each weather's profiles over the planes (K, nz) and cosines over a plane
+        (K, ny, nx), and the weather over each column (ny, nx)."""
         origin = np.asarray(origin, float)
         nx, ny, nz = (int(v) for v in shape)
         xs = origin[0] + (np.arange(nx) + 0.5) * cell
         ys = origin[1] + (np.arange(ny) + 0.5) * cell
         zs = origin[2] + (np.arange(nz) + 0.5) * cell
-        still, wind = self.profiles(zs)
-        cos = self.cosine(xs[None, :], ys[:, None])  # (ny, nx)
+        K = len(self.weathers) if self.zoned else 1
+        still = np.empty((K, nz))
+        wind = np.empty((K, nz))
+        cos = np.empty((K, ny, nx))
+        for k in range(K):
+            still[k], wind[k] = self.profiles(zs, k)
+            cos[k] = self.cosine(xs[None, :], ys[:, None], k)
+        if not self.zoned:
+            return still, wind, cos, None
+        x, y = np.meshgrid(xs, ys)
+        zone = self.zone_at(np.column_stack([x.ravel(), y.ravel()])).reshape(ny, nx)
+        return still, wind, cos, zone
+
+    def extremes(self, origin, cell: float, shape) -> tuple[float, float]:
+        """The slowest and fastest effective speed over the cells of a grid (of the
+        weathers over it)."""
+        still, wind, cos, zone = self._grid(origin, cell, shape)
+        low, high = np.inf, -np.inf
+        for k in ([0] if zone is None else np.unique(zone).tolist()):
+            c = cos[k] if zone is None else cos[k][zone == k]
+            ends = wind[k][:, None] * np.array([c.min(), c.max()])[None, :]
+            low = min(low, float((still[k] + ends.min(axis=1)).min()))
+            high = max(high, float((still[k] + ends.max(axis=1)).max()))
+        return low, high
+
+    def ratios(self, origin, cell: float, shape, speed: float) -> np.ndarray:
+        """(nx ny nz,) float32, x fastest: each cell's effective speed of sound over
+        ``speed`` (the grid's: at least the fastest), at most 1 (``_Weather.ratios``). The
+        cosines of a plane are formed once; each plane's speeds in double, then rounded."""
+        still, wind, cos, zone = self._grid(origin, cell, shape)
+        nz = still.shape[1]
+        ny, nx = cos.shape[1:]
         out = np.empty((nz, ny, nx), np.float32)
+        if zone is None:
+            for z in range(nz):
+                out[z] = np.minimum((still[0, z] + wind[0, z] * cos[0]) / speed, 1.0)
+            return out.reshape(-1)
+        iy, ix = np.indices(zone.shape)
+        column_cos = cos[zone, iy, ix]  # each column's cosine to its own weather's wind
         for z in range(nz):
-            out[z] = np.minimum((still[z] + wind[z] * cos) / speed, 1.0)
+            out[z] = np.minimum((still[zone, z] + wind[zone, z] * column_cos) / speed, 1.0)
         return out.reshape(-1)

@@ -188,3 +188,27 @@ def test_entry_point_outdoors_under_wind(monkeypatch):
     assert info["wave_open"] == 1.0 and info["wave_stencil"] == 7.0
     assert info["wave_speed_min_m_s"] < info["wave_speed_max_m_s"]
     assert np.all(np.isfinite(responses[0].signals))
+
+
+@pytest.mark.gpu
+def test_entry_point_outdoors_under_zoned_weather(monkeypatch):
+    """The same ground under an atmosphere with a zone of its own over the listener's half
+    (the wind dropped and turned, a warmer ground), its polygon clipped to the ground's
+    extent: each cell takes its column's weather; the run equal to the oracle's."""
+    from types import SimpleNamespace
+
+    from acoustic_spaces_amd import gpu
+    from acoustic_spaces_amd.weather import Atmosphere, WeatherZone
+
+    monkeypatch.setattr(gpu, "HipWaveSolver", _Checked)
+    ground = scenes.box_mesh((12.0, 8.0, 1.0), low=(0.0, 0.0, -1.0), inward=False)
+    rows = np.full((ground.materials, 8), 0.2)
+    lee = WeatherZone(polygon=((6.5, -2.0), (20.0, -2.0), (20.0, 10.0), (6.5, 10.0)), wind_speed_m_s=1.5,
+                      wind_direction_deg=120.0, temperature_c=26.0)
+    atmosphere = Atmosphere(wind_speed_m_s=6.0, temperature_gradient_k_m=-0.03, zones=(lee,))
+    model = scenes.trace_model(ground, rows, outdoor=SimpleNamespace(extent=(0.0, 0.0, 12.0, 8.0)), ground=0.0,
+                               atmosphere=atmosphere)
+    responses, info = wv.wave_responses(model, [3.0, 4.0, 1.0], [[9.0, 4.2, 1.5]], ["far"], max_band_hz=63,
+                                        length_s=0.1, sample_rate=8000.0, speed_of_sound=343.0)
+    assert info["wave_speed_min_m_s"] < info["wave_speed_max_m_s"]
+    assert np.all(np.isfinite(responses[0].signals)) and np.abs(responses[0].signals).max() > 0

@@ -2,8 +2,10 @@
 457-627, over its Atmosphere, scene/model.py:3270-3480). CPU: the profiles and the field
 against closed forms that the formulas fix (the still air's speed from the temperature, the
 wind at its reference height, the cosine to the wind downwind, upwind, across and straight
-above the source), the layout of a windy scene against the reference's set-up given the same
-field. GPU: the scaled kernels stepping that field, bit for bit against the oracle."""
+above the source), zones resolved over the atmosphere, their polygons by the even-odd rule
+and clipped to the ground, each column of a zoned grid equal to its own weather's field; the
+layouts of windy and zoned scenes against the reference's set-up given the same field. GPU:
+the scaled kernels stepping those fields, bit for bit against the oracle."""
 
 import math
 
@@ -12,7 +14,7 @@ import pytest
 
 import cases
 from acoustic_spaces_amd import scenes
-from acoustic_spaces_amd.weather import Atmosphere, Weather
+from acoustic_spaces_amd.weather import Atmosphere, Weather, WeatherZone, clip, inside
 
 
 def test_profiles_closed_forms():
@@ -90,5 +92,111 @@ def test_windy_open_block_steps_equal_the_oracle():
     drive = sc.drive[:300]
     with HipWaveSolver(layout) as gpu:
         a = np.concatenate([gpu.run(drive[:131]), gpu.run(drive[131:])], axis=1)
+    b = OracleSolver(layout).run(drive)
+    assert np.array_equal(a, b) and np.any(a != 0)
+
+
+SQUARE = ((0.0, 0.0), (2.0, 0.0), (2.0, 2.0), (0.0, 2.0))
+
+
+def test_a_zone_is_given_over_the_atmosphere():
+    """What a zone gives replaces the atmosphere's, what it leaves out stays; a table given
+    drops the gradient and log terms, a gradient or log term given drops the table
+    (scene/model.py:3377-3413)."""
+    base = Atmosphere(wind_speed_m_s=6.0, wind_direction_deg=30.0, temperature_gradient_k_m=0.01, roughness_m=0.1)
+    r = base.resolved(WeatherZone(polygon=SQUARE, wind_speed_m_s=2.0))
+    assert (r.wind_speed_m_s, r.wind_direction_deg, r.temperature_gradient_k_m, r.roughness_m) == (2.0, 30.0, 0.01, 0.1)
+    table = ((0.0, 10.0, 0.0), (20.0, 14.0, 8.0))
+    r = base.resolved(WeatherZone(polygon=SQUARE, table=table))
+    assert r.table == table and r.temperature_gradient_k_m == 0.0 and r.temperature_log_k == 0.0
+    r = Atmosphere(table=table).resolved(WeatherZone(polygon=SQUARE, temperature_gradient_k_m=-0.01))
+    assert r.table is None and r.temperature_gradient_k_m == -0.01
+    with pytest.raises(ValueError, match="not both"):
+        WeatherZone(polygon=SQUARE, table=table, temperature_log_k=1.0)
+    with pytest.raises(ValueError, match="three"):
+        WeatherZone(polygon=((0.0, 0.0), (1.0, 0.0)))
+    zoned = Atmosphere(zones=(WeatherZone(polygon=SQUARE, temperature_c=30.0), WeatherZone(polygon=SQUARE)))
+    assert zoned.ground_temperatures(15.0) == [15.0, 30.0, 15.0]
+    # Still where every weather is (a zone's own ground temperature alone leaves it so, as
+    # in the reference's Atmosphere.still); a zone's wind is not.
+    assert zoned.still
+    assert not Atmosphere(zones=(WeatherZone(polygon=SQUARE, wind_speed_m_s=3.0),)).still
+
+
+def test_polygons_hold_by_the_even_odd_rule_and_clip_to_the_extent():
+    assert inside([[1.0, 1.0], [3.0, 1.0], [-0.5, 1.0], [1.0, 2.5]], SQUARE).tolist() == [True, False, False, False]
+    ell = ((0.0, 0.0), (2.0, 0.0), (2.0, 1.0), (1.0, 1.0), (1.0, 2.0), (0.0, 2.0))
+    assert inside([[0.5, 1.5], [1.5, 1.5], [1.5, 0.5]], ell).tolist() == [True, False, True]
+    c = clip(((-1.0, -1.0), (1.0, -1.0), (1.0, 1.0), (-1.0, 1.0)), (0.0, 0.0, 5.0, 5.0))
+    area = 0.5 * abs(np.sum(c[:, 0] * np.roll(c[:, 1], -1) - np.roll(c[:, 0], -1) * c[:, 1]))
+    assert abs(area - 1.0) < 1e-12 and c.min() == 0.0 and c.max() == 1.0
+    assert clip(((10.0, 10.0), (11.0, 10.0), (11.0, 11.0)), (0.0, 0.0, 5.0, 5.0)) is None
+
+
+def _zoned():
+    own = dict(wind_speed_m_s=6.0, temperature_gradient_k_m=-0.01)
+    a = WeatherZone(polygon=((0.1, 0.1), (0.8, 0.1), (0.8, 0.7), (0.1, 0.7)), wind_speed_m_s=1.0,
+                    wind_direction_deg=90.0)
+    b = WeatherZone(polygon=((0.5, 0.5), (1.3, 0.5), (1.3, 1.2), (0.5, 1.2)), temperature_c=30.0,
+                    temperature_gradient_k_m=0.02)
+    return Atmosphere(**own), Atmosphere(**own, zones=(a, b)), a, b
+
+
+def test_zoned_ratios_take_each_columns_weather():
+    """Each column of cells takes the ratios of the weather over it, the first zone where
+    zones overlap, the atmosphere's own outside them; the grid's extremes span them."""
+    own, zoned, a, b = _zoned()
+    src, origin, cell, shape = [0.2, 0.3, 0.5], np.array([-0.05, -0.05, -0.05]), 0.1, (15, 14, 9)
+    w = Weather(zoned, 20.0, 0.0, src, extent=(0.0, 0.0, 1.4, 1.3))
+    low, high = w.extremes(origin, cell, shape)
+    r = w.ratios(origin, cell, shape, high).reshape(9, 14, 15)
+    parts = [Weather(own, 20.0, 0.0, src), Weather(zoned.resolved(a), 20.0, 0.0, src),
+             Weather(zoned.resolved(b), 30.0, 0.0, src)]
+    each = [p.ratios(origin, cell, shape, high).reshape(9, 14, 15) for p in parts]
+    xs = origin[0] + (np.arange(15) + 0.5) * cell
+    ys = origin[1] + (np.arange(14) + 0.5) * cell
+    x, y = np.meshgrid(xs, ys)
+    zone = w.zone_at(np.column_stack([x.ravel(), y.ravel()])).reshape(14, 15)
+    assert set(np.unique(zone).tolist()) == {0, 1, 2}
+    assert w.zone_at([[0.6, 0.6]])[0] == 1  # in both: the first holds
+    expect = np.where(zone[None] == 1, each[1], np.where(zone[None] == 2, each[2], each[0]))
+    assert np.array_equal(r, expect)
+    assert r.max() == np.float32(1.0)
+    speeds = [p.extremes(origin, cell, shape) for p in parts]
+    assert low >= min(s[0] for s in speeds) and high <= max(s[1] for s in speeds)
+    points = np.column_stack([x.ravel(), y.ravel(), np.full(x.size, 0.35)])
+    at = w.at(points)
+    for k, p in enumerate(parts):
+        rows = zone.ravel() == k
+        assert np.array_equal(at[rows], p.at(points[rows]))
+
+
+def test_zoned_layout_equals_the_reference(reference_available):
+    from acoustic_spaces_amd.solver import WaveSolver
+    from oracle.reference import ReferenceSolver
+
+    _, zoned, _, _ = _zoned()
+    sc = scenes.with_weather(cases.open_block(), zoned)
+    A = ReferenceSolver.from_scene(sc, threads=4).layout()
+    B = WaveSolver.from_scene(sc, threads=4).layout()
+    for k in A:
+        assert A[k].dtype == B[k].dtype and np.array_equal(A[k], B[k]), k
+
+
+@pytest.mark.gpu
+def test_zoned_open_block_steps_equal_the_oracle():
+    """The open block under zoned weather (a calm zone with the wind turned across, and a
+    warm zone with an inversion, overlapping): the stepper against the oracle, bit for bit."""
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from acoustic_spaces_amd.solver import WaveSolver
+    from oracle.stepper import OracleSolver
+
+    _, zoned, _, _ = _zoned()
+    sc = scenes.with_weather(cases.open_block(), zoned)
+    layout = WaveSolver.from_scene(sc, on_device=True).layout()
+    assert int(layout["scaled"][0]) == 1
+    drive = sc.drive[:300]
+    with HipWaveSolver(layout) as gpu:
+        a = gpu.run(drive)
     b = OracleSolver(layout).run(drive)
     assert np.array_equal(a, b) and np.any(a != 0)
