@@ -22,7 +22,7 @@ in plan, clipped to the ground's extent) with weather of their own, each given o
 atmosphere's (what a zone gives replaces the atmosphere's, what it leaves out stays). A cell
 takes the weather of the first zone whose polygon holds its centre's (x, y) (even-odd rule,
 edges half-open along y), else the atmosphere's own; the cosine is to that weather's wind.
-The zones' named presets are not here. The speeds are formed plane by plane in double and
+A zone may start from a named preset weather. The speeds are formed plane by plane in double and
 rounded once to float, as the reference's are.
 """
 
@@ -36,6 +36,16 @@ import numpy as np
 SOUND_SPEED_0C = 331.3  # m/s in still air at 0 degrees C (air.py:23)
 KELVIN = 273.15
 
+# The weathers a zone can start from whole (scene/model.py:2910-2922): a moderate wind, 5 m/s
+# at its height; a sunny day's lapse (the air cooling with height); a clear night's
+# inversion; still air.
+PRESETS = {
+    "neutral": {},
+    "windy": {"wind_speed_m_s": 5.0},
+    "sunny_day": {"temperature_log_k": -0.5, "temperature_gradient_k_m": -0.0098},
+    "clear_night": {"temperature_log_k": 0.5, "temperature_gradient_k_m": 0.0},
+}
+
 
 @dataclasses.dataclass
 class WeatherZone:
@@ -44,7 +54,8 @@ class WeatherZone:
     atmosphere's: the wind's speed, direction, height and profile, the temperature's
     gradient and logarithmic term, the roughness, a measured table, and the temperature at
     the ground (C). Giving the gradient or the log term drops the atmosphere's table; giving
-    a table drops them (scene/model.py:3377-3413)."""
+    a table drops them (scene/model.py:3377-3413). A ``preset`` (``PRESETS``) gives that
+    weather whole first, its wind blowing the atmosphere's way at its height."""
 
     polygon: tuple
     wind_speed_m_s: float | None = None
@@ -57,8 +68,11 @@ class WeatherZone:
     table: tuple | None = None
     temperature_c: float | None = None
     name: str | None = None
+    preset: str | None = None
 
     def __post_init__(self):
+        if self.preset is not None and self.preset not in PRESETS:
+            raise ValueError(f"no atmosphere preset named {self.preset!r} (the presets are {', '.join(PRESETS)})")
         corners = np.asarray(self.polygon, float)
         if corners.ndim != 2 or corners.shape[1] != 2 or len(corners) < 3:
             raise ValueError("a weather zone's polygon is three (x, y) corners or more")
@@ -116,12 +130,21 @@ class Atmosphere:
         """The atmosphere with ``zone``'s weather given over it."""
         values = {f.name: getattr(self, f.name) for f in dataclasses.fields(self)}
         values["zones"] = ()
-        for mine, theirs in (("wind_speed_m_s", "wind_speed_m_s"), ("wind_direction_deg", "wind_direction_deg"),
-                             ("wind_height_m", "wind_height_m"), ("wind_profile", "wind_profile"),
-                             ("temperature_gradient_k_m", "temperature_gradient_k_m"),
-                             ("temperature_log_k", "temperature_log_k"), ("roughness_m", "roughness_m")):
-            if getattr(zone, theirs) is not None:
-                values[mine] = getattr(zone, theirs)
+        if zone.preset is not None:
+            # The preset's weather whole: no wind unless it has one (then blowing the
+            # atmosphere's way, at its height, by its profile), no gradient terms, no table.
+            preset = PRESETS[zone.preset]
+            values.update(wind_speed_m_s=0.0, temperature_gradient_k_m=0.0, temperature_log_k=0.0, table=None)
+            values.update(preset)
+            if "wind_speed_m_s" not in preset and any(
+                    getattr(zone, f) is not None for f in ("wind_speed_m_s", "wind_direction_deg", "wind_height_m",
+                                                           "wind_profile")):
+                # A zone's wind over a preset without one starts from the atmosphere's wind.
+                values["wind_speed_m_s"] = self.wind_speed_m_s
+        for name in ("wind_speed_m_s", "wind_direction_deg", "wind_height_m", "wind_profile",
+                     "temperature_gradient_k_m", "temperature_log_k", "roughness_m"):
+            if getattr(zone, name) is not None:
+                values[name] = getattr(zone, name)
         if zone.table is not None:
             values["table"] = zone.table
         elif zone.temperature_gradient_k_m is not None or zone.temperature_log_k is not None:

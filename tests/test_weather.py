@@ -115,6 +115,15 @@ def test_a_zone_is_given_over_the_atmosphere():
         WeatherZone(polygon=SQUARE, table=table, temperature_log_k=1.0)
     with pytest.raises(ValueError, match="three"):
         WeatherZone(polygon=((0.0, 0.0), (1.0, 0.0)))
+    # Presets give their weather whole, the wind the atmosphere's way at its height.
+    r = base.resolved(WeatherZone(polygon=SQUARE, preset="windy"))
+    assert (r.wind_speed_m_s, r.wind_direction_deg, r.temperature_gradient_k_m, r.temperature_log_k) == (5.0, 30.0, 0.0, 0.0)
+    r = base.resolved(WeatherZone(polygon=SQUARE, preset="sunny_day"))
+    assert (r.wind_speed_m_s, r.temperature_gradient_k_m, r.temperature_log_k) == (0.0, -0.0098, -0.5)
+    r = base.resolved(WeatherZone(polygon=SQUARE, preset="clear_night", wind_direction_deg=200.0))
+    assert (r.wind_speed_m_s, r.wind_direction_deg, r.temperature_log_k) == (6.0, 200.0, 0.5)
+    with pytest.raises(ValueError, match="no atmosphere preset"):
+        WeatherZone(polygon=SQUARE, preset="stormy")
     zoned = Atmosphere(zones=(WeatherZone(polygon=SQUARE, temperature_c=30.0), WeatherZone(polygon=SQUARE)))
     assert zoned.ground_temperatures(15.0) == [15.0, 30.0, 15.0]
     # Still where every weather is (a zone's own ground temperature alone leaves it so, as
