@@ -1473,6 +1473,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (L.nx < 3 || L.ny < 3 || L.nz < 3) refuse("the grid is too small");
     if (L.n_weights < (L.stencil == 7 ? 6u : 18u)) refuse("the layout's weights are too few for its stencil");
     if (L.types == nullptr) refuse("the layout has no cell types");
+    if (L.flow_mask != nullptr) refuse("moving air: this build of the stepper takes still air only");
     // The probes' ranges: from 0, never decreasing, ending at the taps (a range running
     // backwards would make a probe's wavefront loop near 2^32 times; one past the taps would
     // read past the cells).

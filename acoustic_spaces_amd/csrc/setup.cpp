@@ -402,6 +402,15 @@ public:
     double guard = 0;  // kGuard: distance from a plane within which an end counts as on it
     bool on_device = false;  // the cut links and the flood on the current GPU
     size_t flood_rounds = 0;  // the device flood's rounds
+    // Moving air (see set_up_flow): the shifts a step along x and y and each cell's row
+    // mask (plain), or the wind on each cell's +x, +y, +z faces and the loops (round); the
+    // mask then 1 where the air moves the cell, 2 at the border of the moving air.
+    std::vector<float> flow_x, flow_y;
+    std::vector<uint8_t> flow_mask;
+    std::array<std::vector<float>, 3> flow_wind, flow_loop;
+    bool flow_round = false;
+    float flow_fastest = 0.0f;
+    size_t flow_sweeps = 0;
 
     Stencil st;
     std::array<float, 18> weights{};  // the stencil's lowered weights, as the layout hands them out
@@ -448,6 +457,11 @@ public:
         return x == 0 || y == 0 || z == 0 || x + 1 == nx || y + 1 == ny || z + 1 == nz;
     }
     double cell_scale(size_t i) const { return scale.empty() ? 1.0 : static_cast<double>(scale[i]); }
+    // Cells from i to the grid's edge, the nearest way (0: on the edge).
+    size_t depth(size_t i) const {
+        const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
+        return std::min({x, y, z, nx - 1 - x, ny - 1 - y, nz - 1 - z});
+    }
     // Whether a face cuts the link from i along k (k < st.count): the bit of the link's end
     // whose direction owns it.
     bool is_cut(size_t i, int k) const {
@@ -1203,6 +1217,220 @@ public:
         }
     }
 
+    // --- moving air ---------------------------------------------------------------------
+
+    bool moving(size_t i) const { return type[i] == kInterior || type[i] == kSpecial; }
+
+    // Moving air (set_up_flow, fdtd.hpp:1680-1737): each moving cell's shifts a step along x
+    // and y (under half a cell), and which of its four neighbours in the plane are air across
+    // open links. Plain where along each axis the air moves a cell its two neighbours are air
+    // moving as it does (or not moving at all): then each cell's mask says, per axis (4 bits
+    // each, x low), which of its row's cells one and two away move with it. Otherwise the
+    // wind round the solids (set_up_round).
+    void set_up_flow(const float *fx, const float *fy) {
+        flow_x.assign(n, 0.0f);
+        flow_y.assign(n, 0.0f);
+        flow_mask.assign(n, 0);
+        std::vector<uint8_t> near(n, 0);  // bit k: the neighbour along k (-x, +x, -y, +y)
+        for (size_t i = 0; i < n; ++i) {
+            if (!moving(i)) continue;
+            if (!(std::isfinite(fx[i]) && std::isfinite(fy[i]) && std::abs(fx[i]) < 0.5f && std::abs(fy[i]) < 0.5f))
+                throw std::invalid_argument("the air must move less than half a cell per step");
+            flow_x[i] = fx[i];
+            flow_y[i] = fy[i];
+            flow_fastest = std::max({flow_fastest, std::abs(fx[i]), std::abs(fy[i])});
+            const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
+            for (int k = 0; k < 4; ++k)
+                if (!blocked_at(i, k, x, y, z) && type[neighbour(i, k)] != kSolid) near[i] |= static_cast<uint8_t>(1u << k);
+        }
+        const auto moves = [&](size_t from, int k) { return ((near[from] >> k) & 1u) && moving(neighbour(from, k)); };
+        bool plain = true;
+        for (size_t i = 0; i < n && plain; ++i) {
+            if (!moving(i)) continue;
+            for (int axis = 0; axis < 2 && plain; ++axis) {
+                const std::vector<float> &f = axis == 0 ? flow_x : flow_y;
+                if (f[i] == 0.0f) continue;
+                for (int k = 2 * axis; k < 2 * axis + 2; ++k) {
+                    const size_t j = neighbour(i, k);
+                    plain = plain && ((near[i] >> k) & 1u) && (!moving(j) || f[j] == f[i]);
+                }
+            }
+        }
+        if (!plain) {
+            set_up_round();
+            return;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            if (!moving(i)) continue;
+            unsigned m = 0;
+            for (int axis = 0; axis < 2; ++axis) {
+                const int lo = 2 * axis, hi = lo + 1;
+                const unsigned at = 4u * static_cast<unsigned>(axis);
+                const bool below = moves(i, lo), above = moves(i, hi);
+                if (below) m |= 1u << at;
+                if (above) m |= 2u << at;
+                if (below && moves(neighbour(i, lo), lo)) m |= 4u << at;
+                if (above && moves(neighbour(i, hi), hi)) m |= 8u << at;
+            }
+            flow_mask[i] = static_cast<uint8_t>(m);
+        }
+    }
+
+    // The wind round the solids (set_up_round, fdtd.hpp:1739-1935): the given wind made free
+    // of sources by a potential lambda solved on the faces open to the wind,
+    //   sum over a cell's open faces of k (lambda - lambda') = -(the given wind's net flux),
+    // k a face's weight (the two cells' mean given speed, or the moving one's beside still
+    // air), by conjugate gradients scaled by the diagonal, in double, to a net flux of 1e-9 of
+    // the fastest given speed. Every sum runs plane by plane along z, each plane in index
+    // order, then over the planes in order. Then the faces' winds, the loops (a twelfth of
+    // each three moving cells in a row, halved), and which cells the air moves.
+    void set_up_round() {
+        const size_t stride[3] = {1, sy, sz};
+        const size_t least = layer > 0 ? 1 : 2;
+        std::vector<float> speed(n, 0.0f);
+        std::vector<uint8_t> faces(n, 0);  // bit d: the face towards +d is open to the wind
+        const std::vector<float> *given[2] = {&flow_x, &flow_y};
+        for (size_t i = 0; i < n; ++i) {
+            if (type[i] == kSolid || depth(i) < least) continue;
+            speed[i] = moving(i) ? std::hypot(flow_x[i], flow_y[i]) : 0.0f;
+            const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
+            for (int d = 0; d < 3; ++d) {
+                const size_t j = i + stride[d];
+                if (type[j] == kSolid || depth(j) < least || blocked_at(i, 2 * d + 1, x, y, z)) continue;
+                if (moving(i) || moving(j)) faces[i] |= static_cast<uint8_t>(1u << d);
+            }
+        }
+        const auto mean = [&](const std::vector<float> &f, size_t i, size_t j) {
+            const double sum = static_cast<double>(f[i]) + static_cast<double>(f[j]);
+            return moving(i) && moving(j) ? 0.5 * sum : sum;
+        };
+        const auto through = [&](size_t i, int d) { return d < 2 ? mean(*given[d], i, i + stride[d]) : 0.0; };
+        std::vector<double> lambda(n, 0.0), residual(n, 0.0), direction(n, 0.0), image(n, 0.0);
+        std::vector<float> diagonal(n, 0.0f);
+        // term(i) summed (or its largest) over the moving cells, plane by plane.
+        const auto over = [&](bool largest, auto &&term) {
+            double total = 0.0;
+            for (size_t z = 0; z < nz; ++z) {
+                double plane = 0.0;
+                for (size_t i = z * sz; i < (z + 1) * sz; ++i) {
+                    if (!moving(i)) continue;
+                    const double v = term(i);
+                    plane = largest ? std::max(plane, v) : plane + v;
+                }
+                total = largest ? std::max(total, plane) : total + plane;
+            }
+            return total;
+        };
+        const double top = over(true, [&](size_t i) {
+            double flux = 0.0, weights = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const size_t s = stride[d];
+                if ((faces[i] >> d) & 1u) {
+                    flux += through(i, d);
+                    weights += mean(speed, i, i + s);
+                }
+                if ((faces[i - s] >> d) & 1u) {
+                    flux -= through(i - s, d);
+                    weights += mean(speed, i - s, i);
+                }
+            }
+            residual[i] = weights > 0.0 ? -flux : 0.0;
+            diagonal[i] = static_cast<float>(weights);
+            return static_cast<double>(speed[i]);
+        });
+        const auto scaled = [&](size_t i) {
+            return diagonal[i] > 0.0f ? residual[i] / static_cast<double>(diagonal[i]) : 0.0;
+        };
+        double carried = over(false, [&](size_t i) {
+            direction[i] = scaled(i);
+            return residual[i] * direction[i];
+        });
+        const double enough = 1e-9 * top;
+        double left = over(true, [&](size_t i) { return std::abs(residual[i]); });
+        const size_t most = 1000 + 50 * (nx + ny + nz);
+        while (left > enough) {
+            if (++flow_sweeps > most) throw std::runtime_error("the wind round the model's solids did not settle");
+            const double bent = over(false, [&](size_t i) {
+                double sum = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    const size_t s = stride[d];
+                    if ((faces[i] >> d) & 1u) sum += mean(speed, i, i + s) * (direction[i] - direction[i + s]);
+                    if ((faces[i - s] >> d) & 1u) sum += mean(speed, i - s, i) * (direction[i] - direction[i - s]);
+                }
+                image[i] = sum;
+                return direction[i] * sum;
+            });
+            if (!(bent > 0.0)) break;
+            const double step = carried / bent;
+            left = over(true, [&](size_t i) {
+                lambda[i] += step * direction[i];
+                residual[i] -= step * image[i];
+                return std::abs(residual[i]);
+            });
+            const double next = over(false, [&](size_t i) { return residual[i] * scaled(i); });
+            const double turn = next / carried;
+            carried = next;
+            over(false, [&](size_t i) {
+                direction[i] = scaled(i) + turn * direction[i];
+                return 0.0;
+            });
+        }
+        // The faces' winds and the fastest of them.
+        for (int d = 0; d < 3; ++d) flow_wind[static_cast<size_t>(d)].assign(n, 0.0f);
+        double fastest = 0.0;
+        for (size_t z = 0; z < nz; ++z) {
+            double plane = 0.0;
+            for (size_t i = z * sz; i < (z + 1) * sz; ++i) {
+                for (int d = 0; d < 3; ++d) {
+                    if (!((faces[i] >> d) & 1u)) continue;
+                    const size_t j = i + stride[d];
+                    const double u = through(i, d) - mean(speed, i, j) * (lambda[j] - lambda[i]);
+                    flow_wind[static_cast<size_t>(d)][i] = static_cast<float>(u);
+                    plane = std::max(plane, std::abs(u));
+                }
+            }
+            fastest = std::max(fastest, plane);
+        }
+        flow_fastest = static_cast<float>(fastest);
+        // A wind nowhere a millionth of the given one (a closed room's) is none.
+        if (static_cast<double>(flow_fastest) <= 1e-6 * top) {
+            flow_fastest = 0.0f;
+            for (auto &w : flow_wind) std::fill(w.begin(), w.end(), 0.0f);
+        }
+        if (!(flow_fastest < 0.5f)) {
+            throw std::invalid_argument("the air must move less than half a cell per step, and the wind round the "
+                                        "model's solids moves " + std::to_string(flow_fastest) + " at its fastest");
+        }
+        flow_x.clear();
+        flow_y.clear();
+        flow_x.shrink_to_fit();
+        flow_y.shrink_to_fit();
+        for (int d = 0; d < 3; ++d) {
+            std::vector<float> &loop = flow_loop[static_cast<size_t>(d)];
+            const std::vector<float> &w = flow_wind[static_cast<size_t>(d)];
+            loop.assign(n, 0.0f);
+            const size_t s = stride[d];
+            for (size_t i = 0; i < n; ++i) {
+                if (!((faces[i] >> d) & 1u) || !((faces[i + s] >> d) & 1u)) continue;
+                if (!(moving(i) && moving(i + s) && moving(i + 2 * s))) continue;
+                loop[i] = (w[i] + w[i + s]) * (1.0f / 24.0f);
+            }
+        }
+        for (size_t i = 0; i < n; ++i) {
+            if (!moving(i) || depth(i) < 2) continue;
+            bool moved = false, border = false;
+            for (int d = 0; d < 3; ++d) {
+                const size_t s = stride[d];
+                const std::vector<float> &w = flow_wind[static_cast<size_t>(d)], &l = flow_loop[static_cast<size_t>(d)];
+                moved = moved || w[i] != 0.0f || w[i - s] != 0.0f || l[i] != 0.0f || l[i - s] != 0.0f ||
+                        l[i - 2 * s] != 0.0f;
+                border = border || (w[i] != 0.0f && !moving(i + s)) || (w[i - s] != 0.0f && !moving(i - s));
+            }
+            flow_mask[i] = moved ? (border ? 2 : 1) : 0;
+        }
+        flow_round = true;
+    }
+
     static constexpr const char *kNoSight = "no cell centre of the wave grid is in sight of the source";
 
     // The passes over the whole grid on the current GPU (setup_device.hip): the cut links,
@@ -1334,8 +1562,10 @@ public:
     }
 
     void build(const double *mrq, const int64_t *branch_material, size_t branch_count, const V3 &src,
-               const std::vector<V3> &probes, size_t per_listener, const float *speed) {
+               const std::vector<V3> &probes, size_t per_listener, const float *speed, const float *fx,
+               const float *fy) {
         check_arguments(probes.size(), per_listener);
+        if ((fx == nullptr) != (fy == nullptr)) throw std::invalid_argument("the air's motion needs both of its components");
         branches = make_branches(mrq, branch_material, branch_count, materials);
         st = make_stencil(points, static_cast<ptrdiff_t>(sy), static_cast<ptrdiff_t>(sz));
         for (size_t k = 0; k < 18; ++k) weights[k] = st.d[k].weight;
@@ -1361,6 +1591,8 @@ public:
             place_source(src);
             place_probes(probes, src, per_listener);
             clock.mark("source, probes");
+            if (fx != nullptr) set_up_flow(fx, fy);
+            clock.mark("moving air");
             return;
         } else {
             cut_links();
@@ -1386,6 +1618,8 @@ public:
         place_source(src);
         place_probes(probes, src, per_listener);
         clock.mark("source, probes");
+        if (fx != nullptr) set_up_flow(fx, fy);
+        clock.mark("moving air");
     }
 };
 
@@ -1415,7 +1649,8 @@ as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *f
                              const double *origin, double cell, size_t nx, size_t ny, size_t nz,
                              const double *source, const double *probes, size_t nprobes, double edge_admittance,
                              size_t layer, double layer_damping, size_t stencil, const float *speed,
-                             size_t probes_per_listener, int threads, unsigned flags) {
+                             const float *flow_x, const float *flow_y, size_t probes_per_listener, int threads,
+                             unsigned flags) {
     g_setup_error.clear();
     as_setup *h = new (std::nothrow) as_setup();
     if (h == nullptr) {
@@ -1472,7 +1707,7 @@ as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *f
         std::vector<V3> points;
         for (size_t p = 0; p < nprobes; ++p) points.push_back(V3{probes[3 * p], probes[3 * p + 1], probes[3 * p + 2]});
         s.build(branches, branch_material, nbranch, V3{source[0], source[1], source[2]}, points, probes_per_listener,
-                speed);
+                speed, flow_x, flow_y);
         return h;
     } catch (const std::exception &e) {
         g_setup_error = e.what();
@@ -1490,7 +1725,7 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
                           size_t probes_per_listener, int threads) {
     return as_setup_create_ex(vertices, nv, faces, nf, face_material, beta, materials, branches, branch_material,
                               nbranch, origin, cell, nx, ny, nz, source, probes, nprobes, edge_admittance, layer,
-                              layer_damping, stencil, speed, probes_per_listener, threads, 0u);
+                              layer_damping, stencil, speed, nullptr, nullptr, probes_per_listener, threads, 0u);
 }
 
 const char *as_setup_create_error(void) { return g_setup_error.c_str(); }
@@ -1540,6 +1775,13 @@ int as_setup_layout(const as_setup *h, as_layout *out) {
     out->taps = s.probe_cells.size();
     out->probe_cells = s.probe_cells.data();
     out->probe_weights = s.probe_weights.data();
+    out->flow_x = s.flow_x.empty() ? nullptr : s.flow_x.data();
+    out->flow_y = s.flow_y.empty() ? nullptr : s.flow_y.data();
+    out->flow_mask = s.flow_mask.empty() ? nullptr : s.flow_mask.data();
+    for (size_t d = 0; d < 3; ++d) {
+        out->wind[d] = s.flow_wind[d].empty() ? nullptr : s.flow_wind[d].data();
+        out->loop[d] = s.flow_loop[d].empty() ? nullptr : s.flow_loop[d].data();
+    }
     return AS_OK;
 }
 

@@ -35,6 +35,7 @@ class Layout(ctypes.Structure):
         ("layer_cells", U64), ("layer_cell", P), ("keep", P * 6), ("push", P * 6),
         ("probes", U64), ("probe_start", P), ("taps", U64), ("probe_cells", P),
         ("probe_weights", P), ("coef_b64", P), ("coef_inv64", P), ("branch_gain64", P),
+        ("flow_x", P), ("flow_y", P), ("flow_mask", P), ("wind", P * 3), ("loop", P * 3),
     ]
 
 
@@ -142,10 +143,6 @@ _TYPES = {
 def pack_layout(layout: dict):
     """An as_layout over the arrays of a layout dict (WaveSolver::layout's names,
     reference fdtd.hpp:727-782); returns (struct, arrays kept alive)."""
-    flows = [k for k in ("flow_x", "flow_y", "flow_mask") if len(layout.get(k, ())) > 0]
-    if flows:
-        raise ValueError("the HIP stepper steps still air only: moving air is stepped on the CPU, "
-                         "as the reference's own GPU stepper leaves it (engine/wave.py:1159-1178)")
     A = {}
     for name, dt in _TYPES.items():
         A[name] = np.ascontiguousarray(layout.get(name, np.zeros(0, dt)), dtype=dt)
@@ -205,6 +202,28 @@ def pack_layout(layout: dict):
                 raise ValueError("the matched layer's coefficients do not cover the grid")
             s.keep[2 * a + j] = keep.ctypes.data if len(keep) else None
             s.push[2 * a + j] = push.ctypes.data if len(push) else None
+    # Moving air: plain (flow_x, flow_y and the row masks) or round (each face's wind and the
+    # loops, the mask 1 or 2), each over the whole grid.
+    A["flow_mask"] = np.ascontiguousarray(layout.get("flow_mask", np.zeros(0, np.uint8)), dtype=np.uint8)
+    for name in ("flow_x", "flow_y", *(f"{k}_{ax}" for k in ("wind", "loop") for ax in _AXES)):
+        A[name] = np.ascontiguousarray(layout.get(name, np.zeros(0, np.float32)), dtype=np.float32)
+    flowing = len(A["flow_mask"]) > 0
+    plain = [A[k] for k in ("flow_x", "flow_y")]
+    round_ = [A[f"{k}_{ax}"] for k in ("wind", "loop") for ax in _AXES]
+    if flowing:
+        if len(A["flow_mask"]) != n:
+            raise ValueError("the moving air's mask does not cover the grid")
+        if all(len(a) == n for a in plain) and all(len(a) == 0 for a in round_):
+            s.flow_x, s.flow_y = A["flow_x"].ctypes.data, A["flow_y"].ctypes.data
+        elif all(len(a) == n for a in round_) and all(len(a) == 0 for a in plain):
+            for d, ax in enumerate(_AXES):
+                s.wind[d] = A[f"wind_{ax}"].ctypes.data
+                s.loop[d] = A[f"loop_{ax}"].ctypes.data
+        else:
+            raise ValueError("moving air takes the shifts (flow_x, flow_y) or the wind and loops over the "
+                             "whole grid, not both")
+    elif any(len(a) for a in plain + round_):
+        raise ValueError("moving air without its mask (flow_mask)")
     # Optional double coefficients (precision 64): coef_b64, coef_inv64, branch_gain64.
     for name, count in (("coef_b64", s.specials), ("coef_inv64", s.specials), ("branch_gain64", s.branches)):
         if name in layout:

@@ -41,7 +41,9 @@ def _lib():
         lib.as_setup_create.argtypes = [P, S, P, S, P, P, S, P, P, S, P, D, S, S, S, P, P, S, D, S, D,
                                         S, P, S, ctypes.c_int]
         lib.as_setup_create_ex.restype = P
-        lib.as_setup_create_ex.argtypes = [*lib.as_setup_create.argtypes, ctypes.c_uint]
+        base = list(lib.as_setup_create.argtypes)
+        # ... speed, then flow_x and flow_y, then probes_per_listener, threads, and the flags.
+        lib.as_setup_create_ex.argtypes = [*base[:-2], P, P, *base[-2:], ctypes.c_uint]
         lib.as_setup_create_error.restype = ctypes.c_char_p
         lib.as_setup_create_error.argtypes = []
         lib.as_setup_layout.restype = ctypes.c_int
@@ -91,9 +93,8 @@ class WaveSolver:
                  fitted_walls=False, vertices=None, faces=None, on_device=False):
         if fitted_walls:
             raise ValueError("fitted walls: not laid out by this set-up; take walls on cell faces")
-        if flow_x is not None or flow_y is not None:
-            raise ValueError("moving air: stepped on the CPU only, as the reference's GPU stepper "
-                             "leaves it; leave the air's motion out")
+        if (flow_x is None) != (flow_y is None):
+            raise ValueError("the air's motion needs both of its components")
         lib = _lib()
         fm = np.ascontiguousarray(face_material, np.int64)
         if len(fm) != bvh.face_count():
@@ -112,13 +113,19 @@ class WaveSolver:
             sp = np.ascontiguousarray(speed, np.float32)
             if len(sp) != n:
                 raise ValueError("WaveSolver: one speed per cell of the grid")
-        self._keep = (bvh, fm, b, br, bm, o, s, p, sp)
+        fx = fy = None
+        if flow_x is not None:
+            fx = np.ascontiguousarray(flow_x, np.float32).reshape(-1)
+            fy = np.ascontiguousarray(flow_y, np.float32).reshape(-1)
+            if len(fx) != n or len(fy) != n:
+                raise ValueError("WaveSolver: the air's motion, one shift per cell of the grid")
+        self._keep = (bvh, fm, b, br, bm, o, s, p, sp, fx, fy)
         ptr = lambda a: None if a is None or len(a) == 0 else a.ctypes.data  # noqa: E731
         h = lib.as_setup_create_ex(
             bvh.vertices.ctypes.data, len(bvh.vertices), bvh.faces.ctypes.data, len(bvh.faces),
             fm.ctypes.data, b.ctypes.data, len(b), ptr(br), ptr(bm), len(bm), o.ctypes.data,
             float(cell), int(nx), int(ny), int(nz), s.ctypes.data, ptr(p), len(p),
-            float(edge_admittance), int(layer), float(layer_damping), int(stencil), ptr(sp),
+            float(edge_admittance), int(layer), float(layer_damping), int(stencil), ptr(sp), ptr(fx), ptr(fy),
             int(probes_per_listener), int(threads), SETUP_ON_DEVICE if on_device else 0)
         if not h:
             raise ValueError(lib.as_setup_create_error().decode())
@@ -135,7 +142,7 @@ class WaveSolver:
         self.open = bool(info[5])
         self.open_sides = tuple(bool((int(info[6]) >> k) & 1) for k in range(6))
         self.source_on_face = False
-        self.flowing = False
+        self.flowing = fx is not None
         self._stepper = None
         self._layout = None
 
@@ -203,12 +210,13 @@ class WaveSolver:
             "branch_before": _array(L.branch_before, L.branches, f32),
             "branch_gain": _array(L.branch_gain, L.branches, f32),
             "layer_cells": _array(L.layer_cell, L.layer_cells, np.int64),
-            "flow_x": np.zeros(0, f32), "flow_y": np.zeros(0, f32),
-            "flow_mask": np.zeros(0, np.uint8),
+            "flow_x": _array(L.flow_x, n if L.flow_x else 0, f32),
+            "flow_y": _array(L.flow_y, n if L.flow_y else 0, f32),
+            "flow_mask": _array(L.flow_mask, n if L.flow_mask else 0, np.uint8),
         }
-        for ax in "xyz":
-            out[f"wind_{ax}"] = np.zeros(0, f32)
-            out[f"loop_{ax}"] = np.zeros(0, f32)
+        for d, ax in enumerate("xyz"):
+            out[f"wind_{ax}"] = _array(L.wind[d], n if L.wind[d] else 0, f32)
+            out[f"loop_{ax}"] = _array(L.loop[d], n if L.loop[d] else 0, f32)
         lengths = (L.nx + 1, L.ny + 1, L.nz + 1)
         for a, ax in enumerate("xyz"):
             for j, face in enumerate(("", "face_")):

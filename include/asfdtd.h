@@ -40,8 +40,11 @@ extern "C" {
  * and its length in elements, in the reference's types. keep/push are per axis, the
  * centres' then the faces' coefficients: keep_x, keep_face_x, keep_y, keep_face_y,
  * keep_z, keep_face_z (each of length n_axis + 1, or 0 without a matched layer).
- * Moving air (flow_*, wind_*, loop_*) is not part of it: the reference steps moving air on
- * the CPU only (engine/wave.py:1159-1178), and so does this library refuse it.
+ * Moving air (fdtd.hpp:1680-2120), where the layout has it (flow_mask not NULL): plain,
+ * flow_x and flow_y (each cell's shifts a step) with flow_mask the row masks; or the wind
+ * round the solids, wind[3] (the wind on each cell's +x, +y, +z faces) and loop[3], with
+ * flow_mask 1 where the air moves the cell and 2 at the border of the moving air. The
+ * reference steps moving air on its CPU only (engine/wave.py:1159-1178); here the GPU does.
  */
 typedef struct as_layout {
     uint64_t nx, ny, nz;
@@ -79,6 +82,10 @@ typedef struct as_layout {
      * FP64 reference, finite differences); NULL: the floats above. Given, every special is
      * stepped by its own blocks. */
     const double *coef_b64, *coef_inv64, *branch_gain64;
+    /* Moving air, or NULLs (see above): [nx ny nz] each. */
+    const float *flow_x, *flow_y;
+    const uint8_t *flow_mask;
+    const float *wind[3], *loop[3];
 } as_layout;
 
 typedef struct as_fdtd as_fdtd;
@@ -216,10 +223,14 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
                           double edge_admittance, size_t layer, double layer_damping, size_t stencil,
                           const float *speed, size_t probes_per_listener, int threads);
 /*
- * as_setup_create with flags. AS_SETUP_ON_DEVICE: the passes over the whole grid, the cut
- * links (fdtd.hpp:446-477), the air's flood (498-517), the matched layer (521-542) and the
- * boundary cells (544-625), run on the current GPU (hipMalloc'd temporaries, freed before
- * the call returns); the layout, and every refusal, is the host set-up's bit for bit. Grids of 2^32 cells or more are refused in this mode.
+ * as_setup_create with the air's motion and flags. flow_x, flow_y (both or neither, [nx ny
+ * nz]): each cell's shifts a step along x and y, the air moving under half a cell a step
+ * (the binding's flow_x, flow_y; set_up_flow, fdtd.hpp:1680-1935, solving the wind round
+ * the solids where the shifts do not hold as given). AS_SETUP_ON_DEVICE: the passes over
+ * the whole grid, the cut links (fdtd.hpp:446-477), the air's flood (498-517), the
+ * matched layer (521-542) and the boundary cells (544-625), run on the current GPU
+ * (hipMalloc'd temporaries, freed before the call returns); the layout, and every refusal,
+ * is the host set-up's bit for bit. Grids of 2^32 cells or more are refused in this mode.
  */
 #define AS_SETUP_ON_DEVICE 1u
 as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
@@ -228,7 +239,8 @@ as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *f
                              const double *origin, double cell, size_t nx, size_t ny, size_t nz,
                              const double *source, const double *probes, size_t nprobes,
                              double edge_admittance, size_t layer, double layer_damping, size_t stencil,
-                             const float *speed, size_t probes_per_listener, int threads, unsigned flags);
+                             const float *speed, const float *flow_x, const float *flow_y,
+                             size_t probes_per_listener, int threads, unsigned flags);
 const char *as_setup_create_error(void);
 /* Points `out` at the set-up's layout arrays (WaveSolver::layout, fdtd.hpp:727-782); they
  * live as long as the handle. */

@@ -66,8 +66,8 @@ void *asref_create(const double *vertices, std::size_t nv, const std::int64_t *f
                    const double *origin, double cell, std::size_t nx, std::size_t ny, std::size_t nz,
                    const double *source, const double *probes, std::size_t nprobes,
                    double edge_admittance, std::size_t layer, double layer_damping, std::size_t threads,
-                   std::size_t stencil, const float *speed, std::size_t probes_per_listener, int fitted,
-                   char *err, std::size_t errlen) {
+                   std::size_t stencil, const float *speed, const float *flow_x, const float *flow_y,
+                   std::size_t probes_per_listener, int fitted, char *err, std::size_t errlen) {
     try {
         auto h = std::make_unique<Handle>();
         h->bvh = std::make_unique<as::Bvh>(vertices, nv, faces, nf);
@@ -84,7 +84,7 @@ void *asref_create(const double *vertices, std::size_t nv, const std::int64_t *f
             *h->bvh, face_material, beta, materials, branches, branch_material, nbranch,
             as::Vec3{origin[0], origin[1], origin[2]}, cell, nx, ny, nz,
             as::Vec3{source[0], source[1], source[2]}, points, edge_admittance, layer, layer_damping,
-            threads, stencil, speed, nullptr, nullptr, probes_per_listener, fitted != 0,
+            threads, stencil, speed, flow_x, flow_y, probes_per_listener, fitted != 0,
             fitted != 0 ? &mesh : nullptr);
         h->solver->layout([&](const char *name, const auto &values) {
             using T = typename std::decay_t<decltype(values)>::value_type;

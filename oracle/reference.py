@@ -37,7 +37,7 @@ def _load():
     D = ctypes.c_double
     lib.asref_create.restype = P
     lib.asref_create.argtypes = [P, S, P, S, P, P, S, P, P, S, P, D, S, S, S, P, P, S, D, S, D, S,
-                                 S, P, S, ctypes.c_int, ctypes.c_char_p, S]
+                                 S, P, P, P, S, ctypes.c_int, ctypes.c_char_p, S]
     lib.asref_layout_count.restype = ctypes.c_int
     lib.asref_layout_count.argtypes = [P]
     lib.asref_layout_entry.restype = ctypes.c_int
@@ -68,7 +68,8 @@ class ReferenceSolver:
 
     def __init__(self, vertices, faces, face_material, beta, branches, branch_material, origin,
                  cell, shape, source, probes, edge_admittance=1.0, layer=0, layer_damping=0.0,
-                 threads=1, stencil=7, speed=None, probes_per_listener=1, fitted_walls=False):
+                 threads=1, stencil=7, speed=None, probes_per_listener=1, fitted_walls=False,
+                 flow_x=None, flow_y=None):
         lib = _load()
         self._keep = [
             np.ascontiguousarray(vertices, np.float64),
@@ -83,12 +84,15 @@ class ReferenceSolver:
         ]
         v, f, fm, b, br, bm, o, s, p = self._keep
         sp = None if speed is None else np.ascontiguousarray(speed, np.float32)
+        fx = None if flow_x is None else np.ascontiguousarray(flow_x, np.float32)
+        fy = None if flow_y is None else np.ascontiguousarray(flow_y, np.float32)
+        self._keep += [sp, fx, fy]
         err = ctypes.create_string_buffer(4096)
         nx, ny, nz = (int(x) for x in shape)
         h = lib.asref_create(_ptr(v), len(v), _ptr(f), len(f), _ptr(fm), _ptr(b), len(b), _ptr(br),
                              _ptr(bm), len(bm), _ptr(o), float(cell), nx, ny, nz, _ptr(s), _ptr(p),
                              len(p), float(edge_admittance), int(layer), float(layer_damping),
-                             int(threads), int(stencil), _ptr(sp), int(probes_per_listener),
+                             int(threads), int(stencil), _ptr(sp), _ptr(fx), _ptr(fy), int(probes_per_listener),
                              1 if fitted_walls else 0, err, len(err))
         if not h:
             raise ValueError(err.value.decode())
