@@ -1290,6 +1290,13 @@ struct as_fdtd {
     uint32_t *boxes = nullptr, *box_bits = nullptr, *cells = nullptr;
     uint8_t *types = nullptr;
     float *scale = nullptr;
+    // Moving air (see carry_kernel), padded like the fields: plain (flow_x, flow_y and the
+    // row masks) or round (each cell's +x, +y, +z faces' wind and loops, the mask 0/1/2),
+    // and the carrying's two stage fields.
+    bool flowing = false, flow_round = false;
+    uint8_t *flow_mask = nullptr;
+    float *flow_x = nullptr, *flow_y = nullptr, *flow_wind[3] = {}, *flow_loop[3] = {};
+    float *flow_t1 = nullptr, *flow_t2 = nullptr;
     uint32_t *tiles = nullptr;
     uint64_t *tile_lanes = nullptr;  // per tile block and row: its lanes that march (see interior_tile)
     uint32_t *sp_cell = nullptr, *link_base = nullptr, *state_base = nullptr;
@@ -1473,7 +1480,15 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (L.nx < 3 || L.ny < 3 || L.nz < 3) refuse("the grid is too small");
     if (L.n_weights < (L.stencil == 7 ? 6u : 18u)) refuse("the layout's weights are too few for its stencil");
     if (L.types == nullptr) refuse("the layout has no cell types");
-    if (L.flow_mask != nullptr) refuse("moving air: this build of the stepper takes still air only");
+    if (L.flow_mask != nullptr) {
+        if (precision != 32) refuse("moving air is stepped in float only, the reference's scheme");
+        const bool plain = L.flow_x != nullptr && L.flow_y != nullptr;
+        const bool round = L.wind[0] != nullptr && L.wind[1] != nullptr && L.wind[2] != nullptr &&
+                           L.loop[0] != nullptr && L.loop[1] != nullptr && L.loop[2] != nullptr;
+        if (plain == round) refuse("moving air takes the shifts (flow_x, flow_y) or the wind and loops, not both");
+    } else if (L.flow_x != nullptr || L.flow_y != nullptr || L.wind[0] != nullptr || L.loop[0] != nullptr) {
+        refuse("moving air without its mask (flow_mask)");
+    }
     // The probes' ranges: from 0, never decreasing, ending at the taps (a range running
     // backwards would make a probe's wavefront loop near 2^32 times; one past the taps would
     // read past the cells).
@@ -1510,7 +1525,8 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     if (tuning && tuning->tile_order > 0) h->tile_order = tuning->tile_order == 2 ? 2 : 1;
     if (tuning && tuning->no_faces > 0) h->no_faces = 1;
     if (tuning) h->keep_history = tuning->keep_history > 0 ? 1 : (tuning->keep_history < 0 ? -1 : 0);
-    const bool pairs_on = tuning && tuning->pairs > 0;
+    // Two steps a launch leave no room between the steps for moving air's carrying.
+    const bool pairs_on = tuning && tuning->pairs > 0 && L.flow_mask == nullptr;
     const uint64_t nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
     const uint64_t px = (nx + static_cast<uint64_t>(align) - 1) / static_cast<uint64_t>(align) * static_cast<uint64_t>(align);
     const uint64_t n_pad = px * ny * nz;
@@ -2089,6 +2105,33 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         h->box_bits = upload(h, box_bits);
         h->cells = upload(h, cells);
     }
+    if (L.flow_mask != nullptr) {
+        // Moving air, padded: cells off the grid's rows read 0 (solid).
+        const auto padded = [&](const auto *src) {
+            using U = std::remove_cv_t<std::remove_pointer_t<decltype(src)>>;
+            std::vector<U> out(n_pad, U{});
+            for (uint64_t z = 0; z < nz; ++z)
+                for (uint64_t y = 0; y < ny; ++y)
+                    std::copy(src + nx * (y + ny * z), src + nx * (y + ny * z) + nx,
+                              out.begin() + static_cast<ptrdiff_t>(px * (y + ny * z)));
+            return out;
+        };
+        h->flowing = true;
+        h->flow_round = L.wind[0] != nullptr;
+        h->flow_mask = upload(h, padded(L.flow_mask));
+        if (h->flow_round) {
+            for (int d = 0; d < 3; ++d) {
+                h->flow_wind[d] = upload(h, padded(L.wind[d]));
+                h->flow_loop[d] = upload(h, padded(L.loop[d]));
+            }
+        } else {
+            h->flow_x = upload(h, padded(L.flow_x));
+            h->flow_y = upload(h, padded(L.flow_y));
+        }
+        h->flow_t1 = static_cast<float *>(zeros(h, n_pad * sizeof(float)));
+        h->flow_t2 = static_cast<float *>(zeros(h, n_pad * sizeof(float)));
+        h->pairs = 0;
+    }
     h->layer_state = zeros(h, 3 * L.layer_cells * e);
     if (L.layer_cells > 0) {
         h->layer_face[0] = zeros(h, 3 * n_pad * e);
@@ -2170,6 +2213,198 @@ StepArgs<T> step_args(const as_fdtd *h) {
     return s;
 }
 
+// ---- Moving air: the carrying (fdtd.hpp:1937-2120) ----------------------------------
+//
+// Before each step both pressures, the current and the previous, are carried along with
+// the air by the three-stage strong-stability-preserving Runge-Kutta method, a launch a
+// stage (each stage reads only the stage before, so the grid's order does not matter):
+//   t1 = q + d(q),  t2 = 3/4 q + 1/4 (t1 + d(t1)),  q = q + 2/3 ((t2 + d(t2)) - q),
+// d(q) = -v . grad q over a step at each cell the air moves, cells it does not keeping their
+// values to the bit. Plain: per axis, the row's differences as its mask allows (the
+// fourth-order central one with two moving cells either side, else half the differences
+// across the faces and a twelfth of each loop of three, the cell's own value out and
+// silence in through a face to air that does not move), times the cell's shift. Round:
+// per axis, the two faces' winds times the differences across them, halved, and the three
+// loops; at the border of the moving air, through a face to still air the cell's value goes
+// out and silence comes in. The arithmetic, float with no contraction, is the reference's
+// in its order.
+struct Carry {
+    const uint8_t *__restrict__ types;
+    const uint8_t *__restrict__ mask;
+    const float *__restrict__ fx;
+    const float *__restrict__ fy;
+    const float *__restrict__ wind[3];
+    const float *__restrict__ loop[3];
+    float *t1, *t2;
+    uint32_t n;
+    int px;
+    long long sxy;
+};
+
+// q's change along a row (stride s) at i over a cell, as air moving by `shift` along it
+// carries it; m: which of the row's cells one (bits 0, 1) and two (bits 2, 3) below and
+// above move with it.
+__device__ __forceinline__ float row_gradient(const float *q, long long i, long long s, unsigned m, float shift) {
+    if ((m & 15u) == 15u) return (8.0f * (q[i + s] - q[i - s]) - (q[i + 2 * s] - q[i - 2 * s])) * (1.0f / 12.0f);
+    const bool below = m & 1u, above = m & 2u;
+    const float c = q[i];
+    float faces = 0.0f, loops = 0.0f;
+    if (above) {
+        faces += 0.5f * (q[i + s] - c);
+        if (m & 8u) loops += q[i + s] - q[i + 2 * s];
+    } else if (shift < 0.0f) {
+        faces -= c;
+    }
+    if (below) {
+        faces += 0.5f * (c - q[i - s]);
+        if (m & 4u) loops += q[i - 2 * s] - q[i - s];
+    } else if (shift > 0.0f) {
+        faces += c;
+    }
+    if (below && above) loops += q[i + s] - q[i - s];
+    return faces + loops * (1.0f / 12.0f);
+}
+
+__device__ __forceinline__ float carried_plain(const Carry &c, const float *q, long long i) {
+    const float sx = c.fx[i], sy = c.fy[i];
+    const unsigned m = c.mask[i];
+    return -(sx * row_gradient(q, i, 1, m & 15u, sx) + sy * row_gradient(q, i, c.px, m >> 4, sy));
+}
+
+__device__ __forceinline__ float carried_round(const Carry &c, const float *q, long long i) {
+    const long long stride[3] = {1, c.px, c.sxy};
+    const float v = q[i];
+    float sum = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const long long s = stride[d];
+        const float *w = c.wind[d], *l = c.loop[d];
+        const float above = q[i + s], below = q[i - s];
+        sum += 0.5f * (w[i] * (above - v) + w[i - s] * (v - below)) + l[i] * (above - q[i + 2 * s]) +
+               l[i - s] * (above - below) + l[i - 2 * s] * (q[i - 2 * s] - below);
+    }
+    return -sum;
+}
+
+__device__ __forceinline__ float carried_border(const Carry &c, const float *q, long long i) {
+    const long long stride[3] = {1, c.px, c.sxy};
+    const float v = q[i];
+    float sum = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const long long s = stride[d];
+        const float *w = c.wind[d], *l = c.loop[d];
+        const float above = q[i + s], below = q[i - s];
+        if (c.mask[i + s]) {
+            sum += 0.5f * w[i] * (above - v);
+        } else if (w[i] < 0.0f) {
+            sum -= w[i] * v;
+        }
+        if (c.mask[i - s]) {
+            sum += 0.5f * w[i - s] * (v - below);
+        } else if (w[i - s] > 0.0f) {
+            sum += w[i - s] * v;
+        }
+        sum += l[i] * (above - q[i + 2 * s]) + l[i - s] * (above - below) + l[i - 2 * s] * (q[i - 2 * s] - below);
+    }
+    return -sum;
+}
+
+// Stage STAGE (0, 1, 2) of carrying q: each cell off the solid from the stage before.
+template <bool ROUND, int STAGE>
+__global__ __launch_bounds__(kBlock) void carry_kernel(const Carry c, float *q) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.n || c.types[i] == 0) return;
+    const bool still = ROUND ? c.mask[i] == 0 : (c.fx[i] == 0.0f && c.fy[i] == 0.0f);
+    if (STAGE == 2 && still) return;
+    const float *from = STAGE == 0 ? q : (STAGE == 1 ? c.t1 : c.t2);
+    float d = 0.0f;
+    if (!still) {
+        if (ROUND) {
+            d = c.mask[i] == 2 ? carried_border(c, from, i) : carried_round(c, from, i);
+        } else {
+            d = carried_plain(c, from, i);
+        }
+    }
+    if (STAGE == 0) {
+        c.t1[i] = still ? q[i] : q[i] + d;
+    } else if (STAGE == 1) {
+        c.t2[i] = still ? q[i] : 0.75f * q[i] + 0.25f * (c.t1[i] + d);
+    } else {
+        q[i] = q[i] + (2.0f / 3.0f) * ((c.t2[i] + d) - q[i]);
+    }
+}
+
+Carry carry_args(const as_fdtd *h) {
+    Carry c{};
+    c.types = h->types;
+    c.mask = h->flow_mask;
+    c.fx = h->flow_x;
+    c.fy = h->flow_y;
+    for (int d = 0; d < 3; ++d) {
+        c.wind[d] = h->flow_wind[d];
+        c.loop[d] = h->flow_loop[d];
+    }
+    c.t1 = h->flow_t1;
+    c.t2 = h->flow_t2;
+    c.n = static_cast<uint32_t>(h->n_pad);
+    c.px = h->px;
+    c.sxy = h->sxy;
+    return c;
+}
+
+// The three stages of carrying q, on `stream`.
+void carry(const as_fdtd *h, float *q, hipStream_t stream) {
+    const Carry c = carry_args(h);
+    const dim3 grid(static_cast<uint32_t>((h->n_pad + kBlock - 1) / kBlock)), block(kBlock);
+    if (h->flow_round) {
+        hipLaunchKernelGGL((carry_kernel<true, 0>), grid, block, 0, stream, c, q);
+        hipLaunchKernelGGL((carry_kernel<true, 1>), grid, block, 0, stream, c, q);
+        hipLaunchKernelGGL((carry_kernel<true, 2>), grid, block, 0, stream, c, q);
+    } else {
+        hipLaunchKernelGGL((carry_kernel<false, 0>), grid, block, 0, stream, c, q);
+        hipLaunchKernelGGL((carry_kernel<false, 1>), grid, block, 0, stream, c, q);
+        hipLaunchKernelGGL((carry_kernel<false, 2>), grid, block, 0, stream, c, q);
+    }
+}
+
+// A step as one launch of `blocks` blocks (the roles in the order step_kernel reads them).
+template <typename T>
+void launch_fused(const as_fdtd *h, const StepArgs<T> &s, uint32_t blocks, hipStream_t stream) {
+    const dim3 grid(blocks), block(kBlock);
+    if (h->wide && h->scaled) {
+        hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, 0, stream, s);
+    } else if (h->wide) {
+        hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, 0, stream, s);
+    } else if (h->scaled) {
+        hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, 0, stream, s);
+    } else {
+        hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, 0, stream, s);
+    }
+}
+
+// A step as a launch per kind of cell (the split launch shape); the probes last, where asked.
+template <typename T>
+void launch_split(const as_fdtd *h, const StepArgs<T> &s, hipStream_t stream, bool probes) {
+    if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
+    if (s.face_blocks) hipLaunchKernelGGL((face_kernel<T>), dim3(s.face_blocks), dim3(kBlock), 0, stream, s);
+    if (s.tile_blocks) {
+        const dim3 grid(s.tile_blocks), block(kBlock);
+        if (h->wide && h->scaled) {
+            hipLaunchKernelGGL((interior_kernel<T, true, true>), grid, block, 0, stream, s);
+        } else if (h->wide) {
+            hipLaunchKernelGGL((interior_kernel<T, true, false>), grid, block, 0, stream, s);
+        } else if (h->scaled) {
+            hipLaunchKernelGGL((interior_kernel<T, false, true>), grid, block, 0, stream, s);
+        } else {
+            hipLaunchKernelGGL((interior_kernel<T, false, false>), grid, block, 0, stream, s);
+        }
+    }
+    if (s.layer_blocks) hipLaunchKernelGGL((layer_kernel<T>), dim3(s.layer_blocks), dim3(kBlock), 0, stream, s);
+    const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
+    if (probes && probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
+}
+
 // One step from the fields in slots prev (p(k - 1), overwritten by p(k + 1)) and prev ^ 1
 // (p(k)): the reference's swaps (fdtd.hpp:883-885, 900-903) are the slots' exchange.
 template <typename T>
@@ -2183,36 +2418,31 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned 
     s.layer_face_in = static_cast<const T *>(h->layer_face[prev & 1]);
     s.layer_face_out = static_cast<T *>(h->layer_face[(prev & 1) ^ 1]);
     s.step = k;
-    if (h->split) {
-        if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
-        if (s.face_blocks) hipLaunchKernelGGL((face_kernel<T>), dim3(s.face_blocks), dim3(kBlock), 0, stream, s);
-        if (s.tile_blocks) {
-            const dim3 grid(s.tile_blocks), block(kBlock);
-            if (h->wide && h->scaled) {
-                hipLaunchKernelGGL((interior_kernel<T, true, true>), grid, block, 0, stream, s);
-            } else if (h->wide) {
-                hipLaunchKernelGGL((interior_kernel<T, true, false>), grid, block, 0, stream, s);
-            } else if (h->scaled) {
-                hipLaunchKernelGGL((interior_kernel<T, false, true>), grid, block, 0, stream, s);
-            } else {
-                hipLaunchKernelGGL((interior_kernel<T, false, false>), grid, block, 0, stream, s);
+    if constexpr (std::is_same<T, float>::value) {
+        if (h->flowing) {
+            // The probes read p(k) as it stands, then both pressures are carried with the
+            // air (fdtd.hpp:838-846), then the step, without its probe blocks.
+            const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
+            if (probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
+            carry(h, static_cast<float *>(h->field[prev ^ 1]), stream);
+            carry(h, static_cast<float *>(h->field[prev]), stream);
+            StepArgs<T> t = s;
+            const uint32_t rest = blocks - t.probe_blocks;
+            t.probe_blocks = 0;
+            t.probes = 0;
+            if (h->split) {
+                launch_split<T>(h, t, stream, false);
+                return;
             }
+            if (rest > 0) launch_fused<T>(h, t, rest, stream);
+            return;
         }
-        if (s.layer_blocks) hipLaunchKernelGGL((layer_kernel<T>), dim3(s.layer_blocks), dim3(kBlock), 0, stream, s);
-        const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
-        if (probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
+    }
+    if (h->split) {
+        launch_split<T>(h, s, stream, true);
         return;
     }
-    const dim3 grid(blocks), block(kBlock);
-    if (h->wide && h->scaled) {
-        hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, 0, stream, s);
-    } else if (h->wide) {
-        hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, 0, stream, s);
-    } else if (h->scaled) {
-        hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, 0, stream, s);
-    } else {
-        hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, 0, stream, s);
-    }
+    launch_fused<T>(h, s, blocks, stream);
 }
 
 template <typename T>
@@ -2586,6 +2816,9 @@ int as_fdtd_reset(as_fdtd *h) {
         for (void *f : h->layer_face) {
             if (f != nullptr) check(hipMemsetAsync(f, 0, 3 * h->n_pad * e, h->stream), "hipMemsetAsync");
         }
+        for (float *t : {h->flow_t1, h->flow_t2}) {
+            if (t != nullptr) check(hipMemsetAsync(t, 0, h->n_pad * sizeof(float), h->stream), "hipMemsetAsync");
+        }
         check(hipStreamSynchronize(h->stream), "reset");
         h->done = 0;
         h->prev = 0;
@@ -2668,6 +2901,7 @@ int record_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t co
 
 void check_adjoint(const as_fdtd *h) {
     if (h->face_specials > 0) refuse("the adjoint needs a handle made with no_faces set");
+    if (h->flowing) refuse("the adjoint steps still air only: moving air has no transposed carrying here");
     if (h->layer_cells > 0 && (h->px > (1 << 30) || h->ny > (1 << 30)))
         refuse("the adjoint's matched layer needs a grid under 2^30 cells along each axis");
 }

@@ -213,6 +213,7 @@ def pack_layout(layout: dict):
     if flowing:
         if len(A["flow_mask"]) != n:
             raise ValueError("the moving air's mask does not cover the grid")
+        s.flow_mask = A["flow_mask"].ctypes.data
         if all(len(a) == n for a in plain) and all(len(a) == 0 for a in round_):
             s.flow_x, s.flow_y = A["flow_x"].ctypes.data, A["flow_y"].ctypes.data
         elif all(len(a) == n for a in round_) and all(len(a) == 0 for a in plain):

@@ -16,6 +16,9 @@
  *   specials   CSR links, branch terms, source, coef_inv last              (fdtd.hpp:860-881)
  *   layer      split-field PML from b into a                               (fdtd.hpp:2242-2266)
  *   then a and b swap, and the branch velocities wa and wb with them       (fdtd.hpp:883-885)
+ * In moving air both pressures, b then a, are carried along with the air between the record
+ * and the interior (fdtd.hpp:838-846): three Runge-Kutta stages (fdtd.hpp:2026-2111) of the
+ * plain rows' operator (fdtd.hpp:1945-1974) or the wind round the solids' (fdtd.hpp:1976-2021).
  *
  * With precision 64 the same scheme runs in double: the fields, states and arithmetic
  * are double, the coefficients the layout's floats widened (an FP64 run of the same
@@ -57,12 +60,18 @@ typedef struct {
     uint64_t probes;
     const uint64_t *probe_start, *probe_cells;
     const double *probe_weights;
+    /* Moving air, or NULL: the cells' masks and either the plain shifts or the faces' winds
+       and loops along each axis (x, y, z), each [n]. */
+    const uint8_t *flow_mask;
+    const float *flow_x, *flow_y;
+    const float *wind[3], *loop[3];
 } oracle_layout;
 
 typedef struct {
     oracle_layout L;
     int precision;
     void *u0, *u1, *w0, *w1, *layer_state;
+    void *t1, *t2; /* the carrying's two stages, in moving air */
     ptrdiff_t offset[18];
 } oracle_solver;
 
@@ -83,6 +92,8 @@ oracle_solver *oracle_create(const oracle_layout *layout, int precision) {
     s->w0 = calloc(layout->states + 1, e);
     s->w1 = calloc(layout->states + 1, e);
     s->layer_state = calloc(9 * layout->layer_cells + 1, e);
+    s->t1 = calloc(layout->flow_mask ? n : 1, e);
+    s->t2 = calloc(layout->flow_mask ? n : 1, e);
     const ptrdiff_t sy = (ptrdiff_t)layout->nx, sz = (ptrdiff_t)(layout->nx * layout->ny);
     for (int k = 0; k < 18; ++k) s->offset[k] = kSteps[k][0] + kSteps[k][1] * sy + kSteps[k][2] * sz;
     return s;
@@ -95,6 +106,8 @@ void oracle_destroy(oracle_solver *s) {
     free(s->w0);
     free(s->w1);
     free(s->layer_state);
+    free(s->t1);
+    free(s->t2);
     free(s);
 }
 
@@ -106,10 +119,106 @@ void oracle_reset(oracle_solver *s) {
     memset(s->w0, 0, (s->L.states + 1) * e);
     memset(s->w1, 0, (s->L.states + 1) * e);
     memset(s->layer_state, 0, (9 * s->L.layer_cells + 1) * e);
+    memset(s->t1, 0, (s->L.flow_mask ? n : 1) * e);
+    memset(s->t2, 0, (s->L.flow_mask ? n : 1) * e);
 }
 
+/* Moving air, one body for both precisions. ROW: a quantity's change along a row (stride st)
+ * at cell i over a cell, as air moving by `shift` carries it; m's bits say which of the row's
+ * cells one (bits 0, 1: below, above) and two (bits 2, 3) away move with it (fdtd.hpp:1945-1966).
+ * PLAIN: -(shift_x ROW_x + shift_y ROW_y) (fdtd.hpp:1969-1974). ROUND: over each axis the two
+ * faces' winds times the differences across them, halved, and the three loops through the cell
+ * (fdtd.hpp:1976-1991); BORDER: the same where a face leads to air that does not move, whose
+ * cell's own value leaves through it against the wind (fdtd.hpp:1993-2021). CARRY: q's three
+ * stages, q + d, 3/4 q + 1/4 (t1 + d), q + 2/3 ((t2 + d) - q), solid cells skipped and still ones
+ * kept (fdtd.hpp:2026-2111). */
+#define ORACLE_FLOW(NAME, REAL)                                                                 \
+    static REAL NAME##_row(const REAL *q, size_t i, size_t st, unsigned m, REAL shift) {        \
+        if ((m & 15u) == 15u)                                                                    \
+            return ((REAL)8 * (q[i + st] - q[i - st]) - (q[i + 2 * st] - q[i - 2 * st])) *       \
+                   ((REAL)1 / (REAL)12);                                                         \
+        const REAL c = q[i];                                                                     \
+        REAL faces = 0, loops = 0;                                                               \
+        if (m & 2u) {                                                                            \
+            faces += (REAL)0.5 * (q[i + st] - c);                                                \
+            if (m & 8u) loops += q[i + st] - q[i + 2 * st];                                      \
+        } else if (shift < 0) {                                                                  \
+            faces -= c;                                                                          \
+        }                                                                                        \
+        if (m & 1u) {                                                                            \
+            faces += (REAL)0.5 * (c - q[i - st]);                                                \
+            if (m & 4u) loops += q[i - 2 * st] - q[i - st];                                      \
+        } else if (shift > 0) {                                                                  \
+            faces += c;                                                                          \
+        }                                                                                        \
+        if ((m & 3u) == 3u) loops += q[i + st] - q[i - st];                                      \
+        return faces + loops * ((REAL)1 / (REAL)12);                                             \
+    }                                                                                            \
+    static REAL NAME##_plain(const oracle_layout *L, const REAL *q, size_t i) {                  \
+        const REAL sx = (REAL)L->flow_x[i], sy = (REAL)L->flow_y[i];                             \
+        const unsigned m = L->flow_mask[i];                                                      \
+        return -(sx * NAME##_row(q, i, 1, m & 15u, sx) + sy * NAME##_row(q, i, L->nx, m >> 4, sy)); \
+    }                                                                                            \
+    static REAL NAME##_round(const oracle_layout *L, const REAL *q, size_t i, int border) {      \
+        const size_t stride[3] = {1, L->nx, L->nx * L->ny};                                      \
+        const REAL c = q[i];                                                                     \
+        REAL sum = 0;                                                                            \
+        for (int d = 0; d < 3; ++d) {                                                            \
+            const size_t st = stride[d];                                                         \
+            const float *w = L->wind[d], *l = L->loop[d];                                        \
+            const REAL up = q[i + st], down = q[i - st];                                         \
+            const REAL l0 = (REAL)l[i] * (up - q[i + 2 * st]), l1 = (REAL)l[i - st] * (up - down);  \
+            const REAL l2 = (REAL)l[i - 2 * st] * (q[i - 2 * st] - down);                        \
+            if (!border) {                                                                       \
+                sum += (REAL)0.5 * ((REAL)w[i] * (up - c) + (REAL)w[i - st] * (c - down)) + l0 + l1 + l2; \
+                continue;                                                                        \
+            }                                                                                    \
+            if (L->flow_mask[i + st]) {                                                          \
+                sum += (REAL)0.5 * (REAL)w[i] * (up - c);                                        \
+            } else if (w[i] < 0.0f) {                                                            \
+                sum -= (REAL)w[i] * c;                                                           \
+            }                                                                                    \
+            if (L->flow_mask[i - st]) {                                                          \
+                sum += (REAL)0.5 * (REAL)w[i - st] * (c - down);                                 \
+            } else if (w[i - st] > 0.0f) {                                                       \
+                sum += (REAL)w[i - st] * c;                                                      \
+            }                                                                                    \
+            sum += l0 + l1 + l2;                                                                 \
+        }                                                                                        \
+        return -sum;                                                                             \
+    }                                                                                            \
+    static void NAME##_carry(const oracle_layout *L, REAL *q, REAL *t1, REAL *t2) {              \
+        const size_t n = (size_t)(L->nx * L->ny * L->nz);                                        \
+        const int round = L->wind[0] != NULL;                                                    \
+        for (int stage = 0; stage < 3; ++stage) {                                                \
+            const REAL *from = stage == 0 ? q : (stage == 1 ? t1 : t2);                          \
+            for (size_t i = 0; i < n; ++i) {                                                     \
+                if (L->types[i] == 0) continue;                                                  \
+                const int still = round ? L->flow_mask[i] == 0                                   \
+                                        : (L->flow_x[i] == 0.0f && L->flow_y[i] == 0.0f);        \
+                if (still) {                                                                     \
+                    if (stage == 0) t1[i] = q[i];                                                \
+                    if (stage == 1) t2[i] = q[i];                                                \
+                    continue;                                                                    \
+                }                                                                                \
+                const REAL d = round ? NAME##_round(L, from, i, L->flow_mask[i] == 2)            \
+                                     : NAME##_plain(L, from, i);                                 \
+                if (stage == 0) {                                                                \
+                    t1[i] = q[i] + d;                                                            \
+                } else if (stage == 1) {                                                         \
+                    t2[i] = (REAL)0.75 * q[i] + (REAL)0.25 * (t1[i] + d);                        \
+                } else {                                                                         \
+                    q[i] = q[i] + ((REAL)2 / (REAL)3) * ((t2[i] + d) - q[i]);                    \
+                }                                                                                \
+            }                                                                                    \
+        }                                                                                        \
+    }
+
+ORACLE_FLOW(flow_f32, float)
+ORACLE_FLOW(flow_f64, double)
+
 /* One body for both precisions: REAL is float or double. */
-#define ORACLE_RUN(NAME, REAL)                                                                  \
+#define ORACLE_RUN(NAME, FLOW, REAL)                                                            \
     static void NAME(oracle_solver *s, const float *signal, size_t steps, double *out) {        \
         const oracle_layout *L = &s->L;                                                          \
         const size_t nx = L->nx, ny = L->ny, nz = L->nz, n = nx * ny * nz;                       \
@@ -129,6 +238,11 @@ void oracle_reset(oracle_solver *s) {
                 for (uint64_t k = L->probe_start[p]; k < L->probe_start[p + 1]; ++k)             \
                     value += L->probe_weights[k] * (double)b[L->probe_cells[k]];                 \
                 out[p * steps + step] = value;                                                   \
+            }                                                                                    \
+            /* moving air: both pressures carried along with it */                               \
+            if (L->flow_mask) {                                                                  \
+                FLOW##_carry(L, b, (REAL *)s->t1, (REAL *)s->t2);                                \
+                FLOW##_carry(L, a, (REAL *)s->t1, (REAL *)s->t2);                                \
             }                                                                                    \
             /* interior */                                                                       \
             for (size_t i = 0; i < n; ++i) {                                                     \
@@ -233,8 +347,8 @@ void oracle_reset(oracle_solver *s) {
         free(term);                                                                              \
     }
 
-ORACLE_RUN(run_f32, float)
-ORACLE_RUN(run_f64, double)
+ORACLE_RUN(run_f32, flow_f32, float)
+ORACLE_RUN(run_f64, flow_f64, double)
 
 void oracle_run(oracle_solver *s, const float *signal, size_t steps, double *out) {
 #ifdef ORACLE_HAS_MXCSR

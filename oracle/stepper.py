@@ -31,6 +31,7 @@ class _Layout(ctypes.Structure):
         ("branch_now", P), ("branch_before", P), ("branch_gain", P),
         ("layer_cells", U64), ("layer_cell", P), ("keep", P * 6), ("push", P * 6),
         ("probes", U64), ("probe_start", P), ("probe_cells", P), ("probe_weights", P),
+        ("flow_mask", P), ("flow_x", P), ("flow_y", P), ("wind", P * 3), ("loop", P * 3),
     ]
 
 
@@ -106,6 +107,20 @@ class OracleSolver:
             s.push[2 * a] = L[f"push_{ax}"].ctypes.data
             s.push[2 * a + 1] = L[f"push_face_{ax}"].ctypes.data
         s.probes = len(L["probe_start"]) - 1
+        # Moving air: the masks and the plain shifts, or the faces' winds and loops.
+        mask = L.get("flow_mask", np.zeros(0))
+        if len(mask):
+            L["flow_mask"] = np.ascontiguousarray(mask, np.uint8)
+            s.flow_mask = L["flow_mask"].ctypes.data
+            if len(L.get("flow_x", ())):
+                for k in ("flow_x", "flow_y"):
+                    L[k] = np.ascontiguousarray(L[k], np.float32)
+                s.flow_x, s.flow_y = L["flow_x"].ctypes.data, L["flow_y"].ctypes.data
+            else:
+                for a, ax in enumerate(_AXES):
+                    for k in ("wind", "loop"):
+                        L[f"{k}_{ax}"] = np.ascontiguousarray(L[f"{k}_{ax}"], np.float32)
+                        getattr(s, k)[a] = L[f"{k}_{ax}"].ctypes.data
         self._struct = s
         self.probes = int(s.probes)
         self.shape = tuple(shape)
