@@ -1297,6 +1297,12 @@ struct as_fdtd {
     uint8_t *flow_mask = nullptr;
     float *flow_x = nullptr, *flow_y = nullptr, *flow_wind[3] = {}, *flow_loop[3] = {};
     float *flow_t1 = nullptr, *flow_t2 = nullptr;
+    // The pressure squared over the air (cells not solid) as each step's carrying takes it:
+    // a sum per block of the carrying's grid over the last run's steps, and their total; the
+    // air's cells and the last run's steps (mean_square).
+    double *flow_squares = nullptr, *flow_total = nullptr;
+    uint32_t flow_blocks = 0;
+    uint64_t air_cells = 0, run_steps = 0;
     uint32_t *tiles = nullptr;
     uint64_t *tile_lanes = nullptr;  // per tile block and row: its lanes that march (see interior_tile)
     uint32_t *sp_cell = nullptr, *link_base = nullptr, *state_base = nullptr;
@@ -2130,6 +2136,10 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         }
         h->flow_t1 = static_cast<float *>(zeros(h, n_pad * sizeof(float)));
         h->flow_t2 = static_cast<float *>(zeros(h, n_pad * sizeof(float)));
+        h->flow_blocks = static_cast<uint32_t>((n_pad + kBlock - 1) / kBlock);
+        h->flow_squares = static_cast<double *>(zeros(h, h->flow_blocks * sizeof(double)));
+        h->flow_total = static_cast<double *>(zeros(h, sizeof(double)));
+        for (uint64_t i = 0; i < n; ++i) h->air_cells += L.types[i] != 0;
         h->pairs = 0;
     }
     h->layer_state = zeros(h, 3 * L.layer_cells * e);
@@ -2236,6 +2246,7 @@ struct Carry {
     const float *__restrict__ wind[3];
     const float *__restrict__ loop[3];
     float *t1, *t2;
+    double *squares;  // a sum per block (the first stage of the current pressure only)
     uint32_t n;
     int px;
     long long sxy;
@@ -2310,11 +2321,27 @@ __device__ __forceinline__ float carried_border(const Carry &c, const float *q, 
     return -sum;
 }
 
-// Stage STAGE (0, 1, 2) of carrying q: each cell off the solid from the stage before.
-template <bool ROUND, int STAGE>
+// Stage STAGE (0, 1, 2) of carrying q: each cell off the solid from the stage before. With
+// SQUARES (the first stage of the current pressure), q's squares over the air as it comes
+// (fdtd.hpp:2069-2072) are summed over the block in double, in a fixed order, and added to
+// the block's own slot: the sums are the same run after run.
+template <bool ROUND, int STAGE, bool SQUARES>
 __global__ __launch_bounds__(kBlock) void carry_kernel(const Carry c, float *q) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= c.n || c.types[i] == 0) return;
+    const bool air = i < c.n && c.types[i] != 0;
+    if constexpr (SQUARES) {
+        __shared__ double part[kBlock];
+        const double v = air ? static_cast<double>(q[i]) : 0.0;
+        part[threadIdx.x] = v * v;
+        __syncthreads();
+#pragma unroll
+        for (unsigned w = kBlock / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) c.squares[blockIdx.x] += part[0];
+    }
+    if (!air) return;
     const bool still = ROUND ? c.mask[i] == 0 : (c.fx[i] == 0.0f && c.fy[i] == 0.0f);
     if (STAGE == 2 && still) return;
     const float *from = STAGE == 0 ? q : (STAGE == 1 ? c.t1 : c.t2);
@@ -2347,25 +2374,64 @@ Carry carry_args(const as_fdtd *h) {
     }
     c.t1 = h->flow_t1;
     c.t2 = h->flow_t2;
+    c.squares = h->flow_squares;
     c.n = static_cast<uint32_t>(h->n_pad);
     c.px = h->px;
     c.sxy = h->sxy;
     return c;
 }
 
-// The three stages of carrying q, on `stream`.
-void carry(const as_fdtd *h, float *q, hipStream_t stream) {
-    const Carry c = carry_args(h);
-    const dim3 grid(static_cast<uint32_t>((h->n_pad + kBlock - 1) / kBlock)), block(kBlock);
-    if (h->flow_round) {
-        hipLaunchKernelGGL((carry_kernel<true, 0>), grid, block, 0, stream, c, q);
-        hipLaunchKernelGGL((carry_kernel<true, 1>), grid, block, 0, stream, c, q);
-        hipLaunchKernelGGL((carry_kernel<true, 2>), grid, block, 0, stream, c, q);
+// The three stages of carrying q, on `stream`; `squares`: q's squares summed on the way.
+template <bool ROUND>
+void carry_stages(const Carry &c, float *q, bool squares, hipStream_t stream) {
+    const dim3 grid((c.n + kBlock - 1) / kBlock), block(kBlock);
+    if (squares) {
+        hipLaunchKernelGGL((carry_kernel<ROUND, 0, true>), grid, block, 0, stream, c, q);
     } else {
-        hipLaunchKernelGGL((carry_kernel<false, 0>), grid, block, 0, stream, c, q);
-        hipLaunchKernelGGL((carry_kernel<false, 1>), grid, block, 0, stream, c, q);
-        hipLaunchKernelGGL((carry_kernel<false, 2>), grid, block, 0, stream, c, q);
+        hipLaunchKernelGGL((carry_kernel<ROUND, 0, false>), grid, block, 0, stream, c, q);
     }
+    hipLaunchKernelGGL((carry_kernel<ROUND, 1, false>), grid, block, 0, stream, c, q);
+    hipLaunchKernelGGL((carry_kernel<ROUND, 2, false>), grid, block, 0, stream, c, q);
+}
+
+void carry(const as_fdtd *h, float *q, bool squares, hipStream_t stream) {
+    const Carry c = carry_args(h);
+    if (h->flow_round) {
+        carry_stages<true>(c, q, squares, stream);
+    } else {
+        carry_stages<false>(c, q, squares, stream);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(double *p, uint32_t count) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) p[i] = 0.0;
+}
+
+// The blocks' sums into one, in a fixed order: each lane its strided share in turn, then
+// the lanes pairwise. One block.
+__global__ __launch_bounds__(kBlock) void total_kernel(const double *__restrict__ parts, uint32_t count,
+                                                       double *__restrict__ total) {
+    __shared__ double lane[kBlock];
+    double sum = 0.0;
+    for (uint32_t k = threadIdx.x; k < count; k += kBlock) sum += parts[k];
+    lane[threadIdx.x] = sum;
+    __syncthreads();
+#pragma unroll
+    for (unsigned w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) lane[threadIdx.x] += lane[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = lane[0];
+}
+
+// A run of `steps` starts: in moving air its squares start from zero (by a kernel, so that a
+// caller may capture it).
+void begin_run(as_fdtd *h, size_t steps) {
+    if (!h->flowing) return;
+    hipLaunchKernelGGL(fill_kernel, dim3((h->flow_blocks + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream,
+                       h->flow_squares, h->flow_blocks);
+    h->run_steps = steps;
 }
 
 // A step as one launch of `blocks` blocks (the roles in the order step_kernel reads them).
@@ -2424,8 +2490,8 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned 
             // air (fdtd.hpp:838-846), then the step, without its probe blocks.
             const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
             if (probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
-            carry(h, static_cast<float *>(h->field[prev ^ 1]), stream);
-            carry(h, static_cast<float *>(h->field[prev]), stream);
+            carry(h, static_cast<float *>(h->field[prev ^ 1]), true, stream);
+            carry(h, static_cast<float *>(h->field[prev]), false, stream);
             StepArgs<T> t = s;
             const uint32_t rest = blocks - t.probe_blocks;
             t.probe_blocks = 0;
@@ -2703,6 +2769,7 @@ int as_fdtd_run_device(as_fdtd *h, const float *d_signal, size_t steps, double *
     if (h == nullptr) return AS_EINVAL;
     try {
         if (steps == 0) return AS_OK;
+        begin_run(h, steps);
         if (h->precision == 64) {
             launch_steps<double>(h, d_signal, steps, d_out, steps);
         } else {
@@ -2735,6 +2802,7 @@ int as_fdtd_run(as_fdtd *h, const float *signal, size_t steps, double *out) {
             check(hipMalloc(reinterpret_cast<void **>(&h->out_buf), h->out_cap * sizeof(double)), "hipMalloc");
         }
         std::vector<double> chunk;
+        begin_run(h, steps);
         for (size_t done = 0; done < steps; done += piece) {
             const size_t m = std::min(piece, steps - done);
             check(hipMemcpyAsync(h->signal_buf, signal + done, m * sizeof(float), hipMemcpyHostToDevice, h->stream),
@@ -2778,6 +2846,7 @@ int as_fdtd_run_static(as_fdtd *h, const float *signal, size_t steps, double *ou
         unsigned char *base = static_cast<unsigned char *>(scratch_address(h->device));
         double *d_out = reinterpret_cast<double *>(base);
         float *d_signal = reinterpret_cast<float *>(base + round_up_256(piece * probe_bytes));
+        begin_run(h, steps);
         for (size_t done = 0; done < steps; done += piece) {
             const size_t m = std::min(piece, steps - done);
             check(hipMemcpyAsync(d_signal, signal + done, m * sizeof(float), hipMemcpyHostToDevice, h->stream),
@@ -2795,6 +2864,24 @@ int as_fdtd_run_static(as_fdtd *h, const float *signal, size_t steps, double *ou
             // The next piece reuses the staging: this one's copies end first.
             check(hipStreamSynchronize(h->stream), "step");
         }
+        return AS_OK;
+    } catch (const Failure &f) {
+        return fail(h, f);
+    }
+}
+
+int as_fdtd_mean_square(as_fdtd *h, double *out) {
+    if (h == nullptr || out == nullptr) return AS_EINVAL;
+    try {
+        *out = 0.0;
+        if (!h->flowing || h->run_steps == 0 || h->air_cells == 0) return AS_OK;
+        hipLaunchKernelGGL(total_kernel, dim3(1), dim3(kBlock), 0, h->stream, h->flow_squares, h->flow_blocks,
+                           h->flow_total);
+        double total = 0.0;
+        check(hipMemcpyAsync(&total, h->flow_total, sizeof(double), hipMemcpyDeviceToHost, h->stream),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(h->stream), "mean square");
+        *out = total / (static_cast<double>(h->run_steps) * static_cast<double>(h->air_cells));
         return AS_OK;
     } catch (const Failure &f) {
         return fail(h, f);
@@ -2822,6 +2909,7 @@ int as_fdtd_reset(as_fdtd *h) {
         check(hipStreamSynchronize(h->stream), "reset");
         h->done = 0;
         h->prev = 0;
+        h->run_steps = 0;
         return AS_OK;
     } catch (const Failure &f) {
         return fail(h, f);

@@ -1801,6 +1801,15 @@ int as_setup_info(const as_setup *h, uint64_t *out) {
     return AS_OK;
 }
 
+int as_setup_wind(const as_setup *h, int *round, double *fastest, uint64_t *sweeps) {
+    if (h == nullptr) return AS_EINVAL;
+    const Setup &s = h->s;
+    if (round) *round = s.flow_round ? 1 : 0;
+    if (fastest) *fastest = static_cast<double>(s.flow_fastest);
+    if (sweeps) *sweeps = s.flow_sweeps;
+    return AS_OK;
+}
+
 int as_setup_probe(const as_setup *h, size_t p, int *found, int *in_sight) {
     if (h == nullptr || p >= h->s.probe_found.size()) return AS_EINVAL;
     if (found) *found = h->s.probe_found[p];

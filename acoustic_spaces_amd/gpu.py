@@ -118,6 +118,14 @@ class HipWaveSolver:
         self._check(self._lib.as_fdtd_adjoint(self._h, g.ctypes.data, sens_g.ctypes.data, sens_gain.ctypes.data))
         return sens_g, sens_gain
 
+    @property
+    def mean_square(self) -> float:
+        """In moving air, the pressure squared over the grid's air and the steps of the last
+        run, on average; 0 in still air (the binding's mean_square, native_module.cpp:1606-1608)."""
+        out = ctypes.c_double()
+        self._check(self._lib.as_fdtd_mean_square(self._h, ctypes.byref(out)))
+        return float(out.value)
+
     def reset(self) -> None:
         """Zeroes the fields and states (WaveSolver::reset, fdtd.hpp:906-915)."""
         self._check(self._lib.as_fdtd_reset(self._h))

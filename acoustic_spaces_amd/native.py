@@ -54,7 +54,7 @@ _lib = None
 SYMBOLS = (
     "as_fdtd_create", "as_fdtd_create_error", "as_fdtd_run", "as_fdtd_run_device", "as_fdtd_run_static",
     "as_fdtd_reset", "as_fdtd_field", "as_fdtd_stream", "as_fdtd_info", "as_fdtd_last_error",
-    "as_fdtd_destroy", "as_fdtd_gradient", "as_fdtd_record", "as_fdtd_adjoint",
+    "as_fdtd_destroy", "as_fdtd_gradient", "as_fdtd_record", "as_fdtd_adjoint", "as_fdtd_mean_square",
 )
 
 #: The device post-processing's symbols (acoustic_spaces_amd.device_responses binds them).
@@ -62,7 +62,7 @@ RESPONSE_SYMBOLS = ("as_first_order_device",)
 
 #: The set-up's symbols (acoustic_spaces_amd.solver binds them).
 SETUP_SYMBOLS = ("as_setup_create", "as_setup_create_ex", "as_setup_create_error", "as_setup_layout", "as_setup_info",
-                 "as_setup_probe", "as_setup_destroy")
+                 "as_setup_wind", "as_setup_probe", "as_setup_destroy")
 
 
 def load():
@@ -95,6 +95,8 @@ def load():
     lib.as_fdtd_run_static.argtypes = [P, P, SIZE, P]
     lib.as_fdtd_reset.restype = ctypes.c_int
     lib.as_fdtd_reset.argtypes = [P]
+    lib.as_fdtd_mean_square.restype = ctypes.c_int
+    lib.as_fdtd_mean_square.argtypes = [P, P]
     lib.as_fdtd_field.restype = ctypes.c_int
     lib.as_fdtd_field.argtypes = [P, P]
     lib.as_fdtd_stream.restype = P

@@ -50,6 +50,8 @@ def _lib():
         lib.as_setup_layout.argtypes = [P, ctypes.POINTER(native.Layout)]
         lib.as_setup_info.restype = ctypes.c_int
         lib.as_setup_info.argtypes = [P, P]
+        lib.as_setup_wind.restype = ctypes.c_int
+        lib.as_setup_wind.argtypes = [P, P, P, P]
         lib.as_setup_probe.restype = ctypes.c_int
         lib.as_setup_probe.argtypes = [P, S, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         lib.as_setup_destroy.restype = None
@@ -143,6 +145,12 @@ class WaveSolver:
         self.open_sides = tuple(bool((int(info[6]) >> k) & 1) for k in range(6))
         self.source_on_face = False
         self.flowing = fx is not None
+        # Moving air's set-up (the binding's wind_round, wind_fastest, wind_sweeps).
+        rnd, fastest, sweeps = ctypes.c_int(), ctypes.c_double(), ctypes.c_uint64()
+        lib.as_setup_wind(h, ctypes.byref(rnd), ctypes.byref(fastest), ctypes.byref(sweeps))
+        self.wind_round = bool(rnd.value)
+        self.wind_fastest = float(fastest.value)
+        self.wind_sweeps = int(sweeps.value)
         self._stepper = None
         self._layout = None
 

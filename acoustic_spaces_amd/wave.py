@@ -8,10 +8,11 @@ stepping seam, for a scene of this package's (`scenes.Scene`):
 - the grid is stepped on the GPU (`HipWaveSolver`);
 - each listener's pressure and gradient become W, X, Y, Z (`_first_order`, 757-802), in a
   `WaveResponse` (330-454) with the reference's fields, `fed`, `field` (resampled to a
-  render rate), `high_pass` and `energy`.
-
-The reference's model types, band grids from materials, moving air and its CPU stepper are
-not here: a scene carries its own grid and admittances, and the stepping is the GPU's.
+  render rate), `high_pass` and `energy`;
+and the entry point itself, `wave_responses`, from a model read as the reference's
+TraceModel: band grids from its materials, the admittance fits, the weather's speeds of
+sound, moving air and the watch over it. The stepping is the GPU's: the reference's CPU
+and WebGPU steppers are not here.
 """
 
 from __future__ import annotations
@@ -352,6 +353,13 @@ LAYER_CELLS = 8                              # the matched layer round an open m
 LAYER_DAMPING = 0.5
 LAYER_ROOM = 0.15   # room past the points, a share of their farthest spread across each axis
 LONG_RUN = 2e9      # cell updates beyond which a run reports its tenths
+#: Moving air is watched for sound that grows (engine/wave.py:177-187): the pressure squared
+#: over the grid's air is averaged over pieces of the run at least WATCH_S long (s); from the
+#: end of the source's pulse on, a piece more than WATCH_RISE times the lowest before it, and
+#: more than WATCH_FLOOR of the largest, stops the run.
+WATCH_S = 0.25
+WATCH_RISE = 2.0
+WATCH_FLOOR = 1e-6
 
 
 def band_top(band: int, total: int = len(analysis.OCTAVE_BANDS_HZ)) -> float:
@@ -430,11 +438,13 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
     is the isotropic one (``scheme`` "auto") unless the model is open, where the standard
     one with its 8-cell matched layer steps it (outdoors at once); outdoors under a weather,
     each cell's effective speed of sound from the source (the wind's component along the
-    horizontal added). Differences, each refused or documented: ``device`` "cpu" (this
-    package steps on the GPU), ``wind`` "moving" (the reference steps moving air on the CPU
-    only), ``walls`` "fitted" (not laid out here), materials with an impedance (a porous
-    ground's fit), and the room round an open model's points taken on every side (the
-    reference leaves it off sides its surfaces close, ``_open_sides``)."""
+    horizontal added), or with ``wind`` "moving" the still air's speed and the wind moving
+    the air (each step's carrying, the run watched for sound that grows), a closed grid's
+    air still under any wind. Moving air is stepped on the GPU too (the reference steps it
+    on the CPU only). Differences, each refused or documented: ``device`` "cpu" (this
+    package steps on the GPU), ``walls`` "fitted" (not laid out here), materials with an
+    impedance (a porous ground's fit), and the room round an open model's points taken on
+    every side (the reference leaves it off sides its surfaces close, ``_open_sides``)."""
     from acoustic_spaces_amd.admittance import MAX_ABSORPTION, fit_admittance
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from acoustic_spaces_amd.solver import Bvh, WaveSolver
@@ -496,11 +506,13 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
         scene = getattr(room, "scene", None)
         atmosphere = getattr(scene, "atmosphere", None)
         if atmosphere is not None and not atmosphere.still:
-            if wind == "moving" and atmosphere.wind(np.array([10.0]))[0] != 0.0:
-                raise ValueError("moving air: the reference steps it on the CPU only; set simulation.wave.wind "
-                                 "to effective")
             extent = getattr(room.outdoor, "extent", None)  # the ground's, for the zones' polygons
             weather = Weather(atmosphere, float(scene.air.temperature_c), float(model.ground), source, extent)
+    # Whether the wind blows somewhere over the scene: then with wind "moving" the air moves
+    # (temperature alone needs no moving air), and a closed grid's air is still.
+    windy = weather is not None and weather.winds_over(
+        max(float(vertices[:, 2].max()), float(points[:, 2].max())) + 100.0)
+    moving = windy and wind == "moving"
 
     def layout(courant, layer, speed, per_wavelength):
         grid = band_grid(bands - 1, sample_rate, speed, per_wavelength, total, courant)
@@ -523,11 +535,29 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
                              f"cells{what} (at most {MAX_CELLS:,}); lower max_band_hz or points_per_wavelength")
         return grid, origin, shape
 
-    def prepare(stencil, layer):
+    def prepare(stencil, layer, still_air=False):
         points_n, courant = STENCILS[stencil]
-        speeds, extremes, still = None, None, speed_of_sound
+        speeds, extremes, still, flow = None, None, speed_of_sound, (None, None)
         if weather is None:
             grid, origin, shape = layout(courant, layer, speed_of_sound, points_per_wavelength)
+        elif moving or still_air:
+            # Fine enough for sound going against the fastest wind in the slowest still air,
+            # in steps for the fastest still air (the air's motion is carried apart; still
+            # air, ``still_air``, has none).
+            slowest = fastest = float(weather.at(source)[0])
+            for _ in range(8):
+                grid, origin, shape = layout(courant, layer, fastest, points_per_wavelength * fastest / slowest)
+                low, high, top = weather.moving_extremes(origin, grid.cell, shape)
+                if still_air:
+                    top = 0.0
+                if low - top >= slowest and high <= fastest:
+                    break
+                slowest, fastest = min(slowest, low - top), max(fastest, high)
+            speeds, *flow = weather.moving_fields(origin, grid.cell, shape, courant * grid.cell * grid.rate,
+                                                  1.0 / (grid.cell * grid.rate))
+            if still_air:
+                flow = (None, None)
+            extremes, still = (low - top, high + top), low
         else:
             # Fine enough for the slowest sound, in steps for the fastest, over the grid
             # those give (the reference's eight rounds).
@@ -545,10 +575,18 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
         branch_material = np.concatenate([np.full(len(b), m, np.int64) for m, b in enumerate(branches)]) \
             if branches else np.zeros(0, np.int64)
         probes = np.vstack([probe_points(p, grid.cell) for p in listeners])
-        solver = WaveSolver(bvh, face_material, np.array([f.beta for f in fits]),
-                            np.vstack(branches) if branches else np.zeros((0, 3)), branch_material, origin,
-                            grid.cell, int(shape[0]), int(shape[1]), int(shape[2]), source, probes, 1.0, layer,
-                            LAYER_DAMPING, threads, points_n, speeds, probes_per_listener=PROBES, on_device=True)
+        try:
+            solver = WaveSolver(bvh, face_material, np.array([f.beta for f in fits]),
+                                np.vstack(branches) if branches else np.zeros((0, 3)), branch_material, origin,
+                                grid.cell, int(shape[0]), int(shape[1]), int(shape[2]), source, probes, 1.0, layer,
+                                LAYER_DAMPING, threads, points_n, speeds, probes_per_listener=PROBES,
+                                flow_x=flow[0], flow_y=flow[1], on_device=True)
+        except ValueError as error:
+            if "the wind round the model's solids" not in str(error):
+                raise
+            raise ValueError(f"the wave solver's moving air: {error} (on cells of {100 * grid.cell:.0f} cm; a sharp "
+                             "edge speeds the wind up, the more the finer the grid): set simulation.wave.wind to "
+                             "effective, or lower max_band_hz or points_per_wavelength") from error
         return solver, grid, fits, probes, courant, extremes, still
 
     if scheme == "isotropic" and outdoor:
@@ -560,17 +598,25 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
         if not built[0].open:
             built = None
     if built is None:
-        built = prepare("standard" if scheme == "standard" else "isotropic", 0)
+        # A closed model's air is still whichever way the wind is taken: no wind blows
+        # through its walls (outdoors, a source shut in a closed building).
+        built = prepare("standard" if scheme == "standard" else "isotropic", 0, windy)
         if built[0].open:
             if scheme == "isotropic":
                 raise ValueError("the isotropic stencil has no matched layer for an open model: set "
                                  "simulation.wave.scheme to auto or standard")
             built = prepare("standard", LAYER_CELLS)
     solver, grid, fits, probes, courant, extremes, still = built
+    still_inside = windy and not solver.open  # the wind left out of a closed model
+    stilled = moving or still_inside  # the grid's speeds are the still air's
+
+    def speed_at(point):
+        return weather.still_at(point) if stilled else float(weather.at(point)[0])
+
     h = grid.cell
     at_source = 1.0
     if weather is not None:
-        at_source = min(float(weather.at(source)[0]) / (courant * h * grid.rate), 1.0)
+        at_source = min(speed_at(source) / (courant * h * grid.rate), 1.0)
     signal, delay = source_signal(grid.rate, h, courant, at_source)
     samples = int(math.ceil(length_s * grid.rate))
     whole = max(1, round(grid.rate / sample_rate))  # steps per output sample
@@ -592,28 +638,68 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
     where = ""
     if extremes is not None:
         where = f", sound at {extremes[0]:.0f} to {extremes[1]:.0f} m/s"
+    if still_inside:
+        where += ", the wind left out: the air inside the closed model is still"
+    # The fastest wind round the model's solids (m/s), where it was made to flow round them.
+    fastest = solver.wind_fastest * h * grid.rate if solver.wind_round else None
+    if solver.flowing:
+        where += ", the air moving"
+    if fastest is not None:
+        where += f", round the model's solids at up to {fastest:.0f} m/s"
     say(f"{label}: wave solver, {band_name} ({solver.air_cells:,} cells of {100 * h:.0f} cm, {steps:,} steps "
         f"on the GPU{where})")
     who = f"source {label!r}" if label else "the source"
+    unstable = "set simulation.wave.wind to effective"
+    # Long runs report every tenth, and moving air runs in the pieces it is watched over
+    # (WATCH_S): once the source's pulse has ended, the pressure squared over the air,
+    # averaged over a piece, may not rise past WATCH_RISE times its lowest so far.
     tenths = 10 if solver.air_cells * steps > LONG_RUN else 1
+    watch = max(1, round(WATCH_S * grid.rate))
+    chunks = max(tenths, steps // watch) if solver.flowing else tenths
     with HipWaveSolver(solver) as stepper:
-        if tenths == 1:
+        if chunks == 1:
             recorded = stepper.run(drive)
         else:
             recorded = np.empty((len(probes), steps))
-            bounds = np.linspace(0, steps, tenths + 1).astype(np.int64)
-            for c in range(tenths):
-                if c:
-                    say(f"{label}: wave solver, {band_name}, {100 * c // tenths} %")
+            bounds = np.linspace(0, steps, chunks + 1).astype(np.int64)
+            ended = lead - delay + len(signal)  # the first step after the source's pulse
+            lowest, largest = math.inf, 0.0
+            squares, since = 0.0, 0  # the watched piece so far: its squares' sum, its first step
+            for c in range(chunks):
+                if c and c * tenths // chunks > (c - 1) * tenths // chunks:
+                    say(f"{label}: wave solver, {band_name}, {100 * (c * tenths // chunks) // tenths} %")
                 recorded[:, bounds[c]: bounds[c + 1]] = stepper.run(drive[bounds[c]: bounds[c + 1]])
+                if not solver.flowing:
+                    continue
+                squares += stepper.mean_square * float(bounds[c + 1] - bounds[c])
+                if not math.isfinite(squares):
+                    recorded[:, bounds[c + 1]:] = math.nan  # refused below
+                    break
+                if bounds[c + 1] - since < watch:
+                    continue
+                level = squares / float(bounds[c + 1] - since)
+                start, squares, since = since, 0.0, int(bounds[c + 1])
+                largest = max(largest, level)
+                if start >= ended:
+                    if level > WATCH_RISE * lowest and level > WATCH_FLOOR * largest:
+                        raise ValueError(f"{who}: the wave solver's sound grew in the moving air instead of dying "
+                                         f"away ({10 * math.log10(level / lowest):.1f} dB over its lowest level, "
+                                         f"{(since - lead) / grid.rate:.1f} s after emission): {unstable}")
+                    lowest = min(lowest, level)
     if not np.all(np.isfinite(recorded)):
-        raise ValueError(f"{who}: the wave solver's run did not stay finite")
+        raise ValueError(f"{who}: the wave solver's run did not stay finite" + (f": {unstable}" if solver.flowing else ""))
     info = {"wave_cells": float(solver.air_cells), "wave_cell_m": h, "wave_steps": float(steps),
             "wave_stencil": float(solver.stencil), "wave_gpu": 1.0}
     if solver.open:
         info["wave_open"] = 1.0
     if extremes is not None:
         info["wave_speed_min_m_s"], info["wave_speed_max_m_s"] = extremes
+    if solver.flowing:
+        info["wave_moving_air"] = 1.0
+    if fastest is not None:
+        info["wave_wind_fastest_m_s"] = fastest
+    if still_inside:
+        info["wave_still_inside"] = 1.0
     slowest = speed_of_sound if extremes is None else extremes[0]
     per_wavelength = slowest / (band_top(bands - 1, total) * h)
     info["wave_slow_percent"] = 100.0 * slow_by(solver.stencil, per_wavelength, still / (courant * h * grid.rate))
@@ -625,7 +711,7 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
         part = slice(PROBES * i, PROBES * (i + 1))
         local = courant
         if weather is not None:
-            local *= float(weather.at(listeners[i])[0]) / (courant * h * grid.rate)
+            local *= speed_at(listeners[i]) / (courant * h * grid.rate)
         signals = first_order(recorded[part], found[part], name, local, cutoff(courant))
         responses.append(WaveResponse(grid.rate, signals, bands, h, total, lead / grid.rate))
     return responses, info

@@ -320,6 +320,68 @@ class Weather:
         zone = self.zone_at(np.column_stack([x.ravel(), y.ravel()])).reshape(ny, nx)
         return still, wind, cos, zone
 
+    # Moving air: each cell keeps the still air's speed of sound, and the wind moves the air.
+
+    def winds_over(self, top: float) -> bool:
+        """Whether the wind blows anywhere from the ground up to ``top`` (m, the scene's
+        coordinates), under any of the weathers (engine/wave.py:906-912: 512 heights)."""
+        z = np.linspace(self.ground, top, 512)
+        return any(bool(np.any(w.wind(z - self.ground) != 0.0)) for w in self.weathers)
+
+    def still_grid(self, origin, cell: float, shape):
+        """The still air's speed of sound and the wind's speed over the planes (K, nz), each
+        weather's wind direction (K, 2) and the weather over each column (ny, nx), or None
+        without zones (K = 1) (``_Weather._still_grid``, engine/wave.py:578-593)."""
+        origin = np.asarray(origin, float)
+        nx, ny, nz = (int(v) for v in shape)
+        zs = origin[2] + (np.arange(nz) + 0.5) * cell
+        K = len(self.weathers) if self.zoned else 1
+        still, wind = np.empty((K, nz)), np.empty((K, nz))
+        for k in range(K):
+            still[k], wind[k] = self.profiles(zs, k)
+        towards = np.array([self.weathers[k].direction() for k in range(K)])
+        if not self.zoned:
+            return still, wind, towards, None
+        x, y = np.meshgrid(origin[0] + (np.arange(nx) + 0.5) * cell, origin[1] + (np.arange(ny) + 0.5) * cell)
+        zone = self.zone_at(np.column_stack([x.ravel(), y.ravel()])).reshape(ny, nx)
+        return still, wind, towards, zone
+
+    def still_at(self, point) -> float:
+        """The still air's speed of sound (m/s) at ``point`` (3,), under the weather over it."""
+        p = np.asarray(point, float).reshape(3)
+        k = int(self.zone_at(p[None, :])[0]) if self.zoned else 0
+        return float(self.profiles(p[2:3], k)[0][0])
+
+    def moving_extremes(self, origin, cell: float, shape) -> tuple[float, float, float]:
+        """The slowest and fastest still air's speed of sound over the grid's cells and the
+        wind's fastest speed (m/s), of the weathers over it (engine/wave.py:600-607)."""
+        still, wind, _, zone = self.still_grid(origin, cell, shape)
+        if zone is not None:
+            used = np.unique(zone)
+            still, wind = still[used], wind[used]
+        return float(still.min()), float(still.max()), float(np.abs(wind).max())
+
+    def moving_fields(self, origin, cell: float, shape, speed: float, shift: float):
+        """(nx ny nz,) float32 each, x fastest: each cell's still air's speed of sound over
+        ``speed`` (the grid's), at most 1, and the air's velocity along x and y in cells per
+        step (the wind's speed times its direction's component times ``shift``, a step over a
+        cell); each plane in double, then rounded (engine/wave.py:609-627)."""
+        still, wind, towards, zone = self.still_grid(origin, cell, shape)
+        nx, ny, nz = (int(v) for v in shape)
+        speeds = np.empty((nz, ny, nx), np.float32)
+        fx = np.empty((nz, ny, nx), np.float32)
+        fy = np.empty((nz, ny, nx), np.float32)
+        for z in range(nz):
+            if zone is None:
+                speeds[z] = min(still[0, z] / speed, 1.0)
+                fx[z] = wind[0, z] * towards[0, 0] * shift
+                fy[z] = wind[0, z] * towards[0, 1] * shift
+            else:
+                speeds[z] = np.minimum(still[zone, z] / speed, 1.0)
+                fx[z] = wind[zone, z] * towards[zone, 0] * shift
+                fy[z] = wind[zone, z] * towards[zone, 1] * shift
+        return speeds.reshape(-1), fx.reshape(-1), fy.reshape(-1)
+
     def extremes(self, origin, cell: float, shape) -> tuple[float, float]:
         """The slowest and fastest effective speed over the cells of a grid (of the
         weathers over it)."""

@@ -150,6 +150,13 @@ int as_fdtd_run_static(as_fdtd *h, const float *signal, size_t steps, double *ou
 /* Zeroes the state (WaveSolver::reset, fdtd.hpp:906-915). */
 int as_fdtd_reset(as_fdtd *h);
 
+/* In moving air, the pressure squared over the grid's air (its cells not solid) as each step
+ * of the last run carried it, averaged over those cells and the run's steps; 0 in still air,
+ * before any run and after a reset (WaveSolver::mean_square, fdtd.hpp:711, 895-899). The sum
+ * runs in double in an order of this library's own (a block's cells pairwise, then the blocks),
+ * so it equals the reference's to rounding. Waits for the handle's stream. */
+int as_fdtd_mean_square(as_fdtd *h, double *out);
+
 /* The current pressure, the field the next step's probes read, widened to double
  * (nx ny nz values, x fastest). */
 int as_fdtd_field(as_fdtd *h, double *out);
@@ -248,6 +255,12 @@ int as_setup_layout(const as_setup *h, as_layout *out);
 /* [0] air cells, [1] boundary cells, [2] branch states, [3] layer cells, [4] probes,
  * [5] open, [6] open sides (bit k for -x, +x, -y, +y, -z, +z), [7] source on a face. */
 int as_setup_info(const as_setup *h, uint64_t *out);
+/* The moving air's set-up (WaveSolver::wind_round, wind_fastest, wind_sweeps, fdtd.hpp:705-707;
+ * the binding's properties, native_module.cpp:1596-1605): whether the wind was made to flow
+ * round the model's solids; the fastest the air moves along an axis, in cells per step
+ * (through a face between two cells where round, else the largest given component; 0 in
+ * still air); the sweeps its solve took (0 without one). Any pointer may be NULL. */
+int as_setup_wind(const as_setup *h, int *round, double *fastest, uint64_t *sweeps);
 /* probe_found and probe_in_sight of probe p (fdtd.hpp:713-716). */
 int as_setup_probe(const as_setup *h, size_t p, int *found, int *in_sight);
 void as_setup_destroy(as_setup *h);

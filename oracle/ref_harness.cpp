@@ -138,6 +138,16 @@ void asref_info(void *handle, std::size_t *out) {
     out[7] = s.source_on_face() ? 1 : 0;
 }
 
+// moving air: flowing, wind_round, wind_fastest, wind_sweeps, mean_square (of the last run)
+void asref_flow(void *handle, double *out) {
+    const as::WaveSolver &s = *static_cast<Handle *>(handle)->solver;
+    out[0] = s.flowing() ? 1.0 : 0.0;
+    out[1] = s.wind_round() ? 1.0 : 0.0;
+    out[2] = s.wind_fastest();
+    out[3] = static_cast<double>(s.wind_sweeps());
+    out[4] = s.mean_square();
+}
+
 void asref_open_sides(void *handle, int *out) {
     const auto &sides = static_cast<Handle *>(handle)->solver->open_sides();
     for (int k = 0; k < 6; ++k) out[k] = sides[static_cast<std::size_t>(k)] ? 1 : 0;

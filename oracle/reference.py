@@ -49,6 +49,7 @@ def _load():
     lib.asref_destroy.argtypes = [P]
     lib.asref_info.argtypes = [P, P]
     lib.asref_open_sides.argtypes = [P, P]
+    lib.asref_flow.argtypes = [P, P]
     lib.asref_probe_found.restype = ctypes.c_int
     lib.asref_probe_found.argtypes = [P, S]
     lib.asref_probe_in_sight.restype = ctypes.c_int
@@ -141,6 +142,14 @@ class ReferenceSolver:
 
     def reset(self):
         _load().asref_reset(self._h)
+
+    def flow(self) -> dict:
+        """Moving air: flowing, wind_round, wind_fastest, wind_sweeps and the last run's
+        mean_square, as the binding's properties give them."""
+        out = np.zeros(5, np.float64)
+        _load().asref_flow(self._h, _ptr(out))
+        return {"flowing": bool(out[0]), "wind_round": bool(out[1]), "wind_fastest": float(out[2]),
+                "wind_sweeps": int(out[3]), "mean_square": float(out[4])}
 
     def probe_found(self, p) -> bool:
         return bool(_load().asref_probe_found(self._h, p))

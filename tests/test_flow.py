@@ -93,6 +93,21 @@ def test_flow_layout_equals_the_reference(reference_available, name):
         assert not any(np.any(A[f"wind_{ax}"]) for ax in "xyz") and not np.any(A["flow_mask"])
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_wind_set_up_equals_the_references(reference_available, name):
+    """Whether the wind was made to flow round the solids, how fast it then moves and the
+    sweeps its solve took: the reference's own; and the air over which the mean square is
+    taken is the grid's cells that are not solid."""
+    sc, ref, mine = _pair(name)
+    want = ref.flow()
+    assert want["flowing"] and mine.flowing
+    assert (mine.wind_round, mine.wind_fastest, mine.wind_sweeps) == (
+        want["wind_round"], want["wind_fastest"], want["wind_sweeps"])
+    assert mine.wind_round == (name != "plain")
+    assert (mine.wind_fastest == 0.0) == (name == "round_box") and mine.wind_fastest < 0.5
+    assert ref.air_cells == mine.air_cells == np.count_nonzero(mine.layout()["types"])
+
+
 def test_flow_refusals_match_the_reference(reference_available):
     from acoustic_spaces_amd.solver import WaveSolver
     from oracle.reference import ReferenceSolver
@@ -147,13 +162,24 @@ def test_moving_air_steps_equal_the_reference(reference_available, name, shape):
         assert np.array_equal(host[k], layout[k]), k
     drive = sc.drive[:240]
     tuning = {"fused": {}, "split": {"split": True}, "no graphs": {"graph_steps": -1}}[shape]
+    pieces = (drive[:97], drive[97:])
+    a, b, ms_gpu, ms_ref = [], [], [], []
     with HipWaveSolver(layout, **tuning) as gpu:
-        a = np.concatenate([gpu.run(drive[:97]), gpu.run(drive[97:])], axis=1)
+        for piece in pieces:
+            a.append(gpu.run(piece))
+            ms_gpu.append(gpu.mean_square)
+            b.append(ref.run(piece))
+            ms_ref.append(ref.flow()["mean_square"])
         gpu.reset()
+        assert gpu.mean_square == 0.0
         again = gpu.run(drive[:40])
-    b = ref.run(drive)
+    a, b = np.concatenate(a, axis=1), np.concatenate(b, axis=1)
     assert np.array_equal(a, b) and np.any(a != 0)
     assert np.array_equal(again, b[:, :40])
+    # The pressure squared over the air, each run's own: the reference's to rounding (its
+    # sums run in another order).
+    assert all(m > 0 for m in ms_ref)
+    np.testing.assert_allclose(ms_gpu, ms_ref, rtol=1e-12)
 
 
 @pytest.mark.gpu

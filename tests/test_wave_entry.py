@@ -144,6 +144,10 @@ class _Checked:
         _Checked.runs += 1
         return a
 
+    @property
+    def mean_square(self):
+        return self.gpu.mean_square
+
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("scheme", ["auto", "standard"])
@@ -212,3 +216,68 @@ def test_entry_point_outdoors_under_zoned_weather(monkeypatch):
                                         length_s=0.1, sample_rate=8000.0, speed_of_sound=343.0)
     assert info["wave_speed_min_m_s"] < info["wave_speed_max_m_s"]
     assert np.all(np.isfinite(responses[0].signals)) and np.abs(responses[0].signals).max() > 0
+
+
+def _ground_and_barrier(atmosphere, barrier=True):
+    ground = scenes.box_mesh((12.0, 8.0, 1.0), low=(0.0, 0.0, -1.0), inward=False)
+    mesh = scenes.merge(ground, scenes.box_mesh((0.5, 6.0, 2.5), low=(5.5, 1.0, 0.0), inward=False)) \
+        if barrier else ground
+    rows = np.full((mesh.materials, 8), 0.2)
+    return scenes.trace_model(mesh, rows, outdoor=True, ground=0.0, atmosphere=atmosphere)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("barrier", [False, True])
+def test_entry_point_outdoors_in_moving_air(monkeypatch, barrier):
+    """Open ground under a wind, with wind "moving": each cell the still air's speed of sound
+    and the wind moving the air, carried before each step; as given over bare ground (it
+    crosses no surface), made to flow round a barrier across it. The run, in the pieces the
+    watch reads the mean square over, equals the oracle's bit for bit, piece by piece; the
+    info says so; and the sound differs from the effective speed's."""
+    from acoustic_spaces_amd import gpu
+    from acoustic_spaces_amd.weather import Atmosphere
+
+    monkeypatch.setattr(gpu, "HipWaveSolver", _Checked)
+    model = _ground_and_barrier(Atmosphere(wind_speed_m_s=8.0, temperature_gradient_k_m=-0.02), barrier)
+    kw = dict(max_band_hz=63, length_s=0.6, sample_rate=8000.0, speed_of_sound=343.0)
+    _Checked.runs = 0
+    lines = []
+    responses, info = wv.wave_responses(model, [3.0, 4.0, 1.0], [[9.0, 4.2, 1.5]], ["far"], wind="moving",
+                                        say=lines.append, **kw)
+    print(lines[0], info, flush=True)
+    assert _Checked.runs >= 2  # watched pieces
+    assert info["wave_moving_air"] == 1.0 and info["wave_open"] == 1.0 and "wave_still_inside" not in info
+    assert ("wave_wind_fastest_m_s" in info) == barrier and "the air moving" in lines[0]
+    if barrier:
+        # Faster than the wind at the barrier's top (about 6 m/s), under half a cell a step.
+        assert 5.0 < info["wave_wind_fastest_m_s"] < 0.5 * 343.0 * math.sqrt(3.0)
+    signals = responses[0].signals
+    assert np.all(np.isfinite(signals)) and np.abs(signals).max() > 0
+    effective, info_e = wv.wave_responses(model, [3.0, 4.0, 1.0], [[9.0, 4.2, 1.5]], ["far"], **kw)
+    assert "wave_moving_air" not in info_e
+    assert effective[0].signals.shape == signals.shape and not np.array_equal(effective[0].signals, signals)
+
+
+@pytest.mark.gpu
+def test_entry_point_in_a_closed_building_outdoors_under_wind(monkeypatch):
+    """A source and listener shut in a closed building outdoors, under a wind: the grid is
+    closed, its air still whichever way the wind is taken (the still air's speeds, no
+    carrying), and the info says the wind was left out."""
+    from acoustic_spaces_amd import gpu
+    from acoustic_spaces_amd.weather import Atmosphere
+
+    monkeypatch.setattr(gpu, "HipWaveSolver", _Checked)
+    mesh = scenes.box_mesh((4.1, 3.3, 2.7))
+    rows = np.full((mesh.materials, 8), 0.2)
+    model = scenes.trace_model(mesh, rows, outdoor=True, ground=0.0, volume_m3=4.1 * 3.3 * 2.7,
+                               atmosphere=Atmosphere(wind_speed_m_s=8.0, temperature_gradient_k_m=-0.02))
+    out = {}
+    for wind in ("effective", "moving"):
+        _Checked.runs = 0
+        responses, info = wv.wave_responses(model, [1.0, 1.1, 1.2], [[2.5, 2.0, 1.4]], ["in"], max_band_hz=63,
+                                            length_s=0.25, sample_rate=8000.0, speed_of_sound=343.0, wind=wind)
+        assert _Checked.runs == 1
+        assert info["wave_still_inside"] == 1.0 and "wave_open" not in info and "wave_moving_air" not in info
+        assert info["wave_stencil"] == 19.0
+        out[wind] = responses[0].signals
+    assert np.array_equal(out["effective"], out["moving"]) and np.abs(out["moving"]).max() > 0

@@ -209,3 +209,64 @@ def test_zoned_open_block_steps_equal_the_oracle():
         a = gpu.run(drive)
     b = OracleSolver(layout).run(drive)
     assert np.array_equal(a, b) and np.any(a != 0)
+
+
+def test_moving_fields_are_the_still_air_and_the_wind():
+    """Moving air's fields (engine/wave.py:575-627): each cell's speed is the still air's at
+    its height over the grid's, the same over a plane, at most 1; the air's velocity points
+    the wind's way with its speed times a step over a cell; the extremes span them. Zoned,
+    each column takes its own weather's fields, the first zone where zones overlap."""
+    own, zoned, a, b = _zoned()
+    src, origin, cell, shape = [0.2, 0.3, 0.5], np.array([-0.05, -0.05, -0.05]), 0.1, (15, 14, 9)
+    w = Weather(own, 20.0, 0.0, src)
+    low, high, top = w.moving_extremes(origin, cell, shape)
+    speed, shift = high * 1.01, 0.37
+    speeds, fx, fy = (f.reshape(9, 14, 15) for f in w.moving_fields(origin, cell, shape, speed, shift))
+    z = origin[2] + (np.arange(9) + 0.5) * cell
+    assert all(np.all(p == p[0, 0]) for f in (speeds, fx, fy) for p in f)
+    still = own.sound_speed(z, 20.0)
+    np.testing.assert_allclose(speeds[:, 0, 0], still / speed, rtol=1e-7)
+    assert np.isclose(low, still.min()) and np.isclose(high, still.max()) and speeds.max() < 1.0
+    wind = own.wind(z)
+    np.testing.assert_allclose(np.hypot(fx[:, 0, 0], fy[:, 0, 0]), wind * shift, rtol=1e-6)
+    assert np.all(fy == 0) and np.isclose(top, np.abs(wind).max())  # the wind blows along +x
+    assert w.still_at([1.0, 1.0, z[4]]) == still[4]
+    # A speed above the grid's is held at 1.
+    assert np.all(w.moving_fields(origin, cell, shape, low * 0.5, shift)[0] == 1.0)
+    # Zoned: each column its weather's, the first zone where two overlap.
+    wz = Weather(zoned, 20.0, 0.0, src, extent=(0.0, 0.0, 1.4, 1.3))
+    got = [f.reshape(9, 14, 15) for f in wz.moving_fields(origin, cell, shape, speed, shift)]
+    parts = [Weather(own, 20.0, 0.0, src), Weather(zoned.resolved(a), 20.0, 0.0, src),
+             Weather(zoned.resolved(b), 30.0, 0.0, src)]
+    each = [[f.reshape(9, 14, 15) for f in p.moving_fields(origin, cell, shape, speed, shift)] for p in parts]
+    xs, ys = origin[0] + (np.arange(15) + 0.5) * cell, origin[1] + (np.arange(14) + 0.5) * cell
+    x, y = np.meshgrid(xs, ys)
+    zone = wz.zone_at(np.column_stack([x.ravel(), y.ravel()])).reshape(14, 15)
+    for f in range(3):
+        expect = np.where(zone[None] == 1, each[1][f], np.where(zone[None] == 2, each[2][f], each[0][f]))
+        assert np.array_equal(got[f], expect)
+    # Zone a's wind blows along +y: its columns move along y (above the ground, where the
+    # log profile's wind is none), x only by cos 90 degrees' rounding.
+    fx_a, fy_a = got[1][1:, zone == 1], got[2][1:, zone == 1]
+    assert np.all(fy_a > 0) and np.all(np.abs(fx_a) < 1e-12 * fy_a)
+    lo_z, hi_z, top_z = wz.moving_extremes(origin, cell, shape)
+    assert lo_z == min(p.moving_extremes(origin, cell, shape)[0] for p in parts)
+    assert hi_z == max(p.moving_extremes(origin, cell, shape)[1] for p in parts)
+    assert top_z == max(p.moving_extremes(origin, cell, shape)[2] for p in parts)
+    for point in ([0.3, 0.3, 0.35], [1.0, 1.0, 0.35], [0.05, 1.25, 0.35]):
+        k = int(wz.zone_at([point])[0])
+        assert wz.still_at(point) == parts[k].still_at(point)
+
+
+def test_wind_over_the_scene():
+    """Whether any weather's wind blows between the ground and a height: a temperature
+    gradient alone is no wind; a zone's wind over a calm atmosphere is."""
+    calm = Atmosphere(temperature_gradient_k_m=-0.02)
+    assert not Weather(calm, 20.0, 0.0, [0, 0, 1]).winds_over(50.0)
+    assert Weather(Atmosphere(wind_speed_m_s=2.0), 20.0, 0.0, [0, 0, 1]).winds_over(50.0)
+    gusty = Atmosphere(temperature_gradient_k_m=-0.02, zones=(WeatherZone(polygon=SQUARE, wind_speed_m_s=3.0),))
+    assert Weather(gusty, 20.0, 0.0, [0, 0, 1]).winds_over(50.0)
+    # A measured table whose wind starts above the height asked about.
+    table = ((0.0, 20.0, 0.0), (60.0, 20.0, 0.0), (80.0, 19.0, 4.0))
+    assert not Weather(Atmosphere(table=table), 20.0, 0.0, [0, 0, 1]).winds_over(55.0)
+    assert Weather(Atmosphere(table=table), 20.0, 0.0, [0, 0, 1]).winds_over(70.0)
