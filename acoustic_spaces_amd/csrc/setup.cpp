@@ -1290,16 +1290,21 @@ public:
         std::vector<float> speed(n, 0.0f);
         std::vector<uint8_t> faces(n, 0);  // bit d: the face towards +d is open to the wind
         const std::vector<float> *given[2] = {&flow_x, &flow_y};
-        for (size_t i = 0; i < n; ++i) {
-            if (type[i] == kSolid || depth(i) < least) continue;
-            speed[i] = moving(i) ? std::hypot(flow_x[i], flow_y[i]) : 0.0f;
-            const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
-            for (int d = 0; d < 3; ++d) {
-                const size_t j = i + stride[d];
-                if (type[j] == kSolid || depth(j) < least || blocked_at(i, 2 * d + 1, x, y, z)) continue;
-                if (moving(i) || moving(j)) faces[i] |= static_cast<uint8_t>(1u << d);
+        const size_t T = slabs();
+        // Faces before any weight: a face's flag is read from the cell below it, which may
+        // lie in another slab.
+        in_slabs(T, [&](size_t, size_t z0, size_t z1) {
+            for (size_t i = z0 * sz; i < z1 * sz; ++i) {
+                if (type[i] == kSolid || depth(i) < least) continue;
+                speed[i] = moving(i) ? std::hypot(flow_x[i], flow_y[i]) : 0.0f;
+                const size_t x = i % nx, y = (i / nx) % ny, z = i / sz;
+                for (int d = 0; d < 3; ++d) {
+                    const size_t j = i + stride[d];
+                    if (type[j] == kSolid || depth(j) < least || blocked_at(i, 2 * d + 1, x, y, z)) continue;
+                    if (moving(i) || moving(j)) faces[i] |= static_cast<uint8_t>(1u << d);
+                }
             }
-        }
+        });
         const auto mean = [&](const std::vector<float> &f, size_t i, size_t j) {
             const double sum = static_cast<double>(f[i]) + static_cast<double>(f[j]);
             return moving(i) && moving(j) ? 0.5 * sum : sum;
@@ -1307,90 +1312,125 @@ public:
         const auto through = [&](size_t i, int d) { return d < 2 ? mean(*given[d], i, i + stride[d]) : 0.0; };
         std::vector<double> lambda(n, 0.0), residual(n, 0.0), direction(n, 0.0), image(n, 0.0);
         std::vector<float> diagonal(n, 0.0f);
-        // term(i) summed (or its largest) over the moving cells, plane by plane.
-        const auto over = [&](bool largest, auto &&term) {
-            double total = 0.0;
-            for (size_t z = 0; z < nz; ++z) {
-                double plane = 0.0;
-                for (size_t i = z * sz; i < (z + 1) * sz; ++i) {
-                    if (!moving(i)) continue;
-                    const double v = term(i);
-                    plane = largest ? std::max(plane, v) : plane + v;
-                }
-                total = largest ? std::max(total, plane) : total + plane;
+        // The solve runs on T threads, each on its slab of planes, in step: term(i) summed (or
+        // its largest) over the moving cells of each plane in index order, the planes' sums
+        // in two buffers taken in turn, then every thread takes the planes in order to the
+        // same total. One wait a sum: a thread can be one sum ahead, writing the other buffer.
+        std::vector<double> planes[2] = {std::vector<double>(nz, 0.0), std::vector<double>(nz, 0.0)};
+        std::atomic<size_t> arrived{0}, generation{0};
+        const auto wait_all = [&]() {
+            const size_t g = generation.load(std::memory_order_acquire);
+            if (arrived.fetch_add(1, std::memory_order_acq_rel) + 1 == T) {
+                arrived.store(0, std::memory_order_relaxed);
+                generation.fetch_add(1, std::memory_order_release);
+                return;
             }
-            return total;
+            while (generation.load(std::memory_order_acquire) == g) std::this_thread::yield();
         };
-        const double top = over(true, [&](size_t i) {
-            double flux = 0.0, weights = 0.0;
-            for (int d = 0; d < 3; ++d) {
-                const size_t s = stride[d];
-                if ((faces[i] >> d) & 1u) {
-                    flux += through(i, d);
-                    weights += mean(speed, i, i + s);
-                }
-                if ((faces[i - s] >> d) & 1u) {
-                    flux -= through(i - s, d);
-                    weights += mean(speed, i - s, i);
-                }
-            }
-            residual[i] = weights > 0.0 ? -flux : 0.0;
-            diagonal[i] = static_cast<float>(weights);
-            return static_cast<double>(speed[i]);
-        });
-        const auto scaled = [&](size_t i) {
-            return diagonal[i] > 0.0f ? residual[i] / static_cast<double>(diagonal[i]) : 0.0;
-        };
-        double carried = over(false, [&](size_t i) {
-            direction[i] = scaled(i);
-            return residual[i] * direction[i];
-        });
-        const double enough = 1e-9 * top;
-        double left = over(true, [&](size_t i) { return std::abs(residual[i]); });
         const size_t most = 1000 + 50 * (nx + ny + nz);
-        while (left > enough) {
-            if (++flow_sweeps > most) throw std::runtime_error("the wind round the model's solids did not settle");
-            const double bent = over(false, [&](size_t i) {
-                double sum = 0.0;
+        size_t sweeps = 0;
+        double top = 0.0;  // the fastest given speed
+        in_slabs(T, [&](size_t t, size_t z0, size_t z1) {
+            int turn_of_buffer = 0;
+            const auto over = [&](bool largest, auto &&term) {
+                std::vector<double> &plane_sums = planes[turn_of_buffer];
+                turn_of_buffer ^= 1;
+                for (size_t z = z0; z < z1; ++z) {
+                    double plane = 0.0;
+                    for (size_t i = z * sz; i < (z + 1) * sz; ++i) {
+                        if (!moving(i)) continue;
+                        const double v = term(i);
+                        plane = largest ? std::max(plane, v) : plane + v;
+                    }
+                    plane_sums[z] = plane;
+                }
+                wait_all();
+                double total = 0.0;
+                for (size_t z = 0; z < nz; ++z) total = largest ? std::max(total, plane_sums[z]) : total + plane_sums[z];
+                return total;
+            };
+            const double fastest_given = over(true, [&](size_t i) {
+                double flux = 0.0, weights = 0.0;
                 for (int d = 0; d < 3; ++d) {
                     const size_t s = stride[d];
-                    if ((faces[i] >> d) & 1u) sum += mean(speed, i, i + s) * (direction[i] - direction[i + s]);
-                    if ((faces[i - s] >> d) & 1u) sum += mean(speed, i - s, i) * (direction[i] - direction[i - s]);
+                    if ((faces[i] >> d) & 1u) {
+                        flux += through(i, d);
+                        weights += mean(speed, i, i + s);
+                    }
+                    if ((faces[i - s] >> d) & 1u) {
+                        flux -= through(i - s, d);
+                        weights += mean(speed, i - s, i);
+                    }
                 }
-                image[i] = sum;
-                return direction[i] * sum;
+                residual[i] = weights > 0.0 ? -flux : 0.0;
+                diagonal[i] = static_cast<float>(weights);
+                return static_cast<double>(speed[i]);
             });
-            if (!(bent > 0.0)) break;
-            const double step = carried / bent;
-            left = over(true, [&](size_t i) {
-                lambda[i] += step * direction[i];
-                residual[i] -= step * image[i];
-                return std::abs(residual[i]);
+            const auto scaled = [&](size_t i) {
+                return diagonal[i] > 0.0f ? residual[i] / static_cast<double>(diagonal[i]) : 0.0;
+            };
+            double carried = over(false, [&](size_t i) {
+                direction[i] = scaled(i);
+                return residual[i] * direction[i];
             });
-            const double next = over(false, [&](size_t i) { return residual[i] * scaled(i); });
-            const double turn = next / carried;
-            carried = next;
-            over(false, [&](size_t i) {
-                direction[i] = scaled(i) + turn * direction[i];
-                return 0.0;
-            });
-        }
+            const double enough = 1e-9 * fastest_given;
+            double left = over(true, [&](size_t i) { return std::abs(residual[i]); });
+            size_t mine = 0;
+            while (left > enough) {
+                if (++mine > most) throw std::runtime_error("the wind round the model's solids did not settle");
+                // (direction's neighbours in the slabs beside are final: the last sum's
+                // wait came after every thread wrote its own.)
+                const double bent = over(false, [&](size_t i) {
+                    double sum = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        const size_t s = stride[d];
+                        if ((faces[i] >> d) & 1u) sum += mean(speed, i, i + s) * (direction[i] - direction[i + s]);
+                        if ((faces[i - s] >> d) & 1u) sum += mean(speed, i - s, i) * (direction[i] - direction[i - s]);
+                    }
+                    image[i] = sum;
+                    return direction[i] * sum;
+                });
+                if (!(bent > 0.0)) break;
+                const double step = carried / bent;
+                left = over(true, [&](size_t i) {
+                    lambda[i] += step * direction[i];
+                    residual[i] -= step * image[i];
+                    return std::abs(residual[i]);
+                });
+                const double next = over(false, [&](size_t i) { return residual[i] * scaled(i); });
+                const double turn = next / carried;
+                carried = next;
+                over(false, [&](size_t i) {
+                    direction[i] = scaled(i) + turn * direction[i];
+                    return 0.0;
+                });
+            }
+            if (t == 0) {
+                sweeps = mine;
+                top = fastest_given;
+            }
+        });
+        flow_sweeps = sweeps;
         // The faces' winds and the fastest of them.
         for (int d = 0; d < 3; ++d) flow_wind[static_cast<size_t>(d)].assign(n, 0.0f);
-        double fastest = 0.0;
-        for (size_t z = 0; z < nz; ++z) {
-            double plane = 0.0;
-            for (size_t i = z * sz; i < (z + 1) * sz; ++i) {
-                for (int d = 0; d < 3; ++d) {
-                    if (!((faces[i] >> d) & 1u)) continue;
-                    const size_t j = i + stride[d];
-                    const double u = through(i, d) - mean(speed, i, j) * (lambda[j] - lambda[i]);
-                    flow_wind[static_cast<size_t>(d)][i] = static_cast<float>(u);
-                    plane = std::max(plane, std::abs(u));
+        std::vector<double> &fastest_of = planes[0];
+        in_slabs(T, [&](size_t, size_t z0, size_t z1) {
+            for (size_t z = z0; z < z1; ++z) {
+                double plane = 0.0;
+                for (size_t i = z * sz; i < (z + 1) * sz; ++i) {
+                    for (int d = 0; d < 3; ++d) {
+                        if (!((faces[i] >> d) & 1u)) continue;
+                        const size_t j = i + stride[d];
+                        const double u = through(i, d) - mean(speed, i, j) * (lambda[j] - lambda[i]);
+                        flow_wind[static_cast<size_t>(d)][i] = static_cast<float>(u);
+                        plane = std::max(plane, std::abs(u));
+                    }
                 }
+                fastest_of[z] = plane;
             }
-            fastest = std::max(fastest, plane);
-        }
+        });
+        double fastest = 0.0;
+        for (size_t z = 0; z < nz; ++z) fastest = std::max(fastest, fastest_of[z]);
         flow_fastest = static_cast<float>(fastest);
         // A wind nowhere a millionth of the given one (a closed room's) is none.
         if (static_cast<double>(flow_fastest) <= 1e-6 * top) {
@@ -1405,29 +1445,33 @@ public:
         flow_y.clear();
         flow_x.shrink_to_fit();
         flow_y.shrink_to_fit();
-        for (int d = 0; d < 3; ++d) {
-            std::vector<float> &loop = flow_loop[static_cast<size_t>(d)];
-            const std::vector<float> &w = flow_wind[static_cast<size_t>(d)];
-            loop.assign(n, 0.0f);
-            const size_t s = stride[d];
-            for (size_t i = 0; i < n; ++i) {
-                if (!((faces[i] >> d) & 1u) || !((faces[i + s] >> d) & 1u)) continue;
-                if (!(moving(i) && moving(i + s) && moving(i + 2 * s))) continue;
-                loop[i] = (w[i] + w[i + s]) * (1.0f / 24.0f);
-            }
-        }
-        for (size_t i = 0; i < n; ++i) {
-            if (!moving(i) || depth(i) < 2) continue;
-            bool moved = false, border = false;
+        for (int d = 0; d < 3; ++d) flow_loop[static_cast<size_t>(d)].assign(n, 0.0f);
+        in_slabs(T, [&](size_t, size_t z0, size_t z1) {
             for (int d = 0; d < 3; ++d) {
+                std::vector<float> &loop = flow_loop[static_cast<size_t>(d)];
+                const std::vector<float> &w = flow_wind[static_cast<size_t>(d)];
                 const size_t s = stride[d];
-                const std::vector<float> &w = flow_wind[static_cast<size_t>(d)], &l = flow_loop[static_cast<size_t>(d)];
-                moved = moved || w[i] != 0.0f || w[i - s] != 0.0f || l[i] != 0.0f || l[i - s] != 0.0f ||
-                        l[i - 2 * s] != 0.0f;
-                border = border || (w[i] != 0.0f && !moving(i + s)) || (w[i - s] != 0.0f && !moving(i - s));
+                for (size_t i = z0 * sz; i < z1 * sz; ++i) {
+                    if (!((faces[i] >> d) & 1u) || !((faces[i + s] >> d) & 1u)) continue;
+                    if (!(moving(i) && moving(i + s) && moving(i + 2 * s))) continue;
+                    loop[i] = (w[i] + w[i + s]) * (1.0f / 24.0f);
+                }
             }
-            flow_mask[i] = moved ? (border ? 2 : 1) : 0;
-        }
+        });
+        in_slabs(T, [&](size_t, size_t z0, size_t z1) {
+            for (size_t i = z0 * sz; i < z1 * sz; ++i) {
+                if (!moving(i) || depth(i) < 2) continue;
+                bool moved = false, border = false;
+                for (int d = 0; d < 3; ++d) {
+                    const size_t s = stride[d];
+                    const std::vector<float> &w = flow_wind[static_cast<size_t>(d)], &l = flow_loop[static_cast<size_t>(d)];
+                    moved = moved || w[i] != 0.0f || w[i - s] != 0.0f || l[i] != 0.0f || l[i - s] != 0.0f ||
+                            l[i - 2 * s] != 0.0f;
+                    border = border || (w[i] != 0.0f && !moving(i + s)) || (w[i - s] != 0.0f && !moving(i - s));
+                }
+                flow_mask[i] = moved ? (border ? 2 : 1) : 0;
+            }
+        });
         flow_round = true;
     }
 
