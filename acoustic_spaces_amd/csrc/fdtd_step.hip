@@ -55,6 +55,10 @@ constexpr int kTileRows = 4; // rows (y) of a tile: one wavefront each
 constexpr int kBlock = kTileX * kTileRows;
 constexpr uint32_t kProbesPerBlock = kBlock / kTileX;  // one wavefront a probe
 constexpr uint32_t kXcds = 8;  // MI355X's dies: blocks go to them round robin by index
+// Moving air over plain rows (plain_carry_kernel): a block's tile of a plane, and its frame.
+constexpr int kCarryX = 64, kCarryY = 16, kCarryHalo = 6;
+constexpr int kFrameX = kCarryX + 2 * kCarryHalo, kFrameY = kCarryY + 2 * kCarryHalo;
+constexpr int kFrame = kFrameX * kFrameY;
 constexpr uint32_t kAllInterior = 1u << 31;  // a tile's flag: every cell of it is interior
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;    // a block with no tile
 // Cell classes on the device: the layout's 0 solid, 1 interior, 2 special (here: a general
@@ -2136,7 +2140,9 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         }
         h->flow_t1 = static_cast<float *>(zeros(h, n_pad * sizeof(float)));
         h->flow_t2 = static_cast<float *>(zeros(h, n_pad * sizeof(float)));
-        h->flow_blocks = static_cast<uint32_t>((n_pad + kBlock - 1) / kBlock);
+        // A slot a block of the carrying's launches: a cell a thread, or a plain tile a block.
+        const uint64_t tiles = ((nx + kCarryX - 1) / kCarryX) * ((ny + kCarryY - 1) / kCarryY) * nz;
+        h->flow_blocks = static_cast<uint32_t>(std::max<uint64_t>((n_pad + kBlock - 1) / kBlock, tiles));
         h->flow_squares = static_cast<double *>(zeros(h, h->flow_blocks * sizeof(double)));
         h->flow_total = static_cast<double *>(zeros(h, sizeof(double)));
         for (uint64_t i = 0; i < n; ++i) h->air_cells += L.types[i] != 0;
@@ -2403,6 +2409,131 @@ void carry(const as_fdtd *h, float *q, bool squares, hipStream_t stream) {
     }
 }
 
+// Plain rows, both pressures a step in one launch. The air moves a cell along its row in x and
+// in y only, so each plane is carried on its own. A block takes a tile of kCarryX x kCarryY
+// cells of one plane and its frame: the tile and kCarryHalo cells round it, two for each
+// stage's reach. The frame's pressures, shifts, masks and kinds go to LDS; the first stage is
+// taken over the frame less two cells a side, the second less four, the third over the tile,
+// each from the stage before in LDS. A halo cell is carried by every block whose frame holds
+// it, by the same operations, so each value equals the one-stage-a-launch carrying's.
+struct PlainCarry {
+    const uint8_t *__restrict__ types;
+    const uint8_t *__restrict__ mask;
+    const float *__restrict__ fx;
+    const float *__restrict__ fy;
+    float *a, *b;     // p(k - 1) and p(k), carried in place
+    double *squares;  // a sum a block: b's squares over the tile's air as they come
+    uint32_t nx, ny, tiles_x, tiles_y;
+    int px;
+    long long sxy;
+};
+
+__global__ __launch_bounds__(kBlock) void plain_carry_kernel(const PlainCarry c) {
+    __shared__ float q[kFrame], t1[kFrame], t2[kFrame], sx[kFrame], sy[kFrame];
+    __shared__ uint8_t m[kFrame], kind[kFrame];  // kind: 0 solid or off the grid, 1 still, 2 moved
+    __shared__ double part[kBlock];
+    const uint32_t per_plane = c.tiles_x * c.tiles_y;
+    const uint32_t z = blockIdx.x / per_plane, r0 = blockIdx.x % per_plane;
+    const int x0 = static_cast<int>(r0 % c.tiles_x) * kCarryX - kCarryHalo;
+    const int y0 = static_cast<int>(r0 / c.tiles_x) * kCarryY - kCarryHalo;
+    const long long base = static_cast<long long>(z) * c.sxy;
+    // The frame's cell f, at (x0 + f % kFrameX, y0 + f / kFrameX): its index, -1 off the grid.
+    const auto cell = [&](int f) -> long long {
+        const int x = x0 + f % kFrameX, y = y0 + f / kFrameX;
+        if (x < 0 || y < 0 || x >= static_cast<int>(c.nx) || y >= static_cast<int>(c.ny)) return -1;
+        return base + static_cast<long long>(y) * c.px + x;
+    };
+    for (int f = threadIdx.x; f < kFrame; f += kBlock) {
+        const long long i = cell(f);
+        uint8_t k = 0, bits = 0;
+        float vx = 0.0f, vy = 0.0f;
+        if (i >= 0 && c.types[i] != 0) {
+            vx = c.fx[i];
+            vy = c.fy[i];
+            bits = c.mask[i];
+            k = vx == 0.0f && vy == 0.0f ? 1 : 2;
+        }
+        kind[f] = k;
+        m[f] = bits;
+        sx[f] = vx;
+        sy[f] = vy;
+    }
+    // -v . grad p at frame cell f, over a step (carried_plain's operations).
+    const auto carried = [&](const float *p, int f) {
+        const unsigned bits = m[f];
+        return -(sx[f] * row_gradient(p, f, 1, bits & 15u, sx[f]) + sy[f] * row_gradient(p, f, kFrameX, bits >> 4, sy[f]));
+    };
+    for (int field = 0; field < 2; ++field) {
+        float *p = field == 0 ? c.b : c.a;
+        __syncthreads();  // the frame's kinds are in; the field before has left q, t1, t2
+        for (int f = threadIdx.x; f < kFrame; f += kBlock) {
+            const long long i = cell(f);
+            q[f] = i >= 0 ? p[i] : 0.0f;
+        }
+        __syncthreads();
+        if (field == 0) {
+            double sum = 0.0;
+            for (int r = threadIdx.x; r < kCarryX * kCarryY; r += kBlock) {
+                const int f = (kCarryHalo + r / kCarryX) * kFrameX + kCarryHalo + r % kCarryX;
+                if (kind[f] != 0) sum += static_cast<double>(q[f]) * static_cast<double>(q[f]);
+            }
+            part[threadIdx.x] = sum;
+            __syncthreads();
+#pragma unroll
+            for (unsigned w = kBlock / 2; w > 0; w >>= 1) {
+                if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) c.squares[blockIdx.x] += part[0];
+        }
+        constexpr int w1 = kFrameX - 4, h1 = kFrameY - 4;
+        for (int r = threadIdx.x; r < w1 * h1; r += kBlock) {
+            const int f = (2 + r / w1) * kFrameX + 2 + r % w1;
+            if (kind[f] == 0) continue;
+            t1[f] = kind[f] == 1 ? q[f] : q[f] + carried(q, f);
+        }
+        __syncthreads();
+        constexpr int w2 = kFrameX - 8, h2 = kFrameY - 8;
+        for (int r = threadIdx.x; r < w2 * h2; r += kBlock) {
+            const int f = (4 + r / w2) * kFrameX + 4 + r % w2;
+            if (kind[f] == 0) continue;
+            t2[f] = kind[f] == 1 ? q[f] : 0.75f * q[f] + 0.25f * (t1[f] + carried(t1, f));
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < kCarryX * kCarryY; r += kBlock) {
+            const int f = (kCarryHalo + r / kCarryX) * kFrameX + kCarryHalo + r % kCarryX;
+            if (kind[f] != 2) continue;
+            p[cell(f)] = q[f] + (2.0f / 3.0f) * ((t2[f] + carried(t2, f)) - q[f]);
+        }
+    }
+}
+
+// Both pressures carried a step: plain rows by tiles of planes in one launch (unless the
+// launch shape is split), else three launches of a stage each per pressure; b's squares
+// summed on the way.
+void carry_both(const as_fdtd *h, float *a, float *b, hipStream_t stream) {
+    if (h->flow_round || h->split) {
+        carry(h, b, true, stream);
+        carry(h, a, false, stream);
+        return;
+    }
+    PlainCarry c{};
+    c.types = h->types;
+    c.mask = h->flow_mask;
+    c.fx = h->flow_x;
+    c.fy = h->flow_y;
+    c.a = a;
+    c.b = b;
+    c.squares = h->flow_squares;
+    c.nx = h->nx;
+    c.ny = h->ny;
+    c.tiles_x = (h->nx + kCarryX - 1) / kCarryX;
+    c.tiles_y = (h->ny + kCarryY - 1) / kCarryY;
+    c.px = h->px;
+    c.sxy = h->sxy;
+    hipLaunchKernelGGL(plain_carry_kernel, dim3(c.tiles_x * c.tiles_y * h->nz), dim3(kBlock), 0, stream, c);
+}
+
 __global__ __launch_bounds__(kBlock) void fill_kernel(double *p, uint32_t count) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < count) p[i] = 0.0;
@@ -2490,8 +2621,7 @@ void launch_one(as_fdtd *h, StepArgs<T> &s, uint32_t blocks, int prev, unsigned 
             // air (fdtd.hpp:838-846), then the step, without its probe blocks.
             const uint32_t probe_blocks = (h->probes + kProbesPerBlock - 1) / kProbesPerBlock;
             if (probe_blocks) hipLaunchKernelGGL((probe_kernel<T>), dim3(probe_blocks), dim3(kBlock), 0, stream, s);
-            carry(h, static_cast<float *>(h->field[prev ^ 1]), true, stream);
-            carry(h, static_cast<float *>(h->field[prev]), false, stream);
+            carry_both(h, static_cast<float *>(h->field[prev]), static_cast<float *>(h->field[prev ^ 1]), stream);
             StepArgs<T> t = s;
             const uint32_t rest = blocks - t.probe_blocks;
             t.probe_blocks = 0;
