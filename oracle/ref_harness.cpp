@@ -148,6 +148,46 @@ void asref_flow(void *handle, double *out) {
     out[4] = s.mean_square();
 }
 
+// Fitted walls: the fit's rows (CutCells::Results, cut_cells.hpp:1249-1265) as the solver
+// keeps them (fdtd.hpp:723). Returns the row count (-1 without fitted walls); where the
+// pointers are not null, copies the rows' cells (n), air volumes (n), open areas (18 n), wall
+// areas (n (materials + 1)) and slivers (6 n), and a few of the fit's counts into stats:
+// cubes, regions, ties, farthest, conduit links, conduit scaled, far volume, diagonal area,
+// dropped area, edge area, air volume, least conduit factor, conduit wall moved.
+long asref_cut_rows(void *handle, std::int64_t *cells, double *volume, double *open, double *wall, double *sliver,
+                    double *stats) {
+    const as::CutCells::Results *cut = static_cast<Handle *>(handle)->solver->cut_cells();
+    if (cut == nullptr) return -1;
+    const std::size_t n = cut->cells().size();
+    auto put = [](auto *dst, const auto &src) {
+        if (dst != nullptr && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    put(cells, cut->cells());
+    put(volume, cut->air_volume());
+    put(open, cut->open_area());
+    put(wall, cut->wall_area());
+    put(sliver, cut->sliver());
+    if (stats != nullptr) {
+        const auto &s = cut->stats();
+        const double v[] = {static_cast<double>(s.cubes), static_cast<double>(s.regions), static_cast<double>(s.ties),
+                            static_cast<double>(s.farthest), static_cast<double>(s.conduit_links),
+                            static_cast<double>(s.conduit_scaled), s.far_volume, s.diagonal_area, s.dropped_area,
+                            s.edge_area, s.air_volume, s.conduit_least, s.conduit_wall_moved};
+        std::memcpy(stats, v, sizeof v);
+    }
+    return static_cast<long>(n);
+}
+
+// Fitted walls: the solver's counts of the sliver rule and the clamp (fdtd.hpp:1294-1301):
+// raised, most raised, sliver links, both, thick, layer whole links, plain rows.
+void asref_fitted_counts(void *handle, double *out) {
+    const auto &c = static_cast<Handle *>(handle)->solver->fitted_counts();
+    const double v[] = {static_cast<double>(c.raised), c.most_raised, static_cast<double>(c.sliver_links),
+                        static_cast<double>(c.sliver_both), static_cast<double>(c.sliver_thick),
+                        static_cast<double>(c.layer_whole_links), static_cast<double>(c.plain_rows)};
+    std::memcpy(out, v, sizeof v);
+}
+
 void asref_open_sides(void *handle, int *out) {
     const auto &sides = static_cast<Handle *>(handle)->solver->open_sides();
     for (int k = 0; k < 6; ++k) out[k] = sides[static_cast<std::size_t>(k)] ? 1 : 0;

@@ -151,6 +151,36 @@ class ReferenceSolver:
         return {"flowing": bool(out[0]), "wind_round": bool(out[1]), "wind_fastest": float(out[2]),
                 "wind_sweeps": int(out[3]), "mean_square": float(out[4])}
 
+    _CUT_STATS = ("cubes", "regions", "ties", "farthest", "conduit_links", "conduit_scaled", "far_volume",
+                  "diagonal_area", "dropped_area", "edge_area", "air_volume", "conduit_least",
+                  "conduit_wall_moved")
+    _FITTED_COUNTS = ("raised", "most_raised", "sliver_links", "sliver_both", "sliver_thick",
+                      "layer_whole_links", "plain_rows")
+
+    def cut_rows(self, materials: int) -> dict | None:
+        """Fitted walls: the fit's rows as the solver keeps them (cut_cells.hpp:1249-1265):
+        cells, air volumes, open areas (n x 18), wall areas (n x (materials + 1)), slivers
+        (n x 6), the fit's counts and the solver's counts of the sliver rule; None without
+        fitted walls."""
+        lib = _load()
+        P = ctypes.c_void_p
+        lib.asref_cut_rows.restype = ctypes.c_long
+        lib.asref_cut_rows.argtypes = [P, P, P, P, P, P, P]
+        lib.asref_fitted_counts.argtypes = [P, P]
+        n = lib.asref_cut_rows(self._h, None, None, None, None, None, None)
+        if n < 0:
+            return None
+        out = {"cells": np.zeros(n, np.int64), "air_volume": np.zeros(n), "open_area": np.zeros((n, 18)),
+               "wall_area": np.zeros((n, materials + 1)), "sliver": np.zeros((n, 6))}
+        stats = np.zeros(len(self._CUT_STATS))
+        lib.asref_cut_rows(self._h, _ptr(out["cells"]), _ptr(out["air_volume"]), _ptr(out["open_area"]),
+                           _ptr(out["wall_area"]), _ptr(out["sliver"]), _ptr(stats))
+        counts = np.zeros(len(self._FITTED_COUNTS))
+        lib.asref_fitted_counts(self._h, _ptr(counts))
+        out["stats"] = dict(zip(self._CUT_STATS, stats.tolist()))
+        out["counts"] = dict(zip(self._FITTED_COUNTS, counts.tolist()))
+        return out
+
     def probe_found(self, p) -> bool:
         return bool(_load().asref_probe_found(self._h, p))
 
