@@ -19,7 +19,7 @@ CSRC = PACKAGE / "csrc"
 LIBRARY = PACKAGE / "libasfdtd.so"
 
 HIP_SOURCES = ["fdtd_step.hip", "responses.hip", "setup_device.hip"]
-HOST_SOURCES = ["setup.cpp"]
+HOST_SOURCES = ["setup.cpp", "fitted.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AS_OFFLOAD_ARCH", "gfx950")

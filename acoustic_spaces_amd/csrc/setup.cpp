@@ -58,6 +58,7 @@
 #include <vector>
 
 #include "asfdtd.h"
+#include "fitted.h"
 #include "setup_device.h"
 
 namespace {
@@ -310,6 +311,28 @@ public:
         state_start.push_back(state_branch.size());
     }
 
+    // Appends a cell whose stored weights and coefficients are given (fitted walls): a link
+    // for each direction whose area is positive.
+    void append_given(size_t i, const double *area, const float *weight, float a, float b, float inv,
+                      const State *states, size_t nstates) {
+        cell.push_back(static_cast<int64_t>(i));
+        for (int k = 0; k < 18; ++k) {
+            if (!(area[k] > 0.0)) continue;
+            link_direction.push_back(static_cast<uint8_t>(k));
+            link_weight.push_back(weight[k]);
+        }
+        link_start.push_back(link_direction.size());
+        coef_a.push_back(a);
+        coef_b.push_back(b);
+        coef_inv.push_back(inv);
+        source.push_back(0.0f);
+        for (size_t q = 0; q < nstates; ++q) {
+            state_branch.push_back(states[q].branch);
+            state_weight.push_back(states[q].weight);
+        }
+        state_start.push_back(state_branch.size());
+    }
+
     // Appends another list's cells (their ranges shifted by this list's counts).
     void extend(const SpecialList &o) {
         const uint64_t links0 = link_direction.size(), states0 = state_branch.size();
@@ -429,6 +452,18 @@ public:
     size_t air = 0, boundary_cells = 0;
     bool open = false;
     std::array<bool, 6> open_sides{};
+    // Fitted walls: the mesh as given, each special's volume (cells^3), whether the source
+    // lies on a face.
+    bool fitted_walls = false;
+    fitted::Mesh mesh;
+    std::vector<double> special_volume;
+    bool source_on_face = false;
+    // The fit's rows and cell kinds, kept for cut_cells() (the binding's, native_module.cpp:
+    // 1614-1632) and its counts.
+    std::vector<int64_t> cut_rows;
+    std::vector<double> cut_air_volume, cut_open_area, cut_wall_area, cut_sliver;
+    std::vector<uint8_t> cut_kind;
+    fitted::FitStats cut_stats;
 
     // triangle bins for crossing queries
     size_t bin = 8, bx = 0, by = 0, bz = 0;
@@ -1605,6 +1640,409 @@ public:
         boundary_cells = sp.size();
     }
 
+    // --- fitted walls ----------------------------------------------------------------------
+
+    // The sliver rule's factor for a sliver f cells thick, lambda = 1 + f:
+    // 3 lambda / ((lambda + 1/2)(lambda + 1)), at most 1 (fdtd.hpp:950-955).
+    static double sliver_factor(double f) {
+        const double lambda = 1.0 + f;
+        return 3.0 * lambda / ((lambda + 0.5) * (lambda + 1.0));
+    }
+
+    // A cell's reading on the fit (the fitted stencil, fdtd.hpp:1455-1510): the eight cells
+    // round the located point that `keep` takes and whose centres it reaches crossing no
+    // face, weighed trilinearly; along an axis where the point is tied to a centre, the
+    // cells past it only where none before it is seen; scaled to sum to one. `edge` is set
+    // where the point lies within a cell of the grid's edge.
+    template <typename Keep>
+    std::vector<std::pair<size_t, double>> seen_fitted(const fitted::Fit &fit, const fitted::Located &L, Keep keep,
+                                                       bool *at_edge = nullptr, std::vector<int64_t> *crossed = nullptr) const {
+        std::vector<std::pair<size_t, double>> outv;
+        const long long size[3] = {static_cast<long long>(nx), static_cast<long long>(ny), static_cast<long long>(nz)};
+        for (int a = 0; a < 3; ++a) {
+            const long long j = L.j[a];
+            if (!(j >= 1 && (j < size[a] - 3 || (j == size[a] - 3 && L.tied[a])))) {
+                if (at_edge) *at_edge = true;
+                return outv;
+            }
+        }
+        struct Seen {
+            size_t cell;
+            double w;
+            int past;
+        };
+        Seen seen[8];
+        int count = 0, least = 0;
+        for (int corner = 0; corner < 8; ++corner) {
+            double w = 1.0;
+            int past = 0;
+            size_t i = 0, stride = 1;
+            for (int a = 0; a < 3; ++a) {
+                const int up = (corner >> a) & 1;
+                if (!L.tied[a]) w *= up ? L.lo[a] : L.hi[a];
+                else if (up) past += 1 << (2 * (a + 1));
+                i += static_cast<size_t>(L.j[a] + up) * stride;
+                stride *= a == 0 ? nx : (a == 1 ? ny : 1);
+            }
+            if (w <= 0.0 || !keep(i)) continue;
+            int64_t c[3];
+            fit.centre_q(i, c);
+            const int64_t f = fit.way(L.q, c);
+            if (f >= 0) {
+                if (crossed) crossed->push_back(f);
+                continue;
+            }
+            if (count == 0 || past < least) least = past;
+            seen[count++] = {i, w, past};
+        }
+        double total = 0.0;
+        for (int k = 0; k < count; ++k) {
+            if (seen[k].past != least) continue;
+            outv.emplace_back(seen[k].cell, seen[k].w);
+            total += seen[k].w;
+        }
+        for (auto &cw : outv) cw.second /= total;
+        return outv;
+    }
+
+    // The reading on the fit (fdtd.hpp:1515-1598): the cubic block of the located point's
+    // four planes per axis where its cells are air, seen and linked in the block, else
+    // linear or single along an axis, else trilinear (seen_fitted). Each weight factor
+    // x - m is formed as lo - (m - 1), so a tied point weighs 0 exactly.
+    std::vector<std::pair<size_t, double>> reading_fitted(const fitted::Fit &fit, const fitted::Located &L, bool cubic) const {
+        const auto visible = [&]() { return seen_fitted(fit, L, [&](size_t i) { return fit.kind[i] != fitted::kSolid; }); };
+        if (!cubic) return visible();
+        const long long size[3] = {static_cast<long long>(nx), static_cast<long long>(ny), static_cast<long long>(nz)};
+        long long first[3];
+        double w[3][kReach];
+        for (int a = 0; a < 3; ++a) {
+            const long long j = L.j[a];
+            if (!(j >= 1 && j < size[a] - 3)) return visible();
+            first[a] = j - 1;
+            for (int k = 0; k < kReach; ++k) {
+                double v = 1.0;
+                for (int m = 0; m < kReach; ++m)
+                    if (m != k) v *= (L.lo[a] - static_cast<double>(m - 1)) / static_cast<double>(k - m);
+                w[a][k] = v;
+            }
+        }
+        const auto index = [&](int x, int y, int z) {
+            return static_cast<size_t>(first[0] + x) + static_cast<size_t>(first[1] + y) * sy +
+                   static_cast<size_t>(first[2] + z) * sz;
+        };
+        const auto clear = [&](size_t i, size_t j) {
+            int64_t a[3], b[3];
+            fit.centre_q(i, a);
+            fit.centre_q(j, b);
+            return fit.way(a, b) < 0;
+        };
+        uint64_t own = 0;
+        for (int z = 0; z < kReach; ++z)
+            for (int y = 0; y < kReach; ++y)
+                for (int x = 0; x < kReach; ++x) {
+                    const size_t i = index(x, y, z);
+                    if (fit.kind[i] != fitted::kAir) continue;
+                    int64_t c[3];
+                    fit.centre_q(i, c);
+                    if (fit.way(L.q, c) < 0) own |= block_bit(x, y, z);
+                }
+        unsigned kept[3] = {0xFu, 0xFu, 0xFu};
+        for (bool shrunk = true; shrunk;) {
+            shrunk = false;
+            for (int a = 0; a < 3; ++a) {
+                const uint64_t across = own & block_slab((a + 1) % 3, kept[(a + 1) % 3]) & block_slab((a + 2) % 3, kept[(a + 2) % 3]);
+                for (int j = 0; j < kReach; ++j) {
+                    if (((kept[a] >> j) & 1u) && !(across & block_slab(a, 1u << j))) {
+                        kept[a] &= ~(1u << j);
+                        shrunk = true;
+                    }
+                }
+            }
+        }
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            if (kept[a] == 0xFu) {
+                lo[a] = 0;
+                hi[a] = kReach;
+            } else if ((kept[a] & 0x6u) == 0x6u) {
+                lo[a] = 1;
+                hi[a] = 3;
+                w[a][1] = L.hi[a];
+                w[a][2] = L.lo[a];
+            } else if (kept[a] & 0x6u) {
+                lo[a] = (kept[a] & 0x2u) ? 1 : 2;
+                hi[a] = lo[a] + 1;
+                w[a][lo[a]] = 1.0;
+            } else {
+                return visible();
+            }
+        }
+        std::vector<std::pair<size_t, double>> outv;
+        for (int z = lo[2]; z < hi[2]; ++z)
+            for (int y = lo[1]; y < hi[1]; ++y)
+                for (int x = lo[0]; x < hi[0]; ++x) {
+                    const size_t i = index(x, y, z);
+                    if (!(own & block_bit(x, y, z)) || (x + 1 < hi[0] && !clear(i, i + 1)) ||
+                        (y + 1 < hi[1] && !clear(i, i + sy)) || (z + 1 < hi[2] && !clear(i, i + sz))) {
+                        return visible();
+                    }
+                    const double wi = w[0][x] * w[1][y] * w[2][z];
+                    if (wi != 0.0) outv.emplace_back(i, wi);
+                }
+        return outv;
+    }
+
+    // The layout on fitted walls (set_up_fitted, fdtd.hpp:970-1217; add_special_fitted
+    // 1221-1264): cells' kinds from the fit; every row, and every plain cell the sliver rule
+    // reaches, a special cell with coefficients of its own (a row whose coefficients are a
+    // plain cell's to the bit stays plain); the source's and probes' cells from the fit.
+    void build_fitted(const V3 &src, const std::vector<V3> &probes, size_t per_listener) {
+        SetupClock clock;
+        fitted::GridSpec g;
+        g.origin[0] = origin.x;
+        g.origin[1] = origin.y;
+        g.origin[2] = origin.z;
+        g.cell = cell;
+        g.n[0] = nx;
+        g.n[1] = ny;
+        g.n[2] = nz;
+        g.layer = layer;
+        const double source[3] = {src.x, src.y, src.z};
+        const fitted::Fit fit(mesh, face_material.data(), materials, g, source, threads);
+        clock.mark("fit");
+        type.assign(n, kSolid);
+        air = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (fit.kind[i] == fitted::kAir) {
+                type[i] = kInterior;
+                ++air;
+            } else if (fit.kind[i] == fitted::kLayer) {
+                type[i] = kLayer;
+                layer_cells.push_back(static_cast<int64_t>(i));
+                ++air;
+            }
+        }
+        open_sides = fit.open_sides;
+        open = false;
+        for (const bool b : open_sides) open = open || b;
+        layer_profile();
+
+        const std::vector<int64_t> &rows = fit.rows;
+        const size_t columns = materials + 1;
+        const auto row_of = [&rows](size_t i) -> size_t {
+            const auto it = std::lower_bound(rows.begin(), rows.end(), static_cast<int64_t>(i));
+            return it != rows.end() && *it == static_cast<int64_t>(i) ? static_cast<size_t>(it - rows.begin()) : npos;
+        };
+        // The sliver rule: a link through the face opposite a sliver of thickness f > 0 that
+        // a row owns is scaled by w(f) on both sides; where both its cells' rules touch it,
+        // by the product of their factors.
+        struct Touch {
+            size_t key;  // 3 (the link's lower cell) + its axis
+            double factor;
+        };
+        std::vector<Touch> touches;
+        for (size_t r = 0; r < rows.size(); ++r) {
+            const auto o = static_cast<size_t>(rows[r]);
+            for (int k = 0; k < 6; ++k) {
+                const double f = fit.sliver[6 * r + static_cast<size_t>(k)];
+                if (!(f > 0.0)) continue;
+                const size_t m = neighbour(o, st.d[static_cast<size_t>(k)].opposite);
+                if (edge(m) || type[m] == kSolid || type[m] == kLayer) continue;
+                touches.push_back({3 * std::min(o, m) + static_cast<size_t>(k / 2), sliver_factor(f)});
+            }
+        }
+        std::sort(touches.begin(), touches.end(), [](const Touch &x, const Touch &y) {
+            return x.key != y.key ? x.key < y.key : x.factor < y.factor;
+        });
+        std::vector<std::pair<size_t, double>> factors;
+        std::vector<size_t> far_cells;
+        for (size_t t = 0; t < touches.size();) {
+            const size_t key = touches[t].key;
+            double product = 1.0;
+            for (; t < touches.size() && touches[t].key == key; ++t) product *= touches[t].factor;
+            factors.emplace_back(key, product);
+            const size_t axis = key % 3, lower = key / 3, upper = lower + (axis == 0 ? 1 : (axis == 1 ? sy : sz));
+            for (const size_t c : {lower, upper})
+                if (row_of(c) == npos) far_cells.push_back(c);
+        }
+        std::sort(far_cells.begin(), far_cells.end());
+        far_cells.erase(std::unique(far_cells.begin(), far_cells.end()), far_cells.end());
+        const auto factor_of = [&factors](size_t i, size_t j, int axis) {
+            const size_t key = 3 * std::min(i, j) + static_cast<size_t>(axis);
+            const auto it = std::lower_bound(factors.begin(), factors.end(), std::make_pair(key, -1.0));
+            return it != factors.end() && it->first == key ? it->second : 1.0;
+        };
+        const auto room = [this](size_t i) { return type[i] == kInterior || type[i] == kSpecial; };
+        // A cell's open area towards direction k, after the sliver rule.
+        const auto area_towards = [&](size_t i, int k) {
+            const size_t r = row_of(i);
+            const size_t j = neighbour(i, k);
+            double a = 0.0;
+            if (r != npos) a = fit.open_area[18 * r + static_cast<size_t>(k)];
+            else if (k < 6 && type[i] != kSolid && !edge(j) && type[j] != kSolid) a = fit.face_area(i, k / 2);
+            if (k < 6 && a > 0.0 && room(i) && room(j)) a *= factor_of(i, j, k / 2);
+            return a;
+        };
+        const double exact = st.d[0].exact, plane = st.d[0].wall, half_g = exact / 2.0;
+        struct FittedCell {
+            double volume = 1.0, air_volume = 1.0, wall = 0.0;
+            std::array<double, 18> area{};
+            std::vector<std::pair<int32_t, double>> states;
+        };
+        const auto make = [&](size_t i) {
+            FittedCell c;
+            const size_t r = row_of(i);
+            c.air_volume = r != npos ? fit.air_volume[r] : fit.cell_volume(i);
+            for (int k = 0; k < 18; ++k) c.area[static_cast<size_t>(k)] = area_towards(i, k);
+            if (r != npos) {
+                for (size_t m = 0; m + 1 < columns; ++m) {
+                    const double W = fit.wall_area[columns * r + m];
+                    if (!(W > 0.0)) continue;
+                    const double weight = plane * W;
+                    c.wall += weight * beta[m];
+                    for (const int32_t j : branches.of_material[m]) {
+                        c.wall += weight * (0.5 / branches.a[static_cast<size_t>(j)]);
+                        c.states.emplace_back(j, weight);
+                    }
+                }
+                const double edge_wall = fit.wall_area[columns * r + columns - 1];
+                if (edge_wall > 0.0) c.wall += plane * edge_wall * edge_admittance;
+            }
+            double total = 0.0;
+            for (const double a : c.area) total += a;
+            c.volume = std::max(c.air_volume, half_g * total);
+            return c;
+        };
+        // Makes cell i special with c's coefficients, unless `spare` and they are a plain
+        // cell's to the bit; returns whether it did.
+        const auto add = [&](size_t i, const FittedCell &c, bool spare) {
+            const double s = cell_scale(i), r = std::sqrt(s) / c.volume;
+            float weight[18] = {};
+            double total = 0.0;
+            for (int k = 0; k < 18; ++k) {
+                const double a = c.area[static_cast<size_t>(k)];
+                if (!(a > 0.0)) continue;
+                weight[k] = lowered(s * (exact * a / c.volume));
+                total += static_cast<double>(weight[k]);
+            }
+            const double centre = 2.0 - total;
+            const float coef_a = down(centre);
+            const auto coef_b = static_cast<float>(1.0 - r * c.wall);
+            const auto coef_inv = static_cast<float>(1.0 / (1.0 + r * c.wall));
+            if (spare && scale.empty() && c.states.empty() && coef_a == st.centre && coef_b == 1.0f && coef_inv == 1.0f) {
+                bool plain = true;
+                for (int k = 0; k < 18 && plain; ++k) plain = weight[k] == (k < 6 ? weights[static_cast<size_t>(k)] : 0.0f);
+                if (plain) return false;
+            }
+            type[i] = kSpecial;
+            std::vector<SpecialList::State> states;
+            for (const auto &[j, w] : c.states) states.push_back({j, static_cast<float>(r * w)});
+            specials.append_given(i, c.area.data(), weight, coef_a, coef_b, coef_inv, states.data(), states.size());
+            special_volume.push_back(c.volume);
+            return true;
+        };
+        std::vector<size_t> candidates(far_cells);
+        for (const int64_t i : rows) candidates.push_back(static_cast<size_t>(i));
+        std::sort(candidates.begin(), candidates.end());
+        for (const size_t i : candidates) add(i, make(i), true);
+        specials.sorted = specials.size();
+        boundary_cells = specials.size();
+        clock.mark("fitted specials");
+
+        // The source: each of its cells fed w / V.
+        const fitted::Located S = fit.locate(&source[0]);
+        source_on_face = fit.on_face(S.q);
+        for (const auto &cw : seen_fitted(fit, S, [&](size_t i) { return fit.kind[i] != fitted::kSolid; }))
+            if (fit.kind[cw.first] == fitted::kLayer) throw std::invalid_argument("fitted walls: the source is in the matched layer");
+        const auto feed = reading_fitted(fit, S, true);
+        if (feed.empty()) {
+            bool at_edge = false;
+            std::vector<int64_t> crossed;
+            seen_fitted(fit, S, [&](size_t i) { return fit.kind[i] != fitted::kSolid; }, &at_edge, &crossed);
+            if (at_edge) {
+                throw std::invalid_argument("fitted walls: the source lies within " + std::to_string(kReach / 2) +
+                                            " cells of the grid's edge, where the solver has no cells round it: move it "
+                                            "inward or enlarge the grid.");
+            }
+            throw std::invalid_argument("fitted walls: no cell centre round the source can be reached from it without "
+                                        "crossing the model: the source lies in a space narrower than a cell. Move it, "
+                                        "or use a finer grid.");
+        }
+        std::array<double, 18> stencil_weights{};
+        for (int k = 0; k < 18; ++k) stencil_weights[static_cast<size_t>(k)] = st.d[static_cast<size_t>(k)].exact;
+        for (const auto &[i, w] : feed) {
+            size_t k = specials.index_of(i);
+            if (k == npos) {
+                if (row_of(i) != npos) {
+                    add(i, make(i), false);
+                } else {
+                    type[i] = kSpecial;
+                    specials.append(i, stencil_weights.data(), st.count, cell_scale(i), 0.0, nullptr, 0);
+                    special_volume.push_back(fit.cell_volume(i));
+                }
+                k = specials.size() - 1;
+            }
+            specials.source[k] += static_cast<float>(cell_scale(i) * w / special_volume[k]);
+        }
+        // The probes: cubic, or trilinear within three cells of the source; whether each is
+        // in sight of its listener.
+        std::vector<fitted::Located> at;
+        for (const V3 &p : probes) {
+            const bool close = norm(p - src) < (kNear - 1e-9) * cell;
+            const double q[3] = {p.x, p.y, p.z};
+            at.push_back(fit.locate(q));
+            const auto cells = reading_fitted(fit, at.back(), !close);
+            for (const auto &[i, w] : cells) {
+                probe_cells.push_back(i);
+                probe_weights.push_back(w);
+            }
+            probe_start.push_back(probe_cells.size());
+            probe_found.push_back(cells.empty() ? 0 : 1);
+        }
+        for (size_t k = 0; k < probes.size(); ++k) {
+            const size_t l = k - k % per_listener;
+            if (k == l) {
+                probe_in_sight.push_back(1);
+            } else if (!fit.in_grid_box(at[l].q) || !fit.in_grid_box(at[k].q)) {
+                probe_in_sight.push_back(0);
+            } else {
+                probe_in_sight.push_back(fit.way(at[l].q, at[k].q) < 0 ? 1 : 0);
+            }
+        }
+        clock.mark("fitted source, probes");
+        cut_rows = fit.rows;
+        cut_air_volume = fit.air_volume;
+        cut_open_area = fit.open_area;
+        cut_wall_area = fit.wall_area;
+        cut_sliver = fit.sliver;
+        cut_kind = fit.kind;
+        cut_stats = fit.stats;
+
+        // The links' self-check: each link stands for one area from both of its cells,
+        // reaches only air, is a whole face towards the matched layer, and a cell's links
+        // weigh at most 2 s together.
+        for (size_t k = 0; k < specials.size(); ++k) {
+            const auto i = static_cast<size_t>(specials.cell[k]);
+            double sum = 0.0;
+            for (uint64_t l = specials.link_start[k]; l < specials.link_start[k + 1]; ++l) {
+                const int d = specials.link_direction[l];
+                const size_t j = neighbour(i, d);
+                sum += static_cast<double>(specials.link_weight[l]);
+                const double here = area_towards(i, d), there = area_towards(j, st.d[static_cast<size_t>(d)].opposite);
+                const char *fault = nullptr;
+                if (edge(j) || type[j] == kSolid) fault = "a link reaches a cell without air";
+                else if (std::abs(here - there) > 1e-12 * std::max(here, there)) fault = "a link stands for different areas from its two cells";
+                else if (type[j] == kLayer && (d >= 6 || std::abs(here - fit.face_area(i, d / 2)) > 1e-12))
+                    fault = "a link to the matched layer is not a whole face";
+                if (fault) throw std::logic_error(std::string("fitted walls: self-check of the links failed at cell ") + std::to_string(i) + ": " + fault);
+            }
+            if (sum > 2.0 * cell_scale(i)) {
+                throw std::logic_error("fitted walls: self-check of the links failed at cell " + std::to_string(i) +
+                                       ": its links weigh more than 2");
+            }
+        }
+    }
+
     void build(const double *mrq, const int64_t *branch_material, size_t branch_count, const V3 &src,
                const std::vector<V3> &probes, size_t per_listener, const float *speed, const float *fx,
                const float *fy) {
@@ -1623,6 +2061,25 @@ public:
                 }
                 scale[i] = static_cast<float>(r * r);
             }
+        }
+        if (fitted_walls) {
+            // The reference's own refusals (fdtd.hpp:429-445), then this set-up's.
+            if (points != 7) {
+                throw std::invalid_argument("fitted walls: they need the 7-point stencil, and the 19-point one was "
+                                            "asked for: set the stencil to 7, or take walls on cell faces");
+            }
+            if (fx != nullptr) {
+                throw std::invalid_argument("fitted walls: they are not available in moving air: leave the air's "
+                                            "motion out, or take walls on cell faces");
+            }
+            if (speed != nullptr) {
+                throw std::invalid_argument("fitted walls: they take one speed of sound for the whole grid, and a "
+                                            "speed per cell was given: leave the speed of each cell out, or take "
+                                            "walls on cell faces");
+            }
+            if (on_device) throw std::invalid_argument("fitted walls: the fit runs on the host only");
+            build_fitted(src, probes, per_listener);
+            return;
         }
         SetupClock clock;
         make_bins();
@@ -1704,13 +2161,15 @@ as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *f
     try {
         Setup &s = h->s;
         if (nf == 0) throw std::invalid_argument("the mesh has no faces");
-        if (flags & ~static_cast<unsigned>(AS_SETUP_ON_DEVICE)) throw std::invalid_argument("unknown set-up flags");
+        if (flags & ~static_cast<unsigned>(AS_SETUP_ON_DEVICE | AS_SETUP_FITTED)) throw std::invalid_argument("unknown set-up flags");
         s.on_device = (flags & AS_SETUP_ON_DEVICE) != 0;
+        s.fitted_walls = (flags & AS_SETUP_FITTED) != 0;
+        s.mesh = fitted::Mesh{vertices, nv, faces, nf};
         if (nx * ny * nz >= (1ull << 40)) throw std::invalid_argument("the grid is too large");
         s.materials = materials;
         s.beta.assign(beta, beta + materials);
         s.face_material.assign(face_material, face_material + nf);
-        for (size_t f = 0; f < nf; ++f) {
+        for (size_t f = 0; f < nf && !s.fitted_walls; ++f) {
             V3 p[3];
             for (int k = 0; k < 3; ++k) {
                 const int64_t v = faces[3 * f + static_cast<size_t>(k)];
@@ -1841,7 +2300,7 @@ int as_setup_info(const as_setup *h, uint64_t *out) {
     uint64_t sides = 0;
     for (size_t k = 0; k < 6; ++k) sides |= (s.open_sides[k] ? 1ull : 0ull) << k;
     out[6] = sides;
-    out[7] = 0;  // the source on a face: refused here
+    out[7] = s.source_on_face ? 1 : 0;  // fitted walls only: on cell faces it is refused
     return AS_OK;
 }
 
@@ -1858,6 +2317,33 @@ int as_setup_probe(const as_setup *h, size_t p, int *found, int *in_sight) {
     if (h == nullptr || p >= h->s.probe_found.size()) return AS_EINVAL;
     if (found) *found = h->s.probe_found[p];
     if (in_sight) *in_sight = h->s.probe_in_sight[p];
+    return AS_OK;
+}
+
+int as_setup_cut_cells(const as_setup *h, uint64_t *rows, const int64_t **cell, const double **air_volume,
+                       const double **open_area, const double **wall_area, const double **sliver,
+                       const double **special_volume, const uint8_t **kind, double *stats) {
+    if (h == nullptr) return AS_EINVAL;
+    const Setup &s = h->s;
+    if (!s.fitted_walls) return AS_EINVAL;
+    if (rows) *rows = s.cut_rows.size();
+    if (cell) *cell = s.cut_rows.data();
+    if (air_volume) *air_volume = s.cut_air_volume.data();
+    if (open_area) *open_area = s.cut_open_area.data();
+    if (wall_area) *wall_area = s.cut_wall_area.data();
+    if (sliver) *sliver = s.cut_sliver.data();
+    if (special_volume) *special_volume = s.special_volume.data();
+    if (kind) *kind = s.cut_kind.data();
+    if (stats) {
+        const fitted::FitStats &c = s.cut_stats;
+        const double v[] = {static_cast<double>(c.cubes), static_cast<double>(c.regions), static_cast<double>(c.ties),
+                            static_cast<double>(c.farthest), static_cast<double>(c.conduit_links),
+                            static_cast<double>(c.conduit_scaled), c.far_volume, c.diagonal_area, c.dropped_area,
+                            c.edge_area, c.air_volume, c.conduit_least, c.conduit_wall_moved,
+                            static_cast<double>(c.faces), static_cast<double>(c.planes), static_cast<double>(c.leaves),
+                            static_cast<double>(c.squares), static_cast<double>(c.quantum_bits)};
+        std::memcpy(stats, v, sizeof v);
+    }
     return AS_OK;
 }
 

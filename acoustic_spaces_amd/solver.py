@@ -13,8 +13,11 @@ also the reference solver's drop-in where it is stepped directly.
 air's flood, on the GPU (csrc/setup_device.hip); the layout and every refusal are the host
 set-up's bit for bit.
 
-Not here: moving air (the reference steps it on the CPU only), fitted walls (refused with
-a ValueError), and sources or probes that lie on a face (the reference's leans): refused.
+``fitted_walls=True`` lays the grid out on fitted walls (csrc/fitted.cpp): each cell a face
+meets is cut exactly and holds its own air volume, open areas and wall areas
+(set_up_fitted, fdtd.hpp:970-1264), for closed meshes that lie off the grid's planes.
+
+Not here: sources or probes that lie on a face (the reference's leans): refused.
 """
 
 from __future__ import annotations
@@ -31,6 +34,7 @@ D = ctypes.c_double
 
 _setup_lib = None
 SETUP_ON_DEVICE = 1  # AS_SETUP_ON_DEVICE: the cut links and the air's flood on the GPU
+SETUP_FITTED = 2  # AS_SETUP_FITTED: fitted walls, each cut cell's air, open faces and walls exact
 
 
 def _lib():
@@ -56,6 +60,8 @@ def _lib():
         lib.as_setup_probe.argtypes = [P, S, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         lib.as_setup_destroy.restype = None
         lib.as_setup_destroy.argtypes = [P]
+        lib.as_setup_cut_cells.restype = ctypes.c_int
+        lib.as_setup_cut_cells.argtypes = [P, P, P, P, P, P, P, P, P, P]
         _setup_lib = lib
     return _setup_lib
 
@@ -93,8 +99,6 @@ class WaveSolver:
                  nz, source, probes, edge_admittance=1.0, layer=0, layer_damping=0.0, threads=0,
                  stencil=7, speed=None, flow_x=None, flow_y=None, probes_per_listener=1,
                  fitted_walls=False, vertices=None, faces=None, on_device=False):
-        if fitted_walls:
-            raise ValueError("fitted walls: not laid out by this set-up; take walls on cell faces")
         if (flow_x is None) != (flow_y is None):
             raise ValueError("the air's motion needs both of its components")
         lib = _lib()
@@ -128,7 +132,8 @@ class WaveSolver:
             fm.ctypes.data, b.ctypes.data, len(b), ptr(br), ptr(bm), len(bm), o.ctypes.data,
             float(cell), int(nx), int(ny), int(nz), s.ctypes.data, ptr(p), len(p),
             float(edge_admittance), int(layer), float(layer_damping), int(stencil), ptr(sp), ptr(fx), ptr(fy),
-            int(probes_per_listener), int(threads), SETUP_ON_DEVICE if on_device else 0)
+            int(probes_per_listener), int(threads),
+            (SETUP_ON_DEVICE if on_device else 0) | (SETUP_FITTED if fitted_walls else 0))
         if not h:
             raise ValueError(lib.as_setup_create_error().decode())
         self._h = h
@@ -143,7 +148,7 @@ class WaveSolver:
         self._probes = int(info[4])
         self.open = bool(info[5])
         self.open_sides = tuple(bool((int(info[6]) >> k) & 1) for k in range(6))
-        self.source_on_face = False
+        self.source_on_face = bool(info[7])
         self.flowing = fx is not None
         # Moving air's set-up (the binding's wind_round, wind_fastest, wind_sweeps).
         rnd, fastest, sweeps = ctypes.c_int(), ctypes.c_double(), ctypes.c_uint64()
@@ -153,6 +158,8 @@ class WaveSolver:
         self.wind_sweeps = int(sweeps.value)
         self._stepper = None
         self._layout = None
+        self._fitted = bool(fitted_walls)
+        self._materials = len(b)
 
     @classmethod
     def from_scene(cls, scene, threads=0, **kw):
@@ -171,6 +178,54 @@ class WaveSolver:
     @property
     def stencil(self) -> int:
         return self._stencil
+
+    _CUT_STATS = ("cubes", "regions", "ties", "farthest", "conduit_links", "conduit_scaled", "far_volume",
+                  "diagonal_area", "dropped_area", "edge_area", "air_volume", "conduit_least",
+                  "conduit_wall_moved", "faces", "planes", "leaves", "squares", "quantum_bits")
+
+    @property
+    def fitted_walls(self) -> bool:
+        """Whether the model's faces were fitted to the grid (the binding's fitted_walls)."""
+        return self._fitted
+
+    def cut_cells(self) -> dict:
+        """With fitted walls, the fit's rows as the reference's ``cut_cells()`` hands them out
+        (native_module.cpp:1614-1632): cell, air_volume (cells^3), open_area (n, 18),
+        wall_area (n, materials + 1; the last column the grid's edge), leak_area (zeros),
+        sliver (n, 6), and the solver's special cells with the volume each is stepped with;
+        empty otherwise."""
+        if not self._fitted:
+            return {}
+        rows = ctypes.c_uint64()
+        ptrs = [ctypes.c_void_p() for _ in range(7)]
+        stats = np.zeros(len(self._CUT_STATS))
+        if _lib().as_setup_cut_cells(self._h, ctypes.byref(rows), *(ctypes.byref(p) for p in ptrs),
+                                     stats.ctypes.data) != 0:
+            raise RuntimeError("as_setup_cut_cells failed")
+        n = rows.value
+        cell, vol, open_, wall, sliver, svol, kind = (p.value for p in ptrs)
+        lay = self.layout()
+        k = len(lay["special"])
+        w = self._materials + 1
+        self.cut_stats = dict(zip(self._CUT_STATS, stats.tolist()))
+        return {
+            "cell": _array(cell, n, np.int64),
+            "air_volume": _array(vol, n, np.float64),
+            "open_area": _array(open_, 18 * n, np.float64).reshape(n, 18),
+            "wall_area": _array(wall, w * n, np.float64).reshape(n, w),
+            "leak_area": np.zeros(n),
+            "sliver": _array(sliver, 6 * n, np.float64).reshape(n, 6),
+            "special": lay["special"],
+            "volume": _array(svol, k, np.float64),
+        }
+
+    def cut_air(self) -> np.ndarray:
+        """With fitted walls, each cell's kind in the fit (0 solid, 1 air, 2 matched layer)."""
+        if not self._fitted:
+            return np.zeros(0, np.uint8)
+        kind = ctypes.c_void_p()
+        _lib().as_setup_cut_cells(self._h, None, None, None, None, None, None, None, ctypes.byref(kind), None)
+        return _array(kind.value, self._shape[0] * self._shape[1] * self._shape[2], np.uint8)
 
     def probe_found(self, p: int) -> bool:
         found = ctypes.c_int()

@@ -240,6 +240,14 @@ as_setup *as_setup_create(const double *vertices, size_t nv, const int64_t *face
  * is the host set-up's bit for bit. Grids of 2^32 cells or more are refused in this mode.
  */
 #define AS_SETUP_ON_DEVICE 1u
+/*
+ * AS_SETUP_FITTED: fitted walls (the reference's set_up_fitted, fdtd.hpp:970-1217, over its
+ * CutCells fit, cut_cells.hpp): each cell a face meets is cut exactly, and every air cell
+ * that owns cut air holds its own air volume, open areas towards its 18 neighbours and wall
+ * areas; the source and probes read the fit's cells. Closed meshes off the grid's planes
+ * only; host only (not with AS_SETUP_ON_DEVICE), in still air.
+ */
+#define AS_SETUP_FITTED 2u
 as_setup *as_setup_create_ex(const double *vertices, size_t nv, const int64_t *faces, size_t nf,
                              const int64_t *face_material, const double *beta, size_t materials,
                              const double *branches, const int64_t *branch_material, size_t nbranch,
@@ -263,6 +271,21 @@ int as_setup_info(const as_setup *h, uint64_t *out);
 int as_setup_wind(const as_setup *h, int *round, double *fastest, uint64_t *sweeps);
 /* probe_found and probe_in_sight of probe p (fdtd.hpp:713-716). */
 int as_setup_probe(const as_setup *h, size_t p, int *found, int *in_sight);
+/*
+ * Fitted walls only (AS_SETUP_FITTED; AS_EINVAL otherwise): the fit's rows, as the
+ * reference's WaveSolver.cut_cells() hands them out (cpp/bindings/native_module.cpp:1614-1632):
+ * their count, and pointers (living as long as the handle) to each row's cell (ascending),
+ * air volume (cells^3), open area towards the 18 directions (rows x 18, cell faces), wall
+ * area per material then the grid's edge (rows x (materials + 1)), sliver thickness per
+ * axial face (rows x 6, cells); each special cell's volume after the clamp (in the layout's
+ * order); each cell's kind in the fit (0 solid, 1 air, 2 matched layer). stats, where not
+ * NULL, gets 18 counts: cubes, regions, ties, farthest, conduit links, conduits scaled, far
+ * volume, diagonal area, dropped area, edge area, air volume, least conduit factor, conduit
+ * wall moved, faces, planes, leaves, squares, quantum bits. Any pointer may be NULL.
+ */
+int as_setup_cut_cells(const as_setup *h, uint64_t *rows, const int64_t **cell, const double **air_volume,
+                       const double **open_area, const double **wall_area, const double **sliver,
+                       const double **special_volume, const uint8_t **kind, double *stats);
 void as_setup_destroy(as_setup *h);
 
 #ifdef __cplusplus
