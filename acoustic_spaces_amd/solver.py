@@ -207,7 +207,6 @@ class WaveSolver:
         lay = self.layout()
         k = len(lay["special"])
         w = self._materials + 1
-        self.cut_stats = dict(zip(self._CUT_STATS, stats.tolist()))
         return {
             "cell": _array(cell, n, np.int64),
             "air_volume": _array(vol, n, np.float64),
@@ -218,6 +217,16 @@ class WaveSolver:
             "special": lay["special"],
             "volume": _array(svol, k, np.float64),
         }
+
+    @property
+    def cut_stats(self) -> dict:
+        """With fitted walls, the fit's counts (the binding's cut_stats, native_module.cpp:
+        1631-1655, those this fit keeps); empty otherwise."""
+        if not self._fitted:
+            return {}
+        stats = np.zeros(len(self._CUT_STATS))
+        _lib().as_setup_cut_cells(self._h, None, None, None, None, None, None, None, None, stats.ctypes.data)
+        return dict(zip(self._CUT_STATS, stats.tolist()))
 
     def cut_air(self) -> np.ndarray:
         """With fitted walls, each cell's kind in the fit (0 solid, 1 air, 2 matched layer)."""

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -350,6 +351,11 @@ LINK_WEIGHTS = {7: (1.0 / 3.0, 0.0), 19: (0.25, 0.125)}
 MAX_CELLS = 40_000_000                      # grids of more cells are refused
 WAVE_BANDS_HZ = analysis.OCTAVE_BANDS_HZ[:4]  # the highest band a run may reach
 LAYER_CELLS = 8                              # the matched layer round an open model
+# The configuration's refusals of fitted walls (scene/model.py:3679-3686).
+FITTED_WALLS_STENCIL = ("simulation.wave.walls: fitted works on the standard (7-point) stencil only: set "
+                        "simulation.wave.scheme to standard or auto, or simulation.wave.walls to cell_faces")
+FITTED_WALLS_OUTDOORS = ("simulation.wave.walls: fitted is not available outdoors: set simulation.wave.walls to "
+                         "cell_faces")
 LAYER_DAMPING = 0.5
 LAYER_ROOM = 0.15   # room past the points, a share of their farthest spread across each axis
 LONG_RUN = 2e9      # cell updates beyond which a run reports its tenths
@@ -442,9 +448,11 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
     the air (each step's carrying, the run watched for sound that grows), a closed grid's
     air still under any wind. Moving air is stepped on the GPU too (the reference steps it
     on the CPU only). Differences, each refused or documented: ``device`` "cpu" (this
-    package steps on the GPU), ``walls`` "fitted" (not laid out here), materials with an
-    impedance (a porous ground's fit), and the room round an open model's points taken on
-    every side (the reference leaves it off sides its surfaces close, ``_open_sides``)."""
+    package steps on the GPU), materials with an impedance (a porous ground's fit), and the
+    room round an open model's points taken on every side (the reference leaves it off
+    sides its surfaces close, ``_open_sides``). ``walls`` "fitted" fits the model to the
+    grid with this package's fit (closed models off the grid's planes; the standard
+    stencil; refused outdoors, as the reference refuses it)."""
     from acoustic_spaces_amd.admittance import MAX_ABSORPTION, fit_admittance
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from acoustic_spaces_amd.solver import Bvh, WaveSolver
@@ -463,8 +471,9 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
         raise ValueError(f"unknown wind {wind!r}; use effective or moving")
     if walls not in ("cell_faces", "fitted"):
         raise ValueError(f"unknown walls {walls!r}; use cell_faces or fitted")
-    if walls == "fitted":
-        raise ValueError("fitted walls: not laid out by this package; set simulation.wave.walls to cell_faces")
+    fitted = walls == "fitted"
+    if fitted and scheme == "isotropic":
+        raise ValueError(FITTED_WALLS_STENCIL)
     for what, value in (("points_per_wavelength", points_per_wavelength), ("length_s", length_s),
                         ("sample_rate", sample_rate), ("speed_of_sound", speed_of_sound)):
         if not (math.isfinite(value) and value > 0.0):
@@ -513,6 +522,9 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
     windy = weather is not None and weather.winds_over(
         max(float(vertices[:, 2].max()), float(points[:, 2].max())) + 100.0)
     moving = windy and wind == "moving"
+    if fitted and outdoor:
+        raise ValueError(FITTED_WALLS_OUTDOORS)
+    fit_seconds = []  # with fitted walls, the set-up's time on each grid made
 
     def layout(courant, layer, speed, per_wavelength):
         grid = band_grid(bands - 1, sample_rate, speed, per_wavelength, total, courant)
@@ -576,11 +588,14 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
             if branches else np.zeros(0, np.int64)
         probes = np.vstack([probe_points(p, grid.cell) for p in listeners])
         try:
+            began = time.perf_counter()
             solver = WaveSolver(bvh, face_material, np.array([f.beta for f in fits]),
                                 np.vstack(branches) if branches else np.zeros((0, 3)), branch_material, origin,
                                 grid.cell, int(shape[0]), int(shape[1]), int(shape[2]), source, probes, 1.0, layer,
                                 LAYER_DAMPING, threads, points_n, speeds, probes_per_listener=PROBES,
-                                flow_x=flow[0], flow_y=flow[1], on_device=True)
+                                flow_x=flow[0], flow_y=flow[1], on_device=not fitted, fitted_walls=fitted)
+            if fitted:
+                fit_seconds.append(time.perf_counter() - began)
         except ValueError as error:
             if "the wind round the model's solids" not in str(error):
                 raise
@@ -600,7 +615,7 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
     if built is None:
         # A closed model's air is still whichever way the wind is taken: no wind blows
         # through its walls (outdoors, a source shut in a closed building).
-        built = prepare("standard" if scheme == "standard" else "isotropic", 0, windy)
+        built = prepare("standard" if scheme == "standard" or fitted else "isotropic", 0, windy)
         if built[0].open:
             if scheme == "isotropic":
                 raise ValueError("the isotropic stencil has no matched layer for an open model: set "
@@ -700,6 +715,9 @@ def wave_responses(model, source, listeners, names, *, max_band_hz: int, length_
         info["wave_wind_fastest_m_s"] = fastest
     if still_inside:
         info["wave_still_inside"] = 1.0
+    if fitted:
+        info["wave_fitted_walls"] = 1.0
+        info["wave_fit_s"] = sum(fit_seconds)  # on every grid made
     slowest = speed_of_sound if extremes is None else extremes[0]
     per_wavelength = slowest / (band_top(bands - 1, total) * h)
     info["wave_slow_percent"] = 100.0 * slow_by(solver.stencil, per_wavelength, still / (courant * h * grid.rate))

@@ -2,12 +2,15 @@
 
 A step is one time step of the FDTD scheme over the whole grid of the hall workload: the
 30 x 20 x 12 m hall with a ceiling of six tilted planes and side walls splayed 5 degrees, at
-10 cm cells (305 x 257 x 125 cells, 7,568,516 air cells), walls on cell faces with a
-constant admittance and three first-order (relaxation) admittance branches per material,
-one source, 16 receivers; 2 s of response is 11,882 steps. The grid is laid out by this
-package's set-up (csrc/setup.cpp) and stepped by the HIP kernels (csrc/fdtd_step.hip),
-in the launch shape the autotuner finds fastest on the scene's drive (part of the set-up);
-the drive and the probes' output stay in device memory over the timed region.
+10 cm cells (305 x 257 x 125 cells, about 7.57M air cells), fitted walls (each cell a face
+cuts holds its own air volume, open areas and wall areas, diagonal links where the planes
+are tilted), a constant admittance and three first-order (relaxation) admittance branches
+per material, one source, 16 receivers; 2 s of response is 11,882 steps. The grid is laid
+out by this package's fit (csrc/fitted.cpp, csrc/setup.cpp) and stepped by the HIP
+kernels (csrc/fdtd_step.hip), in the launch shape the autotuner finds fastest on the
+scene's drive (part of the set-up); the drive and the probes' output stay in device memory
+over the timed region. ``--workload hall-faces`` times the same hall with its walls on cell
+faces.
 
 Printed: one JSON line with Mcell-updates/s (air cells x steps / s), the step kernel's
 roofline (algorithmic bytes per step over its time per step, against HBM's 8 TB/s) and
@@ -99,17 +102,19 @@ def cpu_baseline(scene, target_s: float = 12.0, compare=None, others=()) -> dict
     cores = len(os.sched_getaffinity(0))
     from oracle import reference
 
+    fitted = bool(getattr(scene, "fitted_walls", False))
     if reference.available():
-        solver = reference.ReferenceSolver.from_scene(scene, threads=cores)
+        solver = reference.ReferenceSolver.from_scene(scene, threads=cores, fitted_walls=fitted)
         kind, used = "reference", solver.threads
         air = solver.air_cells
     else:
         from acoustic_spaces_amd.solver import WaveSolver
         from oracle.stepper import OracleSolver
 
-        solver = OracleSolver(WaveSolver.from_scene(scene).layout())
+        mine = WaveSolver.from_scene(scene, fitted_walls=fitted)
+        solver = OracleSolver(mine.layout())
         kind, used = "port", 1
-        air = int(np.count_nonzero(WaveSolver.from_scene(scene).layout()["types"]))
+        air = mine.air_cells
     drive = scene.drive
     t = time.perf_counter()
     solver.run(drive[:8])
@@ -134,7 +139,7 @@ def cpu_baseline(scene, target_s: float = 12.0, compare=None, others=()) -> dict
     for k, (other, gpu) in enumerate(others):
         if kind != "reference":
             break
-        ref = reference.ReferenceSolver.from_scene(other, threads=cores)
+        ref = reference.ReferenceSolver.from_scene(other, threads=cores, fitted_walls=fitted)
         y = ref.run(other.drive[: gpu.shape[1]])
         out[f"gpu_f32_vs_this_source{k + 2}"] = compare_outputs(octave_outputs(gpu, other.rate),
                                                                  octave_outputs(y, other.rate), gpu, y)
@@ -284,16 +289,29 @@ def plain(scene, name, args, traffic_name=None):
 
     torch.zeros(1, device="cuda")
     torch.cuda.synchronize()
-    t = time.perf_counter()
-    solver = WaveSolver.from_scene(scene, on_device=True)
-    layout = solver.layout()
-    setup_s = time.perf_counter() - t
-    t = time.perf_counter()
-    host_layout = WaveSolver.from_scene(scene).layout()
-    host_setup_s = time.perf_counter() - t
-    if sorted(host_layout) != sorted(layout) or not all(np.array_equal(layout[k], host_layout[k]) for k in layout):
-        raise SystemExit("the device set-up's layout differs from the host's")
-    del host_layout
+    fitted = bool(getattr(scene, "fitted_walls", False))
+    host_threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+    if fitted:
+        # The fit runs on the host's threads.
+        t = time.perf_counter()
+        solver = WaveSolver.from_scene(scene, fitted_walls=True)
+        layout = solver.layout()
+        setup = {"host": round(time.perf_counter() - t, 3), "host_threads": host_threads,
+                 "host_is": "the fit of the mesh to the grid (csrc/fitted.cpp) and the special cells' coefficients"}
+    else:
+        t = time.perf_counter()
+        solver = WaveSolver.from_scene(scene, on_device=True)
+        layout = solver.layout()
+        setup_s = time.perf_counter() - t
+        t = time.perf_counter()
+        host_layout = WaveSolver.from_scene(scene).layout()
+        host_setup_s = time.perf_counter() - t
+        if sorted(host_layout) != sorted(layout) or not all(np.array_equal(layout[k], host_layout[k]) for k in layout):
+            raise SystemExit("the device set-up's layout differs from the host's")
+        del host_layout
+        setup = {"device": round(setup_s, 3), "host": round(host_setup_s, 3),
+                 "device_is": "every pass over the whole grid on the GPU (cut links, flood, layer, wall cells); the source's and probes' readings on the host",
+                 "host_threads": host_threads}
     if args.tile_z or args.graph_steps or args.face_blocks >= 0:
         launch = {"tile_z": args.tile_z, "graph_steps": args.graph_steps,
                   "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks), "pairs": 0}
@@ -317,9 +335,7 @@ def plain(scene, name, args, traffic_name=None):
     config = {
         "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
         "special_cells": info["specials"], "grid": list(scene.grid.shape),
-        "setup_s": {"device": round(setup_s, 3), "host": round(host_setup_s, 3),
-                    "device_is": "every pass over the whole grid on the GPU (cut links, flood, layer, wall cells); the source's and probes' readings on the host",
-                    "host_threads": int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())},
+        "setup_s": setup,
         "launch": launch, "launch_from": launch_from,
         "timed_launches": launch_mode(launch["graph_steps"], K) + (
             f", two steps by two launches (boxes of {launch['pairs'] if launch['pairs'] > 1 else 8} planes)"
@@ -327,7 +343,10 @@ def plain(scene, name, args, traffic_name=None):
         "lead_in_steps": lead,
         "lead_in": "steps of the drive from rest before the warm-up, so that the timed steps "
                    "meet the spreading pulse (live fields), not fields at rest",
-        "precision": "float (bit-identical to the reference's CPU scheme)",
+        "precision": ("float: the scheme bit-identical to the reference's CPU scheme on one layout; this "
+                      "package's fitted layout has the reference's cell kinds, specials, links and states, its "
+                      "coefficients within a float of the reference's" if fitted else
+                      "float (bit-identical to the reference's CPU scheme)"),
     }
     traffic = measured_traffic(traffic_name) if traffic_name else None
     stepper.close()
@@ -354,7 +373,7 @@ def source_outputs(scene, args) -> dict:
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from acoustic_spaces_amd.solver import WaveSolver
 
-    layout = WaveSolver.from_scene(scene).layout()
+    layout = WaveSolver.from_scene(scene, fitted_walls=bool(getattr(scene, "fitted_walls", False))).layout()
     y32 = HipWaveSolver.run_once(layout, scene.drive, precision=32)
     y64 = HipWaveSolver.run_once(layout, scene.drive, precision=64)
     o32, o64 = octave_outputs(y32, scene.rate), octave_outputs(y64, scene.rate)
@@ -695,7 +714,8 @@ def main(argv=None) -> int:
     ap.add_argument("--lead-in", type=int, default=256,
                     help="steps from rest before the warm-up (set-up, untimed), so that the timed "
                          "steps meet live fields")
-    ap.add_argument("--workload", choices=("hall", "shoebox", "outdoor", "rooms", "adjoint"), default="hall")
+    ap.add_argument("--workload", choices=("hall", "hall-faces", "shoebox", "outdoor", "rooms", "adjoint"),
+                    default="hall")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-outputs", action="store_true",
                     help="skip the receiver outputs (the hall's and shoebox's band levels and T30, "
@@ -728,19 +748,26 @@ def main(argv=None) -> int:
 
     from acoustic_spaces_amd import scenes
 
-    if args.workload == "hall":
+    if args.workload in ("hall", "hall-faces"):
+        fitted = args.workload == "hall"
         args.steps = args.steps or 11882
         sources = scenes.hall()
+        for sc in sources:
+            sc.fitted_walls = fitted
         scene = sources[0]
         args.outputs = not args.no_outputs
-        line = plain(scene, "hall", args, traffic_name="step_kernel_hall_traffic")
+        line = plain(scene, "hall" if fitted else "hall, walls on cell faces", args,
+                     traffic_name="step_kernel_hall_fitted_traffic" if fitted else "step_kernel_hall_traffic")
         if args.outputs:
             # The second source's run (the config's two runs), its outputs as the first's.
             line["config"]["outputs_source2"] = source_outputs(sources[1], args)
+        walls = ("fitted walls (this package's fit: each cut cell's air volume, open areas towards its "
+                 f"18 neighbours and wall areas; {line['config']['special_cells']:,} special cells)"
+                 if fitted else "walls on cell faces")
         line["config"]["workload"] = (
             "hall 30x20x12 m, ceiling of 6 tilted planes, side walls splayed 5 deg, 10 cm cells "
             f"{scene.grid.shape[0]}x{scene.grid.shape[1]}x{scene.grid.shape[2]}, "
-            f"{line['config']['air_cells']:,} air cells, walls on cell faces, 3 relaxation branches per "
+            f"{line['config']['air_cells']:,} air cells, {walls}, 3 relaxation branches per "
             "material, 2 sources (the timed steps from the first; the outputs of both), 16 receivers")
     elif args.workload == "shoebox":
         args.steps = args.steps or 5941
@@ -761,7 +788,7 @@ def main(argv=None) -> int:
         scene = None
     if not args.no_cpu and scene is not None:
         line["cpu_baseline"] = cpu_baseline(scene, compare=args.signals, others=args.others)
-    if args.workload == "hall" and not args.no_ratios:
+    if args.workload in ("hall", "hall-faces") and not args.no_ratios:
         # The metric's other two parts, on their own workloads, in the same run.
         line["batched_over_sequential"] = batched_over_sequential()
         line["gradient_over_forward"] = gradient_over_forward()

@@ -222,3 +222,25 @@ def test_a_face_on_a_plane_of_the_grid_is_refused():
     sc.grid.origin = np.array([-2.0 * sc.grid.cell, sc.grid.origin[1], sc.grid.origin[2]])
     with pytest.raises(ValueError, match="plane of the grid"):
         WaveSolver.from_scene(sc, fitted_walls=True)
+
+
+def test_full_hall_topology_equals_the_fixture():
+    """The hall at 10 cm (7.57M air cells): every layout array whose bits come from the
+    fit's decisions equals the reference's (tests/golden/hall_fitted.json, written by
+    make_fitted_hall.py from the reference's own set-up), for both sources."""
+    import hashlib
+    import json
+    from pathlib import Path
+
+    from acoustic_spaces_amd.solver import WaveSolver
+
+    fixture = json.loads((Path(__file__).parent / "golden" / "hall_fitted.json").read_text())
+    for k, scene in enumerate(scenes.hall()):
+        want = fixture[f"source{k + 1}"]
+        solver = WaveSolver.from_scene(scene, fitted_walls=True)
+        layout = solver.layout()
+        for name, digest in want["topology_sha256"].items():
+            assert hashlib.sha256(np.ascontiguousarray(layout[name]).tobytes()).hexdigest() == digest, name
+        assert (solver.air_cells, solver.boundary_cells, solver.branch_states) == (
+            want["air_cells"], want["boundary_cells"], want["branch_states"])
+        assert math.isclose(solver.cut_stats["air_volume"], want["fit_air_volume"], rel_tol=1e-13)

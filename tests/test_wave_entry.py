@@ -109,7 +109,7 @@ def test_refusals():
     kw = dict(max_band_hz=63, length_s=0.1, sample_rate=8000.0, speed_of_sound=343.0)
     src, lis = np.array([1.0, 1.1, 1.2]), np.array([[2.5, 2.0, 1.4]])
     for bad, match in (({"device": "tpu"}, "device"), ({"device": "cpu"}, "GPU"), ({"scheme": "x"}, "scheme"),
-                       ({"wind": "x"}, "wind"), ({"walls": "x"}, "walls"), ({"walls": "fitted"}, "fitted"),
+                       ({"wind": "x"}, "wind"), ({"walls": "x"}, "walls"), ({"walls": "fitted", "scheme": "isotropic"}, "fitted"),
                        ({"max_band_hz": 1000}, "max_band_hz"), ({"length_s": -1.0}, "length_s")):
         with pytest.raises(ValueError, match=match):
             wv.wave_responses(model, src, lis, ["a"], **(kw | bad))
