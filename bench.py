@@ -324,7 +324,7 @@ def plain(scene, name, args, traffic_name=None):
     # A timed run shorter than the tuned graph replays graphs of a length that divides it
     # (0 is the library's default, 64).
     launch["graph_steps"] = graph_length(launch["graph_steps"] or 64, K)
-    stepper = HipWaveSolver(layout, **launch)
+    stepper = HipWaveSolver(layout, split=args.split, **launch)
     info = stepper.info()
     ms, out = time_steps(stepper, drive, W, K, lead=lead)
     if not np.all(np.isfinite(out)):
@@ -729,6 +729,9 @@ def main(argv=None) -> int:
                     help="face specials by blocks of their own (1) or in their tiles (0); with any "
                          "of --tile-z, --graph-steps, --face-blocks the plain workloads skip the "
                          "autotuner")
+    ap.add_argument("--split", action="store_true",
+                    help="launch the step's roles (probes, general specials, face specials, tiles, layer) "
+                         "as kernels of their own, for a profile of each; the line then times that")
     ap.add_argument("--calibrate", action="store_true",
                     help="time the runtime's own device-to-device copy kernel instead (the bandwidth "
                          "reference and the rocprofv3 counter scale)")
