@@ -18,7 +18,7 @@
 // holding interior cells or face specials are launched, in a list made at set-up and
 // ordered by die. A special whose links are axial and weigh as the interior's (a face
 // special) is a 16-byte record with its coefficients and branches in small tables; the
-// other specials sit in slabs, blocks of 256 sorted by their counts of links and states,
+// other specials sit in slabs, blocks of 256 in the order of the tiles holding them,
 // slot-major, each chunk of six links' directions packed into one word. The matched
 // layer's nine state values are nine planes of its cells (structure of arrays).
 //
@@ -125,8 +125,8 @@ struct StepArgs {
     int tile_z;
     long long sxy;
     T axial, diagonal, centre, total;
-    // specials, in blocks of kBlock sorted by their counts of links and states; each
-    // block's links and states in a slab of (its widest count) x kBlock, slot-major
+    // specials, in blocks of kBlock (by default in the order of the tiles holding them);
+    // each block's links and states in a slab of (its widest count) x kBlock, slot-major
     uint32_t specials, special_blocks;
     const uint32_t *__restrict__ sp_cell;
     const uint8_t *__restrict__ sp_links;   // links of each special
@@ -501,13 +501,12 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
 // branches' coefficients); chunks of links and states past the first, rare, follow one
 // by one.
 template <typename T>
-__device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t block) {
+__device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
     // States taken with the first loads (the rest one by one, in the same order); three in
     // double keep the step within five wavefronts a SIMD.
     constexpr int kStateChunk = sizeof(T) == 8 ? 3 : 4;
-    const uint32_t t = threadIdx.x;
-    const uint32_t k = block * kBlock + t;
     if (k >= s.specials) return;
+    const uint32_t block = k / kBlock, t = k % kBlock;
     const int i = static_cast<int>(s.sp_cell[k]);
     const int nl = s.sp_links[k], ns = s.sp_states[k];
     const float ca = s.coef_a[k], src = s.source[k];
@@ -619,6 +618,12 @@ __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t bloc
             s.wo[q] = s.wa[q] + (rise * static_cast<T>(s.branch_gain[kd]) + term);
         }
     }
+}
+
+// The special blocks' specials: block b's thread t steps special b kBlock + t.
+template <typename T>
+__device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t block) {
+    special_one<T>(s, block * kBlock + threadIdx.x);
 }
 
 // n / d for n < 2^31 by a multiply: magic = floor(2^32 (2^shift - d) / d) + 1 with
@@ -1735,8 +1740,9 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         }
     });
 
-    // General specials: sorted by their counts of links and states (a special's update reads
-    // only its own cell, links and states, so their order is free), in blocks of kBlock;
+    // General specials: in the order of the interior tiles holding them, or by their counts
+    // of links and states, or by cell (a special's update reads only its own cell, links and
+    // states, so their order is free), in blocks of kBlock;
     // each block's links and states in slabs of (its largest count) x kBlock, slot-major.
     // A block's links are its chunks' packed directions and the links' weights, slot-major;
     // each state names its branch, whose coefficients sit in small tables.
@@ -1746,9 +1752,23 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     for (uint64_t k = 0; k < K; ++k)
         if (!face[k]) order.push_back(k);
     const uint64_t G = order.size();
-    std::stable_sort(order.begin(), order.end(), [&](uint64_t p, uint64_t q) {
-        return links_of(p) != links_of(q) ? links_of(p) < links_of(q) : states_of(p) < states_of(q);
-    });
+    // By tile by default: a block's specials then lie close together, and their gathers
+    // share cache lines (the fitted hall: 34.96 to 32.10 us a step against by counts).
+    const int special_order = tuning && tuning->special_order > 0 ? tuning->special_order : 3;
+    if (special_order == 1) {
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t p, uint64_t q) {
+            return links_of(p) != links_of(q) ? links_of(p) < links_of(q) : states_of(p) < states_of(q);
+        });
+    } else if (special_order == 3) {
+        // By the interior tile (64 x 4 x 8 cells) that holds the cell, tiles in grid order.
+        const auto tile_key = [&](uint64_t k) {
+            const uint64_t c = static_cast<uint64_t>(L.special[k]);
+            const uint64_t x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+            return ((z / 8) * ((ny + 3) / 4) + y / 4) * ((nx + 63) / 64) + x / 64;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t p, uint64_t q) { return tile_key(p) < tile_key(q); });
+    }
+    // special_order 2: the layout's order (ascending cells, then the source's).
     const uint64_t blocks = (G + kBlock - 1) / kBlock;
     std::vector<uint32_t> sp_cell(G), link_base(blocks), state_base(blocks);
     std::vector<uint8_t> sp_links(G), sp_states(G);

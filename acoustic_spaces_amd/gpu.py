@@ -33,7 +33,8 @@ class HipWaveSolver:
 
     def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
                  graph_steps: int = 0, split: bool = False, tile_order: int = 0,
-                 no_faces: bool = False, face_blocks=None, pairs: bool = False, keep_history=None) -> None:
+                 no_faces: bool = False, face_blocks=None, pairs: bool = False, keep_history=None,
+                 special_order: int = 0) -> None:
         lib = native.load()
         layout = solver if isinstance(solver, dict) else solver.layout()
         struct, keep = native.pack_layout(layout)
@@ -41,7 +42,7 @@ class HipWaveSolver:
                                int(tile_order), 1 if no_faces else 0,
                                0 if face_blocks is None else (1 if face_blocks else -1),
                                int(pairs) if not isinstance(pairs, bool) and int(pairs) > 1 else (1 if pairs else 0),
-                               0 if keep_history is None else (1 if keep_history else -1))
+                               0 if keep_history is None else (1 if keep_history else -1), int(special_order))
         h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
         if not h:
             message = lib.as_fdtd_create_error().decode()

@@ -45,7 +45,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int), ("row_align", ctypes.c_int),
                 ("graph_steps", ctypes.c_int), ("split", ctypes.c_int),
                 ("tile_order", ctypes.c_int), ("no_faces", ctypes.c_int),
-                ("face_blocks", ctypes.c_int), ("pairs", ctypes.c_int), ("keep_history", ctypes.c_int)]
+                ("face_blocks", ctypes.c_int), ("pairs", ctypes.c_int), ("keep_history", ctypes.c_int),
+                ("special_order", ctypes.c_int)]
 
 
 _lib = None

@@ -1,5 +1,5 @@
 """The stepper's autotuner: tile depth, steps per graph, where the face specials are
-stepped and the steps a launch takes, chosen by timing them.
+stepped, the steps a launch takes and the general specials' order, chosen by timing them.
 
 For a layout it times each candidate launch shape (tile_z planes per tile, graph_steps
 steps per captured graph, face specials by blocks of their own or in their tiles, one step
@@ -30,15 +30,23 @@ ONE_STEP = tuple({"tile_z": tz, "graph_steps": gs, "face_blocks": fb, "pairs": 0
 # twice, in boxes marching 8 or 16 planes in registers (float, 7 points, one speed of sound).
 TWO_STEPS = tuple({"tile_z": 8, "graph_steps": 64, "face_blocks": fb, "pairs": planes}
                   for planes in (8, 16) for fb in (True, False))
-CANDIDATES = ONE_STEP + TWO_STEPS
+# The general specials' order (the library's default, 3: by the tile holding them), against
+# by their counts of links and states.
+ORDERS = tuple({"tile_z": 8, "graph_steps": 64, "face_blocks": True, "pairs": 0, "special_order": order}
+               for order in (1,))
+CANDIDATES = ONE_STEP + TWO_STEPS + ORDERS
 TIE = 0.01
 
 
 def applicable(layout: dict, candidates, precision: int = 32) -> tuple:
     """The candidates the layout can take: two steps by two launches only for float grids
-    of 7 points with one speed of sound."""
+    of 7 points with one speed of sound; the specials' order only where general specials
+    (links of their own weights or diagonal, as fitted walls make) are many."""
     pairs_ok = precision == 32 and int(layout["stencil"][0]) == 7 and not int(layout["scaled"][0])
-    return tuple(c for c in candidates if pairs_ok or not c.get("pairs"))
+    weights = np.asarray(layout["link_weight"])
+    own = int(np.count_nonzero((np.asarray(layout["link_direction"]) >= 6) | (weights != weights.max(initial=0))))
+    return tuple(c for c in candidates
+                 if (pairs_ok or not c.get("pairs")) and (own >= 1000 or not c.get("special_order")))
 
 
 def cache_path() -> Path:
@@ -103,7 +111,8 @@ def time_shape(layout: dict, steps: int = 256, warmup: int = 64, precision: int 
 def autotune(layout: dict, steps: int = 256, candidates=CANDIDATES, use_cache: bool = True,
              precision: int = 32, drive=None) -> dict:
     """The fastest launch shape for the layout (the first candidate within TIE of the
-    fastest): the candidate's keys (tile_z, graph_steps, face_blocks, pairs), "us_per_step" and
+    fastest): the candidate's keys (tile_z, graph_steps, face_blocks, pairs, and where it
+    was timed special_order), "us_per_step" and
     "timings"; from the cache when it holds this layout's
     signature. With ``drive`` each candidate is timed on live fields (after 256 steps of it),
     which can rank shapes otherwise than zero fields do."""

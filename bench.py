@@ -312,14 +312,15 @@ def plain(scene, name, args, traffic_name=None):
         setup = {"device": round(setup_s, 3), "host": round(host_setup_s, 3),
                  "device_is": "every pass over the whole grid on the GPU (cut links, flood, layer, wall cells); the source's and probes' readings on the host",
                  "host_threads": host_threads}
-    if args.tile_z or args.graph_steps or args.face_blocks >= 0:
+    if args.tile_z or args.graph_steps or args.face_blocks >= 0 or args.special_order:
         launch = {"tile_z": args.tile_z, "graph_steps": args.graph_steps,
-                  "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks), "pairs": 0}
+                  "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks), "pairs": 0,
+                  "special_order": args.special_order}
         launch_from = "flags (0 or null: the library's default)"
     else:
         # Set-up, outside the timed region: the launch shape timed on the scene's drive.
         shape = tune.autotune(layout, steps=512, drive=drive)
-        launch = {k: shape.get(k, 0) for k in ("tile_z", "graph_steps", "face_blocks", "pairs")}
+        launch = {k: shape.get(k, 0) for k in ("tile_z", "graph_steps", "face_blocks", "pairs", "special_order")}
         launch_from = "the autotuner, each candidate timed on the scene's drive (live fields)"
     # A timed run shorter than the tuned graph replays graphs of a length that divides it
     # (0 is the library's default, 64).
@@ -729,6 +730,9 @@ def main(argv=None) -> int:
                     help="face specials by blocks of their own (1) or in their tiles (0); with any "
                          "of --tile-z, --graph-steps, --face-blocks the plain workloads skip the "
                          "autotuner")
+    ap.add_argument("--special-order", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="the general specials' order: 1 by their counts, 2 by cell, 3 by tile (0: the "
+                         "library's, 3)")
     ap.add_argument("--split", action="store_true",
                     help="launch the step's roles (probes, general specials, face specials, tiles, layer) "
                          "as kernels of their own, for a profile of each; the line then times that")

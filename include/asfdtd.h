@@ -114,6 +114,9 @@ typedef struct as_tuning {
                           backward pass recomputes each segment from its own; 0, the library's
                           choice: the history where it takes at most 16 GiB and a quarter of
                           the free device memory */
+    int special_order; /* the general specials' order in their blocks: 1 by their counts of links
+                          and states, 2 by cell (the layout's order), 3 by the interior tile
+                          holding the cell; 0, the library's choice (3) */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the

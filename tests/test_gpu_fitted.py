@@ -58,3 +58,26 @@ def test_fitted_hall_receivers_match_the_reference(source):
     print(f"source {source}: |dG| <= {dg.max():.2e} dB, |dT30| <= {dt.max():.2e} s", flush=True)
     assert dg.max() < 0.05
     assert dt.max() < 0.01
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("split", [False, True])
+def test_special_orders_step_alike(order, split):
+    """The general specials in each order (by counts, by cell, by tile), fused and split:
+    the coarse hall's fitted layout, 300 steps in two calls, bit for bit the oracle's."""
+    import sys
+
+    sys.path.insert(0, str(Path(__file__).parent))
+    import cases
+    from acoustic_spaces_amd.gpu import HipWaveSolver
+    from oracle.stepper import OracleSolver
+
+    scene = cases.coarse_hall()
+    layout = _fitted(scene)
+    gpu = HipWaveSolver(layout, special_order=order, split=split)
+    cpu = OracleSolver(layout)
+    d = scene.drive[:300]
+    assert np.array_equal(gpu.run(d[:151]), cpu.run(d[:151]))
+    assert np.array_equal(gpu.run(d[151:]), cpu.run(d[151:]))
+    assert np.array_equal(gpu.field(), cpu.field())
