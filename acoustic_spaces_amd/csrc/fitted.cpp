@@ -415,7 +415,7 @@ struct Fit::Impl {
             const int k = e - 53 + s;
             if (k >= 0) {
                 if (k > 70) throw Refused("fitted walls: a coordinate outside the lattice's frame");
-                v = static_cast<i128>(M) << k;
+                v = static_cast<i128>(M) * (static_cast<i128>(1) << k);
             } else {
                 const int r = -k;
                 const u128 u = mag(M);
@@ -479,7 +479,7 @@ struct Fit::Impl {
         s = 61 - p;
         for (int a = 0; a < 3; ++a) {
             const double X0 = std::ldexp(std::nearbyint(std::ldexp((lo[a] + hi[a]) / 2.0, -(p - 2))), p - 2);
-            X0s[a] = static_cast<i128>(std::nearbyint(std::ldexp(X0, -(p - 2)))) << 59;
+            X0s[a] = static_cast<i128>(std::nearbyint(std::ldexp(X0, -(p - 2)))) * (static_cast<i128>(1) << 59);
         }
         S = std::ldexp(h, s);
         double reach = 0.0;
