@@ -19,6 +19,7 @@ Workloads, with the sizes the solver is measured on:
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 
 import numpy as np
@@ -286,33 +287,115 @@ def shoebox(size=(6.0, 4.0, 3.0), cell=0.05, seed=0, length_s=0.5, receivers=4,
     )
 
 
-def relaxation_branches(alpha_bands, bands_hz, rate):
-    """A constant admittance and one first-order (mass-free) relaxation branch per band
-    for a wall whose normal-incidence absorption is ``alpha_bands`` at ``bands_hz``.
+def _first_order(omega, beta, heights, corners, falling=None):
+    """Y(w) = beta + sum_j Y_j(w): a constant and first-order branches of height g_j and
+    corner w_j, each rising (a relaxation, 1 / (R_j + S_j / (i w)) = g_j i w / (i w + w_j))
+    or, where ``falling`` says so, falling (1 / (i w M_j + R_j) = g_j w_j / (i w + w_j)),
+    with g_j = 1 / R_j."""
+    w = np.asarray(omega, float)[:, None]
+    g = np.asarray(heights, float)[None, :]
+    c = np.asarray(corners, float)[None, :]
+    down = np.zeros(c.shape, bool) if falling is None else np.asarray(falling, bool)[None, :]
+    return beta + np.sum(g * np.where(down, c, 1j * w) / (1j * w + c), axis=1)
 
-    A relaxation branch Y_j(s) = 1 / (R_j + S_j / s) (m = 0, reference impedance.py:168-211)
-    rises from 0 below its corner w_j = S_j / R_j to 1 / R_j above it. Branch j takes its
-    corner at band j's centre and its height from the rise of the target admittance from
-    band j - 1 to band j (the first band's from the constant). Centred in time
-    (fdtd.hpp:391-411) its solver coefficients are m = 0, r = R_j, q = S_j T."""
-    beta = normal_incidence_admittance(alpha_bands)
-    base = float(beta[0]) * 0.5
-    rows = []
-    level = base
+
+def _six_digits(values):
+    """``values`` rounded to six significant digits (by Python's correctly rounded decimal
+    formatting). A fit's result depends in its last bits on the host's linear-algebra
+    kernels, which differ by CPU; rounded, a scene's coefficients are the same on every
+    host."""
+    return [float(f"{float(v):.6g}") for v in values]
+
+
+def _branch_rows(heights, corners, rate, falling=None):
+    """The solver's (m, r, q) of first-order branches centred in time (fdtd.hpp:391-411,
+    m and q in the scheme's steps: m = M / T, q = S T): a rising one m = 0, r = R, q = S T
+    with S = w R; a falling one m = 1 / (g w T), r = R, q = 0."""
     T = 1.0 / rate
-    for j, f in enumerate(bands_hz):
-        rise = max(float(beta[j]) - level, 1e-3)
-        R = 1.0 / rise
-        w = 2.0 * math.pi * f
-        S = w * R
-        rows.append((0.0, R, S * T))
-        level += rise
-    return base, np.array(rows, dtype=float)
+    falling = [False] * len(heights) if falling is None else falling
+    return np.array([(1.0 / (g * w * T), 1.0 / g, 0.0) if down else (0.0, 1.0 / g, w / g * T)
+                     for g, w, down in zip(heights, corners, falling)], dtype=float)
+
+
+def relaxation_branches(alpha_bands, bands_hz, rate):
+    """See _fitted_branches (the fit, kept per input)."""
+    base, rows = _fitted_branches(tuple(float(a) for a in np.asarray(alpha_bands, float)),
+                                  tuple(float(f) for f in bands_hz), float(rate))
+    return base, rows.copy()
+
+
+@functools.lru_cache(maxsize=256)
+def _fitted_branches(alpha_bands, bands_hz, rate):
+    """A constant admittance and one first-order branch per band, fitted so that the wall's
+    normal-incidence absorption 1 - |(1 - Y) / (1 + Y)|^2 is ``alpha_bands`` at the bands'
+    centres ``bands_hz``.
+
+    Branch j takes its corner at band j's centre; it rises (a relaxation, Y_j(s) =
+    1 / (R_j + S_j / s), m = 0, reference impedance.py:168-211) or falls (1 / (s M_j + R_j)).
+    For each choice of rising and falling branches, the constant and the heights 1 / R_j
+    (each at least 1e-6) are the bounded least-squares fit (scipy's trust-region reflective)
+    from the admittance's change from band to band; the choice that fits best is kept (the
+    first on ties, all rising first), its parameters rounded to six digits. First-order
+    branches an octave apart cannot follow every draw (an absorption that rises and falls
+    by octaves): the fit is the closest. Centred in time (fdtd.hpp:391-411) the solver's
+    coefficients are those of _branch_rows."""
+    from itertools import product
+
+    from scipy.optimize import least_squares
+
+    target = np.asarray(alpha_bands, float)
+    corners = 2.0 * math.pi * np.asarray(bands_hz, float)
+    beta = normal_incidence_admittance(target)
+    lower = np.array([0.0] + [1e-6] * len(target))
+    best = None
+    for falling in product((False, True), repeat=len(target)):
+        start = [float(beta.min()) * 0.5] + [max(abs(float(beta[j]) - float(beta[j - 1] if j else 0.0)), 1e-3)
+                                              for j in range(len(target))]
+
+        def misfit(x, falling=falling):
+            y = _first_order(corners, x[0], x[1:], corners, falling)
+            return (1.0 - np.abs((1.0 - y) / (1.0 + y)) ** 2) - target
+
+        fit = least_squares(misfit, np.maximum(start, lower), bounds=(lower, np.inf), method="trf")
+        if best is None or fit.cost < best[0].cost - 1e-18:
+            best = (fit, falling)
+    fit, falling = best
+    x = _six_digits(fit.x)
+    return x[0], _branch_rows(x[1:], corners, rate, falling)
+
+
+def ground_branches(sigma, bands_hz, rate, samples=9):
+    """A constant admittance and one relaxation branch per band, fitted to the Delany-Bazley
+    admittance of flow resistivity ``sigma``: bounded least squares on its real and imaginary
+    parts, relative to its size, at ``samples`` frequencies spaced evenly in log from an
+    octave below the first band to half an octave above the last. The branches' heights and
+    corners are both fitted (the corners start at the bands' centres and stay within two
+    octaves of the bands), then rounded to six digits."""
+    from scipy.optimize import least_squares
+
+    f = np.geomspace(bands_hz[0] / 2.0, bands_hz[-1] * math.sqrt(2.0), samples)
+    target = delany_bazley_admittance(f, sigma)
+    omega = 2.0 * math.pi * f
+    centres = np.log(2.0 * math.pi * np.asarray(bands_hz, float))
+    n = len(centres)
+
+    def misfit(x):
+        d = (_first_order(omega, x[0], x[1:1 + n], np.exp(x[1 + n:])) - target) / np.abs(target)
+        return np.concatenate([d.real, d.imag])
+
+    span = 2.0 * math.log(2.0)
+    lower = np.concatenate([[0.0], [1e-9] * n, [centres[0] - span] * n])
+    upper = np.concatenate([[np.inf], [np.inf] * n, [centres[-1] + span] * n])
+    start = np.concatenate([[float(np.real(target).min()) * 0.5], [float(np.abs(target).mean())] * n, centres])
+    fit = least_squares(misfit, np.clip(start, lower, upper), bounds=(lower, upper), method="trf")
+    x = _six_digits(np.concatenate([fit.x[:1 + n], np.exp(fit.x[1 + n:])]))
+    return x[0], _branch_rows(x[1:1 + n], x[1 + n:], rate)
 
 
 def hall(cell=0.1, seed=1, length_s=2.0, receivers=16, sources=2) -> list:
     """The hall, one Scene per source: per material, absorption U(0.02, 0.3) at 63, 125 and
-    250 Hz as a constant admittance and three relaxation branches; sources and receivers on
+    250 Hz as a constant admittance and three first-order branches, each rising or falling
+    (relaxation_branches); sources and receivers on
     a seeded audience grid 1.2 m above the floor (sources on the stage, 1.5 m up)."""
     rng = np.random.default_rng(seed)
     mesh = hall_mesh()
@@ -368,7 +451,8 @@ def outdoor(cell=0.25, seed=2, length_s=1.0, receivers=32, layer=16, layer_dampi
     barrier's insertion loss), a 16-cell matched layer round the grid. The grid spans
     ``size`` (200 x 200 x 40 m at 25 cm: 800 x 800 x 160 cells) with the block inside its
     layer and margin. The ground's admittance is a constant and two relaxation branches
-    fitted to Delany-Bazley at sigma 200 kPa s/m^2."""
+    fitted to Delany-Bazley at sigma 200 kPa s/m^2, real and imaginary parts, from 31.5 to
+    177 Hz (ground_branches: within 7% of it there)."""
     rng = np.random.default_rng(seed)
     n = np.round(np.asarray(size) / cell).astype(int)
     pad = layer + 2
@@ -395,17 +479,7 @@ def outdoor(cell=0.25, seed=2, length_s=1.0, receivers=32, layer=16, layer_dampi
     rate = C / (COURANT * cell)
     steps = int(round(length_s * rate))
     # Ground materials: the first six (the block's faces); barrier rigid.
-    f = np.array([63.0, 125.0])
-    y = delany_bazley_admittance(f)
-    base = float(np.real(y).min()) * 0.5
-    rows = []
-    level = base
-    T = 1.0 / rate
-    for fj, yj in zip(f, np.real(y)):
-        rise = max(float(yj) - level, 1e-3)
-        R = 1.0 / rise
-        rows.append((0.0, R, 2.0 * math.pi * fj * R * T))
-        level += rise
+    base, rows = ground_branches(200e3, (63.0, 125.0), rate)
     beta = np.zeros(nm)
     beta[:6] = base
     branches, owners = [], []
