@@ -44,6 +44,7 @@
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "asfdtd.h"
@@ -192,6 +193,22 @@ struct StepArgs {
     // interior cells that are not deep.
     uint32_t cell_blocks, cell_count;
     const uint32_t *__restrict__ cells;
+    // The general specials, where few distinct ones repeat over the grid (a plane's cut
+    // cells along its axes): each special's record (its cell, its kind, its first chunk of
+    // link directions, links | states << 8) and the kinds' tables: coefficients, link weights
+    // and chunks of directions (kind_links a kind), states' weights and branches (kind_states
+    // a kind). The states' velocities stay in the blocks' slabs. Null sp_rec: the slabs'
+    // own arrays above. (Last, so that the other arguments keep their places.)
+    const uint4 *__restrict__ sp_rec;
+    const float *__restrict__ kind_coef_a;
+    const T *__restrict__ kind_coef_b;
+    const T *__restrict__ kind_coef_inv;
+    const float *__restrict__ kind_source;
+    const float *__restrict__ kind_weight;
+    const uint32_t *__restrict__ kind_dirs;
+    const float *__restrict__ kind_state_weight;
+    const uint16_t *__restrict__ kind_state_branch;
+    uint32_t kind_links, kind_states;
 };
 
 // A face special (see the tiles' notes): a special cell whose links are axial and weigh
@@ -500,31 +517,50 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
 // everything indexed by its cell and branches (its pressures, its links' far ends, its
 // branches' coefficients); chunks of links and states past the first, rare, follow one
 // by one.
-template <typename T>
+//
+// With KINDS, what is the special's own (coefficients, links, states' weights and branches)
+// comes from its kind's rows of small tables, which stay in cache, and the special's record
+// gives its cell, kind and first chunk of directions in one load: the same operands, so the
+// same arithmetic in the same order.
+template <typename T, bool KINDS>
 __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
     // States taken with the first loads (the rest one by one, in the same order); three in
     // double keep the step within five wavefronts a SIMD.
     constexpr int kStateChunk = sizeof(T) == 8 ? 3 : 4;
     if (k >= s.specials) return;
     const uint32_t block = k / kBlock, t = k % kBlock;
-    const int i = static_cast<int>(s.sp_cell[k]);
-    const int nl = s.sp_links[k], ns = s.sp_states[k];
-    const float ca = s.coef_a[k], src = s.source[k];
-    const T cb = s.coef_b[k], ci = s.coef_inv[k];
+    uint4 r{};
+    if constexpr (KINDS) r = s.sp_rec[k];
+    const uint32_t kd = KINDS ? r.y : 0u;
+    const int i = KINDS ? static_cast<int>(r.x) : static_cast<int>(s.sp_cell[k]);
+    const int nl = KINDS ? static_cast<int>(r.w & 255u) : s.sp_links[k];
+    const int ns = KINDS ? static_cast<int>(r.w >> 8) : s.sp_states[k];
+    const float ca = KINDS ? s.kind_coef_a[kd] : s.coef_a[k];
+    const float src = KINDS ? s.kind_source[kd] : s.source[k];
+    const T cb = KINDS ? s.kind_coef_b[kd] : s.coef_b[k];
+    const T ci = KINDS ? s.kind_coef_inv[kd] : s.coef_inv[k];
     const float sig = s.signal[s.step];
-    const uint32_t lb = s.link_base[block] + t;
-    const uint32_t db = s.link_base[block] / kLinkChunk + t;
+    // Link u's weight at lb + u link_step, chunk c's directions at db + c link_step; state
+    // u's weight and branch at kb + u (the kind's) or its slot (the block's slab).
+    const uint32_t link_step = KINDS ? 1u : kBlock;
+    const uint32_t lb = KINDS ? kd * s.kind_links : s.link_base[block] + t;
+    const uint32_t db = KINDS ? kd * s.kind_links / kLinkChunk : s.link_base[block] / kLinkChunk + t;
+    const uint32_t kb = KINDS ? kd * s.kind_states : 0u;
     const uint32_t sb = s.state_base[block] + t;
+    const float *__restrict__ link_weight = KINDS ? s.kind_weight : s.link_weight;
+    const uint32_t *__restrict__ link_dirs = KINDS ? s.kind_dirs : s.link_dirs;
+    const float *__restrict__ state_weight = KINDS ? s.kind_state_weight : s.state_weight;
+    const uint16_t *__restrict__ state_branch = KINDS ? s.kind_state_branch : s.state_branch;
     const int sy = s.px, sz = static_cast<int>(s.sxy);
     const T *__restrict__ b = s.b;
     int32_t off[kLinkChunk];
     float w[kLinkChunk];
-    const uint32_t dirs = nl > 0 ? s.link_dirs[db] : 0u;
+    const uint32_t dirs = KINDS ? r.z : (nl > 0 ? link_dirs[db] : 0u);
 #pragma unroll
     for (int u = 0; u < kLinkChunk; ++u) {
         if (u < nl) {
             off[u] = step_offset((dirs >> (5 * u)) & 31u, sy, sz);
-            w[u] = s.link_weight[lb + u * kBlock];
+            w[u] = link_weight[lb + u * link_step];
         }
     }
     float sw[kStateChunk], sn[kStateChunk], sbf[kStateChunk];
@@ -534,8 +570,9 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
     for (int u = 0; u < kStateChunk; ++u) {
         if (u < ns) {
             const uint32_t q = sb + u * kBlock;
-            sw[u] = s.state_weight[q];
-            kind[u] = s.state_branch[q];
+            const uint32_t e = KINDS ? kb + u : q;
+            sw[u] = state_weight[e];
+            kind[u] = state_branch[e];
             swa[u] = s.wa[q];
             swb[u] = s.wb[q];
         }
@@ -560,12 +597,12 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
         if (u < nl) sum += static_cast<T>(w[u]) * v[u];
     }
     for (int c = kLinkChunk; c < nl; c += kLinkChunk) {
-        const uint32_t more = s.link_dirs[db + (c / kLinkChunk) * kBlock];
+        const uint32_t more = link_dirs[db + (c / kLinkChunk) * link_step];
 #pragma unroll
         for (int u = 0; u < kLinkChunk; ++u) {
             if (c + u < nl) {
                 off[u] = step_offset((more >> (5 * u)) & 31u, sy, sz);
-                w[u] = s.link_weight[lb + (c + u) * kBlock];
+                w[u] = link_weight[lb + (c + u) * link_step];
             }
         }
 #pragma unroll
@@ -588,9 +625,10 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
     }
     for (int c = kStateChunk; c < ns; ++c) {
         const uint32_t q = sb + c * kBlock;
-        const uint32_t kd = s.state_branch[q];
-        const T term = static_cast<T>(s.branch_now[kd]) * s.wb[q] - static_cast<T>(s.branch_before[kd]) * s.wa[q];
-        flow += static_cast<T>(s.state_weight[q]) * term;
+        const uint32_t e = KINDS ? kb + c : q;
+        const uint32_t br = state_branch[e];
+        const T term = static_cast<T>(s.branch_now[br]) * s.wb[q] - static_cast<T>(s.branch_before[br]) * s.wa[q];
+        flow += static_cast<T>(state_weight[e]) * term;
     }
     T acc = static_cast<T>(ca) * here;
     acc = acc + sum;
@@ -613,17 +651,18 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
         for (int c = kStateChunk; c < ns; ++c) {
             // The term again: wa has not changed since it was taken.
             const uint32_t q = sb + c * kBlock;
-            const uint32_t kd = s.state_branch[q];
-            const T term = static_cast<T>(s.branch_now[kd]) * s.wb[q] - static_cast<T>(s.branch_before[kd]) * s.wa[q];
-            s.wo[q] = s.wa[q] + (rise * static_cast<T>(s.branch_gain[kd]) + term);
+            const uint32_t br = state_branch[KINDS ? kb + c : q];
+            const T term = static_cast<T>(s.branch_now[br]) * s.wb[q] - static_cast<T>(s.branch_before[br]) * s.wa[q];
+            s.wo[q] = s.wa[q] + (rise * static_cast<T>(s.branch_gain[br]) + term);
         }
     }
 }
 
-// The special blocks' specials: block b's thread t steps special b kBlock + t.
-template <typename T>
+// The special blocks' specials: block b's thread t steps special b kBlock + t, from the
+// kinds' tables (KINDS: the handle has them) or the blocks' slabs.
+template <typename T, bool KINDS>
 __device__ __forceinline__ void special_cell(const StepArgs<T> &s, uint32_t block) {
-    special_one<T>(s, block * kBlock + threadIdx.x);
+    special_one<T, KINDS>(s, block * kBlock + threadIdx.x);
 }
 
 // n / d for n < 2^31 by a multiply: magic = floor(2^32 (2^shift - d) / d) + 1 with
@@ -764,7 +803,7 @@ __device__ __forceinline__ void interior_cell(const StepArgs<T> &s, int i) {
 // field, which nothing writes), then special blocks (their gathers are the longest chains
 // of loads), face specials by blocks of their own (unless they are stepped in their
 // tiles), interior tiles, layer cells and listed interior cells.
-template <typename T, bool WIDE, bool SCALED>
+template <typename T, bool WIDE, bool SCALED, bool KINDS>
 __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
     uint32_t block = blockIdx.x;
     if (block < s.probe_blocks) {
@@ -773,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
     }
     block -= s.probe_blocks;
     if (block < s.special_blocks) {
-        special_cell<T>(s, block);
+        special_cell<T, KINDS>(s, block);
         return;
     }
     block -= s.special_blocks;
@@ -803,9 +842,9 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const StepArgs<T> s) {
 }
 
 // The same work as four launches, one per kind of cell (the split launch shape).
-template <typename T>
+template <typename T, bool KINDS>
 __global__ __launch_bounds__(kBlock) void special_kernel(const StepArgs<T> s) {
-    special_cell<T>(s, blockIdx.x);
+    special_cell<T, KINDS>(s, blockIdx.x);
 }
 
 template <typename T>
@@ -1242,7 +1281,10 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(const StepArgs<float> s, c
     }
     block -= s.probe_blocks;
     if (block < s.special_blocks) {
-        special_cell<float>(s, block);
+        if (s.sp_rec != nullptr)
+            special_cell<float, true>(s, block);
+        else
+            special_cell<float, false>(s, block);
         return;
     }
     block -= s.special_blocks;
@@ -1321,6 +1363,13 @@ struct as_fdtd {
     void *coef_b = nullptr, *coef_inv = nullptr, *branch_gain = nullptr;  // float or double, as the precision
     float *state_weight = nullptr, *branch_now = nullptr, *branch_before = nullptr;
     uint16_t *state_branch = nullptr;
+    // The general specials' kinds (see StepArgs::sp_rec), or null sp_rec.
+    uint4 *sp_rec = nullptr;
+    float *kind_coef_a = nullptr, *kind_source = nullptr, *kind_weight = nullptr, *kind_state_weight = nullptr;
+    void *kind_coef_b = nullptr, *kind_coef_inv = nullptr;  // float or double, as the precision
+    uint32_t *kind_dirs = nullptr;
+    uint16_t *kind_state_branch = nullptr;
+    uint32_t kinds = 0, kind_links = 0, kind_states = 0;
     size_t state_slots = 0;   // the states' arrays' length (general slabs, then face states)
     uint32_t face_specials = 0;
     uint32_t *face_tile = nullptr;
@@ -1817,6 +1866,96 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             state_branch[slot] = static_cast<uint16_t>(L.state_branch[j]);
         }
     }
+    // The specials' kinds: specials alike in everything but their cell and their states'
+    // velocities (coefficients, links' directions and weights, states' branches and
+    // weights, bit for bit) share one row of the kinds' tables. Used where a kind serves
+    // four specials or more on average (the cut cells of a plane repeat along the plane's
+    // axes: the fitted hall's 486,785 general specials are 5,003 kinds), or as asked.
+    std::vector<uint32_t> sp_kind(G);
+    std::vector<uint64_t> kind_first;  // per kind: a sorted special of it
+    {
+        std::unordered_map<std::string, uint32_t> seen;
+        std::string key;
+        for (uint64_t k = 0; k < G; ++k) {
+            const uint64_t src = order[k];
+            key.clear();
+            const auto put = [&](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
+            put(&coef_a[k], 4);
+            put(&coef_b[k], 8);
+            put(&coef_inv[k], 8);
+            put(&source[k], 4);
+            const uint64_t nl = links_of(src), ns = states_of(src);
+            put(&nl, 8);
+            put(&ns, 8);
+            for (uint64_t u = 0; u < nl; ++u) {
+                const uint64_t l = L.link_start[src] + u;
+                put(&L.link_direction[l], 1);
+                put(&L.link_weight[l], 4);
+            }
+            for (uint64_t u = 0; u < ns; ++u) {
+                const uint64_t j = L.state_start[src] + u;
+                put(&L.state_branch[j], 4);
+                put(&L.state_weight[j], 4);
+            }
+            const auto [at, fresh] = seen.emplace(key, static_cast<uint32_t>(kind_first.size()));
+            if (fresh) kind_first.push_back(k);
+            sp_kind[k] = at->second;
+        }
+    }
+    const int kinds_asked = tuning ? tuning->special_kinds : 0;
+    const bool use_kinds = G > 0 && (kinds_asked == 2 || (kinds_asked == 0 && kind_first.size() * 4 <= G));
+    std::vector<uint4> sp_rec;
+    std::vector<float> kind_coef_a, kind_source, kind_weight, kind_state_weight;
+    std::vector<double> kind_coef_b, kind_coef_inv;
+    std::vector<uint32_t> kind_dirs;
+    std::vector<uint16_t> kind_state_branch;
+    if (use_kinds) {
+        const uint64_t nk = kind_first.size();
+        uint64_t wl = 0, ws = 0;
+        for (uint64_t q : kind_first) {
+            wl = std::max(wl, links_of(order[q]));
+            ws = std::max(ws, states_of(order[q]));
+        }
+        wl = std::max<uint64_t>((wl + kLinkChunk - 1) / kLinkChunk * kLinkChunk, kLinkChunk);
+        ws = std::max<uint64_t>(ws, 1);
+        if (nk * wl >= (1ull << 32) || nk * ws >= (1ull << 32))
+            throw Failure{AS_ERANGE, "too many links or states in the specials' kinds for 32-bit indices"};
+        kind_coef_a.resize(nk);
+        kind_source.resize(nk);
+        kind_coef_b.resize(nk);
+        kind_coef_inv.resize(nk);
+        kind_weight.assign(nk * wl, 0.0f);
+        kind_dirs.assign(nk * wl / kLinkChunk, 0u);
+        kind_state_weight.assign(nk * ws, 0.0f);
+        kind_state_branch.assign(nk * ws, 0);
+        for (uint64_t c = 0; c < nk; ++c) {
+            const uint64_t k = kind_first[c], src = order[k];
+            kind_coef_a[c] = coef_a[k];
+            kind_source[c] = source[k];
+            kind_coef_b[c] = coef_b[k];
+            kind_coef_inv[c] = coef_inv[k];
+            for (uint64_t u = 0; u < links_of(src); ++u) {
+                const uint64_t l = L.link_start[src] + u;
+                kind_weight[c * wl + u] = L.link_weight[l];
+                kind_dirs[c * wl / kLinkChunk + u / kLinkChunk] |= static_cast<uint32_t>(L.link_direction[l])
+                                                                   << (5 * (u % kLinkChunk));
+            }
+            for (uint64_t u = 0; u < states_of(src); ++u) {
+                const uint64_t j = L.state_start[src] + u;
+                kind_state_weight[c * ws + u] = L.state_weight[j];
+                kind_state_branch[c * ws + u] = static_cast<uint16_t>(L.state_branch[j]);
+            }
+        }
+        sp_rec.resize(G);
+        for (uint64_t k = 0; k < G; ++k) {
+            const uint32_t c = sp_kind[k];
+            sp_rec[k] = make_uint4(sp_cell[k], c, kind_dirs[c * wl / kLinkChunk],
+                                   static_cast<uint32_t>(sp_links[k]) | static_cast<uint32_t>(sp_states[k]) << 8);
+        }
+        h->kinds = static_cast<uint32_t>(nk);
+        h->kind_links = static_cast<uint32_t>(wl);
+        h->kind_states = static_cast<uint32_t>(ws);
+    }
     h->specials = static_cast<uint32_t>(G);
     // Where each layout special and state went, for the adjoint's sensitivities.
     h->sorted_to_special = order;
@@ -2072,6 +2211,15 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->source = upload(h, source);
     h->state_weight = upload(h, state_weight);
     h->state_branch = upload(h, state_branch);
+    h->sp_rec = upload(h, sp_rec);
+    h->kind_coef_a = upload(h, kind_coef_a);
+    h->kind_source = upload(h, kind_source);
+    h->kind_coef_b = kind_coef_b.empty() ? nullptr : upload_as(h, kind_coef_b, precision);
+    h->kind_coef_inv = kind_coef_inv.empty() ? nullptr : upload_as(h, kind_coef_inv, precision);
+    h->kind_weight = upload(h, kind_weight);
+    h->kind_dirs = upload(h, kind_dirs);
+    h->kind_state_weight = upload(h, kind_state_weight);
+    h->kind_state_branch = upload(h, kind_state_branch);
     {
         std::vector<float> now(L.branch_now, L.branch_now + L.branches), before(L.branch_before, L.branch_before + L.branches);
         std::vector<double> gain(L.branches);
@@ -2214,6 +2362,17 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.source = h->source;
     s.state_weight = h->state_weight;
     s.state_branch = h->state_branch;
+    s.sp_rec = h->sp_rec;
+    s.kind_coef_a = h->kind_coef_a;
+    s.kind_coef_b = static_cast<const T *>(h->kind_coef_b);
+    s.kind_coef_inv = static_cast<const T *>(h->kind_coef_inv);
+    s.kind_source = h->kind_source;
+    s.kind_weight = h->kind_weight;
+    s.kind_dirs = h->kind_dirs;
+    s.kind_state_weight = h->kind_state_weight;
+    s.kind_state_branch = h->kind_state_branch;
+    s.kind_links = h->kind_links;
+    s.kind_states = h->kind_states;
     s.branch_now = h->branch_now;
     s.branch_before = h->branch_before;
     s.branch_gain = static_cast<const T *>(h->branch_gain);
@@ -2589,21 +2748,34 @@ void begin_run(as_fdtd *h, size_t steps) {
 template <typename T>
 void launch_fused(const as_fdtd *h, const StepArgs<T> &s, uint32_t blocks, hipStream_t stream) {
     const dim3 grid(blocks), block(kBlock);
-    if (h->wide && h->scaled) {
-        hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, 0, stream, s);
+    if (s.sp_rec != nullptr) {
+        if (h->wide && h->scaled) {
+            hipLaunchKernelGGL((step_kernel<T, true, true, true>), grid, block, 0, stream, s);
+        } else if (h->wide) {
+            hipLaunchKernelGGL((step_kernel<T, true, false, true>), grid, block, 0, stream, s);
+        } else if (h->scaled) {
+            hipLaunchKernelGGL((step_kernel<T, false, true, true>), grid, block, 0, stream, s);
+        } else {
+            hipLaunchKernelGGL((step_kernel<T, false, false, true>), grid, block, 0, stream, s);
+        }
+    } else if (h->wide && h->scaled) {
+        hipLaunchKernelGGL((step_kernel<T, true, true, false>), grid, block, 0, stream, s);
     } else if (h->wide) {
-        hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, 0, stream, s);
+        hipLaunchKernelGGL((step_kernel<T, true, false, false>), grid, block, 0, stream, s);
     } else if (h->scaled) {
-        hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, 0, stream, s);
+        hipLaunchKernelGGL((step_kernel<T, false, true, false>), grid, block, 0, stream, s);
     } else {
-        hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, 0, stream, s);
+        hipLaunchKernelGGL((step_kernel<T, false, false, false>), grid, block, 0, stream, s);
     }
 }
 
 // A step as a launch per kind of cell (the split launch shape); the probes last, where asked.
 template <typename T>
 void launch_split(const as_fdtd *h, const StepArgs<T> &s, hipStream_t stream, bool probes) {
-    if (s.special_blocks) hipLaunchKernelGGL((special_kernel<T>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
+    if (s.special_blocks && s.sp_rec != nullptr)
+        hipLaunchKernelGGL((special_kernel<T, true>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
+    else if (s.special_blocks)
+        hipLaunchKernelGGL((special_kernel<T, false>), dim3(s.special_blocks), dim3(kBlock), 0, stream, s);
     if (s.face_blocks) hipLaunchKernelGGL((face_kernel<T>), dim3(s.face_blocks), dim3(kBlock), 0, stream, s);
     if (s.tile_blocks) {
         const dim3 grid(s.tile_blocks), block(kBlock);
@@ -2753,9 +2925,12 @@ int enqueue_steps(as_fdtd *h, StepArgs<T> s, uint32_t blocks, int prev, size_t k
                 second.layer_face_in = static_cast<const float *>(h->layer_face[(prev & 1) ^ 1]);
                 second.layer_face_out = static_cast<float *>(h->layer_face[prev & 1]);
                 second.step = k + 1;
-                if (second_blocks) {
-                    hipLaunchKernelGGL((step_kernel<float, false, false>), dim3(second_blocks), dim3(kBlock), 0, stream,
-                                       second);
+                if (second_blocks && second.sp_rec != nullptr) {
+                    hipLaunchKernelGGL((step_kernel<float, false, false, true>), dim3(second_blocks), dim3(kBlock), 0,
+                                       stream, second);
+                } else if (second_blocks) {
+                    hipLaunchKernelGGL((step_kernel<float, false, false, false>), dim3(second_blocks), dim3(kBlock), 0,
+                                       stream, second);
                 }
                 prev = next;
             }
@@ -3674,6 +3849,7 @@ int as_fdtd_info(const as_fdtd *h, uint64_t *out) {
     out[7] = h->done;
     out[8] = h->pairs ? static_cast<uint64_t>(h->pair_z) : 0u;
     out[9] = h->pairs ? h->cell_count : 0u;
+    out[10] = h->sp_rec != nullptr ? h->kinds : 0u;
     return AS_OK;
 }
 

@@ -29,12 +29,14 @@ class HipWaveSolver:
     or 8 or 16 for the planes a box marches). ``keep_history`` (record and adjoint): True keeps
     every step's history of the special cells in the record, so that the adjoint recomputes
     nothing; False keeps the checkpoints only, each segment recomputed from its own; None,
-    the library's choice (the history where it fits 16 GiB and a quarter of free memory)."""
+    the library's choice (the history where it fits 16 GiB and a quarter of free memory).
+    ``special_order`` and ``special_kinds`` are as_tuning's (the general specials' order in
+    their blocks, and whether they read their kinds' tables: 2, or their own slabs: 1)."""
 
     def __init__(self, solver, precision: int = 32, tile_z: int = 0, row_align: int = 0,
                  graph_steps: int = 0, split: bool = False, tile_order: int = 0,
                  no_faces: bool = False, face_blocks=None, pairs: bool = False, keep_history=None,
-                 special_order: int = 0) -> None:
+                 special_order: int = 0, special_kinds: int = 0) -> None:
         lib = native.load()
         layout = solver if isinstance(solver, dict) else solver.layout()
         struct, keep = native.pack_layout(layout)
@@ -42,7 +44,8 @@ class HipWaveSolver:
                                int(tile_order), 1 if no_faces else 0,
                                0 if face_blocks is None else (1 if face_blocks else -1),
                                int(pairs) if not isinstance(pairs, bool) and int(pairs) > 1 else (1 if pairs else 0),
-                               0 if keep_history is None else (1 if keep_history else -1), int(special_order))
+                               0 if keep_history is None else (1 if keep_history else -1), int(special_order),
+                               int(special_kinds))
         h = lib.as_fdtd_create(ctypes.byref(struct), int(precision), ctypes.byref(tuning))
         if not h:
             message = lib.as_fdtd_create_error().decode()
@@ -143,10 +146,10 @@ class HipWaveSolver:
         return int(self._lib.as_fdtd_stream(self._h) or 0)
 
     def info(self) -> dict:
-        out = np.zeros(10, np.uint64)
+        out = np.zeros(11, np.uint64)
         self._check(self._lib.as_fdtd_info(self._h, out.ctypes.data))
         keys = ("interior_cells", "specials", "states", "layer_cells", "probes", "tiles",
-                "row_pitch", "steps_done", "pair_planes", "pair_cells")
+                "row_pitch", "steps_done", "pair_planes", "pair_cells", "special_kinds")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def close(self) -> None:

@@ -46,7 +46,7 @@ class Tuning(ctypes.Structure):
                 ("graph_steps", ctypes.c_int), ("split", ctypes.c_int),
                 ("tile_order", ctypes.c_int), ("no_faces", ctypes.c_int),
                 ("face_blocks", ctypes.c_int), ("pairs", ctypes.c_int), ("keep_history", ctypes.c_int),
-                ("special_order", ctypes.c_int)]
+                ("special_order", ctypes.c_int), ("special_kinds", ctypes.c_int)]
 
 
 _lib = None

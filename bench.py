@@ -4,8 +4,9 @@ A step is one time step of the FDTD scheme over the whole grid of the hall workl
 30 x 20 x 12 m hall with a ceiling of six tilted planes and side walls splayed 5 degrees, at
 10 cm cells (305 x 257 x 125 cells, about 7.57M air cells), fitted walls (each cell a face
 cuts holds its own air volume, open areas and wall areas, diagonal links where the planes
-are tilted), a constant admittance and three first-order (relaxation) admittance branches
-per material, one source, 16 receivers; 2 s of response is 11,882 steps. The grid is laid
+are tilted), a constant admittance and three first-order admittance branches (each rising
+or falling, fitted to the material's drawn absorption) per material, one source, 16
+receivers; 2 s of response is 11,882 steps. The grid is laid
 out by this package's fit (csrc/fitted.cpp, csrc/setup.cpp) and stepped by the HIP
 kernels (csrc/fdtd_step.hip), in the launch shape the autotuner finds fastest on the
 scene's drive (part of the set-up); the drive and the probes' output stay in device memory
@@ -312,10 +313,10 @@ def plain(scene, name, args, traffic_name=None):
         setup = {"device": round(setup_s, 3), "host": round(host_setup_s, 3),
                  "device_is": "every pass over the whole grid on the GPU (cut links, flood, layer, wall cells); the source's and probes' readings on the host",
                  "host_threads": host_threads}
-    if args.tile_z or args.graph_steps or args.face_blocks >= 0 or args.special_order:
+    if args.tile_z or args.graph_steps or args.face_blocks >= 0 or args.special_order or args.special_kinds:
         launch = {"tile_z": args.tile_z, "graph_steps": args.graph_steps,
                   "face_blocks": None if args.face_blocks < 0 else bool(args.face_blocks), "pairs": 0,
-                  "special_order": args.special_order}
+                  "special_order": args.special_order, "special_kinds": args.special_kinds}
         launch_from = "flags (0 or null: the library's default)"
     else:
         # Set-up, outside the timed region: the launch shape timed on the scene's drive.
@@ -335,7 +336,7 @@ def plain(scene, name, args, traffic_name=None):
     step_bytes = algorithmic_bytes(info, air, solver.layer_cells, solver.branch_states)
     config = {
         "air_cells": air, "layer_cells": solver.layer_cells, "branch_states": solver.branch_states,
-        "special_cells": info["specials"], "grid": list(scene.grid.shape),
+        "special_cells": info["specials"], "special_kinds": info["special_kinds"], "grid": list(scene.grid.shape),
         "setup_s": setup,
         "launch": launch, "launch_from": launch_from,
         "timed_launches": launch_mode(launch["graph_steps"], K) + (
@@ -733,6 +734,9 @@ def main(argv=None) -> int:
     ap.add_argument("--special-order", type=int, default=0, choices=(0, 1, 2, 3),
                     help="the general specials' order: 1 by their counts, 2 by cell, 3 by tile (0: the "
                          "library's, 3)")
+    ap.add_argument("--special-kinds", type=int, default=0, choices=(0, 1, 2),
+                    help="the general specials read their kinds' tables (2) or their own slabs (1); "
+                         "0: the library's choice")
     ap.add_argument("--split", action="store_true",
                     help="launch the step's roles (probes, general specials, face specials, tiles, layer) "
                          "as kernels of their own, for a profile of each; the line then times that")
@@ -774,8 +778,8 @@ def main(argv=None) -> int:
         line["config"]["workload"] = (
             "hall 30x20x12 m, ceiling of 6 tilted planes, side walls splayed 5 deg, 10 cm cells "
             f"{scene.grid.shape[0]}x{scene.grid.shape[1]}x{scene.grid.shape[2]}, "
-            f"{line['config']['air_cells']:,} air cells, {walls}, 3 relaxation branches per "
-            "material, 2 sources (the timed steps from the first; the outputs of both), 16 receivers")
+            f"{line['config']['air_cells']:,} air cells, {walls}, 3 first-order admittance branches "
+            "per material (rising or falling, fitted to its drawn absorption), 2 sources (the timed steps from the first; the outputs of both), 16 receivers")
     elif args.workload == "shoebox":
         args.steps = args.steps or 5941
         scene = scenes.shoebox()

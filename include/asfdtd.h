@@ -117,6 +117,11 @@ typedef struct as_tuning {
     int special_order; /* the general specials' order in their blocks: 1 by their counts of links
                           and states, 2 by cell (the layout's order), 3 by the interior tile
                           holding the cell; 0, the library's choice (3) */
+    int special_kinds; /* 2: the general specials' coefficients, links and states read from tables
+                          of their distinct kinds (specials alike in all but their cell and their
+                          states' velocities share a row), each special a 16-byte record; 1: each
+                          special's own, from its block's slabs; 0, the library's choice (the
+                          kinds where a kind serves four specials or more on average) */
 } as_tuning;
 
 /* Copies the layout to the device; precision 32 (the reference's float scheme) or 64 (the
@@ -167,10 +172,11 @@ int as_fdtd_field(as_fdtd *h, double *out);
 /* The handle's HIP stream (a hipStream_t), for the caller's events. */
 void *as_fdtd_stream(as_fdtd *h);
 
-/* Counts (out holds 10): [0] cells stepped as interior, [1] specials, [2] states, [3] layer
+/* Counts (out holds 11): [0] cells stepped as interior, [1] specials, [2] states, [3] layer
  * cells, [4] probes, [5] active tiles, [6] row pitch, [7] steps done, [8] planes a box of
  * two steps marches (0: one step a launch), [9] interior cells the second launch of two
- * steps takes one by one. */
+ * steps takes one by one, [10] the general specials' kinds read from tables (0: each
+ * special's own slabs; see as_tuning.special_kinds). */
 int as_fdtd_info(const as_fdtd *h, uint64_t *out);
 
 const char *as_fdtd_last_error(const as_fdtd *h);

@@ -30,10 +30,19 @@ def test_fitted_hall_steps_equal_the_oracle():
     layout = _fitted(scene)
     print(f"fitted hall: {len(layout['special'])} specials", flush=True)
     gpu = HipWaveSolver(layout)
+    # The library's choice: the kinds' tables (486,785 specials of 5,003 kinds).
+    assert 0 < gpu.info()["special_kinds"] <= len(layout["special"]) // 4
     cpu = OracleSolver(layout)
     d = scene.drive[40:190]
-    assert np.array_equal(gpu.run(d[:75]), cpu.run(d[:75]))
-    assert np.array_equal(gpu.run(d[75:]), cpu.run(d[75:]))
+    first, second = cpu.run(d[:75]), cpu.run(d[75:])
+    assert np.array_equal(gpu.run(d[:75]), first)
+    assert np.array_equal(gpu.run(d[75:]), second)
+    assert np.array_equal(gpu.field(), cpu.field())
+    # Each special's own slabs: the same run.
+    slabs = HipWaveSolver(layout, special_kinds=1)
+    assert slabs.info()["special_kinds"] == 0
+    assert np.array_equal(slabs.run(d), np.concatenate([first, second], axis=1))
+    assert np.array_equal(slabs.field(), cpu.field())
 
 
 @pytest.mark.gpu
@@ -63,9 +72,11 @@ def test_fitted_hall_receivers_match_the_reference(source):
 @pytest.mark.gpu
 @pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("split", [False, True])
-def test_special_orders_step_alike(order, split):
-    """The general specials in each order (by counts, by cell, by tile), fused and split:
-    the coarse hall's fitted layout, 300 steps in two calls, bit for bit the oracle's."""
+@pytest.mark.parametrize("kinds", [1, 2])
+def test_special_orders_step_alike(order, split, kinds):
+    """The general specials in each order (by counts, by cell, by tile), fused and split,
+    from their slabs (kinds 1) or their kinds' tables (2): the coarse hall's fitted layout,
+    300 steps in two calls, bit for bit the oracle's."""
     import sys
 
     sys.path.insert(0, str(Path(__file__).parent))
@@ -75,7 +86,8 @@ def test_special_orders_step_alike(order, split):
 
     scene = cases.coarse_hall()
     layout = _fitted(scene)
-    gpu = HipWaveSolver(layout, special_order=order, split=split)
+    gpu = HipWaveSolver(layout, special_order=order, split=split, special_kinds=kinds)
+    assert (gpu.info()["special_kinds"] > 0) == (kinds == 2)
     cpu = OracleSolver(layout)
     d = scene.drive[:300]
     assert np.array_equal(gpu.run(d[:151]), cpu.run(d[:151]))

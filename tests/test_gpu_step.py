@@ -35,14 +35,18 @@ CASES = {
 
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("precision", [32, 64])
-def test_steps_equal_the_oracle(reference_available, name, precision):
+@pytest.mark.parametrize("kinds", [1, 2])
+def test_steps_equal_the_oracle(reference_available, name, precision, kinds):
+    """kinds 1: each general special's own slabs; 2: its kind's row of the kinds' tables."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
     scene, kw = CASES[name]()
     layout = _layout(scene, **kw)
-    print(f"case {name} precision {precision} shape {scene.grid.shape}", flush=True)
-    gpu = HipWaveSolver(layout, precision=precision)
+    print(f"case {name} precision {precision} kinds {kinds} shape {scene.grid.shape}", flush=True)
+    gpu = HipWaveSolver(layout, precision=precision, special_kinds=kinds)
+    if kinds == 1:
+        assert gpu.info()["special_kinds"] == 0
     cpu = OracleSolver(layout, precision=precision)
     drive = scene.drive[: scene.steps]
     # Split so that the fields' parity flips both ways across calls (tests/test_gpu.py:56-85).
@@ -127,17 +131,19 @@ def test_two_steps_a_launch_in_any_pieces(reference_available, pieces, tile_z):
 
 @pytest.mark.parametrize("planes", [8, 16])
 @pytest.mark.parametrize("no_faces", [False, True])
-def test_two_launches_on_general_specials_and_the_layer(reference_available, planes, no_faces):
+@pytest.mark.parametrize("kinds", [1, 2])
+def test_two_launches_on_general_specials_and_the_layer(reference_available, planes, no_faces, kinds):
     """Two steps by two launches with every special general (no_faces) or face specials by
     their own blocks, on the matched-layer scene (boxes of 8 or 16 planes) and on a room
-    with branch states (8 planes): the probes and fields equal the oracle's."""
+    with branch states (8 planes), the general specials from their slabs (kinds 1) or their
+    kinds' tables (2): the probes and fields equal the oracle's."""
     from acoustic_spaces_amd.gpu import HipWaveSolver
     from oracle.stepper import OracleSolver
 
     for scene in (cases.open_block(), cases.with_branches(cases.small_box())):
         layout = _layout(scene)
-        print(f"two launches planes {planes} no_faces {no_faces} shape {scene.grid.shape}", flush=True)
-        gpu = HipWaveSolver(layout, pairs=planes, no_faces=no_faces)
+        print(f"two launches planes {planes} no_faces {no_faces} kinds {kinds} shape {scene.grid.shape}", flush=True)
+        gpu = HipWaveSolver(layout, pairs=planes, no_faces=no_faces, special_kinds=kinds)
         expect = planes if scene.grid.shape[2] - 2 >= planes else 8
         assert gpu.info()["pair_planes"] == expect
         cpu = OracleSolver(layout)
