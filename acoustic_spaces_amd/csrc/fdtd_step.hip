@@ -209,6 +209,9 @@ struct StepArgs {
     const float *__restrict__ kind_state_weight;
     const uint16_t *__restrict__ kind_state_branch;
     uint32_t kind_links, kind_states;
+    // Each row's interior cells as bits, a word per kTileX cells: word (z ny + y) tiles_x +
+    // x / kTileX, bit x mod kTileX (see interior_column).
+    const uint64_t *__restrict__ interior_bits;
 };
 
 // A face special (see the tiles' notes): a special cell whose links are axial and weigh
@@ -360,18 +363,27 @@ __device__ __forceinline__ void face_special_at(const StepArgs<T> &s, uint32_t r
 // twice a plane. Each address lies in the padded grid whatever the cell's class
 // (x < nx <= px, 1 <= y <= ny - 2, 1 <= z <= nz - 2), so the loads need no guard; only
 // interior cells are written.
+//
+// Where the tile is not all interior, a lane's cells are told by the row's interior bits
+// (one word a plane for the wavefront's 64 cells, a scalar load: `word` is the word of the
+// lane's row at plane z0, uniform over the wavefront, `plane` the words a plane), or, with
+// TAPS, by the cells' classes.
 template <typename T, bool SCALED, bool ALL, bool TAPS = false>
-__device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int z0, int z1) {
+__device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int z0, int z1, uint32_t word = 0,
+                                                uint32_t plane = 0) {
     constexpr int kUnroll = 4;
+    constexpr bool kBits = !ALL && !TAPS;
     const int sy = s.px, sz = static_cast<int>(s.sxy);
+    const uint32_t lane = threadIdx.x % kTileX;
     const T *__restrict__ b = s.b;
     T col[kUnroll + 2];  // planes z - 1, z, z + 1, ..., z + kUnroll
     col[0] = b[i - sz];
     col[1] = b[i];
-    for (int z = z0; z < z1; z += kUnroll, i += kUnroll * sz) {
+    for (int z = z0; z < z1; z += kUnroll, i += kUnroll * sz, word += kUnroll * plane) {
         const int count = min(kUnroll, z1 - z);
         T xm[kUnroll], xp[kUnroll], ym[kUnroll], yp[kUnroll], prev[kUnroll];
         uint8_t cls[kUnroll];
+        uint64_t bits[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
             if (u < count) {
@@ -382,12 +394,16 @@ __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int
                 ym[u] = b[j - sy];
                 yp[u] = b[j + sy];
                 prev[u] = s.a[j];
-                if constexpr (!ALL) cls[u] = s.types[j];
+                if constexpr (kBits) bits[u] = s.interior_bits[word + u * plane];
+                else if constexpr (!ALL) cls[u] = s.types[j];
             }
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            if (u < count && (ALL || cls[u] == kClassInterior || (TAPS && cls[u] == kClassTap))) {
+            bool step = ALL;
+            if constexpr (kBits) step = ((bits[u] >> lane) & 1u) != 0;
+            else if constexpr (!ALL) step = cls[u] == kClassInterior || (TAPS && cls[u] == kClassTap);
+            if (u < count && step) {
                 // The reference's order (fdtd.hpp:240-288): -x, +x, -y, +y, -z, +z.
                 T sum = xm[u] + xp[u];
                 sum = sum + ym[u];
@@ -445,7 +461,12 @@ __device__ __forceinline__ void interior_tile(const StepArgs<T> &s, uint32_t blo
             if (all) {
                 interior_column<T, SCALED, true>(s, i, z0, z1);
             } else {
-                interior_column<T, SCALED, false, TAPS>(s, i, z0, z1);
+                // The row's interior bits at plane z0: the same word for the whole wavefront.
+                const uint32_t row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTileX);
+                const uint32_t yr = 1 + ty * kTileRows + row;
+                const uint32_t plane = static_cast<uint32_t>(s.ny) * s.tiles_x;
+                const uint32_t word = (static_cast<uint32_t>(z0) * static_cast<uint32_t>(s.ny) + yr) * s.tiles_x + tx;
+                interior_column<T, SCALED, false, TAPS>(s, i, z0, z1, word, plane);
             }
         }
         if (!all) {
@@ -1356,6 +1377,7 @@ struct as_fdtd {
     uint64_t air_cells = 0, run_steps = 0;
     uint32_t *tiles = nullptr;
     uint64_t *tile_lanes = nullptr;  // per tile block and row: its lanes that march (see interior_tile)
+    uint64_t *interior_bits = nullptr;  // each row's interior cells (see StepArgs::interior_bits)
     uint32_t *sp_cell = nullptr, *link_base = nullptr, *state_base = nullptr;
     uint8_t *sp_links = nullptr, *sp_states = nullptr;
     uint32_t *link_dirs = nullptr;
@@ -1789,6 +1811,16 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         }
     });
 
+    // Each row's interior cells as bits (see StepArgs::interior_bits), planes among threads.
+    std::vector<uint64_t> interior_bits(static_cast<size_t>(nz) * ny * h->tiles_x, 0);
+    in_slices(nz, 1, [&](unsigned, uint64_t a, uint64_t b) {
+        for (uint64_t z = a; z < b; ++z)
+            for (uint64_t y = 0; y < ny; ++y)
+                for (uint64_t x = 0; x < nx; ++x)
+                    if (types[x + px * (y + ny * z)] == kClassInterior)
+                        interior_bits[(z * ny + y) * h->tiles_x + x / kTileX] |= uint64_t{1} << (x % kTileX);
+    });
+
     // General specials: in the order of the interior tiles holding them, or by their counts
     // of links and states, or by cell (a special's update reads only its own cell, links and
     // states, so their order is free), in blocks of kBlock;
@@ -2198,6 +2230,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->scale = upload(h, scale);
     h->tiles = upload(h, tiles);
     h->tile_lanes = upload(h, tile_lanes);
+    h->interior_bits = upload(h, interior_bits);
     h->sp_cell = upload(h, sp_cell);
     h->sp_links = upload(h, sp_links);
     h->sp_states = upload(h, sp_states);
@@ -2332,6 +2365,7 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.scale = h->scale;
     s.tiles = h->tiles;
     s.tile_lanes = h->tile_lanes;
+    s.interior_bits = h->interior_bits;
     s.tile_blocks = h->tile_count;
     s.tiles_x = h->tiles_x;
     s.tiles_y = h->tiles_y;
