@@ -365,21 +365,27 @@ __device__ __forceinline__ void face_special_at(const StepArgs<T> &s, uint32_t r
 // interior cells are written.
 //
 // Where the tile is not all interior, a lane's cells are told by the row's interior bits
-// (one word a plane for the wavefront's 64 cells, a scalar load: `word` is the word of the
-// lane's row at plane z0, uniform over the wavefront, `plane` the words a plane), or, with
-// TAPS, by the cells' classes.
+// (one word a plane for the wavefront's 64 cells: `word` is the word of the lane's row at
+// plane z0, the same for the whole wavefront, `plane` the words a plane), or, with TAPS, by
+// the cells' classes.
 template <typename T, bool SCALED, bool ALL, bool TAPS = false>
 __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int z0, int z1, uint32_t word = 0,
                                                 uint32_t plane = 0) {
     constexpr int kUnroll = 4;
     constexpr bool kBits = !ALL && !TAPS;
+    // A wavefront reads its row's word by a vector load of one address, in order with its
+    // other loads: the index is kept in a vector register (an empty asm statement hides
+    // that it is uniform), since a scalar load's wait would also wait on every other scalar
+    // load in flight.
+    uint32_t at = word;
+    if constexpr (kBits) asm volatile("" : "+v"(at));
     const int sy = s.px, sz = static_cast<int>(s.sxy);
     const uint32_t lane = threadIdx.x % kTileX;
     const T *__restrict__ b = s.b;
     T col[kUnroll + 2];  // planes z - 1, z, z + 1, ..., z + kUnroll
     col[0] = b[i - sz];
     col[1] = b[i];
-    for (int z = z0; z < z1; z += kUnroll, i += kUnroll * sz, word += kUnroll * plane) {
+    for (int z = z0; z < z1; z += kUnroll, i += kUnroll * sz, at += kUnroll * plane) {
         const int count = min(kUnroll, z1 - z);
         T xm[kUnroll], xp[kUnroll], ym[kUnroll], yp[kUnroll], prev[kUnroll];
         uint8_t cls[kUnroll];
@@ -394,7 +400,7 @@ __device__ __forceinline__ void interior_column(const StepArgs<T> &s, int i, int
                 ym[u] = b[j - sy];
                 yp[u] = b[j + sy];
                 prev[u] = s.a[j];
-                if constexpr (kBits) bits[u] = s.interior_bits[word + u * plane];
+                if constexpr (kBits) bits[u] = s.interior_bits[at + u * plane];
                 else if constexpr (!ALL) cls[u] = s.types[j];
             }
         }
