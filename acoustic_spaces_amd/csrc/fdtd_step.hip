@@ -1912,32 +1912,82 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     std::vector<uint32_t> sp_kind(G);
     std::vector<uint64_t> kind_first;  // per kind: a sorted special of it
     {
-        std::unordered_map<std::string, uint32_t> seen;
-        std::string key;
+        // Each special's fields hashed on the set-up's threads; then, in order, a special
+        // joins the first kind of equal hash whose fields equal its own (bit for bit), or
+        // starts a kind.
+        const auto bits32 = [](float v) {
+            uint32_t u;
+            std::memcpy(&u, &v, 4);
+            return u;
+        };
+        const auto bits64 = [](double v) {
+            uint64_t u;
+            std::memcpy(&u, &v, 8);
+            return u;
+        };
+        const auto same = [&](uint64_t p, uint64_t q) {
+            const uint64_t a = order[p], c = order[q];
+            if (bits32(coef_a[p]) != bits32(coef_a[q]) || bits64(coef_b[p]) != bits64(coef_b[q]) ||
+                bits64(coef_inv[p]) != bits64(coef_inv[q]) || bits32(source[p]) != bits32(source[q]) ||
+                links_of(a) != links_of(c) || states_of(a) != states_of(c))
+                return false;
+            for (uint64_t u = 0; u < links_of(a); ++u) {
+                const uint64_t l = L.link_start[a] + u, m = L.link_start[c] + u;
+                if (L.link_direction[l] != L.link_direction[m] || bits32(L.link_weight[l]) != bits32(L.link_weight[m]))
+                    return false;
+            }
+            for (uint64_t u = 0; u < states_of(a); ++u) {
+                const uint64_t j = L.state_start[a] + u, m = L.state_start[c] + u;
+                if (L.state_branch[j] != L.state_branch[m] || bits32(L.state_weight[j]) != bits32(L.state_weight[m]))
+                    return false;
+            }
+            return true;
+        };
+        std::vector<uint64_t> hash(G);
+        in_slices(G, 4096, [&](unsigned, uint64_t a, uint64_t b) {
+            for (uint64_t k = a; k < b; ++k) {
+                const uint64_t src = order[k];
+                uint64_t h = 0x9E3779B97F4A7C15ull;
+                const auto mix = [&h](uint64_t v) {
+                    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+                    h *= 0xBF58476D1CE4E5B9ull;
+                    h ^= h >> 31;
+                };
+                mix(bits32(coef_a[k]));
+                mix(bits64(coef_b[k]));
+                mix(bits64(coef_inv[k]));
+                mix(bits32(source[k]));
+                mix(links_of(src) | states_of(src) << 32);
+                for (uint64_t u = 0; u < links_of(src); ++u) {
+                    const uint64_t l = L.link_start[src] + u;
+                    mix(static_cast<uint64_t>(L.link_direction[l]) << 32 | bits32(L.link_weight[l]));
+                }
+                for (uint64_t u = 0; u < states_of(src); ++u) {
+                    const uint64_t j = L.state_start[src] + u;
+                    mix(static_cast<uint64_t>(static_cast<uint32_t>(L.state_branch[j])) << 32 | bits32(L.state_weight[j]));
+                }
+                hash[k] = h;
+            }
+        });
+        std::unordered_map<uint64_t, uint32_t> first_of;  // hash -> its first kind
+        std::vector<uint32_t> also;                       // per kind: the next kind of its hash
         for (uint64_t k = 0; k < G; ++k) {
-            const uint64_t src = order[k];
-            key.clear();
-            const auto put = [&](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
-            put(&coef_a[k], 4);
-            put(&coef_b[k], 8);
-            put(&coef_inv[k], 8);
-            put(&source[k], 4);
-            const uint64_t nl = links_of(src), ns = states_of(src);
-            put(&nl, 8);
-            put(&ns, 8);
-            for (uint64_t u = 0; u < nl; ++u) {
-                const uint64_t l = L.link_start[src] + u;
-                put(&L.link_direction[l], 1);
-                put(&L.link_weight[l], 4);
+            const auto at = first_of.find(hash[k]);
+            uint32_t c = at == first_of.end() ? ~0u : at->second, last = ~0u;
+            while (c != ~0u && !same(kind_first[c], k)) {
+                last = c;
+                c = also[c];
             }
-            for (uint64_t u = 0; u < ns; ++u) {
-                const uint64_t j = L.state_start[src] + u;
-                put(&L.state_branch[j], 4);
-                put(&L.state_weight[j], 4);
+            if (c == ~0u) {
+                c = static_cast<uint32_t>(kind_first.size());
+                kind_first.push_back(k);
+                also.push_back(~0u);
+                if (last == ~0u)
+                    first_of.emplace(hash[k], c);
+                else
+                    also[last] = c;
             }
-            const auto [at, fresh] = seen.emplace(key, static_cast<uint32_t>(kind_first.size()));
-            if (fresh) kind_first.push_back(k);
-            sp_kind[k] = at->second;
+            sp_kind[k] = c;
         }
     }
     const int kinds_asked = tuning ? tuning->special_kinds : 0;

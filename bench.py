@@ -364,9 +364,15 @@ def plain(scene, name, args, traffic_name=None):
         config["insertion_loss"] = insertion_loss(scene, layout)
     return line_for(f"FDTD Mcell-updates/s ({name}, plain solver)", air / per_step_s / 1e6, K, W, ms, scene.name,
                     config, roofline_of(step_bytes, per_step_s, traffic,
-                                        "pair_kernel + step_kernel<float,false,false> (two steps by two launches)"
+                                        f"pair_kernel + {step_kernel_name(info)} (two steps by two launches)"
                                         if launch.get("pairs") else
-                                        "step_kernel<float,false,false> (one fused launch per step)"))
+                                        f"{step_kernel_name(info)} (one fused launch per step)"))
+
+
+def step_kernel_name(info: dict) -> str:
+    """The fused step kernel a float grid of 7 points with one speed of sound launches: its
+    last argument says whether the general specials read their kinds' tables."""
+    return f"step_kernel<float,false,false,{'true' if info.get('special_kinds') else 'false'}>"
 
 
 def source_outputs(scene, args) -> dict:
@@ -465,7 +471,7 @@ def rooms(args):
                     air / per_step_s / 1e6, K, W, ms_batch,
                     "64 shoeboxes U(3,12)xU(3,10)xU(2.5,5) m at 5 cm, Monte Carlo wall absorption, "
                     "1 source and 4 receivers each, merged into batched grids", config,
-                    roofline_of(step_bytes, per_step_s, None, "step_kernel<float,false,false> over all rooms"))
+                    roofline_of(step_bytes, per_step_s, None, f"{step_kernel_name(info)} over all rooms"))
 
 
 def adjoint(args):
