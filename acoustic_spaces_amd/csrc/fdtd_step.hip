@@ -212,6 +212,10 @@ struct StepArgs {
     // Each row's interior cells as bits, a word per kTileX cells: word (z ny + y) tiles_x +
     // x / kTileX, bit x mod kTileX (see interior_column).
     const uint64_t *__restrict__ interior_bits;
+    // Float handles' kinds packed for wide loads: (coef_a, coef_b, coef_inv, source) a kind,
+    // (now, before, gain, weight) a kind's state; the kinds' link weights in float4 rows.
+    const float4 *__restrict__ kind_coef4;
+    const float4 *__restrict__ kind_state4;
 };
 
 // A face special (see the tiles' notes): a special cell whose links are axial and weigh
@@ -562,10 +566,15 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
     const int i = KINDS ? static_cast<int>(r.x) : static_cast<int>(s.sp_cell[k]);
     const int nl = KINDS ? static_cast<int>(r.w & 255u) : s.sp_links[k];
     const int ns = KINDS ? static_cast<int>(r.w >> 8) : s.sp_states[k];
-    const float ca = KINDS ? s.kind_coef_a[kd] : s.coef_a[k];
-    const float src = KINDS ? s.kind_source[kd] : s.source[k];
-    const T cb = KINDS ? s.kind_coef_b[kd] : s.coef_b[k];
-    const T ci = KINDS ? s.kind_coef_inv[kd] : s.coef_inv[k];
+    // Float kinds: one 16-byte load of the coefficients, float4 rows of link weights and one
+    // 16-byte load a state (its branch's now, before and gain with its weight).
+    constexpr bool kPacked = KINDS && sizeof(T) == 4;
+    float4 cf{};
+    if constexpr (kPacked) cf = s.kind_coef4[kd];
+    const float ca = kPacked ? cf.x : (KINDS ? s.kind_coef_a[kd] : s.coef_a[k]);
+    const float src = kPacked ? cf.w : (KINDS ? s.kind_source[kd] : s.source[k]);
+    const T cb = kPacked ? static_cast<T>(cf.y) : (KINDS ? s.kind_coef_b[kd] : s.coef_b[k]);
+    const T ci = kPacked ? static_cast<T>(cf.z) : (KINDS ? s.kind_coef_inv[kd] : s.coef_inv[k]);
     const float sig = s.signal[s.step];
     // Link u's weight at lb + u link_step, chunk c's directions at db + c link_step; state
     // u's weight and branch at kb + u (the kind's) or its slot (the block's slab).
@@ -583,14 +592,26 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
     int32_t off[kLinkChunk];
     float w[kLinkChunk];
     const uint32_t dirs = KINDS ? r.z : (nl > 0 ? link_dirs[db] : 0u);
+    if constexpr (kPacked) {
+        const float4 *__restrict__ rows = reinterpret_cast<const float4 *>(s.kind_weight + lb);
+        float4 w0{}, w1{};
+        if (nl > 0) w0 = rows[0];
+        if (nl > 4) w1 = rows[1];
+        w[0] = w0.x;
+        w[1] = w0.y;
+        w[2] = w0.z;
+        w[3] = w0.w;
+        w[4] = w1.x;
+        w[5] = w1.y;
+    }
 #pragma unroll
     for (int u = 0; u < kLinkChunk; ++u) {
         if (u < nl) {
             off[u] = step_offset((dirs >> (5 * u)) & 31u, sy, sz);
-            w[u] = link_weight[lb + u * link_step];
+            if constexpr (!kPacked) w[u] = link_weight[lb + u * link_step];
         }
     }
-    float sw[kStateChunk], sn[kStateChunk], sbf[kStateChunk];
+    float sw[kStateChunk], sn[kStateChunk], sbf[kStateChunk], sg[kStateChunk];
     T swa[kStateChunk], swb[kStateChunk];
     uint32_t kind[kStateChunk];
 #pragma unroll
@@ -598,17 +619,27 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
         if (u < ns) {
             const uint32_t q = sb + u * kBlock;
             const uint32_t e = KINDS ? kb + u : q;
-            sw[u] = state_weight[e];
-            kind[u] = state_branch[e];
+            if constexpr (kPacked) {
+                const float4 st = s.kind_state4[e];
+                sn[u] = st.x;
+                sbf[u] = st.y;
+                sg[u] = st.z;
+                sw[u] = st.w;
+            } else {
+                sw[u] = state_weight[e];
+                kind[u] = state_branch[e];
+            }
             swa[u] = s.wa[q];
             swb[u] = s.wb[q];
         }
     }
+    if constexpr (!kPacked) {
 #pragma unroll
-    for (int u = 0; u < kStateChunk; ++u) {
-        if (u < ns) {
-            sn[u] = s.branch_now[kind[u]];
-            sbf[u] = s.branch_before[kind[u]];
+        for (int u = 0; u < kStateChunk; ++u) {
+            if (u < ns) {
+                sn[u] = s.branch_now[kind[u]];
+                sbf[u] = s.branch_before[kind[u]];
+            }
         }
     }
     const T here = b[i];
@@ -672,7 +703,8 @@ __device__ __forceinline__ void special_one(const StepArgs<T> &s, uint32_t k) {
         for (int u = 0; u < kStateChunk; ++u) {
             if (u < ns) {
                 const uint32_t q = sb + u * kBlock;
-                s.wo[q] = swa[u] + (rise * static_cast<T>(s.branch_gain[kind[u]]) + own[u]);
+                const T gain = kPacked ? static_cast<T>(sg[u]) : static_cast<T>(s.branch_gain[kind[u]]);
+                s.wo[q] = swa[u] + (rise * gain + own[u]);
             }
         }
         for (int c = kStateChunk; c < ns; ++c) {
@@ -1397,6 +1429,7 @@ struct as_fdtd {
     void *kind_coef_b = nullptr, *kind_coef_inv = nullptr;  // float or double, as the precision
     uint32_t *kind_dirs = nullptr;
     uint16_t *kind_state_branch = nullptr;
+    float4 *kind_coef4 = nullptr, *kind_state4 = nullptr;  // float handles (see StepArgs::kind_coef4)
     uint32_t kinds = 0, kind_links = 0, kind_states = 0;
     size_t state_slots = 0;   // the states' arrays' length (general slabs, then face states)
     uint32_t face_specials = 0;
@@ -1997,6 +2030,7 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     std::vector<double> kind_coef_b, kind_coef_inv;
     std::vector<uint32_t> kind_dirs;
     std::vector<uint16_t> kind_state_branch;
+    std::vector<float4> kind_coef4, kind_state4;  // float handles: (coef_a, coef_b, coef_inv, source), (now, before, gain, weight)
     if (use_kinds) {
         const uint64_t nk = kind_first.size();
         uint64_t wl = 0, ws = 0;
@@ -2004,7 +2038,8 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
             wl = std::max(wl, links_of(order[q]));
             ws = std::max(ws, states_of(order[q]));
         }
-        wl = std::max<uint64_t>((wl + kLinkChunk - 1) / kLinkChunk * kLinkChunk, kLinkChunk);
+        // Whole chunks of directions, and whole float4 rows of weights (see special_one).
+        wl = std::max<uint64_t>((wl + 2 * kLinkChunk - 1) / (2 * kLinkChunk) * (2 * kLinkChunk), 2 * kLinkChunk);
         ws = std::max<uint64_t>(ws, 1);
         if (nk * wl >= (1ull << 32) || nk * ws >= (1ull << 32))
             throw Failure{AS_ERANGE, "too many links or states in the specials' kinds for 32-bit indices"};
@@ -2016,6 +2051,8 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
         kind_dirs.assign(nk * wl / kLinkChunk, 0u);
         kind_state_weight.assign(nk * ws, 0.0f);
         kind_state_branch.assign(nk * ws, 0);
+        kind_coef4.assign(nk, make_float4(0, 0, 0, 0));
+        kind_state4.assign(nk * ws, make_float4(0, 0, 0, 0));
         for (uint64_t c = 0; c < nk; ++c) {
             const uint64_t k = kind_first[c], src = order[k];
             kind_coef_a[c] = coef_a[k];
@@ -2032,7 +2069,12 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
                 const uint64_t j = L.state_start[src] + u;
                 kind_state_weight[c * ws + u] = L.state_weight[j];
                 kind_state_branch[c * ws + u] = static_cast<uint16_t>(L.state_branch[j]);
+                const int32_t br = L.state_branch[j];
+                kind_state4[c * ws + u] =
+                    make_float4(L.branch_now[br], L.branch_before[br], L.branch_gain[br], L.state_weight[j]);
             }
+            kind_coef4[c] = make_float4(coef_a[k], static_cast<float>(coef_b[k]), static_cast<float>(coef_inv[k]),
+                                        source[k]);
         }
         sp_rec.resize(G);
         for (uint64_t k = 0; k < G; ++k) {
@@ -2309,6 +2351,10 @@ void build(as_fdtd *h, const as_layout &L, int precision, const as_tuning *tunin
     h->kind_dirs = upload(h, kind_dirs);
     h->kind_state_weight = upload(h, kind_state_weight);
     h->kind_state_branch = upload(h, kind_state_branch);
+    if (precision == 32) {
+        h->kind_coef4 = upload(h, kind_coef4);
+        h->kind_state4 = upload(h, kind_state4);
+    }
     {
         std::vector<float> now(L.branch_now, L.branch_now + L.branches), before(L.branch_before, L.branch_before + L.branches);
         std::vector<double> gain(L.branches);
@@ -2461,6 +2507,8 @@ StepArgs<T> step_args(const as_fdtd *h) {
     s.kind_dirs = h->kind_dirs;
     s.kind_state_weight = h->kind_state_weight;
     s.kind_state_branch = h->kind_state_branch;
+    s.kind_coef4 = h->kind_coef4;
+    s.kind_state4 = h->kind_state4;
     s.kind_links = h->kind_links;
     s.kind_states = h->kind_states;
     s.branch_now = h->branch_now;
